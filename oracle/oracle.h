@@ -1,0 +1,231 @@
+/*
+ * oracle.h -- FP64 CPU restatement of mtcode's per-timestep Langevin step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (the mtcode_amd package,
+ * libmdx.so, the drop-in host classes) may include, link or call this code.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
+ * and only as the checker.
+ *
+ * Every function restates one reference function; the citation is given next
+ * to its declaration (paths relative to the reference's Code/ directory).
+ * Arithmetic follows the reference expression by expression (operation order,
+ * the long-double `pi` of Basic/basic.h:32, the literal 1.12246 WCA cut-off,
+ * the 0.999999 acos clamp) so that FP64 results agree to rounding.
+ *
+ * Pinning (what checks each function independently of this file):
+ *   orc_fasterfloor / orc_assign_box / orc_boxes_relationships /
+ *   orc_pairs_reference          : golden fixtures from the compiled reference
+ *                                  (oracle/_ref), brute force O(N^2) pair search
+ *   orc_hs_force / orc_wca_energy: closed form, finite difference of energy
+ *   orc_kf2_*                    : golden fixtures, finite differences of
+ *                                  orc_kf2_energy in position and rotation,
+ *                                  rotational covariance, Newton's third law
+ *   orc_bond_* / orc_bend_*      : golden fixtures (harmonic, FENE, harmonic
+ *                                  angle), finite differences; the cosine angle
+ *                                  form and the active force have no reference
+ *                                  counterpart and are pinned by closed form
+ *                                  and finite differences only
+ *   integrators                  : 1000-step trajectory fixtures from the
+ *                                  compiled reference with injected noise,
+ *                                  rotation-matrix orthonormality, affine-in-
+ *                                  noise checks
+ *   orc_philox4x32_10            : the Random123 known-answer vectors
+ *   orc_noise_uniform            : moments / support (its own definition)
+ */
+#ifndef MTCODE_ORACLE_H
+#define MTCODE_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- geometry */
+
+/* Box: orthorhombic generalisation of `cube` (MDBase/geometry.h:43-66).
+ * For a cube l[0]=l[1]=l[2].  `periodic` follows the reference's single
+ * flag (`periodic = pb[0]`, geometry.h:53) for distances and wrapping. */
+typedef struct {
+    double l[3];
+    int periodic;
+} orc_box;
+
+/* Basic/basic.h:58 */
+int orc_fasterfloor(double x);
+
+/* MDBase/geometry.h:508-521 with dim=[1,n0,n0*n1] (MD.cpp:1071-1078).
+ * n per axis; for a cube all three are the reference's single n. */
+int orc_assign_box(const orc_box *b, const double *x3, const int n[3]);
+void orc_assign_all(const orc_box *b, int N, const double *x, const int n[3], int *cell);
+
+/* MDBase/geometry.h:523-609: 27-cell stencil, wrapped, sorted and deduped per
+ * row.  Writes ncells*ncols ints, returns ncols (27 when every n>=3). */
+int orc_boxes_relationships(const int n[3], int *out);
+
+/* MDBase/geometry.h:145-164 (fabs(dx) > l/2 test; strict r2 < cut^2) */
+int orc_distance_less_than(const orc_box *b, const double *xi, const double *xj, double cut);
+
+/* MDBase/geometry.h:277-294 (SQR(dx) > (l/2)^2 test); un = xi - xj min-imaged */
+void orc_distance_vector(const orc_box *b, const double *xi, const double *xj, double un[3], double *r2);
+
+/* MD::calculatepairs (MD.cpp:1035-1098) + precalculatepairs (MD.cpp:575-713),
+ * serial order.  cells per side = n (the reference derives it from the
+ * stencil table).  Writes up to cap pairs (i,j) into ij, returns the count
+ * (which may exceed cap; then only cap were written). */
+long long orc_pairs_reference(const orc_box *b, int N, const double *x, const int n[3],
+                              double cut, int *ij, long long cap);
+
+/* --------------------------------------------------------------- potentials */
+
+/* HSPotential::force (Potentials/potential.h:112-125), paramlimit 1.12246*sigma */
+double orc_hs_force(double r, double eps, double sigma, double rcut);
+/* WCAPotential::energy inner branch (potential.h:179-185) with attrepsilon=0 */
+double orc_wca_energy(double r, double eps, double sigma, double rcut);
+
+/* pair-list force loop MD::calculateforces (MD.cpp:1684-1764).
+ * kind: 0 = HS/WCA (p0=eps, p1=sigma, p2=rcut<=0 -> 1.12246 sigma)
+ *       1 = harmonic (HarmonicPotential, potential.h:265-297; per-pair k, r0
+ *           via kk/r0 arrays when non-NULL, else p0,p1) -- calculateforcesharmonic
+ *           (MD.cpp:2037-2111) when per-pair r0 are given
+ *       2 = FENE (potential.h:345-376; p0=k, p1=R0)
+ * F is accumulated (+=). */
+void orc_pair_forces(const orc_box *b, int N, const double *x, const int *ij, long long np,
+                     int kind, double p0, double p1, double p2,
+                     const double *kk, const double *r0, double *F);
+
+/* three-body loop MD::calculateforces_threebody (MD.cpp:1925-1983) with
+ * form 0 = BendingPotential::force (Potentials/potential3.h:43-114),
+ * form 1 = cosine bending U = kappa*(1 - cos theta) (no reference form).
+ * k and theta0 per triplet (arrays) ; F accumulated. */
+void orc_angle_forces(const orc_box *b, int N, const double *x, const int *ijk, long long nt,
+                      int form, const double *k, const double *theta0, double *F);
+
+/* Tangential active force (no 3-D reference): F_i += fa * unit(x_next - x_prev)
+ * with prev/next = -1 meaning "self".  Accumulated. */
+void orc_active_forces(const orc_box *b, int N, const double *x, const int *prev,
+                       const int *next, double fa, double *F);
+
+/* Patch set: GeneralPatch (Potentials/ComboPatch/GeneralPatch.cpp:7-121)
+ * layout; TetrahedralPatch / BivalentPatch are single-type instances.
+ * params: rows (eps, d, thetam) in GeneralPatch order: for ti<=tj, for a in
+ * patches(ti), for b in patches(tj). */
+#define ORC_MAXTYPES 8
+typedef struct {
+    int ntypes;
+    const int *type;            /* per particle */
+    int npatch[ORC_MAXTYPES];
+    const double *dirs;         /* body-frame unit vectors, type-major, sum(npatch) x 3 */
+    const double *params;       /* rows x 3 */
+} orc_patchset;
+
+/* KernFrenkelOnePatch2::force_and_torque (Potentials/potentialtheta.h:517-750)
+ * un = unit(x_p1 - x_p2), rij = |x_p1 - x_p2|.  Q1,Q2 row-major 3x3 (lab->body).
+ * out: fx fy fz tix tiy tiz tjx tjy tjz */
+void orc_kf2_force_torque(const double un[3], double rij, const double *Q1, const double *Q2,
+                          const double nb1[3], const double nb2[3], double att, double dis,
+                          double thetam, double out[9]);
+/* KernFrenkelOnePatch2::energy (potentialtheta.h:448-515) */
+double orc_kf2_energy(const double un[3], double rij, const double *Q1, const double *Q2,
+                      const double nb1[3], const double nb2[3], double att, double dis,
+                      double thetam);
+
+/* LangevinNVTR::calculate_forces_and_torques3D(pairs, ComboPatch, F, T)
+ * (Rotational/LangevinR.cpp:911-1047).  F, T (lab frame) accumulated. */
+void orc_patch_forces(const orc_box *b, int N, const double *x, const double *Q,
+                      const int *ij, long long np, const orc_patchset *ps, double maxd2,
+                      double *F, double *T);
+/* sum of pair energies over a pair list (particle_energy, LangevinR.cpp:697-802,
+ * without its extra r2>2.5 test: the maxd2 test of the force loop is used) */
+double orc_patch_energy(const orc_box *b, int N, const double *x, const double *Q,
+                        const int *ij, long long np, const orc_patchset *ps, double maxd2);
+
+/* -------------------------------------------------------------- integrators */
+
+typedef struct {
+    double dt, m, gamma, gammar, kT;
+    double I[3];                /* im[0], im[4], im[8] (LangevinR.cpp:44-47) */
+} orc_params;
+
+/* LangevinNVTR::create_forces_and_torques_sphere (LangevinR.cpp:475-592).
+ * F lab (in/out), T lab in -> body out, RT N x 6. */
+void orc_sphere(const orc_params *P, int N, const double *p, const double *L, const double *Q,
+                double *F, double *T, const double *RT);
+/* LangevinNVTR::advancemom_halfstep (LangevinR.cpp:327-347) */
+void orc_halfkick(const orc_params *P, int N, double *p, double *L, const double *F,
+                  const double *T);
+/* LangevinNVT::advance_pos (Langevin.cpp:205-223) + cube::correct_position_and_momentum
+ * (geometry.h:403-445) */
+void orc_advance_pos(const orc_params *P, const orc_box *b, int N, double *x, double *p);
+/* LangevinNVTR::genfullmat (LangevinR.cpp:392-461) */
+void orc_genfullmat(const orc_params *P, const double *L3, double up[9]);
+/* LangevinNVTR::rotate (LangevinR.cpp:595-695) */
+void orc_rotate(const orc_params *P, int N, double *L, double *Q);
+/* LangevinNVT::advance_mom (Langevin.cpp:254-269), coefficients Langevin.cpp:29-53 */
+void orc_advance_mom(const orc_params *P, int N, double *p, const double *F, const double *R);
+
+/* ------------------------------------------------------------- noise & RNG */
+
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11, "Parallel random numbers:
+ * as easy as 1, 2, 3").  ctr[4], key[2] -> out[4]. */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* Shared injected-noise generator used by the parity trajectories: a pure
+ * function of (seed, idx) with the reference's uniform transform
+ * 3.464101615*u - 1.732050808 (DataStructures/matrix2.cpp:1807-1821), u from a
+ * splitmix64 hash of seed and idx (53-bit mantissa). */
+double orc_noise_uniform(uint64_t seed, uint64_t idx);
+/* fills out[count] with orc_noise_uniform(seed, first + k) */
+void orc_noise_fill(uint64_t seed, uint64_t first, long long count, double *out);
+
+/* -------------------------------------------------------- step compositions */
+
+/* Interaction description for the step compositions. */
+typedef struct {
+    /* WCA (HSPotential) on all listed pairs minus exclusions */
+    int wca_on;
+    double wca_eps, wca_sigma, wca_rcut;
+    /* patches */
+    const orc_patchset *patches;  /* NULL: none */
+    double maxd2;
+    /* bonds */
+    const int *bonds; long long nbonds; int bond_kind;     /* 1 harmonic, 2 FENE */
+    const double *bond_k, *bond_r0;
+    /* angles */
+    const int *angles; long long nangles; int angle_form;
+    const double *angle_k, *angle_theta0;
+    /* exclusions from WCA (pairs) */
+    const int *excl; long long nexcl;
+    /* active */
+    double fa; const int *prev; const int *next;
+    /* neighbour list: cut-off and rebuild period (Condensate.cpp:311,345) */
+    double list_cut; int rebuild_every; int cells_per_side[3];
+} orc_interactions;
+
+/* Conservative forces F (lab), patch torques T (lab) on a frozen
+ * configuration (F, T zeroed here).  Pair list built with list_cut. */
+int orc_compute_forces(const orc_box *b, int N, const double *x, const double *Q,
+                       const orc_interactions *it, double *F, double *T);
+
+/* Condensate::run step order (Condensate.cpp:288-377) with injected noise:
+ * noise holds (nsteps+1) x N x 6 values; row 0 primes F,T before the loop.
+ * x,p,Q,L updated in place.  If snap_every>0, writes x after every
+ * snap_every-th step into snaps (nsteps/snap_every x N x 3). */
+int orc_run_rotors(const orc_params *P, const orc_box *b, int N, double *x, double *p,
+                   double *Q, double *L, const orc_interactions *it, int nsteps,
+                   const double *noise, int snap_every, double *snaps);
+
+/* Bead step order of mainActivePolymer.cpp:381-426 (forces, advance_mom,
+ * advance_pos) with injected noise nsteps x N x 3. */
+int orc_run_beads(const orc_params *P, const orc_box *b, int N, double *x, double *p,
+                  const orc_interactions *it, int nsteps, const double *noise,
+                  int snap_every, double *snaps);
+
+/* measured_temperature (LangevinR.cpp:170-193): out[0..2] <p^2/2m>, out[3..5] <L^2/2I0> */
+void orc_measured_temperature(const orc_params *P, int N, const double *p, const double *L,
+                              double out[6]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

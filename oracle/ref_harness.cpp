@@ -1,0 +1,542 @@
+/*
+ * ref_harness.cpp -- drives the reference's own C++ hot-path classes on
+ * files, to make golden fixtures and to time the reference's CPU path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile from the reference
+ * sources where they lie (the reference headers include their .cpp files);
+ * nothing from the reference is copied here, this file only calls it.  The
+ * binary goes to oracle/_ref/ and is never linked into the product.
+ *
+ * Usage: mtref <command> <job-dir>
+ *   job-dir/job.txt holds "key value" lines; arrays are raw little-endian
+ *   files job-dir/<name>.bin (float64 or int32 as documented per command).
+ * Commands:
+ *   cells      x(N,3) -> cells(N) via cube::assign_box, boxes(tb,ncols) via
+ *              cube::generate_boxes_relationships
+ *   pairs      x -> pairs(P,2) via MD::calculatepairs
+ *   forces     x,Q -> F,T via MD::calculateforces(pairs, HSPotential) then
+ *              LangevinNVTR::calculate_forces_and_torques3D(pairs, ComboPatch)
+ *   bonded     x, bonds, angles -> Fb, Fa via MD::calculateforces(bonds, pot)
+ *              and MD::calculateforces_threebody(triplets, BendingPotential)
+ *   traj_rotor x,p,Q,L -> final state + snapshots, Condensate::run step order
+ *              with the injected noise of orc_noise_uniform
+ *   traj_bead  x,p -> final state, mainActivePolymer step order, injected noise
+ *   time_rotor x,Q -> wall-clock of the Condensate::run loop (OpenMP, rand())
+ */
+#include <stdlib.h>
+#include <iostream>
+#include <fstream>
+#include <stdio.h>
+#include <stdarg.h>
+#include <vector>
+#include <algorithm>
+#include <stdexcept>
+#include <sstream>
+#include <string>
+#include <iomanip>
+#include <map>
+#include <sys/stat.h>
+#include <random>
+#include <parallel/algorithm>
+#include <chrono>
+#include <unistd.h>
+#include <execinfo.h>
+#include <signal.h>
+#if defined(_OPENMP)
+#include <omp.h>
+#else
+typedef int omp_int_t;
+inline omp_int_t omp_get_thread_num() { return 0; }
+inline omp_int_t omp_get_max_threads() { return 1; }
+inline omp_int_t omp_get_num_threads() { return 1; }
+#endif
+using namespace std;
+using namespace std::chrono;
+
+#include "Basic/basic.h"
+#include "DataStructures/vector1.h"
+#include "DataStructures/matrix2.h"
+#include "DataStructures/matrix2.cpp"
+#include "MDBase/MD.h"
+#include "MDBase/Langevin.h"
+#include "MDBase/Rotational/LangevinR.h"
+
+#include "oracle.h"
+
+static string g_dir;
+static map<string, string> g_job;
+
+static void die(const string &m)
+{
+    fprintf(stderr, "mtref: %s\n", m.c_str());
+    exit(2);
+}
+
+static void load_job()
+{
+    ifstream f((g_dir + "/job.txt").c_str());
+    if (!f)
+        die("missing job.txt");
+    string k, v;
+    while (f >> k >> v)
+        g_job[k] = v;
+}
+
+static double jd(const char *k, double def = NAN)
+{
+    auto it = g_job.find(k);
+    if (it == g_job.end()) {
+        if (std::isnan(def))
+            die(string("missing key ") + k);
+        return def;
+    }
+    return atof(it->second.c_str());
+}
+static long long ji(const char *k, long long def = -1234567)
+{
+    auto it = g_job.find(k);
+    if (it == g_job.end()) {
+        if (def == -1234567)
+            die(string("missing key ") + k);
+        return def;
+    }
+    return atoll(it->second.c_str());
+}
+
+template <class T> static vector<T> rd(const string &name, size_t count)
+{
+    vector<T> v(count);
+    FILE *f = fopen((g_dir + "/" + name + ".bin").c_str(), "rb");
+    if (!f)
+        die("missing " + name);
+    if (count && fread(v.data(), sizeof(T), count, f) != count)
+        die("short " + name);
+    fclose(f);
+    return v;
+}
+
+template <class T> static void wr(const string &name, const T *p, size_t count)
+{
+    FILE *f = fopen((g_dir + "/" + name + ".bin").c_str(), "wb");
+    if (!f)
+        die("cannot write " + name);
+    if (count)
+        fwrite(p, sizeof(T), count, f);
+    fclose(f);
+}
+
+static matrix<double> to_mat(const vector<double> &v, int n, int c)
+{
+    matrix<double> m(n, c);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < c; j++)
+            m(i, j) = v[(size_t)i * c + j];
+    return m;
+}
+
+static vector<double> from_mat(matrix<double> &m)
+{
+    int n = m.getnrows(), c = m.getncols();
+    vector<double> v((size_t)n * c);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < c; j++)
+            v[(size_t)i * c + j] = m(i, j);
+    return v;
+}
+
+static cube make_cube()
+{
+    bool per = ji("periodic") != 0;
+    vector1<bool> pb(3, per);
+    return cube(jd("L"), pb, 3);
+}
+
+/* ComboPatch from the job: patch_kind tetra (TetrahedralPatch), bivalent
+ * (BivalentPatch) or general (GeneralPatch with ntypes, npatch_<t>,
+ * numpertype_<t> cumulative, params.bin rows x 3, dirs.bin) */
+static ComboPatch *make_patches()
+{
+    string kind = g_job.count("patch_kind") ? g_job["patch_kind"] : "none";
+    if (kind == "tetra")
+        return new TetrahedralPatch(jd("eps_p"), jd("d"), jd("thetam"));
+    if (kind == "bivalent")
+        return new BivalentPatch(jd("eps_p"), jd("d"), jd("thetam"));
+    if (kind == "general") {
+        int nt = (int)ji("ntypes");
+        vector1<int> np(nt), npt(nt);
+        int nrows = 0, ndirs = 0;
+        for (int t = 0; t < nt; t++) {
+            char k1[64], k2[64];
+            snprintf(k1, sizeof k1, "npatch_%d", t);
+            snprintf(k2, sizeof k2, "numpertype_%d", t);
+            np[t] = (int)ji(k1);
+            npt[t] = (int)ji(k2);
+            ndirs += np[t];
+        }
+        for (int i = 0; i < nt; i++)
+            for (int j = i; j < nt; j++)
+                nrows += np[i] * np[j];
+        matrix<double> params = to_mat(rd<double>("params", (size_t)nrows * 3), nrows, 3);
+        matrix<double> dirs = to_mat(rd<double>("dirs", (size_t)ndirs * 3), ndirs, 3);
+        return new GeneralPatch(np, npt, params, dirs);
+    }
+    return NULL;
+}
+
+static int cmd_cells()
+{
+    int N = (int)ji("N"), n = (int)ji("n");
+    cube geo = make_cube();
+    matrix<double> x = to_mat(rd<double>("x", (size_t)N * 3), N, 3);
+    vector1<int> dim(3);
+    dim[0] = 1;
+    dim[1] = n;
+    dim[2] = n * n;
+    vector<int> cells(N);
+    for (int i = 0; i < N; i++)
+        cells[i] = geo.assign_box(x, i, dim, n);
+    wr("cells", cells.data(), cells.size());
+    int tb;
+    matrix<int> boxes = geo.generate_boxes_relationships(n, tb);
+    vector<int> bx((size_t)boxes.getnrows() * boxes.getncols());
+    for (int i = 0; i < boxes.getnrows(); i++)
+        for (int j = 0; j < boxes.getncols(); j++)
+            bx[(size_t)i * boxes.getncols() + j] = boxes(i, j);
+    wr("boxes", bx.data(), bx.size());
+    int shape[2] = {boxes.getnrows(), boxes.getncols()};
+    wr("boxes_shape", shape, 2);
+    return 0;
+}
+
+static matrix<int> *make_pairs(LangevinNVT &b, cube &geo, int n, double cut)
+{
+    int tb;
+    matrix<int> boxes = geo.generate_boxes_relationships(n, tb);
+    return b.calculatepairs(boxes, cut);
+}
+
+static void write_pairs(matrix<int> *pairs)
+{
+    vector<int> pv((size_t)pairs->getnrows() * 2);
+    for (int i = 0; i < pairs->getnrows(); i++) {
+        pv[2 * (size_t)i] = (*pairs)(i, 0);
+        pv[2 * (size_t)i + 1] = (*pairs)(i, 1);
+    }
+    wr("pairs", pv.data(), pv.size());
+}
+
+static int cmd_pairs()
+{
+    int N = (int)ji("N");
+    cube geo = make_cube();
+    LangevinNVT b(geo);
+    matrix<double> x = to_mat(rd<double>("x", (size_t)N * 3), N, 3);
+    b.setdat(x);
+    matrix<int> *pairs = make_pairs(b, geo, (int)ji("n"), jd("cut"));
+    write_pairs(pairs);
+    delete pairs;
+    return 0;
+}
+
+static int cmd_forces()
+{
+    int N = (int)ji("N");
+    cube geo = make_cube();
+    LangevinNVTR b(geo);
+    matrix<double> x = to_mat(rd<double>("x", (size_t)N * 3), N, 3);
+    matrix<double> Q = to_mat(rd<double>("Q", (size_t)N * 9), N, 9);
+    matrix<double> p0(N, 3), L0(N, 3);
+    b.initialize(x, p0, Q, L0);
+    b.setmaxdistance(jd("maxd2", 2.4));
+    matrix<int> *pairs = make_pairs(b, geo, (int)ji("n"), jd("cut"));
+    write_pairs(pairs);
+    HSPotential wsa(jd("wca_eps"), jd("wca_sigma"));
+    matrix<double> F = b.calculateforces(*pairs, wsa);
+    vector<double> Fw = from_mat(F);
+    wr("F_wca", Fw.data(), Fw.size());
+    matrix<double> T(N, 3);
+    ComboPatch *pots = make_patches();
+    double epot = 0.0;
+    if (pots) {
+        b.calculate_forces_and_torques3D(*pairs, *pots, F, T);
+        /* patch energy with the force loop's own pair filter */
+        double maxd2 = jd("maxd2", 2.4);
+        for (int i = 0; i < pairs->getnrows(); i++) {
+            int p1 = (*pairs)(i, 0), p2 = (*pairs)(i, 1);
+            if (p2 < p1)
+                std::swap(p1, p2);
+            vector1<double> un(3);
+            double dis;
+            matrix<double> xx = b.getdat();
+            geo.distance_vector(xx, p1, p2, un, dis);
+            if (!(dis < maxd2))
+                continue;
+            dis = sqrt(dis);
+            un /= dis;
+            int **q = new int *;
+            if (pots->safe) {
+                pots->UpdateIterator(p1, p2);
+                *q = *pots->p;
+            } else
+                pots->UpdateIteratorSafe(p1, p2, q);
+            for (int tp = 1; tp < (*q)[0] + 1; tp++)
+                epot += pots->potential_bundle[(*q)[tp]]->energy(un, dis, b.getorientation(), p1, p2);
+            delete q;
+        }
+    }
+    vector<double> Fv = from_mat(F), Tv = from_mat(T);
+    wr("F", Fv.data(), Fv.size());
+    wr("T", Tv.data(), Tv.size());
+    wr("epot", &epot, 1);
+    delete pairs;
+    return 0;
+}
+
+static int cmd_bonded()
+{
+    int N = (int)ji("N");
+    cube geo = make_cube();
+    LangevinNVT b(geo);
+    matrix<double> x = to_mat(rd<double>("x", (size_t)N * 3), N, 3);
+    b.setdat(x);
+    int nb = (int)ji("nbonds"), na = (int)ji("nangles");
+    vector<int> bl = rd<int>("bonds", (size_t)nb * 2);
+    vector<int> al = rd<int>("angles", (size_t)na * 3);
+    matrix<int> bonds(nb, 2), tri(na, 3);
+    for (int i = 0; i < nb; i++) {
+        bonds(i, 0) = bl[2 * i];
+        bonds(i, 1) = bl[2 * i + 1];
+    }
+    for (int i = 0; i < na; i++)
+        for (int k = 0; k < 3; k++)
+            tri(i, k) = al[3 * i + k];
+    string kind = g_job["bond_kind"];
+    matrix<double> Fb;
+    if (kind == "harmonic") {
+        HarmonicPotential hp(jd("bond_k"), jd("bond_r0"));
+        Fb = b.calculateforces(bonds, hp);
+    } else if (kind == "fene") {
+        FENEPotential fp(jd("bond_k"), jd("bond_r0"));
+        Fb = b.calculateforces(bonds, fp);
+    } else {
+        /* per-bond rest lengths: calculateforcesharmonic */
+        matrix<double> r0 = to_mat(rd<double>("bond_r0s", (size_t)nb), nb, 1);
+        Fb = b.calculateforcesharmonic(bonds, r0, jd("bond_k"));
+    }
+    BendingPotential bp(jd("angle_k"), jd("angle_theta0"));
+    matrix<double> Fa = b.calculateforces_threebody(tri, bp);
+    vector<double> v1 = from_mat(Fb), v2 = from_mat(Fa);
+    wr("Fb", v1.data(), v1.size());
+    wr("Fa", v2.data(), v2.size());
+    return 0;
+}
+
+static void set_rotor_params(LangevinNVTR &b)
+{
+    /* Condensate::Condensate (Condensate.cpp:70-79) setter order */
+    b.setdt(jd("dt"));
+    b.setgamma(jd("gamma"));
+    b.setgammar(jd("gammar"));
+    b.setkT(jd("kT"));
+    b.setmaxdistance(jd("maxd2", 2.4));
+}
+
+static int cmd_traj_rotor()
+{
+    int N = (int)ji("N"), nsteps = (int)ji("nsteps"), n = (int)ji("n");
+    int rebuild = (int)ji("rebuild", 20), snap = (int)ji("snap_every", 0);
+    uint64_t seed = (uint64_t)ji("noise_seed");
+    cube geo = make_cube();
+    LangevinNVTR b(geo);
+    matrix<double> x = to_mat(rd<double>("x", (size_t)N * 3), N, 3);
+    matrix<double> p = to_mat(rd<double>("p", (size_t)N * 3), N, 3);
+    matrix<double> Q = to_mat(rd<double>("Q", (size_t)N * 9), N, 9);
+    matrix<double> L = to_mat(rd<double>("L", (size_t)N * 3), N, 3);
+    b.initialize(x, p, Q, L);
+    set_rotor_params(b);
+    ComboPatch *pots = make_patches();
+    double cut = jd("cut");
+    int tb;
+    matrix<int> boxes = geo.generate_boxes_relationships(n, tb);
+    matrix<int> *pairs = b.calculatepairs(boxes, cut);
+    HSPotential wsa(jd("wca_eps"), jd("wca_sigma"));
+    matrix<double> F(N, 3), T(N, 3), RT(N, 6);
+    vector<double> snaps;
+    /* Condensate.cpp:318-331 */
+    F = b.calculateforces(*pairs, wsa);
+    if (pots)
+        b.calculate_forces_and_torques3D(*pairs, *pots, F, T);
+    vector<double> nz((size_t)N * 6);
+    orc_noise_fill(seed, 0, (long long)N * 6, nz.data());
+    RT = to_mat(nz, N, 6);
+    b.create_forces_and_torques_sphere(F, T, RT);
+    /* Condensate.cpp:335-377 */
+    for (int i = 0; i < nsteps; i++) {
+        if (i > 0 && i % rebuild == 0) {
+            delete pairs;
+            pairs = b.calculatepairs(boxes, cut);
+        }
+        b.advancemom_halfstep(F, T);
+        b.advance_pos();
+        b.rotate();
+        F = b.calculateforces(*pairs, wsa);
+        T.reset_parallel(0.0);
+        if (pots)
+            b.calculate_forces_and_torques3D(*pairs, *pots, F, T);
+        orc_noise_fill(seed, (uint64_t)(i + 1) * N * 6, (long long)N * 6, nz.data());
+        RT = to_mat(nz, N, 6);
+        b.create_forces_and_torques_sphere(F, T, RT);
+        b.advancemom_halfstep(F, T);
+        if (snap > 0 && (i + 1) % snap == 0) {
+            matrix<double> xm = b.getdat();
+            vector<double> xs = from_mat(xm);
+            snaps.insert(snaps.end(), xs.begin(), xs.end());
+        }
+    }
+    matrix<double> xf = b.getdat(), pf = b.getmom();
+    vector<double> v1 = from_mat(xf), v2 = from_mat(pf), v3 = from_mat(b.getorientation()),
+                   v4 = from_mat(b.getangmom());
+    wr("x_out", v1.data(), v1.size());
+    wr("p_out", v2.data(), v2.size());
+    wr("Q_out", v3.data(), v3.size());
+    wr("L_out", v4.data(), v4.size());
+    wr("snaps", snaps.data(), snaps.size());
+    delete pairs;
+    return 0;
+}
+
+static int cmd_traj_bead()
+{
+    int N = (int)ji("N"), nsteps = (int)ji("nsteps"), n = (int)ji("n");
+    int rebuild = (int)ji("rebuild", 20), snap = (int)ji("snap_every", 0);
+    uint64_t seed = (uint64_t)ji("noise_seed");
+    cube geo = make_cube();
+    LangevinNVT b(geo);
+    matrix<double> x = to_mat(rd<double>("x", (size_t)N * 3), N, 3);
+    matrix<double> p = to_mat(rd<double>("p", (size_t)N * 3), N, 3);
+    b.setdat(x);
+    b.setmom(p);
+    b.setgamma(jd("gamma"));
+    b.setdt(jd("dt"));
+    b.setkT(jd("kT"));
+    int nb = (int)ji("nbonds");
+    vector<int> bl = rd<int>("bonds", (size_t)nb * 2);
+    matrix<int> bonds(nb, 2);
+    for (int i = 0; i < nb; i++) {
+        bonds(i, 0) = bl[2 * i];
+        bonds(i, 1) = bl[2 * i + 1];
+    }
+    HarmonicPotential hp(jd("bond_k"), jd("bond_r0"));
+    HSPotential wsa(jd("wca_eps"), jd("wca_sigma"));
+    double cut = jd("cut");
+    int tb;
+    matrix<int> boxes = geo.generate_boxes_relationships(n, tb);
+    matrix<int> *pairs = b.calculatepairs(boxes, cut);
+    vector<double> nz((size_t)N * 3), snaps;
+    /* mainActivePolymer.cpp:381-426 */
+    for (int i = 0; i < nsteps; i++) {
+        if (i > 0 && i % rebuild == 0) {
+            delete pairs;
+            pairs = b.calculatepairs(boxes, cut);
+        }
+        matrix<double> F1(b.calculateforces(bonds, hp));
+        matrix<double> F2(b.calculateforces(*pairs, wsa));
+        matrix<double> F = F1;
+        F += F2;
+        orc_noise_fill(seed, (uint64_t)i * N * 3, (long long)N * 3, nz.data());
+        matrix<double> R = to_mat(nz, N, 3);
+        b.advance_mom(F, R);
+        b.advance_pos();
+        if (snap > 0 && (i + 1) % snap == 0) {
+            matrix<double> xs = b.getdat();
+            vector<double> v = from_mat(xs);
+            snaps.insert(snaps.end(), v.begin(), v.end());
+        }
+    }
+    matrix<double> xf = b.getdat(), pf = b.getmom();
+    vector<double> v1 = from_mat(xf), v2 = from_mat(pf);
+    wr("x_out", v1.data(), v1.size());
+    wr("p_out", v2.data(), v2.size());
+    wr("snaps", snaps.data(), snaps.size());
+    delete pairs;
+    return 0;
+}
+
+static int cmd_time_rotor()
+{
+    /* The Condensate::run loop (Condensate.cpp:288-377) exactly as the
+     * reference runs it: rand() noise, 3.5 Verlet list rebuilt every 20 steps,
+     * OpenMP inside each call.  Prints one line: seconds per step. */
+    int N = (int)ji("N"), nsteps = (int)ji("nsteps"), n = (int)ji("n");
+    cube geo = make_cube();
+    LangevinNVTR b(geo);
+    matrix<double> x = to_mat(rd<double>("x", (size_t)N * 3), N, 3);
+    matrix<double> Q = to_mat(rd<double>("Q", (size_t)N * 9), N, 9);
+    matrix<double> p(N, 3), L(N, 3);
+    b.initialize(x, p, Q, L);
+    set_rotor_params(b);
+    ComboPatch *pots = make_patches();
+    double cut = jd("cut");
+    srand(1);
+    auto t0 = high_resolution_clock::now();
+    int tb;
+    matrix<int> boxes = geo.generate_boxes_relationships(n, tb);
+    matrix<int> *pairs = b.calculatepairs(boxes, cut);
+    HSPotential wsa(jd("wca_eps"), jd("wca_sigma"));
+    matrix<double> F(N, 3), T(N, 3), RT(N, 6);
+    F = b.calculateforces(*pairs, wsa);
+    b.calculate_forces_and_torques3D(*pairs, *pots, F, T);
+    generate_uniform_random_matrix(RT);
+    b.create_forces_and_torques_sphere(F, T, RT);
+    auto t1 = high_resolution_clock::now();
+    for (int i = 0; i < nsteps; i++) {
+        vector1<double> meas(6);
+        b.measured_temperature(meas);
+        if (i > 0 && i % 20 == 0) {
+            delete pairs;
+            pairs = b.calculatepairs(boxes, cut);
+        }
+        b.advancemom_halfstep(F, T);
+        b.advance_pos();
+        b.rotate();
+        F = b.calculateforces(*pairs, wsa);
+        T.reset_parallel(0.0);
+        b.calculate_forces_and_torques3D(*pairs, *pots, F, T);
+        generate_uniform_random_matrix(RT);
+        b.create_forces_and_torques_sphere(F, T, RT);
+        b.advancemom_halfstep(F, T);
+    }
+    auto t2 = high_resolution_clock::now();
+    double setup = duration_cast<duration<double>>(t1 - t0).count();
+    double loop = duration_cast<duration<double>>(t2 - t1).count();
+    printf("{\"setup_s\": %.6f, \"loop_s\": %.6f, \"steps\": %d, \"N\": %d, \"threads\": %d, "
+           "\"pairs\": %d}\n",
+           setup, loop, nsteps, N, omp_get_max_threads(), pairs->getnrows());
+    delete pairs;
+    return 0;
+}
+
+int main(int argc, char **argv)
+{
+    if (argc != 3)
+        die("usage: mtref <command> <job-dir>");
+    string cmd = argv[1];
+    g_dir = argv[2];
+    load_job();
+    if (cmd == "cells")
+        return cmd_cells();
+    if (cmd == "pairs")
+        return cmd_pairs();
+    if (cmd == "forces")
+        return cmd_forces();
+    if (cmd == "bonded")
+        return cmd_bonded();
+    if (cmd == "traj_rotor")
+        return cmd_traj_rotor();
+    if (cmd == "traj_bead")
+        return cmd_traj_bead();
+    if (cmd == "time_rotor")
+        return cmd_time_rotor();
+    die("unknown command " + cmd);
+    return 2;
+}

@@ -1,0 +1,117 @@
+"""The oracle (oracle/oracle.c) against golden vectors written by the
+reference's own C++ classes (tests/golden/make_golden.py)."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from mtcode_amd import workloads as wl
+
+
+def relmax(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 6, 16])
+def test_cells_and_stencil_bit_exact(golden, n):
+    g = golden("cells_patchy4096.npz")
+    assert np.array_equal(po.assign_all(g["L"], g["x"], n), g[f"cells_n{n}"])
+    boxes = po.boxes_relationships(n)
+    if f"boxes_n{n}" in g:
+        assert np.array_equal(boxes, g[f"boxes_n{n}"])
+    else:
+        h = (boxes.astype(np.int64) * np.arange(1, boxes.shape[1] + 1)).sum(1)
+        assert np.array_equal(h, g[f"boxes_n{n}_row_hash"])
+        assert boxes.sum() == g[f"boxes_n{n}_sum"]
+
+
+def test_pairs_bit_exact_in_reference_order(golden):
+    g = golden("pairs.npz")
+    keys = [k for k in g if k.startswith("pairs512")]
+    assert len(keys) == 6
+    for k in keys:
+        parts = k.split("_")
+        n, cut = int(parts[1][1:]), float(parts[2][3:])
+        pr = po.pairs_reference(g["L512"], g["x512"], n, cut, not k.endswith("nonperiodic"))
+        assert np.array_equal(pr, g[k]), k
+
+
+def test_pairs_full_size_set(golden):
+    g = golden("pairs.npz")
+    s = wl.patchy_tetra()
+    pr = po.pairs_reference(s["box"][0], s["x"], 16, 1.68)
+    keys = np.sort(np.sort(pr, axis=1).astype(np.int64) @ np.array([1, 1 << 20]))
+    assert len(pr) == g["pairs4096_n16_cut1.68_count"]
+    assert np.array_equal(keys, g["pairs4096_n16_cut1.68_keys"])
+
+
+@pytest.mark.parametrize("name,builder", [("forces_patchy4096.npz", lambda: wl.patchy_tetra()),
+                                          ("forces_condensate4096.npz",
+                                           lambda: wl.condensate_ab(4096))])
+def test_forces_torques_energy(golden, name, builder):
+    g = golden(name)
+    s = builder()
+    assert np.array_equal(s["x"], g["x"]) and np.array_equal(s["Q"], g["Q"])
+    F, T = po.compute_forces(s, list_cut=float(g["cut"]), n_side=int(g["n"]))
+    assert relmax(F, g["F"]) < 1e-13
+    assert relmax(T, g["T"]) < 1e-13
+    assert np.count_nonzero(g["T"]) > 20
+    e = po.patch_energy(s, float(g["cut"]), int(g["n"]))
+    assert abs(e - g["epot"]) < 1e-12 * abs(g["epot"])
+
+
+@pytest.mark.parametrize("name,dirs", [("tetra", wl.TETRA_DIRS), ("bivalent", wl.BIVALENT_DIRS)])
+def test_two_particle_sweeps(golden, name, dirs):
+    g = golden("pairsweep.npz")
+    x = g[name + "_x"]
+    N = len(x)
+    ps = {"npatch": [len(dirs)], "dirs": dirs,
+          "params": np.tile([6.0, 1.2, wl.THETAM_095], (len(dirs) ** 2, 1)),
+          "types": np.zeros(N, np.int32), "maxd2": 2.4}
+    L = float(g[name + "_L"])
+    s = {"box": np.repeat(L, 3), "periodic": True, "x": x, "Q": g[name + "_Q"],
+         "wca": (1.0, 1.0, 1.12246), "patches": ps}
+    F, T = po.compute_forces(s, list_cut=3.5, n_side=int(L // 4))
+    assert relmax(F, g[name + "_F"]) < 1e-13
+    assert relmax(T, g[name + "_T"]) < 1e-13
+
+
+def test_bonded(golden):
+    g = golden("bonded.npz")
+    x, L, b, a = g["x"], float(g["L"]), g["bonds"], g["angles"]
+    assert np.array_equal(po.pair_forces(L, x, b, 1, 100.0, 1.0), g["Fb_harmonic"])
+    assert np.array_equal(po.pair_forces(L, x, b, 2, 30.0, 1.5), g["Fb_fene"])
+    Fp = po.pair_forces(L, x, b, 1, 0, 0, kk=np.full(len(b), 100.0), r0=g["r0s"])
+    assert np.array_equal(Fp, g["Fb_perbond"])
+    for th in (0.3, 0.0):
+        Fa = po.angle_forces(L, x, a, 0, np.full(len(a), 50.0), np.full(len(a), th))
+        assert relmax(Fa, g[f"Fa_theta0_{th}"]) < 1e-14
+
+
+@pytest.mark.parametrize("N", [64, 512])
+def test_rotor_trajectory_1000_steps(golden, N):
+    """Condensate::run step order, injected noise, 1000 steps.  The two codes
+    differ only in last-bit rounding (e.g. libm), which the dynamics amplify
+    by about 10x per 100 steps; the first snapshots agree to 1e-13."""
+    g = golden(f"traj_rotor{N}.npz")
+    s = wl.patchy_tetra(N)
+    r = po.run_rotors(s, int(g["nsteps"]), int(g["noise_seed"]), int(g["snap_every"]), 3.5, 20,
+                      int(g["n"]))
+    assert np.max(np.abs(r["snaps"][0] - g["snaps"][0])) < 1e-13
+    assert np.max(np.abs(r["x"] - g["x_out"])) < 1e-7
+    assert np.max(np.abs(r["Q"] - g["Q_out"])) < 1e-7
+    assert relmax(r["p"], g["p_out"]) < 1e-6
+    assert relmax(r["L"], g["L_out"]) < 1e-6
+
+
+def test_bead_trajectory_1000_steps(golden):
+    g = golden("traj_bead128.npz")
+    N = len(g["x"])
+    s = {"N": N, "box": np.repeat(float(g["box"]), 3), "periodic": True, "x": g["x"],
+         "p": np.zeros((N, 3)), "dt": 0.002, "m": 1.0, "gamma": 1.0, "gammar": 1.0, "kT": 1.0,
+         "I": [0.1] * 3, "wca": (1.0, 1.0, 1.12246), "patches": None, "bonds": g["bonds"],
+         "bond_k": np.full(len(g["bonds"]), 100.0), "bond_r0": np.ones(len(g["bonds"])),
+         "bond_kind": 1}
+    r = po.run_beads(s, int(g["nsteps"]), int(g["noise_seed"]), int(g["snap_every"]), 3.5, 20,
+                     int(g["n"]))
+    assert np.max(np.abs(r["x"] - g["x_out"])) < 1e-10
+    assert relmax(r["p"], g["p_out"]) < 1e-10
