@@ -1,0 +1,156 @@
+/*
+ * mdx.h -- C ABI of the MI355X (gfx950) Langevin step engine.
+ *
+ * One engine owns one HIP stream on one GPU and all device memory of a batch
+ * of independent systems ("replicas"; a single system is a batch of one).
+ * Host arrays use the reference's row-major FP64 layout
+ * (DataStructures/matrix2.h:33-36): positions, momenta, forces, torques and
+ * angular momenta N x 3, orientations N x 9 (lab -> body,
+ * MDBase/Rotational/LangevinR.cpp:1-2).  Particles of replica r are stored
+ * after those of replicas 0..r-1.  Host pointers are caller-owned and only
+ * touched during the call.  Every function returns MDX_OK (0) or an error
+ * code; mdx_last_error() gives the text (the reference's error() printed,
+ * blocked on stdin and exited: Basic/basic.h:61-71).  Calls on one engine
+ * must not overlap; different engines are independent.
+ *
+ * Reference paths cited below are relative to the reference's Code/ folder.
+ */
+#ifndef MDX_H
+#define MDX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDX_OK 0
+#define MDX_ERR_ARG 1       /* invalid argument or size */
+#define MDX_ERR_HIP 2       /* HIP runtime failure */
+#define MDX_ERR_STATE 3     /* call out of order (e.g. step before prime) */
+#define MDX_ERR_FORCE 4     /* |f| of one pair term above the guard (MD.cpp:1711-1749) */
+#define MDX_ERR_NAN 5       /* non-finite force (BasicGPU/basic.cu:14-27) */
+#define MDX_ERR_CAPACITY 6  /* output buffer too small; the count is still returned */
+
+#define MDX_PREC_FP64 0     /* everything in FP64 */
+#define MDX_PREC_MIXED 1    /* FP32 pair math and p/Q/L; FP64 positions and decisions */
+
+#define MDX_INTEG_ROTOR 0   /* LangevinNVTR step (Condensate.cpp:335-377) */
+#define MDX_INTEG_BEAD 1    /* LangevinNVT step (mainActivePolymer.cpp:381-426) */
+
+#define MDX_BOND_HARMONIC 1 /* HarmonicPotential / calculateforcesharmonic */
+#define MDX_BOND_FENE 2     /* FENEPotential */
+
+#define MDX_ANGLE_HARMONIC 0 /* BendingPotential: 0.5 k (theta - theta0)^2 */
+#define MDX_ANGLE_COSINE 1   /* k (1 - cos theta) (no reference counterpart) */
+
+typedef struct mdx_engine mdx_engine;
+
+typedef struct mdx_desc {
+    int nreplicas;            /* >= 1 */
+    const int *n_per_replica; /* particle count of each replica */
+    const double *box;        /* nreplicas x 3 box edges (a cube: equal edges; cube, geometry.h:43-66) */
+    int periodic;             /* 1 periodic (cube pb), 0 reflecting walls (geometry.h:433-444) */
+    const double *kT;         /* nreplicas temperatures (LangevinNVT::setkT) */
+    double dt, m, gamma, gammar;  /* LangevinNVT::setdt/setm/setgamma, LangevinNVTR::setgammar */
+    double inertia[3];        /* im[0], im[4], im[8] (LangevinR.cpp:44-47; 0.1 each) */
+    int precision;            /* MDX_PREC_* */
+    int integrator;           /* MDX_INTEG_* */
+    unsigned long long seed;  /* Philox4x32-10 key for the engine's own noise */
+    int device;               /* HIP device ordinal */
+} mdx_desc;
+
+/* Lifecycle.  Replaces constructing LangevinNVT(cube&) / LangevinNVTR(cube&)
+ * (Langevin.cpp:34-52, LangevinR.cpp:41-65) plus the parameter setters
+ * (LangevinR.h:109-137). */
+int mdx_create(const mdx_desc *desc, mdx_engine **out);
+void mdx_destroy(mdx_engine *e);
+const char *mdx_last_error(void);
+
+/* State.  Replaces MD::setdat (MD.cpp:64), LangevinNVT::setmom
+ * (Langevin.cpp:170-174), LangevinNVTR::setorientation / setangularmomenta /
+ * initialize (LangevinR.h:141-163, LangevinR.cpp:308-325); p, Q, L may be
+ * NULL (zero momenta, identity orientation).  Upload drops the primed forces.
+ * Download replaces getdat/getmom/getorientation/getangmom. */
+int mdx_upload_state(mdx_engine *e, const double *x, const double *p, const double *Q,
+                     const double *L);
+int mdx_download_state(mdx_engine *e, double *x, double *p, double *Q, double *L);
+/* total force (lab) and torque (body) carried between steps: the F and T of
+ * Condensate::run after create_forces_and_torques_sphere (Condensate.cpp:370) */
+int mdx_download_forces(mdx_engine *e, double *F, double *T);
+
+/* Interactions. */
+/* HSPotential(eps, sigma) as used by MD::calculateforces (potential.h:79-125);
+ * rcut <= 0 selects the reference's literal 1.12246 sigma (potential.h:84). */
+int mdx_set_pair_wca(mdx_engine *e, double eps, double sigma, double rcut);
+/* GeneralPatch (Potentials/ComboPatch/GeneralPatch.cpp:7-121) with
+ * KernFrenkelOnePatch2 rows (potentialtheta.h:379-797): ntypes <= 8,
+ * type_of_particle N ints, npatch_per_type ntypes ints, dirs sum(npatch) x 3
+ * body-frame unit vectors (type-major), params rows (eps, d, theta_m) in
+ * GeneralPatch order (for ti <= tj, for patch a of ti, for patch b of tj),
+ * maxd2 = LangevinNVTR::max_distance_check (LangevinR.h:70, default 2.4),
+ * compared with r^2.  TetrahedralPatch / BivalentPatch are one-type cases. */
+int mdx_set_patches(mdx_engine *e, int ntypes, const int *type_of_particle,
+                    const int *npatch_per_type, const double *dirs, const double *params,
+                    double maxd2);
+/* per-replica patch rows (same layout as params above) */
+int mdx_set_replica_patch_params(mdx_engine *e, int replica, const double *params);
+/* bonds: MD::calculateforces(bonds, HarmonicPotential|FENEPotential)
+ * (MD.cpp:1684-1764) and calculateforcesharmonic (MD.cpp:2037-2111): per
+ * bond k and r0 (FENE: r0 is R_0). */
+int mdx_set_bonds(mdx_engine *e, long long nbonds, const int *ij, const double *k,
+                  const double *r0, int kind);
+/* angles: MD::calculateforces_threebody(triplets, BendingPotential)
+ * (MD.cpp:1925-1983, potential3.h:43-114), or the cosine form */
+int mdx_set_angles(mdx_engine *e, long long nangles, const int *ijk, const double *k,
+                   const double *theta0, int form);
+/* pairs excluded from the WCA term (no reference counterpart: the reference
+ * has no exclusion list) */
+int mdx_set_exclusions(mdx_engine *e, long long npairs, const int *ij);
+/* tangential active force fa * unit(x_next - x_prev), -1 = the particle itself
+ * (no 3-D reference; cf. PositionForcesDueToAnglesGPUx, MicrotubuleGPU.cu:357-411) */
+int mdx_set_active(mdx_engine *e, double fa, const int *prev, const int *next);
+/* |pair force| guard; 0 disables.  The reference prints "force too big" and
+ * pauses above 1e4 (MD.cpp:1711-1749); here mdx_step returns MDX_ERR_FORCE. */
+int mdx_set_force_guard(mdx_engine *e, double fmax);
+
+/* Frozen-configuration entry points (parity). */
+/* cube::assign_box with dim [1,n,n^2] (geometry.h:508-521, MD.cpp:1071-1078)
+ * for replica r with n[3] cells per side; cell_of receives N_r ints */
+int mdx_cell_assign(mdx_engine *e, int replica, const int n[3], int *cell_of);
+/* MD::calculatepairs (MD.cpp:1035-1098, 575-713) as a set: every i<j of
+ * replica r (local indices) with cube::distance_less_than(i, j, cut)
+ * (geometry.h:145-164), sorted by (i, j).  n[3] cells per side, n >= 1. */
+int mdx_neighbor_pairs(mdx_engine *e, int replica, const int n[3], double cut, int *ij,
+                       long long cap, long long *npairs);
+/* conservative force F (lab) and patch torque T (lab) at the current state:
+ * calculateforces(pairs, HSPotential) + bonds + angles + active +
+ * calculate_forces_and_torques3D(pairs, ComboPatch, F, T) (LangevinR.cpp:911-1047) */
+int mdx_compute_forces(mdx_engine *e, double *F, double *T);
+
+/* Time stepping. */
+/* Rotor integrator: computes the carried F, T at the current state
+ * (Condensate.cpp:318-331).  noise0 (N x 6, nullable: Philox) */
+int mdx_prime(mdx_engine *e, const double *noise0);
+/* nsteps steps; rotor: the loop body of Condensate::run (Condensate.cpp:352-377),
+ * bead: mainActivePolymer.cpp:390-425.  noise: nullable; else nsteps x N x 6
+ * (rotor, row s is used by step s) or nsteps x N x 3 (bead) injected values in
+ * place of generate_uniform_random_matrix (matrix2.cpp:1807-1821). */
+int mdx_step(mdx_engine *e, int nsteps, const double *noise);
+/* FP64 reductions of replica r (or -1: all): out[0..2] <p_k^2/2m>,
+ * out[3..5] <L_k^2/2I_0> (LangevinR.cpp:170-193), out[6] potential energy,
+ * out[7..15] virial sum_pairs r_ij (x) f_ij (calculatestress, MD.cpp:2381-2424) */
+int mdx_observables(mdx_engine *e, int replica, double *out);
+
+/* Diagnostics. */
+/* per-kernel average device time (ms) over nsteps eagerly launched steps,
+ * timed with HIP events on the engine's stream; names[k] are static strings.
+ * Returns the kernel count in *nk (<= cap). */
+int mdx_profile_step(mdx_engine *e, int nsteps, int cap, const char **names, double *ms,
+                     int *nk);
+/* engine facts: out[0] N, out[1] total cells, out[2] cell size (replica 0, x),
+ * out[3] bytes of device memory held, out[4] steps taken */
+int mdx_info(mdx_engine *e, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,318 @@
+"""ctypes front end of libmdx.so (the C ABI of include/mdx.h).
+
+The engine is the product; this module only marshals numpy arrays across the
+C ABI.  It raises if libmdx.so is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmdx.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "mdx.h")
+
+MDX_OK = 0
+MDX_ERR_ARG, MDX_ERR_HIP, MDX_ERR_STATE, MDX_ERR_FORCE, MDX_ERR_NAN, MDX_ERR_CAPACITY = 1, 2, 3, 4, 5, 6
+PREC_FP64, PREC_MIXED = 0, 1
+INTEG_ROTOR, INTEG_BEAD = 0, 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class MdxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mdx error {code}: {msg}")
+        self.code = code
+
+
+class Desc(C.Structure):
+    _fields_ = [
+        ("nreplicas", C.c_int), ("n_per_replica", _ip), ("box", _dp), ("periodic", C.c_int),
+        ("kT", _dp), ("dt", C.c_double), ("m", C.c_double), ("gamma", C.c_double),
+        ("gammar", C.c_double), ("inertia", C.c_double * 3), ("precision", C.c_int),
+        ("integrator", C.c_int), ("seed", C.c_ulonglong), ("device", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def declared_symbols():
+    """Function names declared in include/mdx.h."""
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(mdx_\w+)\s*\(", txt, re.M)))
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m mtcode_amd.build` "
+            "(there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    L.mdx_last_error.restype = C.c_char_p
+    L.mdx_create.argtypes = [C.POINTER(Desc), C.POINTER(C.c_void_p)]
+    L.mdx_destroy.argtypes = [C.c_void_p]
+    L.mdx_destroy.restype = None
+    L.mdx_upload_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+    L.mdx_download_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+    L.mdx_download_forces.argtypes = [C.c_void_p, _dp, _dp]
+    L.mdx_set_pair_wca.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+    L.mdx_set_patches.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _dp, _dp, C.c_double]
+    L.mdx_set_replica_patch_params.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.mdx_set_bonds.argtypes = [C.c_void_p, C.c_longlong, _ip, _dp, _dp, C.c_int]
+    L.mdx_set_angles.argtypes = [C.c_void_p, C.c_longlong, _ip, _dp, _dp, C.c_int]
+    L.mdx_set_exclusions.argtypes = [C.c_void_p, C.c_longlong, _ip]
+    L.mdx_set_active.argtypes = [C.c_void_p, C.c_double, _ip, _ip]
+    L.mdx_set_force_guard.argtypes = [C.c_void_p, C.c_double]
+    L.mdx_cell_assign.argtypes = [C.c_void_p, C.c_int, _ip, _ip]
+    L.mdx_neighbor_pairs.argtypes = [C.c_void_p, C.c_int, _ip, C.c_double, _ip, C.c_longlong,
+                                     C.POINTER(C.c_longlong)]
+    L.mdx_compute_forces.argtypes = [C.c_void_p, _dp, _dp]
+    L.mdx_prime.argtypes = [C.c_void_p, _dp]
+    L.mdx_step.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.mdx_observables.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.mdx_profile_step.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_char_p), _dp,
+                                   _ip]
+    L.mdx_info.argtypes = [C.c_void_p, _dp]
+    _lib = L
+    return L
+
+
+def _d(a):
+    if a is None:
+        return None, None
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    if a is None:
+        return None, None
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_ip)
+
+
+def _check(rc):
+    if rc != MDX_OK:
+        raise MdxError(rc, lib().mdx_last_error().decode())
+
+
+class Engine:
+    """One mdx engine (one HIP stream, one GPU, a batch of replicas)."""
+
+    def __init__(self, n_per_replica, box, kT, dt, m=1.0, gamma=1.0, gammar=1.0,
+                 inertia=(0.1, 0.1, 0.1), periodic=True, precision=PREC_FP64,
+                 integrator=INTEG_ROTOR, seed=0, device=0):
+        L = lib()
+        n = np.atleast_1d(np.asarray(n_per_replica, dtype=np.int32))
+        self.nrep = len(n)
+        self.N = int(n.sum())
+        self.n_per_replica = n
+        self.first = np.concatenate([[0], np.cumsum(n)[:-1]]).astype(int)
+        box = np.asarray(box, dtype=np.float64).reshape(-1)
+        if box.size == 1:
+            box = np.repeat(box, 3)
+        if box.size == 3 and self.nrep > 1:
+            box = np.tile(box, self.nrep)
+        kT = np.broadcast_to(np.asarray(kT, dtype=np.float64), (self.nrep,)).copy()
+        self._keep = [n, box, kT]
+        d = Desc()
+        d.nreplicas = self.nrep
+        d.n_per_replica = n.ctypes.data_as(_ip)
+        d.box = box.ctypes.data_as(_dp)
+        d.periodic = 1 if periodic else 0
+        d.kT = kT.ctypes.data_as(_dp)
+        d.dt, d.m, d.gamma, d.gammar = dt, m, gamma, gammar
+        for k in range(3):
+            d.inertia[k] = inertia[k]
+        d.precision = precision
+        d.integrator = integrator
+        d.seed = seed
+        d.device = device
+        h = C.c_void_p()
+        _check(L.mdx_create(C.byref(d), C.byref(h)))
+        self.h = h
+        self.precision = precision
+        self.integrator = integrator
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mdx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ------------------------------------------------------------ state
+    def upload(self, x, p=None, Q=None, L=None):
+        xs, xp = _d(x)
+        ps, pp = _d(p)
+        qs, qp = _d(Q)
+        ls, lp = _d(L)
+        _check(lib().mdx_upload_state(self.h, xp, pp, qp, lp))
+
+    def download(self):
+        N = self.N
+        x, p, Q, L = np.empty((N, 3)), np.empty((N, 3)), np.empty((N, 9)), np.empty((N, 3))
+        _check(lib().mdx_download_state(self.h, x.ctypes.data_as(_dp), p.ctypes.data_as(_dp),
+                                        Q.ctypes.data_as(_dp), L.ctypes.data_as(_dp)))
+        return {"x": x, "p": p, "Q": Q, "L": L}
+
+    def carried_forces(self):
+        F, T = np.empty((self.N, 3)), np.empty((self.N, 3))
+        _check(lib().mdx_download_forces(self.h, F.ctypes.data_as(_dp), T.ctypes.data_as(_dp)))
+        return F, T
+
+    # ------------------------------------------------------ interactions
+    def set_wca(self, eps=1.0, sigma=1.0, rcut=0.0):
+        _check(lib().mdx_set_pair_wca(self.h, eps, sigma, rcut))
+
+    def set_patches(self, types, npatch, dirs, params, maxd2=2.4):
+        t, tp = _i(types)
+        n, np_ = _i(npatch)
+        dd, dp = _d(np.asarray(dirs, dtype=np.float64).reshape(-1) if len(dirs) else np.zeros(3))
+        pr, pp = _d(params)
+        _check(lib().mdx_set_patches(self.h, len(npatch), tp, np_, dp, pp, float(maxd2)))
+
+    def set_replica_patch_params(self, replica, params):
+        pr, pp = _d(params)
+        _check(lib().mdx_set_replica_patch_params(self.h, int(replica), pp))
+
+    def set_bonds(self, ij, k, r0, kind=1):
+        a, ap = _i(ij)
+        kk, kp = _d(k)
+        rr, rp = _d(r0)
+        _check(lib().mdx_set_bonds(self.h, len(a), ap, kp, rp, int(kind)))
+
+    def set_angles(self, ijk, k, theta0, form=0):
+        a, ap = _i(ijk)
+        kk, kp = _d(k)
+        tt, tp = _d(theta0)
+        _check(lib().mdx_set_angles(self.h, len(a), ap, kp, tp, int(form)))
+
+    def set_exclusions(self, ij):
+        a, ap = _i(ij)
+        _check(lib().mdx_set_exclusions(self.h, len(a), ap))
+
+    def set_active(self, fa, prev, nxt):
+        a, ap = _i(prev)
+        b, bp = _d(None)
+        b, bp = _i(nxt)
+        _check(lib().mdx_set_active(self.h, float(fa), ap, bp))
+
+    def set_force_guard(self, fmax):
+        _check(lib().mdx_set_force_guard(self.h, float(fmax)))
+
+    # ------------------------------------------------------------ parity
+    def cell_assign(self, n, replica=0):
+        n3 = np.broadcast_to(np.asarray(n, dtype=np.int32), (3,)).copy()
+        out = np.empty(int(self.n_per_replica[replica]), dtype=np.int32)
+        _check(lib().mdx_cell_assign(self.h, int(replica), n3.ctypes.data_as(_ip),
+                                     out.ctypes.data_as(_ip)))
+        return out
+
+    def neighbor_pairs(self, n, cut, replica=0):
+        n3 = np.broadcast_to(np.asarray(n, dtype=np.int32), (3,)).copy()
+        cnt = C.c_longlong(0)
+        rc = lib().mdx_neighbor_pairs(self.h, int(replica), n3.ctypes.data_as(_ip), float(cut),
+                                      None, 0, C.byref(cnt))
+        if rc not in (MDX_OK, MDX_ERR_CAPACITY):
+            _check(rc)
+        out = np.empty((max(cnt.value, 1), 2), dtype=np.int32)
+        _check(lib().mdx_neighbor_pairs(self.h, int(replica), n3.ctypes.data_as(_ip), float(cut),
+                                        out.ctypes.data_as(_ip), cnt.value, C.byref(cnt)))
+        return out[: cnt.value]
+
+    def compute_forces(self):
+        F, T = np.empty((self.N, 3)), np.empty((self.N, 3))
+        _check(lib().mdx_compute_forces(self.h, F.ctypes.data_as(_dp), T.ctypes.data_as(_dp)))
+        return F, T
+
+    # ---------------------------------------------------------- stepping
+    def prime(self, noise0=None):
+        nz, npp = _d(noise0)
+        _check(lib().mdx_prime(self.h, npp))
+
+    def step(self, nsteps, noise=None):
+        nz, npp = _d(noise)
+        _check(lib().mdx_step(self.h, int(nsteps), npp))
+
+    def observables(self, replica=-1):
+        out = np.zeros(16)
+        _check(lib().mdx_observables(self.h, int(replica), out.ctypes.data_as(_dp)))
+        return {"ke_t": out[0:3], "ke_r": out[3:6], "epot": out[6], "virial": out[7:16].reshape(3, 3)}
+
+    def profile(self, nsteps):
+        names = (C.c_char_p * 16)()
+        ms = np.zeros(16)
+        nk = C.c_int(0)
+        _check(lib().mdx_profile_step(self.h, int(nsteps), 16, names, ms.ctypes.data_as(_dp),
+                                      C.byref(nk)))
+        return {names[k].decode(): float(ms[k]) for k in range(nk.value)}
+
+    def info(self):
+        out = np.zeros(5)
+        _check(lib().mdx_info(self.h, out.ctypes.data_as(_dp)))
+        return {"N": int(out[0]), "cells": int(out[1]), "cell_size": out[2],
+                "device_bytes": int(out[3]), "steps": int(out[4])}
+
+
+def engine_from_system(s, precision=PREC_FP64, seed=0, device=0, replicas=None):
+    """Build and load an engine from a workload dict (mtcode_amd.workloads),
+    or from a list of them (a batch of replicas of the same kind)."""
+    systems = replicas if replicas is not None else [s]
+    s0 = systems[0]
+    e = Engine([x["N"] for x in systems], np.concatenate([x["box"] for x in systems]),
+               [x["kT"] for x in systems], s0["dt"], s0["m"], s0["gamma"], s0["gammar"],
+               tuple(s0["I"]), s0.get("periodic", True), precision,
+               INTEG_ROTOR if s0["integrator"] == "rotor" else INTEG_BEAD, seed, device)
+    offs = np.concatenate([[0], np.cumsum([x["N"] for x in systems])])
+    cat = lambda key: np.concatenate([x[key] for x in systems])  # noqa: E731
+    e.upload(cat("x"), cat("p"), cat("Q"), cat("L"))
+    if s0.get("wca"):
+        e.set_wca(*s0["wca"])
+    if s0.get("patches") is not None:
+        ps = s0["patches"]
+        e.set_patches(cat_types(systems), ps["npatch"], ps["dirs"], ps["params"], ps["maxd2"])
+        if len(systems) > 1:
+            for r, x in enumerate(systems):
+                e.set_replica_patch_params(r, x["patches"]["params"])
+
+    def shifted(key, width):
+        return np.concatenate([x[key] + offs[r] for r, x in enumerate(systems)]).reshape(-1, width)
+
+    if s0.get("bonds") is not None:
+        e.set_bonds(shifted("bonds", 2), cat("bond_k"), cat("bond_r0"), s0.get("bond_kind", 1))
+    if s0.get("angles") is not None:
+        e.set_angles(shifted("angles", 3), cat("angle_k"), cat("angle_theta0"),
+                     s0.get("angle_form", 0))
+    if s0.get("exclusions") is not None:
+        e.set_exclusions(shifted("exclusions", 2))
+    if s0.get("active") is not None:
+        prev = np.concatenate([np.where(x["active"]["prev"] < 0, -1, x["active"]["prev"] + offs[r])
+                               for r, x in enumerate(systems)])
+        nxt = np.concatenate([np.where(x["active"]["next"] < 0, -1, x["active"]["next"] + offs[r])
+                              for r, x in enumerate(systems)])
+        e.set_active(s0["active"]["fa"], prev, nxt)
+    return e
+
+
+def cat_types(systems):
+    return np.concatenate([x["patches"]["types"] for x in systems]).astype(np.int32)

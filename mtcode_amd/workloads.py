@@ -67,6 +67,22 @@ def nonoverlapping_uniform(n: int, box, rng: np.random.Generator, rmin: float = 
         box = np.repeat(box, 3)
     x = rng.uniform(0.0, 1.0, size=(n, 3)) * box
     x = np.minimum(x, np.nextafter(box, 0))
+    if n > 200000:
+        # large systems: after the first full pass only redrawn particles are
+        # re-checked (a different, equally valid, deterministic sequence)
+        kw = dict(boxsize=box if periodic else None, balanced_tree=False, compact_nodes=False)
+        tree = cKDTree(x, **kw)
+        pairs = tree.query_pairs(rmin, output_type="ndarray")
+        bad = np.unique(pairs[:, 1]) if len(pairs) else np.zeros(0, dtype=int)
+        for _ in range(1000):
+            if len(bad) == 0:
+                return x
+            x[bad] = rng.uniform(0.0, 1.0, size=(len(bad), 3)) * box
+            x[bad] = np.minimum(x[bad], np.nextafter(box, 0))
+            tree = cKDTree(x, **kw)
+            cnt = tree.query_ball_point(x[bad], rmin, return_length=True, workers=8)
+            bad = bad[cnt > 1]
+        raise RuntimeError("could not place particles without overlap")
     for _ in range(1000):
         tree = cKDTree(x, boxsize=box if periodic else None)
         pairs = tree.query_pairs(rmin, output_type="ndarray")

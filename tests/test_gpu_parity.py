@@ -1,0 +1,258 @@
+"""Parity of the HIP engine (through the C ABI) with the reference's golden
+vectors and with the oracle, on an MI355X.
+
+Bars: bit-exact for cell ids and neighbour sets; forces and torques within
+1e-12 (FP64) / 1e-5 (mixed) of the FP64 value, relative to the largest
+force (torque) of the configuration; 1000-step FP64 trajectories with
+injected noise within the chaos-amplified rounding floor measured between
+the oracle and the reference themselves."""
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from mtcode_amd import mdx
+from mtcode_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def relmax(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def engine_for(s, precision=mdx.PREC_FP64, **kw):
+    return mdx.engine_from_system(s, precision=precision, **kw)
+
+
+# ------------------------------------------------------------ binning, pairs
+@pytest.mark.parametrize("n", [1, 2, 3, 6, 16])
+def test_cell_assign_bit_exact(golden, n):
+    g = golden("cells_patchy4096.npz")
+    say("cell_assign n", n)
+    x = g["x"]
+    with mdx.Engine(len(x), g["L"], 1.0, 0.005) as e:
+        e.upload(x)
+        assert np.array_equal(e.cell_assign(n), g[f"cells_n{n}"])
+
+
+def test_neighbor_pairs_bit_exact(golden):
+    g = golden("pairs.npz")
+    for k in [k for k in g if k.startswith("pairs512")]:
+        parts = k.split("_")
+        n, cut = int(parts[1][1:]), float(parts[2][3:])
+        periodic = not k.endswith("nonperiodic")
+        say("neighbor_pairs", k)
+        with mdx.Engine(len(g["x512"]), g["L512"], 1.0, 0.005, periodic=periodic) as e:
+            e.upload(g["x512"])
+            got = e.neighbor_pairs(n, cut)
+        ref = np.sort(np.sort(g[k], axis=1).astype(np.int64) @ np.array([1, 1 << 20]))
+        key = np.sort(got.astype(np.int64) @ np.array([1, 1 << 20]))
+        assert np.all(got[:, 0] < got[:, 1])
+        assert np.array_equal(key, ref), k
+
+
+def test_neighbor_pairs_full_size(golden):
+    g = golden("pairs.npz")
+    s = wl.patchy_tetra()
+    say("neighbor_pairs 4096")
+    with mdx.Engine(s["N"], s["box"], 1.0, 0.005) as e:
+        e.upload(s["x"])
+        got = e.neighbor_pairs(16, 1.68)
+    key = np.sort(got.astype(np.int64) @ np.array([1, 1 << 20]))
+    assert np.array_equal(key, g["pairs4096_n16_cut1.68_keys"])
+
+
+# ------------------------------------------------------------------- forces
+@pytest.mark.parametrize("name,builder", [("forces_patchy4096.npz", lambda: wl.patchy_tetra()),
+                                          ("forces_condensate4096.npz",
+                                           lambda: wl.condensate_ab(4096))])
+def test_forces_fp64_vs_reference(golden, name, builder):
+    g = golden(name)
+    s = builder()
+    say("forces fp64", name)
+    with engine_for(s) as e:
+        F, T = e.compute_forces()
+        obs = e.observables()
+    assert relmax(F, g["F"]) < 1e-12
+    assert relmax(T, g["T"]) < 1e-12
+    # patch + WCA energy: the reference fixture holds the patch part only
+    wca_e = 0.0
+    pr = po.pairs_reference(s["box"][0], s["x"], 16, 1.12246)
+    for a, b in pr:
+        d = s["x"][a] - s["x"][b]
+        d -= s["box"][0] * np.round(d / s["box"][0])
+        wca_e += po.lib().orc_wca_energy(float(np.linalg.norm(d)), 1.0, 1.0, 0)
+    assert abs(obs["epot"] - (g["epot"] + wca_e)) < 1e-9 * max(1.0, abs(g["epot"]))
+
+
+@pytest.mark.parametrize("name,dirs", [("tetra", wl.TETRA_DIRS), ("bivalent", wl.BIVALENT_DIRS)])
+def test_two_particle_sweeps_fp64(golden, name, dirs):
+    g = golden("pairsweep.npz")
+    x = g[name + "_x"]
+    N = len(x)
+    L = float(g[name + "_L"])
+    say("sweep fp64", name)
+    with mdx.Engine(N, L, 1.0, 0.005) as e:
+        e.upload(x, None, g[name + "_Q"], None)
+        e.set_wca(1.0, 1.0, 0.0)
+        e.set_patches(np.zeros(N), [len(dirs)], dirs,
+                      np.tile([6.0, 1.2, wl.THETAM_095], (len(dirs) ** 2, 1)), 2.4)
+        F, T = e.compute_forces()
+    assert relmax(F, g[name + "_F"]) < 1e-12
+    assert relmax(T, g[name + "_T"]) < 1e-12
+
+
+def test_bonded_fp64(golden):
+    g = golden("bonded.npz")
+    x, L, b, a = g["x"], float(g["L"]), g["bonds"], g["angles"]
+    N = len(x)
+    cases = [("Fb_harmonic", dict(k=100.0, r0=np.ones(len(b)), kind=1)),
+             ("Fb_fene", dict(k=30.0, r0=np.full(len(b), 1.5), kind=2)),
+             ("Fb_perbond", dict(k=100.0, r0=g["r0s"], kind=1))]
+    for key, c in cases:
+        say("bonded", key)
+        with mdx.Engine(N, L, 1.0, 0.002, integrator=mdx.INTEG_BEAD) as e:
+            e.upload(x)
+            e.set_bonds(b, np.full(len(b), c["k"]), c["r0"], c["kind"])
+            F, _ = e.compute_forces()
+        assert relmax(F, g[key]) < 1e-13, key
+    for th in (0.3, 0.0):
+        say("angles", th)
+        with mdx.Engine(N, L, 1.0, 0.002, integrator=mdx.INTEG_BEAD) as e:
+            e.upload(x)
+            e.set_angles(a, np.full(len(a), 50.0), np.full(len(a), th), 0)
+            F, _ = e.compute_forces()
+        assert relmax(F, g[f"Fa_theta0_{th}"]) < 1e-12
+
+
+def test_cosine_bending_and_active_vs_oracle():
+    rng = np.random.default_rng(4)
+    s = wl.active_polymer(nchains=6, nbeads=40, phi=0.05, seed=3)
+    say("active polymer small fp64", s["N"])
+    with engine_for(s) as e:
+        F, T = e.compute_forces()
+    Fo, To = po.compute_forces(s, list_cut=3.5)
+    assert relmax(F, Fo) < 1e-12
+    assert relmax(T, To) < 1e-12 or np.abs(To).max() == 0
+    _ = rng
+
+
+# --------------------------------------------------------------- trajectories
+@pytest.mark.parametrize("N", [64, 512])
+def test_rotor_trajectory_fp64_vs_reference(golden, N):
+    g = golden(f"traj_rotor{N}.npz")
+    s = wl.patchy_tetra(N)
+    nsteps, snap = int(g["nsteps"]), int(g["snap_every"])
+    noise = po.noise(int(g["noise_seed"]), 0, (nsteps + 1) * N * 6).reshape(nsteps + 1, N, 6)
+    say("rotor trajectory fp64", N)
+    with engine_for(s) as e:
+        e.prime(noise[0])
+        errs = []
+        for k in range(nsteps // snap):
+            e.step(snap, noise[1 + k * snap: 1 + (k + 1) * snap])
+            st = e.download()
+            errs.append(float(np.max(np.abs(st["x"] - g["snaps"][k]))))
+    say("max |dx| per 100 steps", errs)
+    assert errs[0] < 1e-10
+    assert np.max(np.abs(st["x"] - g["x_out"])) < 1e-6
+    assert np.max(np.abs(st["Q"] - g["Q_out"])) < 1e-6
+    assert relmax(st["p"], g["p_out"]) < 1e-5
+    assert relmax(st["L"], g["L_out"]) < 1e-5
+
+
+def test_bead_trajectory_fp64_vs_reference(golden):
+    g = golden("traj_bead128.npz")
+    N = len(g["x"])
+    nsteps = int(g["nsteps"])
+    noise = po.noise(int(g["noise_seed"]), 0, nsteps * N * 3).reshape(nsteps, N, 3)
+    say("bead trajectory fp64")
+    with mdx.Engine(N, float(g["box"]), 1.0, 0.002, gamma=1.0, integrator=mdx.INTEG_BEAD) as e:
+        e.upload(g["x"])
+        e.set_wca(1.0, 1.0, 0.0)
+        e.set_bonds(g["bonds"], np.full(len(g["bonds"]), 100.0), np.ones(len(g["bonds"])), 1)
+        e.step(nsteps, noise)
+        st = e.download()
+    assert np.max(np.abs(st["x"] - g["x_out"])) < 1e-9
+    assert relmax(st["p"], g["p_out"]) < 1e-9
+
+
+# ------------------------------------------------------------ mixed precision
+def f32_rounded(s):
+    s = dict(s)
+    s["Q"] = s["Q"].astype(np.float32).astype(np.float64)
+    return s
+
+
+@pytest.mark.parametrize("builder", [lambda: wl.patchy_tetra(), lambda: wl.condensate_ab(32768)])
+def test_mixed_forces_within_1e5_of_fp64(builder):
+    s = f32_rounded(builder())
+    say("mixed vs fp64 forces", s["name"], s["N"])
+    with engine_for(s) as e64:
+        F64, T64 = e64.compute_forces()
+    with engine_for(s, mdx.PREC_MIXED) as e32:
+        F32, T32 = e32.compute_forces()
+    ef, et = relmax(F32, F64), relmax(T32, T64)
+    rms = float(np.linalg.norm(F32 - F64) / np.linalg.norm(F64))
+    say("max rel F", ef, "T", et, "rms F", rms)
+    assert ef < 1e-5 and et < 1e-5 and rms < 1e-6
+
+
+def test_determinism_bitwise():
+    s = wl.patchy_tetra(2048)
+    say("determinism")
+    outs = []
+    for _ in range(2):
+        with engine_for(s, mdx.PREC_MIXED, seed=9) as e:
+            e.prime()
+            e.step(50)
+            outs.append(e.download())
+    for k in ("x", "p", "Q", "L"):
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+# ------------------------------------------------------------------ replicas
+def test_replicas_match_independent_runs():
+    reps = wl.replicas(6, seed=5, nmin=200, nmax=900)
+    say("replicas", [r["N"] for r in reps])
+    nsteps = 20
+    Ntot = sum(r["N"] for r in reps)
+    noise = po.noise(3, 0, (nsteps + 1) * Ntot * 6).reshape(nsteps + 1, Ntot, 6)
+    offs = np.concatenate([[0], np.cumsum([r["N"] for r in reps])])
+    with mdx.engine_from_system(None, replicas=reps) as e:
+        e.prime(noise[0])
+        e.step(nsteps, noise[1:])
+        batch = e.download()
+    for r, s in enumerate(reps):
+        sl = slice(offs[r], offs[r + 1])
+        with engine_for(s) as e:
+            e.prime(noise[0, sl])
+            e.step(nsteps, np.ascontiguousarray(noise[1:, sl]))
+            one = e.download()
+        for k in ("x", "p", "Q", "L"):
+            assert np.array_equal(batch[k][sl], one[k]), (r, k)
+
+
+# --------------------------------------------------------------- statistics
+def test_equipartition_with_philox_noise():
+    s = wl.patchy_tetra(4096)
+    s["patches"]["params"][:, 0] = 0.0  # ideal rotors + WCA only
+    say("equipartition")
+    with engine_for(s, mdx.PREC_MIXED, seed=1) as e:
+        e.prime()
+        e.step(2000)
+        acc = np.zeros(6)
+        for _ in range(20):
+            e.step(50)
+            o = e.observables()
+            acc += np.concatenate([o["ke_t"], o["ke_r"]])
+    acc /= 20
+    say("<KE> per component", acc)
+    # the explicit friction scheme of the reference at dt=0.005 gives kT/2 to a few %
+    assert np.all(np.abs(acc - 0.5) < 0.05)
