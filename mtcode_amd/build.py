@@ -19,7 +19,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libmdx.so")
 SOURCES = [os.path.join(HERE, "csrc", "mdx_engine.hip")]
-HEADERS = [os.path.join(HERE, "csrc", "mdx_device.h"), os.path.join(ROOT, "include", "mdx.h")]
+HEADERS = [os.path.join(HERE, "csrc", "mdx_device.h"), os.path.join(HERE, "csrc", "mdx_kernels.h"),
+           os.path.join(ROOT, "include", "mdx.h")]
 ARCH = os.environ.get("MDX_OFFLOAD_ARCH", "gfx950")
 
 

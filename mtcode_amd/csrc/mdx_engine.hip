@@ -1,18 +1,16 @@
-// mdx_engine.hip -- MI355X (gfx950) engine behind include/mdx.h.
+// mdx_engine.hip -- MI355X (gfx950) engine behind include/mdx.h (host side).
 //
 // Per step (rotor integrator, Condensate.cpp:352-377 of the reference):
-//   kick/drift/rotate/bin  -> exclusive scan of cell counts -> fill cell lists
-//   -> order each cell's list -> gather forces + sphere + half kick
-// The gather kernel computes every particle's force and torque by looping
-// over its 27 neighbour cells (full neighbour list, no Newton-3 scatter, no
-// atomics), so results are deterministic.  Inside one mdx_step call the
-// gather kernel of step s also performs the first half of step s+1 (second
-// half kick, next half kick, drift, rotation, binning) so that the per-step
-// state crosses HBM once.  Positions and orientations are double buffered
-// because neighbours read them while the owner writes the next ones.
+//   exclusive scan of cell counts -> fill cell lists -> order each cell's list
+//   -> gather forces + sphere + half kick (+ the next step's half kick, drift,
+//      rotation and binning, fused)
+// The gather kernel (mdx_kernels.h) computes every particle's force and torque
+// by looping over its 27 neighbour cells (full neighbour list, no Newton-3
+// scatter, no atomics), so results are deterministic, and re-sorts the state
+// into cell order as it writes the next step.
 //
 // Reference paths in comments are relative to the reference's Code/ folder.
-#include "mdx_device.h"
+#include "mdx_kernels.h"
 #include "../../include/mdx.h"
 
 #include <hip/hip_runtime.h>
@@ -20,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -49,961 +48,6 @@ static int set_err(int code, const char *fmt, ...)
                            __FILE__, __LINE__);                                                \
     } while (0)
 
-// ------------------------------------------------------------ device structs
-struct DevRep {
-    double l[3], l2s[3];
-    int nc[3];
-    int coff;
-    int first, count;
-    double Rt, Rr;      // sqrt(2 gamma kT / dt), sqrt(2 gamma_r kT / dt) (LangevinR.cpp:51-52)
-    double bead_r;      // sqrt(0.5 kT gamma m dt) (Langevin.cpp:44)
-    int prow;           // first patch row of this replica
-};
-
-struct StepCtl {
-    unsigned long long epoch;     // force evaluations so far (noise counter)
-    unsigned long long call_base; // epoch at the start of the current call
-    unsigned int status;          // bit0 force guard, bit1 non-finite
-    unsigned int pad;
-};
-
-enum { M_CONS = 0, M_PRIME = 1, M_STEP = 2, M_FUSED = 3, M_BEAD = 4, M_ENERGY = 5 };
-
-template <class R> struct KArgs {
-    int N;
-    int rotor;
-    int periodic;
-    const uint32_t *meta; // replica << 8 | type
-    const DevRep *reps;
-    const double *X;      // current positions N x 3
-    double *Xn;           // next positions
-    const R *Q;           // current orientations N x 9
-    R *Qn;
-    R *p, *L, *F, *T;     // momenta, body angular momenta, carried total force/torque
-    double *Fout, *Tout;  // conservative output
-    double *obs;          // per particle energy + virial (10)
-    int *cell, *rank, *count;
-    const int *start, *list;
-    // WCA (HSPotential)
-    int wca;
-    double wca_eps, wca_s6, wca_s12, wca_pl;
-    // patches
-    int has_patches;
-    int np[MAXT];
-    int pstart[MAXT];
-    int rstart[MAXT * MAXT];
-    unsigned long long active_mask; // bit ta * MAXT + tc: type pair with a nonzero row
-    const double *pdirs;
-    const PatchRow *prows;
-    double maxd2;
-    double rmax2;
-    // topology
-    const int *bond_off, *bond_j;
-    const double *bond_k, *bond_r0;
-    int bond_kind;
-    const int *ang_off, *ang_ref; // per particle: angle index * 4 + role
-    const int *ang_ijk;
-    const double *ang_k, *ang_t0;
-    int ang_form;
-    const int *exc_off, *exc_j;
-    const int *act_prev, *act_next;
-    double fa;
-    // integrator
-    double dt, hdt, c1, m, gamma, gammar, I[3];
-    double b_c5c2, b_c5c3q, b_c5c4, b_c2; // bead coefficients without the kT part
-    int reortho_every;
-    // noise
-    uint32_t key0, key1;
-    const double *noise; // injected rows (ncomp per particle) or NULL
-    StepCtl *ctl;
-    double fguard;
-};
-
-// ------------------------------------------------------------ small helpers
-template <class R> __device__ __forceinline__ void lab_normal(const R q[9], const double *nb, R n[3])
-{
-    // n = Q^T n_body (potentialtheta.h:559-565)
-    n[0] = (R)nb[0] * q[0] + (R)nb[1] * q[3] + (R)nb[2] * q[6];
-    n[1] = (R)nb[0] * q[1] + (R)nb[1] * q[4] + (R)nb[2] * q[7];
-    n[2] = (R)nb[0] * q[2] + (R)nb[1] * q[5] + (R)nb[2] * q[8];
-}
-
-__device__ __forceinline__ void lab_normal_d(const float q[9], const double *nb, double n[3])
-{
-    n[0] = nb[0] * (double)q[0] + nb[1] * (double)q[3] + nb[2] * (double)q[6];
-    n[1] = nb[0] * (double)q[1] + nb[1] * (double)q[4] + nb[2] * (double)q[7];
-    n[2] = nb[0] * (double)q[2] + nb[1] * (double)q[5] + nb[2] * (double)q[8];
-}
-
-// LangevinNVTR::genfullmat (LangevinR.cpp:392-461)
-template <class R>
-__device__ __forceinline__ void genfullmat(R dt, const R I[3], const R L[3], R up[9])
-{
-    R tempx = dt * L[0] / I[0];
-    R tempy = dt * L[1] / I[1];
-    R tempz = dt * L[2] / I[2];
-    R qx = tempx / (R)4., qy = tempy / (R)4., hz = tempz / (R)2.;
-    R fac1 = ((R)1 - qx * qx) / ((R)1 + qx * qx);
-    R fac2 = tempx / ((R)2. * ((R)1 + qx * qx));
-    R fac3 = ((R)1 - qy * qy) / ((R)1 + qy * qy);
-    R fac4 = tempy / ((R)2. * ((R)1 + qy * qy));
-    R fac5 = ((R)1 - hz * hz) / ((R)1 + hz * hz);
-    R fac6 = (tempz) / ((R)1 + hz * hz);
-    up[0] = -fac4 * fac4 + fac3 * fac3 * fac5;
-    up[1] = fac3 * (fac2 * fac4 * ((R)1 + fac5) - fac1 * fac6);
-    up[2] = fac3 * (fac1 * fac4 * ((R)1 + fac5) + fac2 * fac6);
-    up[3] = fac3 * (fac2 * fac4 * ((R)1 + fac5) + fac1 * fac6);
-    up[4] = fac1 * fac1 * fac5 + fac2 * fac2 * (-fac3 * fac3 + fac4 * fac4 * fac5);
-    up[5] = -(fac1 * fac2 * (fac3 * fac3 + fac5 - fac4 * fac4 * fac5)) +
-            (fac1 * fac1 + fac2 * fac2) * fac4 * fac6;
-    up[6] = -(fac1 * fac3 * fac4 * ((R)1 + fac5)) + fac2 * fac3 * fac6;
-    up[7] = fac1 * fac2 * (fac3 * fac3 + fac5 - fac4 * fac4 * fac5) +
-            (fac1 * fac1 + fac2 * fac2) * fac4 * fac6;
-    up[8] = -(fac2 * fac2 * fac5) + fac1 * fac1 * (fac3 * fac3 - fac4 * fac4 * fac5);
-}
-
-// LangevinNVTR::rotate (LangevinR.cpp:595-685): L <- R L, Q <- R^T Q
-template <class R> __device__ __forceinline__ void rotate(R dt, const R I[3], R L[3], R q[9])
-{
-    R rr[9];
-    genfullmat(dt, I, L, rr);
-    R jx = L[0], jy = L[1], jz = L[2];
-    L[0] = jx * rr[0] + jy * rr[1] + jz * rr[2];
-    L[1] = jx * rr[3] + jy * rr[4] + jz * rr[5];
-    L[2] = jx * rr[6] + jy * rr[7] + jz * rr[8];
-    R o[9];
-#pragma unroll
-    for (int k = 0; k < 9; k++)
-        o[k] = q[k];
-    q[0] = o[0] * rr[0] + o[3] * rr[3] + o[6] * rr[6];
-    q[1] = o[1] * rr[0] + o[4] * rr[3] + o[7] * rr[6];
-    q[2] = o[2] * rr[0] + o[5] * rr[3] + o[8] * rr[6];
-    q[3] = o[0] * rr[1] + o[3] * rr[4] + o[6] * rr[7];
-    q[4] = o[1] * rr[1] + o[4] * rr[4] + o[7] * rr[7];
-    q[5] = o[2] * rr[1] + o[5] * rr[4] + o[8] * rr[7];
-    q[6] = o[0] * rr[2] + o[3] * rr[5] + o[6] * rr[8];
-    q[7] = o[1] * rr[2] + o[4] * rr[5] + o[7] * rr[8];
-    q[8] = o[2] * rr[2] + o[5] * rr[5] + o[8] * rr[8];
-}
-
-// symmetric re-orthonormalisation in FP64: Q <- 1.5 Q - 0.5 Q Q^T Q (one
-// Newton-Schulz step of Q (Q^T Q)^(-1/2); deviations are ~1e-6 in FP32)
-template <class R> __device__ __forceinline__ void reortho(R q[9])
-{
-    double a[9], m[9];
-    for (int k = 0; k < 9; k++)
-        a[k] = (double)q[k];
-    for (int r = 0; r < 3; r++)
-        for (int c = 0; c < 3; c++)
-            m[3 * r + c] = a[3 * r] * a[3 * c] + a[3 * r + 1] * a[3 * c + 1] + a[3 * r + 2] * a[3 * c + 2];
-    for (int r = 0; r < 3; r++)
-        for (int c = 0; c < 3; c++) {
-            double s = m[3 * r] * a[c] + m[3 * r + 1] * a[3 + c] + m[3 * r + 2] * a[6 + c];
-            q[3 * r + c] = (R)(1.5 * a[3 * r + c] - 0.5 * s);
-        }
-}
-
-// cube::correct_position_and_momentum (geometry.h:403-445) for one coordinate
-template <class R>
-__device__ __forceinline__ void wrap1(double &x, R &p, double l, int periodic)
-{
-    if (periodic) {
-        if (x < 0) {
-            while (x < 0)
-                x += l;
-        } else if (x > l) {
-            while (x > l)
-                x -= l;
-        }
-    } else {
-        if (x < 0) {
-            x = -x;
-            p = -p;
-        } else if (x > l) {
-            x = l - (x - l);
-            p = -p;
-        }
-    }
-}
-
-template <class R> __device__ __forceinline__ void flag_force(const KArgs<R> &a, double f)
-{
-    if (a.fguard > 0 && fabs(f) > a.fguard)
-        atomicOr(&a.ctl->status, 1u);
-}
-
-template <class R>
-__device__ __forceinline__ void load_noise(const KArgs<R> &a, int i, int ncomp, R out[6])
-{
-    unsigned long long ep = a.ctl->epoch;
-    if (a.noise) {
-        const double *row = a.noise + (size_t)(ep - a.ctl->call_base) * (size_t)a.N * ncomp +
-                            (size_t)i * ncomp;
-        for (int k = 0; k < ncomp; k++)
-            out[k] = (R)row[k];
-    } else {
-        U4 c0 = {(uint32_t)i, (uint32_t)ep, (uint32_t)(ep >> 32), 0u};
-        U4 r0 = philox4x32_10(c0, a.key0, a.key1);
-        out[0] = unit_uniform<R>(r0.a);
-        out[1] = unit_uniform<R>(r0.b);
-        out[2] = unit_uniform<R>(r0.c);
-        if (ncomp > 3) {
-            out[3] = unit_uniform<R>(r0.d);
-            U4 c1 = {(uint32_t)i, (uint32_t)ep, (uint32_t)(ep >> 32), 1u};
-            U4 r1 = philox4x32_10(c1, a.key0, a.key1);
-            out[4] = unit_uniform<R>(r1.a);
-            out[5] = unit_uniform<R>(r1.b);
-        }
-    }
-}
-
-template <class R>
-__device__ __forceinline__ void bin_particle(const KArgs<R> &a, int i, const DevRep &rp,
-                                             const double x[3])
-{
-    int cx = axis_cell(x[0], rp.nc[0], rp.l[0]);
-    int cy = axis_cell(x[1], rp.nc[1], rp.l[1]);
-    int cz = axis_cell(x[2], rp.nc[2], rp.l[2]);
-    int c = rp.coff + cx + rp.nc[0] * (cy + rp.nc[1] * cz);
-    a.cell[i] = c;
-    a.rank[i] = atomicAdd(&a.count[c], 1);
-}
-
-// ------------------------------------------------------------- gather kernel
-// One thread per particle.  Accumulates the conservative force (lab) and
-// patch torque (lab) of particle i over its neighbour cells and topology,
-// then applies the mode's integrator epilogue.
-template <class R, int MODE>
-__global__ void __launch_bounds__(256) k_gather(KArgs<R> a)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.N)
-        return;
-    const uint32_t mi = a.meta[i];
-    const int ti = mi & 255;
-    const DevRep rp = a.reps[mi >> 8];
-    const double xi0 = a.X[3 * i], xi1 = a.X[3 * i + 1], xi2 = a.X[3 * i + 2];
-    R Fi[3] = {0, 0, 0}, Ti[3] = {0, 0, 0};
-    double epot = 0, vir[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    constexpr bool FP64 = sizeof(R) == 8;
-
-    // own orientation and patch normals
-    R qi[9];
-    R ni[MAXP][3];
-    const int npi = a.has_patches ? a.np[ti] : 0;
-    if (a.rotor) {
-#pragma unroll
-        for (int k = 0; k < 9; k++)
-            qi[k] = a.Q[9 * (size_t)i + k];
-        for (int ia = 0; ia < npi; ia++)
-            lab_normal(qi, a.pdirs + 3 * (a.pstart[ti] + ia), ni[ia]);
-    }
-    // own exclusions
-    int exc[MAXEXCL];
-    int nexc = 0;
-    if (a.exc_off) {
-        int e0 = a.exc_off[i], e1 = a.exc_off[i + 1];
-        nexc = e1 - e0;
-        for (int k = 0; k < nexc && k < MAXEXCL; k++)
-            exc[k] = a.exc_j[e0 + k];
-    }
-
-    if (a.wca || npi > 0) {
-        const int c = a.cell[i] - rp.coff;
-        const int cx = c % rp.nc[0], cy = (c / rp.nc[0]) % rp.nc[1], cz = c / (rp.nc[0] * rp.nc[1]);
-        int lo[3], hi[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            lo[k] = rp.nc[k] >= 3 ? -1 : 0;
-            hi[k] = rp.nc[k] >= 2 ? 1 : 0;
-        }
-        for (int oz = lo[2]; oz <= hi[2]; oz++) {
-            int z = cz + oz;
-            z = z < 0 ? z + rp.nc[2] : (z >= rp.nc[2] ? z - rp.nc[2] : z);
-            for (int oy = lo[1]; oy <= hi[1]; oy++) {
-                int y = cy + oy;
-                y = y < 0 ? y + rp.nc[1] : (y >= rp.nc[1] ? y - rp.nc[1] : y);
-                for (int ox = lo[0]; ox <= hi[0]; ox++) {
-                    int x = cx + ox;
-                    x = x < 0 ? x + rp.nc[0] : (x >= rp.nc[0] ? x - rp.nc[0] : x);
-                    const int c2 = rp.coff + x + rp.nc[0] * (y + rp.nc[1] * z);
-                    const int s0 = a.start[c2], s1 = a.start[c2 + 1];
-                    for (int s = s0; s < s1; s++) {
-                        const int j = a.list[s];
-                        if (j == i)
-                            continue;
-                        double d[3];
-                        d[0] = mimage(xi0 - a.X[3 * j], rp.l[0], rp.l2s[0], a.periodic);
-                        d[1] = mimage(xi1 - a.X[3 * j + 1], rp.l[1], rp.l2s[1], a.periodic);
-                        d[2] = mimage(xi2 - a.X[3 * j + 2], rp.l[2], rp.l2s[2], a.periodic);
-                        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-                        if (!(r2 < a.rmax2))
-                            continue;
-                        // ---------------- WCA (MD.cpp:1700-1758, potential.h:112-125)
-                        if (a.wca) {
-                            bool ex = false;
-                            for (int k = 0; k < nexc && k < MAXEXCL; k++)
-                                ex |= exc[k] == j;
-                            const double r = sqrt(r2);
-                            if (!ex && r < a.wca_pl) {
-                                if constexpr (FP64) {
-                                    double f1 = hs_force(r, a.wca_eps, a.wca_s6, a.wca_s12);
-                                    flag_force(a, f1);
-                                    for (int k = 0; k < 3; k++) {
-                                        double fac = f1 * d[k] / r;
-                                        Fi[k] += (R)fac;
-                                        if (MODE == M_ENERGY)
-                                            for (int l = 0; l < 3; l++)
-                                                vir[3 * k + l] += 0.5 * d[l] * fac;
-                                    }
-                                    if (MODE == M_ENERGY)
-                                        epot += 0.5 * wca_energy(r, a.wca_eps, a.wca_s6, a.wca_s12);
-                                } else {
-                                    float rf = (float)r;
-                                    float f1 = hs_force(rf, (float)a.wca_eps, (float)a.wca_s6,
-                                                        (float)a.wca_s12);
-                                    flag_force(a, f1);
-                                    float fr = f1 / rf;
-                                    for (int k = 0; k < 3; k++) {
-                                        float fac = fr * (float)d[k];
-                                        Fi[k] += (R)fac;
-                                        if (MODE == M_ENERGY)
-                                            for (int l = 0; l < 3; l++)
-                                                vir[3 * k + l] += 0.5 * d[l] * fac;
-                                    }
-                                    if (MODE == M_ENERGY)
-                                        epot += 0.5 * (double)wca_energy(
-                                                          rf, (float)a.wca_eps,
-                                                          (float)a.wca_s6, (float)a.wca_s12);
-                                }
-                            }
-                        }
-                        // ------------- patches (LangevinR.cpp:911-1047, potentialtheta.h:517-750)
-                        if (npi > 0 && r2 < a.maxd2) {
-                            const int tj = a.meta[j] & 255;
-                            const int npj = a.np[tj];
-                            if (npj == 0)
-                                continue;
-                            // roles: p1 < p2, exchanged when type(p1) > type(p2)
-                            int pa = i < j ? i : j, pc = i < j ? j : i;
-                            int tpa = pa == i ? ti : tj, tpc = pa == i ? tj : ti;
-                            if (tpa > tpc) {
-                                int t = pa;
-                                pa = pc;
-                                pc = t;
-                                t = tpa;
-                                tpa = tpc;
-                                tpc = t;
-                            }
-                            if (!((a.active_mask >> (tpa * MAXT + tpc)) & 1ull))
-                                continue;
-                            const bool i_is_a = pa == i;
-                            const double rij = sqrt(r2);
-                            const double sg = i_is_a ? 1.0 : -1.0;
-                            double ud[3] = {sg * d[0] / rij, sg * d[1] / rij, sg * d[2] / rij};
-                            R qj[9];
-                            for (int k = 0; k < 9; k++)
-                                qj[k] = a.Q[9 * (size_t)j + k];
-                            R nj[MAXP][3];
-                            for (int ib = 0; ib < npj; ib++)
-                                lab_normal(qj, a.pdirs + 3 * (a.pstart[tj] + ib), nj[ib]);
-                            const int npa = a.np[tpa], npc = a.np[tpc];
-                            const int rowbase = rp.prow + a.rstart[tpa * MAXT + tpc];
-                            R ua[3] = {(R)ud[0], (R)ud[1], (R)ud[2]};
-                            R arg1[MAXP], arg2[MAXP];
-                            for (int ia = 0; ia < npa; ia++) {
-                                const R *n1 = i_is_a ? ni[ia] : nj[ia];
-                                arg1[ia] = -(n1[0] * ua[0] + n1[1] * ua[1] + n1[2] * ua[2]);
-                            }
-                            for (int ib = 0; ib < npc; ib++) {
-                                const R *n2 = i_is_a ? nj[ib] : ni[ib];
-                                arg2[ib] = (n2[0] * ua[0] + n2[1] * ua[1] + n2[2] * ua[2]);
-                            }
-                            for (int ia = 0; ia < npa; ia++) {
-                                for (int ib = 0; ib < npc; ib++) {
-                                    const PatchRow pr = a.prows[rowbase + ia * npc + ib];
-                                    if (rij > pr.idist)
-                                        continue;
-                                    R f[3] = {0, 0, 0}, t1[3] = {0, 0, 0}, t2[3] = {0, 0, 0};
-                                    const R *n1 = i_is_a ? ni[ia] : nj[ia];
-                                    const R *n2 = i_is_a ? nj[ib] : ni[ib];
-                                    double e;
-                                    if constexpr (FP64) {
-                                        if (!(arg1[ia] > pr.costm && arg2[ib] > pr.costm))
-                                            continue;
-                                        double fd[3] = {0, 0, 0}, t1d[3] = {0, 0, 0},
-                                               t2d[3] = {0, 0, 0};
-                                        double n1d[3] = {(double)n1[0], (double)n1[1], (double)n1[2]};
-                                        double n2d[3] = {(double)n2[0], (double)n2[1], (double)n2[2]};
-                                        e = kf2_pair_fp64(ud, rij, n1d, n2d, (double)arg1[ia],
-                                                          (double)arg2[ib], pr, fd, t1d, t2d);
-                                        for (int k = 0; k < 3; k++) {
-                                            f[k] = (R)fd[k];
-                                            t1[k] = (R)t1d[k];
-                                            t2[k] = (R)t2d[k];
-                                        }
-                                    } else {
-                                        // cone and d1 = 0 decisions in FP64 near their edges
-                                        double a1 = (double)arg1[ia], a2 = (double)arg2[ib];
-                                        const double tol = 4e-6;
-                                        const int tpi = i_is_a ? ti : tj, tpj = i_is_a ? tj : ti;
-                                        const R *qa = i_is_a ? qi : qj;
-                                        const R *qc = i_is_a ? qj : qi;
-                                        if (fabs(a1 - pr.costm) < tol || a1 > 0.9999) {
-                                            double nd[3];
-                                            lab_normal_d(qa,
-                                                         a.pdirs + 3 * (a.pstart[tpi] + ia), nd);
-                                            a1 = -(nd[0] * ud[0] + nd[1] * ud[1] + nd[2] * ud[2]);
-                                        }
-                                        if (fabs(a2 - pr.costm) < tol || a2 > 0.9999) {
-                                            double nd[3];
-                                            lab_normal_d(qc,
-                                                         a.pdirs + 3 * (a.pstart[tpj] + ib), nd);
-                                            a2 = (nd[0] * ud[0] + nd[1] * ud[1] + nd[2] * ud[2]);
-                                        }
-                                        if (!(a1 > pr.costm && a2 > pr.costm))
-                                            continue;
-                                        bool z1 = fabs(a1 * a1 - 1) < 1E-5;
-                                        bool z2 = fabs(a2 * a2 - 1) < 1E-5;
-                                        float ff[3] = {0, 0, 0}, t1f[3] = {0, 0, 0},
-                                              t2f[3] = {0, 0, 0};
-                                        float uf[3] = {(float)ud[0], (float)ud[1], (float)ud[2]};
-                                        float n1f[3] = {(float)n1[0], (float)n1[1], (float)n1[2]};
-                                        float n2f[3] = {(float)n2[0], (float)n2[1], (float)n2[2]};
-                                        e = kf2_pair_fp32(uf, (float)rij, n1f, n2f,
-                                                          (float)arg1[ia], (float)arg2[ib], z1,
-                                                          z2, (float)pr.eps, (float)pr.d,
-                                                          (float)pr.kk, ff, t1f, t2f);
-                                        for (int k = 0; k < 3; k++) {
-                                            f[k] = (R)ff[k];
-                                            t1[k] = (R)t1f[k];
-                                            t2[k] = (R)t2f[k];
-                                        }
-                                    }
-                                    if (i_is_a) {
-                                        for (int k = 0; k < 3; k++) {
-                                            Fi[k] += f[k];
-                                            Ti[k] += t1[k];
-                                        }
-                                    } else {
-                                        for (int k = 0; k < 3; k++) {
-                                            Fi[k] -= f[k];
-                                            Ti[k] += t2[k];
-                                        }
-                                    }
-                                    if (MODE == M_ENERGY) {
-                                        epot += 0.5 * e;
-                                        for (int k = 0; k < 3; k++)
-                                            for (int l = 0; l < 3; l++)
-                                                vir[3 * k + l] += 0.5 * d[l] * (double)(sg * f[k]);
-                                    }
-                                }
-                            }
-                        }
-                    }
-                }
-            }
-        }
-    }
-
-    // ---------------------------------------- bonds (MD.cpp:1684-1764, 2037-2111)
-    if (a.bond_off) {
-        for (int b = a.bond_off[i]; b < a.bond_off[i + 1]; b++) {
-            const int j = a.bond_j[b];
-            double d[3];
-            d[0] = mimage(xi0 - a.X[3 * j], rp.l[0], rp.l2s[0], a.periodic);
-            d[1] = mimage(xi1 - a.X[3 * j + 1], rp.l[1], rp.l2s[1], a.periodic);
-            d[2] = mimage(xi2 - a.X[3 * j + 2], rp.l[2], rp.l2s[2], a.periodic);
-            const double dis = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-            const double r = sqrt(dis);
-            const double k = a.bond_k[b], r0 = a.bond_r0[b];
-            double f1;
-            if (a.bond_kind == MDX_BOND_FENE)
-                f1 = -k * r / (1 - (r * r) / (r0 * r0)); // FENEPotential::force (potential.h:356)
-            else
-                f1 = -k * (r - r0);
-            flag_force(a, f1);
-            for (int c = 0; c < 3; c++) {
-                double fac = f1 * d[c] / r;
-                Fi[c] += (R)fac;
-                if (MODE == M_ENERGY)
-                    for (int l = 0; l < 3; l++)
-                        vir[3 * c + l] += 0.5 * d[l] * fac;
-            }
-            if (MODE == M_ENERGY) {
-                if (a.bond_kind == MDX_BOND_FENE)
-                    epot += 0.5 * (-(k / 2.) * r0 * r0 * log(1.0 - r * r / (r0 * r0)));
-                else
-                    epot += 0.5 * 0.5 * k * (r - r0) * (r - r0);
-            }
-        }
-    }
-    // ------------------------------------ angles (MD.cpp:1925-1983, potential3.h)
-    if (a.ang_off) {
-        for (int q = a.ang_off[i]; q < a.ang_off[i + 1]; q++) {
-            const int ref = a.ang_ref[q];
-            const int t = ref >> 2, role = ref & 3;
-            const int p1 = a.ang_ijk[3 * t], p2 = a.ang_ijk[3 * t + 1], p3 = a.ang_ijk[3 * t + 2];
-            double ab[3], bc[3];
-            for (int k = 0; k < 3; k++) {
-                ab[k] = mimage(a.X[3 * p2 + k] - a.X[3 * p1 + k], rp.l[k], rp.l2s[k], a.periodic);
-                bc[k] = mimage(a.X[3 * p3 + k] - a.X[3 * p2 + k], rp.l[k], rp.l2s[k], a.periodic);
-            }
-            R abr[3] = {(R)ab[0], (R)ab[1], (R)ab[2]}, bcr[3] = {(R)bc[0], (R)bc[1], (R)bc[2]};
-            R out[3];
-            if (a.ang_form == MDX_ANGLE_HARMONIC)
-                bend_harmonic(abr, bcr, (R)a.ang_k[t], (R)a.ang_t0[t], role, out);
-            else
-                bend_cosine(abr, bcr, (R)a.ang_k[t], role, out);
-            for (int k = 0; k < 3; k++)
-                Fi[k] += out[k];
-            if (MODE == M_ENERGY && role == 1) {
-                double c = (ab[0] * bc[0] + ab[1] * bc[1] + ab[2] * bc[2]) /
-                           sqrt((ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2]) *
-                                (bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2]));
-                c = fmin(1.0, fmax(-1.0, c));
-                if (a.ang_form == MDX_ANGLE_HARMONIC)
-                    epot += 0.5 * a.ang_k[t] * (acos(c) - a.ang_t0[t]) * (acos(c) - a.ang_t0[t]);
-                else
-                    epot += a.ang_k[t] * (1 - c);
-            }
-        }
-    }
-    // -------------------------------------------------- tangential active force
-    if (a.act_prev) {
-        int nx = a.act_next[i] < 0 ? i : a.act_next[i];
-        int pv = a.act_prev[i] < 0 ? i : a.act_prev[i];
-        if (nx != pv) {
-            double t[3];
-            for (int k = 0; k < 3; k++)
-                t[k] = mimage(a.X[3 * nx + k] - a.X[3 * pv + k], rp.l[k], rp.l2s[k], a.periodic);
-            double inv = 1.0 / sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
-            for (int k = 0; k < 3; k++)
-                Fi[k] += (R)(a.fa * t[k] * inv);
-        }
-    }
-
-    if (!(isfinite((double)Fi[0]) && isfinite((double)Fi[1]) && isfinite((double)Fi[2]) &&
-          isfinite((double)Ti[0]) && isfinite((double)Ti[1]) && isfinite((double)Ti[2])))
-        atomicOr(&a.ctl->status, 2u);
-
-    // ------------------------------------------------------------ epilogues
-    if (MODE == M_CONS) {
-        for (int k = 0; k < 3; k++) {
-            a.Fout[3 * i + k] = (double)Fi[k];
-            a.Tout[3 * i + k] = (double)Ti[k];
-        }
-        return;
-    }
-    if (MODE == M_ENERGY) {
-        double *o = a.obs + 10 * (size_t)i;
-        o[0] = epot;
-        for (int k = 0; k < 9; k++)
-            o[1 + k] = vir[k];
-        return;
-    }
-    if (MODE == M_BEAD) {
-        // LangevinNVT::advance_mom (Langevin.cpp:254-269) then advance_pos
-        // (Langevin.cpp:205-223) and binning of the new position
-        R nz[6];
-        load_noise(a, i, 3, nz);
-        const R rr = (R)rp.bead_r;
-        const R c5 = (R)1 - (R)(a.gamma * a.dt / 2.);
-        const R c4 = (R)a.b_c2 * rr;
-        double xn[3] = {xi0, xi1, xi2};
-        for (int k = 0; k < 3; k++) {
-            R pk = a.p[3 * i + k];
-            pk = (R)a.b_c5c2 * pk + (R)a.b_c5c3q * Fi[k] + (c5 * c4 + rr) * nz[k];
-            if (FP64)
-                xn[k] = xn[k] + a.c1 * (double)pk;
-            else
-                xn[k] = xn[k] + (double)((R)a.c1 * pk);
-            wrap1(xn[k], pk, rp.l[k], a.periodic);
-            a.p[3 * i + k] = pk;
-            a.Xn[3 * i + k] = xn[k];
-        }
-        bin_particle(a, i, rp, xn);
-        return;
-    }
-
-    // rotor: create_forces_and_torques_sphere (LangevinR.cpp:475-578)
-    R pl[3] = {a.p[3 * i], a.p[3 * i + 1], a.p[3 * i + 2]};
-    R ll[3] = {a.L[3 * i], a.L[3 * i + 1], a.L[3 * i + 2]};
-    R nz[6];
-    load_noise(a, i, 6, nz);
-    {
-        const R *q = qi;
-        R pbx = q[0] * pl[0] + q[1] * pl[1] + q[2] * pl[2];
-        R pby = q[3] * pl[0] + q[4] * pl[1] + q[5] * pl[2];
-        R pbz = q[6] * pl[0] + q[7] * pl[1] + q[8] * pl[2];
-        const R gm = (R)(-a.gamma / a.m);
-        R fx = gm * pbx + (R)rp.Rt * nz[0];
-        R fy = gm * pby + (R)rp.Rt * nz[1];
-        R fz = gm * pbz + (R)rp.Rt * nz[2];
-        R tbx = (R)(-a.gammar / a.I[0]) * ll[0] + (R)rp.Rr * nz[3];
-        R tby = (R)(-a.gammar / a.I[1]) * ll[1] + (R)rp.Rr * nz[4];
-        R tbz = (R)(-a.gammar / a.I[2]) * ll[2] + (R)rp.Rr * nz[5];
-        R Fx = Fi[0] + fx * q[0] + fy * q[3] + fz * q[6];
-        R Fy = Fi[1] + fx * q[1] + fy * q[4] + fz * q[7];
-        R Fz = Fi[2] + fx * q[2] + fy * q[5] + fz * q[8];
-        R Tx = tbx + Ti[0] * q[0] + Ti[1] * q[1] + Ti[2] * q[2];
-        R Ty = tby + Ti[0] * q[3] + Ti[1] * q[4] + Ti[2] * q[5];
-        R Tz = tbz + Ti[0] * q[6] + Ti[1] * q[7] + Ti[2] * q[8];
-        Fi[0] = Fx;
-        Fi[1] = Fy;
-        Fi[2] = Fz;
-        Ti[0] = Tx;
-        Ti[1] = Ty;
-        Ti[2] = Tz;
-    }
-    if (MODE == M_PRIME) {
-        for (int k = 0; k < 3; k++) {
-            a.F[3 * i + k] = Fi[k];
-            a.T[3 * i + k] = Ti[k];
-        }
-        return;
-    }
-    // advancemom_halfstep (LangevinR.cpp:327-347)
-    const R hdt = (R)a.hdt;
-    for (int k = 0; k < 3; k++) {
-        pl[k] += hdt * Fi[k];
-        ll[k] += hdt * Ti[k];
-    }
-    if (MODE == M_STEP) {
-        for (int k = 0; k < 3; k++) {
-            a.p[3 * i + k] = pl[k];
-            a.L[3 * i + k] = ll[k];
-            a.F[3 * i + k] = Fi[k];
-            a.T[3 * i + k] = Ti[k];
-        }
-        return;
-    }
-    // M_FUSED: first half of the next step with the same F, T
-    for (int k = 0; k < 3; k++) {
-        pl[k] += hdt * Fi[k];
-        ll[k] += hdt * Ti[k];
-    }
-    double xn[3] = {xi0, xi1, xi2};
-    for (int k = 0; k < 3; k++) {
-        if (FP64)
-            xn[k] = xn[k] + a.c1 * (double)pl[k];
-        else
-            xn[k] = xn[k] + (double)((R)a.c1 * pl[k]);
-        wrap1(xn[k], pl[k], rp.l[k], a.periodic);
-    }
-    const R Ir[3] = {(R)a.I[0], (R)a.I[1], (R)a.I[2]};
-    rotate((R)a.dt, Ir, ll, qi);
-    if (!FP64 && a.reortho_every > 0 && ((a.ctl->epoch + 1) % a.reortho_every) == 0)
-        reortho(qi);
-    for (int k = 0; k < 3; k++) {
-        a.p[3 * i + k] = pl[k];
-        a.L[3 * i + k] = ll[k];
-        a.Xn[3 * i + k] = xn[k];
-    }
-    for (int k = 0; k < 9; k++)
-        a.Qn[9 * (size_t)i + k] = qi[k];
-    bin_particle(a, i, rp, xn);
-}
-
-// first half of a rotor step from the carried F, T: half kick, drift + wrap
-// (Langevin.cpp:205-223), rotate (LangevinR.cpp:687-695), then binning
-template <class R> __global__ void __launch_bounds__(256) k_kick_drift(KArgs<R> a)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.N)
-        return;
-    constexpr bool FP64 = sizeof(R) == 8;
-    const DevRep rp = a.reps[a.meta[i] >> 8];
-    const R hdt = (R)a.hdt;
-    R pl[3], ll[3], q[9];
-    double xn[3];
-    for (int k = 0; k < 3; k++) {
-        pl[k] = a.p[3 * i + k] + hdt * a.F[3 * i + k];
-        ll[k] = a.L[3 * i + k] + hdt * a.T[3 * i + k];
-        xn[k] = a.X[3 * i + k];
-    }
-    for (int k = 0; k < 9; k++)
-        q[k] = a.Q[9 * (size_t)i + k];
-    for (int k = 0; k < 3; k++) {
-        if (FP64)
-            xn[k] = xn[k] + a.c1 * (double)pl[k];
-        else
-            xn[k] = xn[k] + (double)((R)a.c1 * pl[k]);
-        wrap1(xn[k], pl[k], rp.l[k], a.periodic);
-    }
-    const R Ir[3] = {(R)a.I[0], (R)a.I[1], (R)a.I[2]};
-    rotate((R)a.dt, Ir, ll, q);
-    if (!FP64 && a.reortho_every > 0 && ((a.ctl->epoch + 1) % a.reortho_every) == 0)
-        reortho(q);
-    for (int k = 0; k < 3; k++) {
-        a.p[3 * i + k] = pl[k];
-        a.L[3 * i + k] = ll[k];
-        a.Xn[3 * i + k] = xn[k];
-    }
-    for (int k = 0; k < 9; k++)
-        a.Qn[9 * (size_t)i + k] = q[k];
-    bin_particle(a, i, rp, xn);
-}
-
-// binning of the current positions (after an upload)
-template <class R> __global__ void __launch_bounds__(256) k_bin(KArgs<R> a)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.N)
-        return;
-    const DevRep rp = a.reps[a.meta[i] >> 8];
-    double x[3] = {a.X[3 * i], a.X[3 * i + 1], a.X[3 * i + 2]};
-    bin_particle(a, i, rp, x);
-}
-
-// --------------------------------------------------------------- cell lists
-// exclusive scan of count[0..n) into start[0..n], count zeroed after use.
-// phase 1: per block (1024 elements) local scan and block total
-__global__ void __launch_bounds__(256) k_scan1(const int *count, int *start, int *bsum, int n)
-{
-    __shared__ int sh[256];
-    const int base = blockIdx.x * 1024 + threadIdx.x * 4;
-    int v[4], s = 0;
-    for (int k = 0; k < 4; k++) {
-        v[k] = base + k < n ? count[base + k] : 0;
-        s += v[k];
-    }
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-        __syncthreads();
-        sh[threadIdx.x] += t;
-        __syncthreads();
-    }
-    int run = sh[threadIdx.x] - s;
-    for (int k = 0; k < 4; k++) {
-        if (base + k < n)
-            start[base + k] = run;
-        run += v[k];
-    }
-    if (threadIdx.x == 255)
-        bsum[blockIdx.x] = sh[255];
-}
-
-// phase 2: exclusive scan of the block totals (one block)
-__global__ void __launch_bounds__(1024) k_scan2(int *bsum, int nb, int *total)
-{
-    __shared__ int sh[1024];
-    const int per = (nb + 1023) / 1024;
-    const int b0 = threadIdx.x * per;
-    int s = 0;
-    for (int k = 0; k < per; k++)
-        if (b0 + k < nb)
-            s += bsum[b0 + k];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-        __syncthreads();
-        sh[threadIdx.x] += t;
-        __syncthreads();
-    }
-    int run = sh[threadIdx.x] - s;
-    for (int k = 0; k < per; k++)
-        if (b0 + k < nb) {
-            int v = bsum[b0 + k];
-            bsum[b0 + k] = run;
-            run += v;
-        }
-    if (threadIdx.x == 1023)
-        *total = sh[1023];
-}
-
-// phase 3: add block offsets, zero the counts, close the array
-__global__ void __launch_bounds__(256) k_scan3(int *count, int *start, const int *bsum, int n,
-                                               const int *total)
-{
-    const int base = blockIdx.x * 1024 + threadIdx.x * 4;
-    const int off = bsum[blockIdx.x];
-    for (int k = 0; k < 4; k++)
-        if (base + k < n) {
-            start[base + k] += off;
-            count[base + k] = 0;
-        }
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        start[n] = *total;
-}
-
-__global__ void __launch_bounds__(256) k_fill(int N, const int *cell, const int *rank,
-                                              const int *start, int *list)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N)
-        list[start[cell[i]] + rank[i]] = i;
-}
-
-// ascending particle order inside each cell (makes the gather order, and so
-// the floating-point sums, independent of atomic arrival order)
-__global__ void __launch_bounds__(256) k_cellsort(int ncells, const int *start, int *list)
-{
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells)
-        return;
-    int s0 = start[c], s1 = start[c + 1];
-    for (int s = s0 + 1; s < s1; s++) {
-        int v = list[s];
-        int t = s - 1;
-        while (t >= s0 && list[t] > v) {
-            list[t + 1] = list[t];
-            t--;
-        }
-        list[t + 1] = v;
-    }
-}
-
-__global__ void k_epoch_inc(StepCtl *ctl)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0)
-        ctl->epoch += 1;
-}
-
-__global__ void k_set_call_base(StepCtl *ctl)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0)
-        ctl->call_base = ctl->epoch;
-}
-
-// ---------------------------------------------------- parity entry points
-// cube::assign_box with a caller-given n for particles [first, first+count)
-__global__ void k_assign_custom(const double *X, int first, int count, double l0, double l1,
-                                double l2, int n0, int n1, int n2, int *out)
-{
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count)
-        return;
-    const double *x = X + 3 * (size_t)(first + k);
-    int cx = axis_cell(x[0], n0, l0), cy = axis_cell(x[1], n1, l1), cz = axis_cell(x[2], n2, l2);
-    out[k] = cx + n0 * (cy + n1 * cz);
-}
-
-// cube::distance_less_than (geometry.h:145-164)
-__device__ __forceinline__ bool dlt(const double *xi, const double *xj, const double l[3],
-                                    int periodic, double cut)
-{
-    double temp = 0.0;
-    const double c2 = cut * cut;
-    for (int k = 0; k < 3; k++) {
-        double dx = xi[k] - xj[k];
-        if (periodic) {
-            if (fabs(dx) > l[k] * 0.5)
-                dx = dx - (dx >= 0 ? fabs(l[k]) : -fabs(l[k]));
-        }
-        dx = dx * dx;
-        if (dx > c2)
-            return false;
-        temp += dx;
-    }
-    return temp < c2;
-}
-
-// pass 0: count (write=0) or write sorted partner lists (write=1) of i < j
-__global__ void k_pairs(const double *X, int first, int count, double l0, double l1, double l2,
-                        int periodic, int n0, int n1, int n2, double cut, const int *cellv,
-                        const int *cstart, const int *clist, int *cnt, const long long *off,
-                        int *ij, long long cap, int write)
-{
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count)
-        return;
-    const double l[3] = {l0, l1, l2};
-    const int nn[3] = {n0, n1, n2};
-    const double *xi = X + 3 * (size_t)(first + k);
-    const int c = cellv[k];
-    const int cc[3] = {c % n0, (c / n0) % n1, c / (n0 * n1)};
-    int lo[3], hi[3];
-    for (int a = 0; a < 3; a++) {
-        lo[a] = nn[a] >= 3 ? -1 : 0;
-        hi[a] = nn[a] >= 2 ? 1 : 0;
-    }
-    int found = 0;
-    long long o = write ? off[k] : 0;
-    for (int oz = lo[2]; oz <= hi[2]; oz++)
-        for (int oy = lo[1]; oy <= hi[1]; oy++)
-            for (int ox = lo[0]; ox <= hi[0]; ox++) {
-                int w[3] = {cc[0] + ox, cc[1] + oy, cc[2] + oz};
-                for (int a = 0; a < 3; a++)
-                    w[a] = w[a] < 0 ? w[a] + nn[a] : (w[a] >= nn[a] ? w[a] - nn[a] : w[a]);
-                int c2 = w[0] + n0 * (w[1] + n1 * w[2]);
-                for (int s = cstart[c2]; s < cstart[c2 + 1]; s++) {
-                    int j = clist[s];
-                    if (j <= k)
-                        continue;
-                    if (dlt(xi, X + 3 * (size_t)(first + j), l, periodic, cut)) {
-                        if (write && o + found < cap) {
-                            ij[2 * (o + found)] = k;
-                            ij[2 * (o + found) + 1] = j;
-                        }
-                        found++;
-                    }
-                }
-            }
-    if (!write) {
-        cnt[k] = found;
-        return;
-    }
-    // order the partners of k ascending
-    long long e = o + found < cap ? o + found : cap;
-    for (long long s = o + 1; s < e; s++) {
-        int v = ij[2 * s + 1];
-        long long t = s - 1;
-        while (t >= o && ij[2 * t + 1] > v) {
-            ij[2 * (t + 1) + 1] = ij[2 * t + 1];
-            t--;
-        }
-        ij[2 * (t + 1) + 1] = v;
-    }
-}
-
-// ------------------------------------------------------------ reductions
-// per-block partial sums (fixed order) of 16 observables over [first, first+count)
-template <class R>
-__global__ void __launch_bounds__(256) k_obs(int first, int count, const R *p, const R *L,
-                                             const double *obs, int has_obs, double m, double I0,
-                                             double *partial)
-{
-    __shared__ double sh[256];
-    double acc[16];
-    for (int k = 0; k < 16; k++)
-        acc[k] = 0;
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < count; s += gridDim.x * blockDim.x) {
-        int i = first + s;
-        for (int k = 0; k < 3; k++) {
-            double pk = (double)p[3 * i + k], lk = L ? (double)L[3 * i + k] : 0.0;
-            acc[k] += (pk * pk) / (2. * m);
-            acc[3 + k] += (lk * lk) / (2. * I0);
-        }
-        if (has_obs)
-            for (int k = 0; k < 10; k++)
-                acc[6 + k] += obs[10 * (size_t)i + k];
-    }
-    for (int k = 0; k < 16; k++) {
-        sh[threadIdx.x] = acc[k];
-        __syncthreads();
-        for (int off = 128; off > 0; off >>= 1) {
-            if (threadIdx.x < off)
-                sh[threadIdx.x] += sh[threadIdx.x + off];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0)
-            partial[16 * blockIdx.x + k] = sh[0];
-        __syncthreads();
-    }
-}
-
-// precision conversion helpers (device <-> host FP64 layout)
-template <class A, class B> __global__ void k_convert(const A *in, B *out, size_t n)
-{
-    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n)
-        out[k] = (B)in[k];
-}
-
 // ================================================================ host side
 namespace {
 
@@ -1026,16 +70,18 @@ struct mdx_engine {
     int device = 0;
     std::vector<int> nper, first;
     std::vector<double> box, kT;
+    std::vector<uint32_t> meta; // by original index
     hipStream_t stream = nullptr;
     // interactions
     int wca = 0;
     double wca_eps = 1, wca_sigma = 1, wca_pl = 1.12246;
     int ntypes = 0;
-    std::vector<int> type, np, pstart, rstart;
+    std::vector<int> np, pstart, rstart;
     std::vector<double> pdirs;
     std::vector<double> prow_raw; // per replica rows x 3
     int nrows = 0;
     double maxd2 = 2.4;
+    double costm_min = 1.0;
     unsigned long long active_mask = 0;
     int bond_kind = MDX_BOND_HARMONIC, ang_form = MDX_ANGLE_HARMONIC;
     long long nbonds = 0, nangles = 0, nexcl = 0;
@@ -1047,20 +93,21 @@ struct mdx_engine {
     int ncells = 0;
     std::vector<DevRep> reps;
     bool grid_dirty = true;
-    // state flags
+    // state
     int cur = 0;
     bool binned = false;
     bool primed = false;
     unsigned long long steps_taken = 0;
     size_t bytes_held = 0;
-    // device buffers
-    Buf X[2], Q[2], P, Lm, Fc, Tc, Fout, Tout, obs;
-    Buf meta, dreps, prows, pdirs_d;
-    Buf cell, rank, count, start, list, bsum, total;
+    // device buffers (double-buffered state in slot order)
+    Buf X[2], Q[2], P[2], Lm[2], Fc[2], Tc[2], slot[2], cell[2];
+    Buf Fout, Tout, obs;
+    Buf dreps, prows, pdirs_d;
+    Buf rank, count, start, list, bsum, total;
     Buf bond_off, bond_j, bond_k, bond_r0;
     Buf ang_off, ang_ref, ang_ijk, ang_k, ang_t0;
     Buf exc_off, exc_j, act_prev, act_next;
-    Buf noise, ctl, partial, tmpd, tmpi, tmpi2, tmpi3, tmpll;
+    Buf noise, ctl, partial, tmpd, tmpi, tmpi2, tmpi3, tmpll, tmpx;
 
     size_t rsz() const { return prec == MDX_PREC_FP64 ? 8 : 4; }
 };
@@ -1102,6 +149,14 @@ template <class T> static int upload(mdx_engine *e, Buf &b, const T *h, size_t n
     return MDX_OK;
 }
 
+template <class T> static int download(mdx_engine *e, const Buf &b, T *h, size_t n)
+{
+    if (n)
+        HIPCHK(hipMemcpyAsync(h, b.p, n * sizeof(T), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return MDX_OK;
+}
+
 static inline unsigned nblk(long long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
 // interaction reach: WCA cut-off and the patch reach min(sqrt(maxd2), 1.4 d)
@@ -1110,7 +165,7 @@ static double reach(const mdx_engine *e)
     double r = 0;
     if (e->wca)
         r = std::max(r, e->wca_pl);
-    if (e->ntypes > 0 && e->active_mask) {
+    if (e->ntypes > 0 && e->active_mask && e->integ == MDX_INTEG_ROTOR) {
         double dmax = 0;
         for (size_t k = 0; k < e->prow_raw.size() / 3; k++)
             if (e->prow_raw[3 * k] != 0.0)
@@ -1124,28 +179,31 @@ static int build_grid(mdx_engine *e)
 {
     if (!e->grid_dirty)
         return MDX_OK;
-    double rc = reach(e);
+    const double rc = reach(e);
     e->reps.resize(e->nrep);
-    int coff = 0;
+    long long coff = 0;
     for (int r = 0; r < e->nrep; r++) {
         DevRep &d = e->reps[r];
         for (int k = 0; k < 3; k++) {
             d.l[k] = e->box[3 * r + k];
             d.l2s[k] = (d.l[k] / 2.) * (d.l[k] / 2.);
+            // cells no smaller than the interaction reach (margin for rounding)
             int n = rc > 0 ? (int)std::floor(d.l[k] / (rc * (1 + 1e-9))) : 1;
             d.nc[k] = std::max(1, std::min(n, 1024));
         }
-        d.coff = coff;
-        coff += d.nc[0] * d.nc[1] * d.nc[2];
+        d.coff = (int)coff;
+        coff += (long long)d.nc[0] * d.nc[1] * d.nc[2];
         d.first = e->first[r];
         d.count = e->nper[r];
-        double kT = e->kT[r];
+        const double kT = e->kT[r];
         d.Rt = std::sqrt(2 * e->gamma * kT / e->dt);
         d.Rr = std::sqrt(2 * e->gammar * kT / e->dt);
         d.bead_r = std::sqrt(0.5 * kT * (e->gamma) * (e->m) * (e->dt));
         d.prow = r * e->nrows;
     }
-    e->ncells = coff;
+    if (coff > (1ll << 30))
+        return set_err(MDX_ERR_ARG, "cell grid too large (%lld cells)", coff);
+    e->ncells = (int)coff;
     int rc2;
     if ((rc2 = upload(e, e->dreps, e->reps.data(), e->reps.size())))
         return rc2;
@@ -1154,7 +212,9 @@ static int build_grid(mdx_engine *e)
     HIPCHK(hipMemsetAsync(e->count.p, 0, sizeof(int) * (size_t)(e->ncells + 1), e->stream));
     if ((rc2 = dalloc(e, e->start, sizeof(int) * (size_t)(e->ncells + 1))))
         return rc2;
-    int nb = (e->ncells + 1023) / 1024;
+    const int nb = (e->ncells + 1023) / 1024;
+    if (nb > 1024 * 1024)
+        return set_err(MDX_ERR_ARG, "cell grid too large for the scan");
     if ((rc2 = dalloc(e, e->bsum, sizeof(int) * (size_t)std::max(nb, 1))))
         return rc2;
     e->grid_dirty = false;
@@ -1167,34 +227,41 @@ template <class R> static KArgs<R> make_args(mdx_engine *e)
 {
     KArgs<R> a;
     memset(&a, 0, sizeof a);
+    const int c = e->cur, n = e->cur ^ 1;
     a.N = e->N;
     a.rotor = e->integ == MDX_INTEG_ROTOR;
     a.periodic = e->periodic;
-    a.meta = (const uint32_t *)e->meta.p;
     a.reps = (const DevRep *)e->dreps.p;
-    a.X = (const double *)e->X[e->cur].p;
-    a.Xn = (double *)e->X[e->cur ^ 1].p;
-    a.Q = (const R *)e->Q[e->cur].p;
-    a.Qn = (R *)e->Q[e->cur ^ 1].p;
-    a.p = (R *)e->P.p;
-    a.L = (R *)e->Lm.p;
-    a.F = (R *)e->Fc.p;
-    a.T = (R *)e->Tc.p;
-    a.Fout = (double *)e->Fout.p;
-    a.Tout = (double *)e->Tout.p;
-    a.obs = (double *)e->obs.p;
-    a.cell = (int *)e->cell.p;
+    a.X = (const PosRec *)e->X[c].p;
+    a.Xn = (PosRec *)e->X[n].p;
+    a.Q = (const R *)e->Q[c].p;
+    a.Qn = (R *)e->Q[n].p;
+    a.p = (const R *)e->P[c].p;
+    a.pn = (R *)e->P[n].p;
+    a.L = (const R *)e->Lm[c].p;
+    a.Ln = (R *)e->Lm[n].p;
+    a.F = (const R *)e->Fc[c].p;
+    a.Fn = (R *)e->Fc[n].p;
+    a.T = (const R *)e->Tc[c].p;
+    a.Tn = (R *)e->Tc[n].p;
+    a.slot = (const int *)e->slot[c].p;
+    a.slotn = (int *)e->slot[n].p;
+    a.cell = (const int *)e->cell[c].p;
+    a.celln = (int *)e->cell[n].p;
     a.rank = (int *)e->rank.p;
     a.count = (int *)e->count.p;
     a.start = (const int *)e->start.p;
     a.list = (const int *)e->list.p;
+    a.Fout = (double *)e->Fout.p;
+    a.Tout = (double *)e->Tout.p;
+    a.obs = (double *)e->obs.p;
     a.wca = e->wca;
     a.wca_eps = e->wca_eps;
-    double s3 = e->wca_sigma * e->wca_sigma * e->wca_sigma;
+    const double s3 = e->wca_sigma * e->wca_sigma * e->wca_sigma;
     a.wca_s6 = s3 * s3;
     a.wca_s12 = a.wca_s6 * a.wca_s6;
     a.wca_pl = e->wca_pl;
-    a.has_patches = e->ntypes > 0 && e->integ == MDX_INTEG_ROTOR;
+    a.has_patches = e->ntypes > 0 && e->integ == MDX_INTEG_ROTOR && e->active_mask != 0;
     for (int t = 0; t < MAXT; t++) {
         a.np[t] = t < e->ntypes ? e->np[t] : 0;
         a.pstart[t] = t < e->ntypes ? e->pstart[t] : 0;
@@ -1205,9 +272,10 @@ template <class R> static KArgs<R> make_args(mdx_engine *e)
     a.pdirs = (const double *)e->pdirs_d.p;
     a.prows = (const PatchRow *)e->prows.p;
     a.maxd2 = e->maxd2;
-    double pr = std::max(e->wca ? e->wca_pl * e->wca_pl * (1 + 1e-12) : 0.0,
-                         a.has_patches ? e->maxd2 : 0.0);
-    a.rmax2 = pr;
+    a.rmax2 = std::max(e->wca ? e->wca_pl * e->wca_pl * (1 + 1e-12) : 0.0,
+                       a.has_patches ? e->maxd2 : 0.0);
+    // the screen of the cone test runs in R precision: FP32 needs a margin
+    a.costm_screen = e->costm_min - (e->prec == MDX_PREC_FP64 ? 1e-12 : 1e-5);
     if (e->nbonds) {
         a.bond_off = (const int *)e->bond_off.p;
         a.bond_j = (const int *)e->bond_j.p;
@@ -1241,21 +309,22 @@ template <class R> static KArgs<R> make_args(mdx_engine *e)
     for (int k = 0; k < 3; k++)
         a.I[k] = e->I[k];
     // LangevinNVT coefficients (Langevin.cpp:44-52); r depends on kT (per replica)
-    double d = (e->gamma * e->dt / 2.), q = (e->dt) / 2.;
-    double c2 = (1.0 / (1.0 + (d))), c3 = (1.0 / (1.0 + (d))) * q, c5 = (1 - (d));
+    const double d = (e->gamma * e->dt / 2.), q = (e->dt) / 2.;
+    const double c2 = (1.0 / (1.0 + (d))), c3 = (1.0 / (1.0 + (d))) * q, c5 = (1 - (d));
     a.b_c2 = c2;
     a.b_c5c2 = c5 * c2;
     a.b_c5c3q = (c5 * (c3) + q);
     a.reortho_every = e->reortho_every;
     a.key0 = (uint32_t)e->seed;
     a.key1 = (uint32_t)(e->seed >> 32);
-    a.noise = (const double *)nullptr;
+    a.noise = nullptr;
     a.ctl = (StepCtl *)e->ctl.p;
     a.fguard = e->fguard;
     return a;
 }
 
-// scan + fill + per-cell ordering of the cell list built by the binning
+// scan + fill + per-cell ordering of the cell list built by the binning of
+// the current slots
 static int finish_bins(mdx_engine *e)
 {
     const int n = e->ncells;
@@ -1267,24 +336,25 @@ static int finish_bins(mdx_engine *e)
     hipLaunchKernelGGL(k_scan3, dim3(nb), dim3(256), 0, e->stream, (int *)e->count.p,
                        (int *)e->start.p, (const int *)e->bsum.p, n, (const int *)e->total.p);
     hipLaunchKernelGGL(k_fill, dim3(nblk(e->N)), dim3(256), 0, e->stream, e->N,
-                       (const int *)e->cell.p, (const int *)e->rank.p, (const int *)e->start.p,
-                       (int *)e->list.p);
+                       (const int *)e->cell[e->cur].p, (const int *)e->rank.p,
+                       (const int *)e->start.p, (int *)e->list.p);
     hipLaunchKernelGGL(k_cellsort, dim3(nblk(n)), dim3(256), 0, e->stream, n,
                        (const int *)e->start.p, (int *)e->list.p);
     HIPCHK(hipGetLastError());
     return MDX_OK;
 }
 
+// cell list of the current slots (no reordering)
 template <class R> static int bin_current(mdx_engine *e)
 {
     int rc = build_grid(e);
     if (rc)
         return rc;
     KArgs<R> a = make_args<R>(e);
+    a.celln = (int *)e->cell[e->cur].p;
     hipLaunchKernelGGL(k_bin<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
     HIPCHK(hipGetLastError());
-    rc = finish_bins(e);
-    if (rc)
+    if ((rc = finish_bins(e)))
         return rc;
     e->binned = true;
     return MDX_OK;
@@ -1307,8 +377,9 @@ static int read_status(mdx_engine *e, unsigned int *st)
     *st = c.status;
     if (c.status) {
         unsigned int z = 0;
-        HIPCHK(hipMemcpy((char *)e->ctl.p + offsetof(StepCtl, status), &z, sizeof z,
-                         hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync((char *)e->ctl.p + offsetof(StepCtl, status), &z, sizeof z,
+                              hipMemcpyHostToDevice, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
     }
     return MDX_OK;
 }
@@ -1335,6 +406,11 @@ static int noise_upload(mdx_engine *e, const double *h, size_t n, const double *
     return MDX_OK;
 }
 
+static void launch_small(mdx_engine *e, void (*k)(StepCtl *))
+{
+    hipLaunchKernelGGL(k, dim3(1), dim3(1), 0, e->stream, (StepCtl *)e->ctl.p);
+}
+
 template <class R> static int do_prime(mdx_engine *e, const double *noise0)
 {
     int rc = bin_current<R>(e);
@@ -1343,10 +419,12 @@ template <class R> static int do_prime(mdx_engine *e, const double *noise0)
     const double *nd;
     if ((rc = noise_upload(e, noise0, (size_t)e->N * 6, &nd)))
         return rc;
-    hipLaunchKernelGGL(k_set_call_base, dim3(1), dim3(1), 0, e->stream, (StepCtl *)e->ctl.p);
+    launch_small(e, k_set_call_base);
     if ((rc = launch_gather<R, M_PRIME>(e, nd)))
         return rc;
-    hipLaunchKernelGGL(k_epoch_inc, dim3(1), dim3(1), 0, e->stream, (StepCtl *)e->ctl.p);
+    e->cur ^= 1;
+    launch_small(e, k_epoch_inc);
+    e->binned = false; // the state moved to cell order; its list is stale
     unsigned int st;
     if ((rc = read_status(e, &st)))
         return rc;
@@ -1354,15 +432,11 @@ template <class R> static int do_prime(mdx_engine *e, const double *noise0)
     return status_to_rc(st);
 }
 
-template <class R> static int do_step_rotor(mdx_engine *e, int nsteps, const double *noise)
+// enqueue nsteps rotor steps (requires primed F, T)
+template <class R> static int enqueue_rotor(mdx_engine *e, int nsteps, const double *nd)
 {
-    if (!e->primed)
-        return set_err(MDX_ERR_STATE, "mdx_step before mdx_prime (the carried F, T are unset)");
     int rc;
-    const double *nd;
-    if ((rc = noise_upload(e, noise, (size_t)nsteps * e->N * 6, &nd)))
-        return rc;
-    hipLaunchKernelGGL(k_set_call_base, dim3(1), dim3(1), 0, e->stream, (StepCtl *)e->ctl.p);
+    launch_small(e, k_set_call_base);
     {
         KArgs<R> a = make_args<R>(e);
         hipLaunchKernelGGL(k_kick_drift<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
@@ -1375,15 +449,28 @@ template <class R> static int do_step_rotor(mdx_engine *e, int nsteps, const dou
         if (s + 1 < nsteps) {
             if ((rc = launch_gather<R, M_FUSED>(e, nd)))
                 return rc;
-            e->cur ^= 1;
         } else {
             if ((rc = launch_gather<R, M_STEP>(e, nd)))
                 return rc;
         }
-        hipLaunchKernelGGL(k_epoch_inc, dim3(1), dim3(1), 0, e->stream, (StepCtl *)e->ctl.p);
+        e->cur ^= 1;
+        launch_small(e, k_epoch_inc);
     }
-    e->binned = true;
+    e->binned = false;
     e->steps_taken += nsteps;
+    return MDX_OK;
+}
+
+template <class R> static int do_step_rotor(mdx_engine *e, int nsteps, const double *noise)
+{
+    if (!e->primed)
+        return set_err(MDX_ERR_STATE, "mdx_step before mdx_prime (the carried F, T are unset)");
+    int rc;
+    const double *nd;
+    if ((rc = noise_upload(e, noise, (size_t)nsteps * e->N * 6, &nd)))
+        return rc;
+    if ((rc = enqueue_rotor<R>(e, nsteps, nd)))
+        return rc;
     unsigned int st;
     if ((rc = read_status(e, &st)))
         return rc;
@@ -1398,12 +485,12 @@ template <class R> static int do_step_bead(mdx_engine *e, int nsteps, const doub
     const double *nd;
     if ((rc = noise_upload(e, noise, (size_t)nsteps * e->N * 3, &nd)))
         return rc;
-    hipLaunchKernelGGL(k_set_call_base, dim3(1), dim3(1), 0, e->stream, (StepCtl *)e->ctl.p);
+    launch_small(e, k_set_call_base);
     for (int s = 0; s < nsteps; s++) {
         if ((rc = launch_gather<R, M_BEAD>(e, nd)))
             return rc;
         e->cur ^= 1;
-        hipLaunchKernelGGL(k_epoch_inc, dim3(1), dim3(1), 0, e->stream, (StepCtl *)e->ctl.p);
+        launch_small(e, k_epoch_inc);
         if ((rc = finish_bins(e)))
             return rc;
     }
@@ -1413,6 +500,54 @@ template <class R> static int do_step_bead(mdx_engine *e, int nsteps, const doub
     if ((rc = read_status(e, &st)))
         return rc;
     return status_to_rc(st);
+}
+
+// ids of the current slots (host copy)
+static int slot_ids(mdx_engine *e, std::vector<PosRec> &rec)
+{
+    rec.resize(e->N);
+    return download(e, e->X[e->cur], rec.data(), rec.size());
+}
+
+template <class R> static int upload_real(mdx_engine *e, Buf &dst, const double *h, size_t n)
+{
+    if (sizeof(R) == 8)
+        return upload(e, dst, h, n);
+    int rc = upload(e, e->tmpd, h, n);
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL((k_convert<double, R>), dim3(nblk((long long)n)), dim3(256), 0, e->stream,
+                       (const double *)e->tmpd.p, (R *)dst.p, n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return MDX_OK;
+}
+
+// slot-ordered device array (width values per particle) to a host array in
+// original order
+template <class R>
+static int download_by_id(mdx_engine *e, const Buf &src, const std::vector<PosRec> &rec, int width,
+                          double *h)
+{
+    const size_t n = (size_t)e->N * width;
+    std::vector<double> tmp(n);
+    if (sizeof(R) == 8) {
+        int rc = download(e, src, tmp.data(), n);
+        if (rc)
+            return rc;
+    } else {
+        int rc = dalloc(e, e->tmpd, n * 8);
+        if (rc)
+            return rc;
+        hipLaunchKernelGGL((k_convert<R, double>), dim3(nblk((long long)n)), dim3(256), 0,
+                           e->stream, (const R *)src.p, (double *)e->tmpd.p, n);
+        HIPCHK(hipGetLastError());
+        if ((rc = download(e, e->tmpd, tmp.data(), n)))
+            return rc;
+    }
+    for (int s = 0; s < e->N; s++)
+        std::memcpy(h + (size_t)rec[s].id * width, tmp.data() + (size_t)s * width, 8 * width);
+    return MDX_OK;
 }
 
 // ================================================================== C ABI
@@ -1427,6 +562,8 @@ int mdx_create(const mdx_desc *d, mdx_engine **out)
     *out = nullptr;
     if (d->nreplicas < 1 || !d->n_per_replica || !d->box || !d->kT)
         return set_err(MDX_ERR_ARG, "nreplicas >= 1 with counts, boxes and kT required");
+    if (d->nreplicas > (1 << 23))
+        return set_err(MDX_ERR_ARG, "too many replicas");
     if (d->precision != MDX_PREC_FP64 && d->precision != MDX_PREC_MIXED)
         return set_err(MDX_ERR_ARG, "unknown precision %d", d->precision);
     if (d->integrator != MDX_INTEG_ROTOR && d->integrator != MDX_INTEG_BEAD)
@@ -1436,6 +573,19 @@ int mdx_create(const mdx_desc *d, mdx_engine **out)
     for (int k = 0; k < 3; k++)
         if (!(d->inertia[k] > 0))
             return set_err(MDX_ERR_ARG, "inertia must be > 0");
+    long long N = 0;
+    for (int r = 0; r < d->nreplicas; r++) {
+        if (d->n_per_replica[r] < 1)
+            return set_err(MDX_ERR_ARG, "replica %d has no particles", r);
+        for (int k = 0; k < 3; k++)
+            if (!(d->box[3 * r + k] > 0))
+                return set_err(MDX_ERR_ARG, "replica %d box edge %d must be > 0", r, k);
+        if (!(d->kT[r] >= 0))
+            return set_err(MDX_ERR_ARG, "replica %d kT must be >= 0", r);
+        N += d->n_per_replica[r];
+    }
+    if (N > (1ll << 30))
+        return set_err(MDX_ERR_ARG, "too many particles");
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));
     if (d->device < 0 || d->device >= ndev)
@@ -1454,60 +604,44 @@ int mdx_create(const mdx_desc *d, mdx_engine **out)
         e->I[k] = d->inertia[k];
     e->seed = d->seed;
     e->device = d->device;
-    long long N = 0;
+    e->N = (int)N;
+    long long f = 0;
     for (int r = 0; r < e->nrep; r++) {
-        if (d->n_per_replica[r] < 1) {
-            delete e;
-            return set_err(MDX_ERR_ARG, "replica %d has no particles", r);
-        }
-        for (int k = 0; k < 3; k++)
-            if (!(d->box[3 * r + k] > 0)) {
-                delete e;
-                return set_err(MDX_ERR_ARG, "replica %d box edge %d must be > 0", r, k);
-            }
-        e->first.push_back((int)N);
+        e->first.push_back((int)f);
         e->nper.push_back(d->n_per_replica[r]);
-        N += d->n_per_replica[r];
+        f += d->n_per_replica[r];
         e->kT.push_back(d->kT[r]);
         for (int k = 0; k < 3; k++)
             e->box.push_back(d->box[3 * r + k]);
+        for (int k = 0; k < d->n_per_replica[r]; k++)
+            e->meta.push_back((uint32_t)r << 8);
     }
-    if (N > (1ll << 30) || e->nrep > (1 << 23)) {
-        delete e;
-        return set_err(MDX_ERR_ARG, "too many particles or replicas");
-    }
-    e->N = (int)N;
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
         delete e;
         return set_err(MDX_ERR_HIP, "hipStreamCreate failed");
     }
-    size_t rs = e->rsz();
+    const size_t rs = e->rsz();
     int rc = 0;
-    rc |= dalloc(e, e->X[0], 24 * N);
-    rc |= dalloc(e, e->X[1], 24 * N);
-    rc |= dalloc(e, e->Q[0], 9 * rs * N);
-    rc |= dalloc(e, e->Q[1], 9 * rs * N);
-    rc |= dalloc(e, e->P, 3 * rs * N);
-    rc |= dalloc(e, e->Lm, 3 * rs * N);
-    rc |= dalloc(e, e->Fc, 3 * rs * N);
-    rc |= dalloc(e, e->Tc, 3 * rs * N);
-    rc |= dalloc(e, e->cell, 4 * N);
+    for (int b = 0; b < 2 && !rc; b++) {
+        rc |= dalloc(e, e->X[b], sizeof(PosRec) * N);
+        rc |= dalloc(e, e->P[b], 3 * rs * N);
+        rc |= dalloc(e, e->slot[b], 4 * N);
+        rc |= dalloc(e, e->cell[b], 4 * N);
+        if (e->integ == MDX_INTEG_ROTOR) {
+            rc |= dalloc(e, e->Q[b], 9 * rs * N);
+            rc |= dalloc(e, e->Lm[b], 3 * rs * N);
+            rc |= dalloc(e, e->Fc[b], 3 * rs * N);
+            rc |= dalloc(e, e->Tc[b], 3 * rs * N);
+        }
+    }
     rc |= dalloc(e, e->rank, 4 * N);
     rc |= dalloc(e, e->list, 4 * N);
-    rc |= dalloc(e, e->meta, 4 * N);
     rc |= dalloc(e, e->ctl, sizeof(StepCtl));
     rc |= dalloc(e, e->total, 16);
     if (rc) {
+        std::string msg = g_err;
         mdx_destroy(e);
-        return set_err(MDX_ERR_HIP, "device allocation failed (%s)", g_err.c_str());
-    }
-    std::vector<uint32_t> meta(N);
-    for (int r = 0; r < e->nrep; r++)
-        for (int k = 0; k < e->nper[r]; k++)
-            meta[e->first[r] + k] = ((uint32_t)r << 8);
-    if ((rc = upload(e, e->meta, meta.data(), meta.size()))) {
-        mdx_destroy(e);
-        return rc;
+        return set_err(MDX_ERR_HIP, "device allocation failed (%s)", msg.c_str());
     }
     *out = e;
     return MDX_OK;
@@ -1520,77 +654,65 @@ void mdx_destroy(mdx_engine *e)
     (void)hipSetDevice(e->device);
     if (e->stream)
         (void)hipStreamSynchronize(e->stream);
-    Buf *bufs[] = {&e->X[0],   &e->X[1],    &e->Q[0],    &e->Q[1],   &e->P,       &e->Lm,
-                   &e->Fc,     &e->Tc,      &e->Fout,    &e->Tout,   &e->obs,     &e->meta,
-                   &e->dreps,  &e->prows,   &e->pdirs_d, &e->cell,   &e->rank,    &e->count,
-                   &e->start,  &e->list,    &e->bsum,    &e->total,  &e->bond_off, &e->bond_j,
-                   &e->bond_k, &e->bond_r0, &e->ang_off, &e->ang_ref, &e->ang_ijk, &e->ang_k,
-                   &e->ang_t0, &e->exc_off, &e->exc_j,   &e->act_prev, &e->act_next, &e->noise,
-                   &e->ctl,    &e->partial, &e->tmpd,    &e->tmpi,   &e->tmpi2,   &e->tmpi3,
-                   &e->tmpll};
+    Buf *bufs[] = {&e->Fout,    &e->Tout,    &e->obs,    &e->dreps,  &e->prows,   &e->pdirs_d,
+                   &e->rank,    &e->count,   &e->start,  &e->list,   &e->bsum,    &e->total,
+                   &e->bond_off, &e->bond_j, &e->bond_k, &e->bond_r0, &e->ang_off, &e->ang_ref,
+                   &e->ang_ijk, &e->ang_k,   &e->ang_t0, &e->exc_off, &e->exc_j,  &e->act_prev,
+                   &e->act_next, &e->noise,  &e->ctl,    &e->partial, &e->tmpd,   &e->tmpi,
+                   &e->tmpi2,   &e->tmpi3,   &e->tmpll,  &e->tmpx};
     for (Buf *b : bufs)
         dfree(*b);
+    for (int b = 0; b < 2; b++) {
+        dfree(e->X[b]);
+        dfree(e->Q[b]);
+        dfree(e->P[b]);
+        dfree(e->Lm[b]);
+        dfree(e->Fc[b]);
+        dfree(e->Tc[b]);
+        dfree(e->slot[b]);
+        dfree(e->cell[b]);
+    }
     if (e->stream)
         (void)hipStreamDestroy(e->stream);
     delete e;
-}
-
-template <class R> static int upload_real(mdx_engine *e, Buf &dst, const double *h, size_t n)
-{
-    if (sizeof(R) == 8)
-        return upload(e, dst, h, n);
-    int rc = upload(e, e->tmpd, h, n);
-    if (rc)
-        return rc;
-    hipLaunchKernelGGL((k_convert<double, R>), dim3(nblk((long long)n)), dim3(256), 0, e->stream,
-                       (const double *)e->tmpd.p, (R *)dst.p, n);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(e->stream));
-    return MDX_OK;
-}
-
-template <class R> static int download_real(mdx_engine *e, const Buf &src, double *h, size_t n)
-{
-    if (sizeof(R) == 8) {
-        HIPCHK(hipMemcpyAsync(h, src.p, n * 8, hipMemcpyDeviceToHost, e->stream));
-        HIPCHK(hipStreamSynchronize(e->stream));
-        return MDX_OK;
-    }
-    int rc = dalloc(e, e->tmpd, n * 8);
-    if (rc)
-        return rc;
-    hipLaunchKernelGGL((k_convert<R, double>), dim3(nblk((long long)n)), dim3(256), 0, e->stream,
-                       (const R *)src.p, (double *)e->tmpd.p, n);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(h, e->tmpd.p, n * 8, hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(hipStreamSynchronize(e->stream));
-    return MDX_OK;
 }
 
 template <class R>
 static int upload_state_t(mdx_engine *e, const double *x, const double *p, const double *Q,
                           const double *L)
 {
-    size_t N = e->N;
+    const size_t N = e->N;
+    e->cur = 0;
+    std::vector<PosRec> rec(N);
+    std::vector<int> slot(N);
+    for (size_t i = 0; i < N; i++) {
+        rec[i].x = x[3 * i];
+        rec[i].y = x[3 * i + 1];
+        rec[i].z = x[3 * i + 2];
+        rec[i].id = (uint32_t)i;
+        rec[i].meta = e->meta[i];
+        slot[i] = (int)i;
+    }
     int rc;
-    if ((rc = upload(e, e->X[e->cur], x, 3 * N)))
+    if ((rc = upload(e, e->X[0], rec.data(), N)) || (rc = upload(e, e->slot[0], slot.data(), N)))
         return rc;
     std::vector<double> z;
-    if (!p || !L) {
+    if (!p || (!L && e->integ == MDX_INTEG_ROTOR))
         z.assign(3 * N, 0.0);
+    if ((rc = upload_real<R>(e, e->P[0], p ? p : z.data(), 3 * N)))
+        return rc;
+    if (e->integ == MDX_INTEG_ROTOR) {
+        if ((rc = upload_real<R>(e, e->Lm[0], L ? L : z.data(), 3 * N)))
+            return rc;
+        std::vector<double> eye;
+        if (!Q) {
+            eye.assign(9 * N, 0.0);
+            for (size_t i = 0; i < N; i++)
+                eye[9 * i] = eye[9 * i + 4] = eye[9 * i + 8] = 1.0;
+        }
+        if ((rc = upload_real<R>(e, e->Q[0], Q ? Q : eye.data(), 9 * N)))
+            return rc;
     }
-    if ((rc = upload_real<R>(e, e->P, p ? p : z.data(), 3 * N)))
-        return rc;
-    if ((rc = upload_real<R>(e, e->Lm, L ? L : z.data(), 3 * N)))
-        return rc;
-    std::vector<double> eye;
-    if (!Q) {
-        eye.assign(9 * N, 0.0);
-        for (size_t i = 0; i < N; i++)
-            eye[9 * i] = eye[9 * i + 4] = eye[9 * i + 8] = 1.0;
-    }
-    if ((rc = upload_real<R>(e, e->Q[e->cur], Q ? Q : eye.data(), 9 * N)))
-        return rc;
     return MDX_OK;
 }
 
@@ -1602,7 +724,7 @@ int mdx_upload_state(mdx_engine *e, const double *x, const double *p, const doub
     HIPCHK(hipSetDevice(e->device));
     for (long long k = 0; k < 3ll * e->N; k++)
         if (!std::isfinite(x[k]))
-            return set_err(MDX_ERR_ARG, "non-finite position at %lld", k / 3);
+            return set_err(MDX_ERR_ARG, "non-finite position at particle %lld", k / 3);
     int rc = e->prec == MDX_PREC_FP64 ? upload_state_t<double>(e, x, p, Q, L)
                                       : upload_state_t<float>(e, x, p, Q, L);
     e->binned = false;
@@ -1615,20 +737,43 @@ int mdx_download_state(mdx_engine *e, double *x, double *p, double *Q, double *L
     if (!e)
         return set_err(MDX_ERR_ARG, "null engine");
     HIPCHK(hipSetDevice(e->device));
-    size_t N = e->N;
-    int rc = MDX_OK;
-    if (x) {
-        HIPCHK(hipMemcpyAsync(x, e->X[e->cur].p, 24 * N, hipMemcpyDeviceToHost, e->stream));
-        HIPCHK(hipStreamSynchronize(e->stream));
+    std::vector<PosRec> rec;
+    int rc = slot_ids(e, rec);
+    if (rc)
+        return rc;
+    if (x)
+        for (int s = 0; s < e->N; s++) {
+            double *o = x + 3 * (size_t)rec[s].id;
+            o[0] = rec[s].x;
+            o[1] = rec[s].y;
+            o[2] = rec[s].z;
+        }
+    const bool f64 = e->prec == MDX_PREC_FP64;
+    const bool rot = e->integ == MDX_INTEG_ROTOR;
+    const int c = e->cur;
+    if (p && (rc = f64 ? download_by_id<double>(e, e->P[c], rec, 3, p)
+                       : download_by_id<float>(e, e->P[c], rec, 3, p)))
+        return rc;
+    if (Q) {
+        if (rot) {
+            if ((rc = f64 ? download_by_id<double>(e, e->Q[c], rec, 9, Q)
+                          : download_by_id<float>(e, e->Q[c], rec, 9, Q)))
+                return rc;
+        } else {
+            for (int i = 0; i < e->N; i++)
+                for (int k = 0; k < 9; k++)
+                    Q[9 * (size_t)i + k] = (k % 4 == 0) ? 1.0 : 0.0;
+        }
     }
-    bool f64 = e->prec == MDX_PREC_FP64;
-    if (p && (rc = f64 ? download_real<double>(e, e->P, p, 3 * N) : download_real<float>(e, e->P, p, 3 * N)))
-        return rc;
-    if (Q && (rc = f64 ? download_real<double>(e, e->Q[e->cur], Q, 9 * N)
-                       : download_real<float>(e, e->Q[e->cur], Q, 9 * N)))
-        return rc;
-    if (L && (rc = f64 ? download_real<double>(e, e->Lm, L, 3 * N) : download_real<float>(e, e->Lm, L, 3 * N)))
-        return rc;
+    if (L) {
+        if (rot) {
+            if ((rc = f64 ? download_by_id<double>(e, e->Lm[c], rec, 3, L)
+                          : download_by_id<float>(e, e->Lm[c], rec, 3, L)))
+                return rc;
+        } else {
+            std::fill(L, L + 3 * (size_t)e->N, 0.0);
+        }
+    }
     return MDX_OK;
 }
 
@@ -1636,15 +781,20 @@ int mdx_download_forces(mdx_engine *e, double *F, double *T)
 {
     if (!e)
         return set_err(MDX_ERR_ARG, "null engine");
-    if (!e->primed)
-        return set_err(MDX_ERR_STATE, "no carried forces (prime or step first)");
+    if (!e->primed || e->integ != MDX_INTEG_ROTOR)
+        return set_err(MDX_ERR_STATE, "no carried forces (rotor engine: prime or step first)");
     HIPCHK(hipSetDevice(e->device));
-    size_t N = e->N;
-    bool f64 = e->prec == MDX_PREC_FP64;
-    int rc;
-    if (F && (rc = f64 ? download_real<double>(e, e->Fc, F, 3 * N) : download_real<float>(e, e->Fc, F, 3 * N)))
+    std::vector<PosRec> rec;
+    int rc = slot_ids(e, rec);
+    if (rc)
         return rc;
-    if (T && (rc = f64 ? download_real<double>(e, e->Tc, T, 3 * N) : download_real<float>(e, e->Tc, T, 3 * N)))
+    const bool f64 = e->prec == MDX_PREC_FP64;
+    const int c = e->cur;
+    if (F && (rc = f64 ? download_by_id<double>(e, e->Fc[c], rec, 3, F)
+                       : download_by_id<float>(e, e->Fc[c], rec, 3, F)))
+        return rc;
+    if (T && (rc = f64 ? download_by_id<double>(e, e->Tc[c], rec, 3, T)
+                       : download_by_id<float>(e, e->Tc[c], rec, 3, T)))
         return rc;
     return MDX_OK;
 }
@@ -1665,6 +815,7 @@ static int patch_rows_upload(mdx_engine *e)
 {
     std::vector<PatchRow> rows(e->prow_raw.size() / 3);
     const double PI = 3.14159265358979323846;
+    e->costm_min = 1.0;
     for (size_t k = 0; k < rows.size(); k++) {
         PatchRow &r = rows[k];
         r.eps = e->prow_raw[3 * k];
@@ -1673,12 +824,13 @@ static int patch_rows_upload(mdx_engine *e)
         r.costm = std::cos(r.thetam);
         r.kk = PI / (2 * r.thetam);
         r.idist = 1.4 * r.d;
+        e->costm_min = std::min(e->costm_min, r.costm);
     }
     // a type pair is active when any of its rows in any replica has eps != 0
     e->active_mask = 0;
     for (int ta = 0; ta < e->ntypes; ta++)
         for (int tc = ta; tc < e->ntypes; tc++) {
-            int n = e->np[ta] * e->np[tc];
+            const int n = e->np[ta] * e->np[tc];
             bool on = false;
             for (int r = 0; r < e->nrep && !on; r++)
                 for (int k = 0; k < n; k++)
@@ -1687,7 +839,7 @@ static int patch_rows_upload(mdx_engine *e)
                 e->active_mask |= 1ull << (ta * MAXT + tc);
         }
     e->grid_dirty = true;
-    return upload(e, e->prows, rows.data(), rows.size());
+    return upload(e, e->prows, rows.data(), std::max<size_t>(rows.size(), 1));
 }
 
 int mdx_set_patches(mdx_engine *e, int ntypes, const int *type_of_particle,
@@ -1699,58 +851,62 @@ int mdx_set_patches(mdx_engine *e, int ntypes, const int *type_of_particle,
     if (!(maxd2 > 0))
         return set_err(MDX_ERR_ARG, "maxd2 must be > 0");
     HIPCHK(hipSetDevice(e->device));
-    int ptot = 0;
-    for (int t = 0; t < ntypes; t++)
-        ptot += npatch_per_type[t] > 0 ? npatch_per_type[t] : 0;
-    if (ptot > 0 && !dirs)
-        return set_err(MDX_ERR_ARG, "patch directions required");
-    e->ntypes = ntypes;
-    e->np.assign(npatch_per_type, npatch_per_type + ntypes);
-    e->pstart.assign(ntypes, 0);
     int tot = 0;
     for (int t = 0; t < ntypes; t++) {
-        if (e->np[t] < 0 || e->np[t] > MAXP)
+        if (npatch_per_type[t] < 0 || npatch_per_type[t] > MAXP)
             return set_err(MDX_ERR_ARG, "type %d: 0..%d patches supported", t, MAXP);
-        e->pstart[t] = tot;
-        tot += e->np[t];
+        tot += npatch_per_type[t];
     }
-    e->rstart.assign(MAXT * MAXT, 0);
-    int hj = 0;
-    for (int i = 0; i < ntypes; i++)
-        for (int j = i; j < ntypes; j++) {
-            e->rstart[i * MAXT + j] = hj;
-            hj += e->np[i] * e->np[j];
-        }
-    e->nrows = hj;
+    if (tot > 0 && !dirs)
+        return set_err(MDX_ERR_ARG, "patch directions required");
     for (int k = 0; k < tot; k++) {
-        double n2 = dirs[3 * k] * dirs[3 * k] + dirs[3 * k + 1] * dirs[3 * k + 1] +
-                    dirs[3 * k + 2] * dirs[3 * k + 2];
+        const double n2 = dirs[3 * k] * dirs[3 * k] + dirs[3 * k + 1] * dirs[3 * k + 1] +
+                          dirs[3 * k + 2] * dirs[3 * k + 2];
         // KernFrenkelOnePatch2 constructor check (potentialtheta.h:415-422)
         if (std::fabs(n2 - 1) > 1E-5)
             return set_err(MDX_ERR_ARG, "patch %d is not a unit vector", k);
     }
+    for (int i = 0; i < e->N; i++)
+        if (type_of_particle[i] < 0 || type_of_particle[i] >= ntypes)
+            return set_err(MDX_ERR_ARG, "particle %d has type %d outside 0..%d", i,
+                           type_of_particle[i], ntypes - 1);
+    int hj = 0;
+    std::vector<int> rstart(MAXT * MAXT, 0);
+    for (int i = 0; i < ntypes; i++)
+        for (int j = i; j < ntypes; j++) {
+            rstart[i * MAXT + j] = hj;
+            hj += npatch_per_type[i] * npatch_per_type[j];
+        }
     for (int k = 0; k < hj; k++)
         if (!(params[3 * k + 1] > 0) || !(params[3 * k + 2] > 0))
             return set_err(MDX_ERR_ARG, "patch row %d: d and theta_m must be > 0", k);
-    e->pdirs.assign(dirs, dirs + 3 * tot);
+    e->ntypes = ntypes;
+    e->np.assign(npatch_per_type, npatch_per_type + ntypes);
+    e->pstart.assign(ntypes, 0);
+    for (int t = 1; t < ntypes; t++)
+        e->pstart[t] = e->pstart[t - 1] + e->np[t - 1];
+    e->rstart = rstart;
+    e->nrows = hj;
+    e->pdirs.assign(dirs ? dirs : params, dirs ? dirs + 3 * tot : params);
     e->prow_raw.clear();
     for (int r = 0; r < e->nrep; r++)
         e->prow_raw.insert(e->prow_raw.end(), params, params + 3 * hj);
     e->maxd2 = maxd2;
-    e->type.assign(type_of_particle, type_of_particle + e->N);
-    std::vector<uint32_t> meta(e->N);
     for (int r = 0; r < e->nrep; r++)
         for (int k = 0; k < e->nper[r]; k++) {
-            int i = e->first[r] + k;
-            int t = e->type[i];
-            if (t < 0 || t >= ntypes)
-                return set_err(MDX_ERR_ARG, "particle %d has type %d outside 0..%d", i, t, ntypes - 1);
-            meta[i] = ((uint32_t)r << 8) | (uint32_t)t;
+            const int i = e->first[r] + k;
+            e->meta[i] = ((uint32_t)r << 8) | (uint32_t)type_of_particle[i];
         }
     int rc;
-    if ((rc = upload(e, e->meta, meta.data(), meta.size())))
+    if ((rc = upload(e, e->tmpi, e->meta.data(), e->meta.size())))
         return rc;
-    if ((rc = upload(e, e->pdirs_d, e->pdirs.data(), std::max<size_t>(e->pdirs.size(), 3))))
+    hipLaunchKernelGGL(k_set_meta, dim3(nblk(e->N)), dim3(256), 0, e->stream,
+                       (PosRec *)e->X[e->cur].p, e->N, (const uint32_t *)e->tmpi.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(e->stream));
+    std::vector<double> pd = e->pdirs;
+    pd.resize(std::max<size_t>(pd.size(), 3), 0.0);
+    if ((rc = upload(e, e->pdirs_d, pd.data(), pd.size())))
         return rc;
     return patch_rows_upload(e);
 }
@@ -1786,10 +942,10 @@ static int check_same_replica(mdx_engine *e, const int *idx, int n, long long ro
 {
     int r0 = -1;
     for (int k = 0; k < n; k++) {
-        int i = idx[k];
+        const int i = idx[k];
         if (i < 0 || i >= e->N)
             return set_err(MDX_ERR_ARG, "entry %lld: particle %d out of range", row, i);
-        int r = (int)(std::upper_bound(e->first.begin(), e->first.end(), i) - e->first.begin()) - 1;
+        const int r = (int)(std::upper_bound(e->first.begin(), e->first.end(), i) - e->first.begin()) - 1;
         if (r0 >= 0 && r != r0)
             return set_err(MDX_ERR_ARG, "entry %lld couples different replicas", row);
         r0 = r;
@@ -1804,6 +960,8 @@ int mdx_set_bonds(mdx_engine *e, long long nbonds, const int *ij, const double *
         return set_err(MDX_ERR_ARG, "bond arrays required");
     if (kind != MDX_BOND_HARMONIC && kind != MDX_BOND_FENE)
         return set_err(MDX_ERR_ARG, "unknown bond kind %d", kind);
+    if (nbonds > (1ll << 29))
+        return set_err(MDX_ERR_ARG, "too many bonds");
     HIPCHK(hipSetDevice(e->device));
     std::vector<std::pair<int, int>> ent;
     ent.reserve(2 * nbonds);
@@ -1813,17 +971,18 @@ int mdx_set_bonds(mdx_engine *e, long long nbonds, const int *ij, const double *
             return rc;
         if (ij[2 * b] == ij[2 * b + 1])
             return set_err(MDX_ERR_ARG, "bond %lld joins a particle to itself", b);
+        if (kind == MDX_BOND_FENE && !(r0[b] > 0))
+            return set_err(MDX_ERR_ARG, "bond %lld: FENE R0 must be > 0", b);
         ent.push_back({ij[2 * b], (int)b});
         ent.push_back({ij[2 * b + 1], (int)b});
     }
     std::vector<int> off, val;
     csr(e->N, ent, off, val);
-    // per-entry partner, k, r0 in CSR order
     std::vector<int> bj(val.size());
     std::vector<double> bk(val.size()), br(val.size());
     for (int i = 0; i < e->N; i++)
         for (int q = off[i]; q < off[i + 1]; q++) {
-            long long b = val[q];
+            const long long b = val[q];
             bj[q] = ij[2 * b] == i ? ij[2 * b + 1] : ij[2 * b];
             bk[q] = k[b];
             br[q] = r0[b];
@@ -1916,9 +1075,23 @@ int mdx_set_active(mdx_engine *e, double fa, const int *prev, const int *next)
 
 int mdx_set_force_guard(mdx_engine *e, double fmax)
 {
-    if (!e || fmax < 0)
+    if (!e || !(fmax >= 0))
         return set_err(MDX_ERR_ARG, "guard must be >= 0");
     e->fguard = fmax;
+    return MDX_OK;
+}
+
+// positions of replica r in original order into e->tmpx (count x 3)
+static int replica_positions(mdx_engine *e, int replica)
+{
+    const int cnt = e->nper[replica];
+    int rc = dalloc(e, e->tmpx, 24 * (size_t)cnt);
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(k_positions_by_id, dim3(nblk(e->N)), dim3(256), 0, e->stream,
+                       (const PosRec *)e->X[e->cur].p, e->N, e->first[replica], cnt,
+                       (double *)e->tmpx.p);
+    HIPCHK(hipGetLastError());
     return MDX_OK;
 }
 
@@ -1929,18 +1102,16 @@ int mdx_cell_assign(mdx_engine *e, int replica, const int n[3], int *cell_of)
     if (n[0] < 1 || n[1] < 1 || n[2] < 1)
         return set_err(MDX_ERR_ARG, "n must be >= 1");
     HIPCHK(hipSetDevice(e->device));
-    int cnt = e->nper[replica];
-    int rc = dalloc(e, e->tmpi, 4 * (size_t)cnt);
-    if (rc)
+    const int cnt = e->nper[replica];
+    int rc;
+    if ((rc = replica_positions(e, replica)) || (rc = dalloc(e, e->tmpi, 4 * (size_t)cnt)))
         return rc;
     const double *b = &e->box[3 * replica];
     hipLaunchKernelGGL(k_assign_custom, dim3(nblk(cnt)), dim3(256), 0, e->stream,
-                       (const double *)e->X[e->cur].p, e->first[replica], cnt, b[0], b[1], b[2],
-                       n[0], n[1], n[2], (int *)e->tmpi.p);
+                       (const double *)e->tmpx.p, cnt, b[0], b[1], b[2], n[0], n[1], n[2],
+                       (int *)e->tmpi.p);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(cell_of, e->tmpi.p, 4 * (size_t)cnt, hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(hipStreamSynchronize(e->stream));
-    return MDX_OK;
+    return download(e, e->tmpi, cell_of, (size_t)cnt);
 }
 
 int mdx_neighbor_pairs(mdx_engine *e, int replica, const int n[3], double cut, int *ij,
@@ -1950,26 +1121,25 @@ int mdx_neighbor_pairs(mdx_engine *e, int replica, const int n[3], double cut, i
         return set_err(MDX_ERR_ARG, "bad arguments");
     if (n[0] < 1 || n[1] < 1 || n[2] < 1 || !(cut > 0))
         return set_err(MDX_ERR_ARG, "n >= 1 and cut > 0 required");
-    const double *b = &e->box[3 * replica];
     HIPCHK(hipSetDevice(e->device));
-    long long nc = (long long)n[0] * n[1] * n[2];
+    const long long nc = (long long)n[0] * n[1] * n[2];
     if (nc > (1ll << 28))
         return set_err(MDX_ERR_ARG, "too many cells");
-    int cnt = e->nper[replica];
+    const int cnt = e->nper[replica];
+    const double *b = &e->box[3 * replica];
     int rc;
-    if ((rc = dalloc(e, e->tmpi, 4 * (size_t)cnt)) || (rc = dalloc(e, e->tmpi2, 4 * (size_t)cnt)) ||
-        (rc = dalloc(e, e->tmpi3, 4 * (size_t)(nc + 1))) ||
-        (rc = dalloc(e, e->tmpll, 8 * (size_t)(cnt + 1))))
+    if ((rc = replica_positions(e, replica)) || (rc = dalloc(e, e->tmpi, 4 * (size_t)cnt)) ||
+        (rc = dalloc(e, e->tmpi2, 4 * (size_t)cnt)) || (rc = dalloc(e, e->tmpi3, 4 * (size_t)(nc + 1))) ||
+        (rc = dalloc(e, e->tmpll, 8 * (size_t)(cnt + 1))) || (rc = dalloc(e, e->partial, 4 * (size_t)cnt)))
         return rc;
-    // host-side counting sort of the cell ids (parity path, not the hot loop)
-    const double *X = (const double *)e->X[e->cur].p;
-    hipLaunchKernelGGL(k_assign_custom, dim3(nblk(cnt)), dim3(256), 0, e->stream, X,
-                       e->first[replica], cnt, b[0], b[1], b[2], n[0], n[1], n[2],
-                       (int *)e->tmpi.p);
+    const double *X = (const double *)e->tmpx.p;
+    hipLaunchKernelGGL(k_assign_custom, dim3(nblk(cnt)), dim3(256), 0, e->stream, X, cnt, b[0],
+                       b[1], b[2], n[0], n[1], n[2], (int *)e->tmpi.p);
     HIPCHK(hipGetLastError());
+    // counting sort of the cell ids on the host (parity path, not the step)
     std::vector<int> cellv(cnt), cstart(nc + 1, 0), clist(cnt);
-    HIPCHK(hipMemcpyAsync(cellv.data(), e->tmpi.p, 4 * (size_t)cnt, hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(hipStreamSynchronize(e->stream));
+    if ((rc = download(e, e->tmpi, cellv.data(), (size_t)cnt)))
+        return rc;
     for (int k = 0; k < cnt; k++)
         cstart[cellv[k] + 1]++;
     for (long long c = 0; c < nc; c++)
@@ -1982,35 +1152,32 @@ int mdx_neighbor_pairs(mdx_engine *e, int replica, const int n[3], double cut, i
     if ((rc = upload(e, e->tmpi3, cstart.data(), cstart.size())) ||
         (rc = upload(e, e->tmpi2, clist.data(), clist.size())))
         return rc;
-    if ((rc = dalloc(e, e->partial, 4 * (size_t)cnt)))
-        return rc;
-    hipLaunchKernelGGL(k_pairs, dim3(nblk(cnt)), dim3(256), 0, e->stream, X, e->first[replica],
-                       cnt, b[0], b[1], b[2], e->periodic, n[0], n[1], n[2], cut,
-                       (const int *)e->tmpi.p, (const int *)e->tmpi3.p, (const int *)e->tmpi2.p,
-                       (int *)e->partial.p, (const long long *)nullptr, (int *)nullptr, 0ll, 0);
+    hipLaunchKernelGGL(k_pairs, dim3(nblk(cnt)), dim3(256), 0, e->stream, X, cnt, b[0], b[1],
+                       b[2], e->periodic, n[0], n[1], n[2], cut, (const int *)e->tmpi.p,
+                       (const int *)e->tmpi3.p, (const int *)e->tmpi2.p, (int *)e->partial.p,
+                       (const long long *)nullptr, (int *)nullptr, 0ll, 0);
     HIPCHK(hipGetLastError());
     std::vector<int> pc(cnt);
-    HIPCHK(hipMemcpyAsync(pc.data(), e->partial.p, 4 * (size_t)cnt, hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(hipStreamSynchronize(e->stream));
+    if ((rc = download(e, e->partial, pc.data(), (size_t)cnt)))
+        return rc;
     std::vector<long long> off(cnt + 1, 0);
     for (int k = 0; k < cnt; k++)
         off[k + 1] = off[k] + pc[k];
     *npairs = off[cnt];
-    if (cap > 0) {
+    if (cap > 0 && off[cnt] > 0) {
+        const long long w = std::min(cap, off[cnt]);
         if ((rc = upload(e, e->tmpll, off.data(), off.size())))
             return rc;
         Buf out;
-        if ((rc = dalloc(e, out, 8 * (size_t)std::min(cap, off[cnt]) + 8)))
+        if ((rc = dalloc(e, out, 8 * (size_t)w)))
             return rc;
-        hipLaunchKernelGGL(k_pairs, dim3(nblk(cnt)), dim3(256), 0, e->stream, X,
-                           e->first[replica], cnt, b[0], b[1], b[2], e->periodic, n[0], n[1],
-                           n[2], cut, (const int *)e->tmpi.p, (const int *)e->tmpi3.p,
-                           (const int *)e->tmpi2.p, (int *)nullptr, (const long long *)e->tmpll.p,
-                           (int *)out.p, std::min(cap, off[cnt]), 1);
+        hipLaunchKernelGGL(k_pairs, dim3(nblk(cnt)), dim3(256), 0, e->stream, X, cnt, b[0],
+                           b[1], b[2], e->periodic, n[0], n[1], n[2], cut, (const int *)e->tmpi.p,
+                           (const int *)e->tmpi3.p, (const int *)e->tmpi2.p, (int *)nullptr,
+                           (const long long *)e->tmpll.p, (int *)out.p, w, 1);
         hipError_t le = hipGetLastError();
         if (le == hipSuccess)
-            le = hipMemcpyAsync(ij, out.p, 8 * (size_t)std::min(cap, off[cnt]),
-                                hipMemcpyDeviceToHost, e->stream);
+            le = hipMemcpyAsync(ij, out.p, 8 * (size_t)w, hipMemcpyDeviceToHost, e->stream);
         if (le == hipSuccess)
             le = hipStreamSynchronize(e->stream);
         e->bytes_held -= out.bytes;
@@ -2026,15 +1193,16 @@ int mdx_neighbor_pairs(mdx_engine *e, int replica, const int n[3], double cut, i
 template <class R> static int compute_forces_t(mdx_engine *e, double *F, double *T)
 {
     int rc;
-    if ((rc = bin_current<R>(e)))
+    if (!e->binned && (rc = bin_current<R>(e)))
         return rc;
     if ((rc = dalloc(e, e->Fout, 24 * (size_t)e->N)) || (rc = dalloc(e, e->Tout, 24 * (size_t)e->N)))
         return rc;
     if ((rc = launch_gather<R, M_CONS>(e, nullptr)))
         return rc;
-    HIPCHK(hipMemcpyAsync(F, e->Fout.p, 24 * (size_t)e->N, hipMemcpyDeviceToHost, e->stream));
-    if (T)
-        HIPCHK(hipMemcpyAsync(T, e->Tout.p, 24 * (size_t)e->N, hipMemcpyDeviceToHost, e->stream));
+    if ((rc = download(e, e->Fout, F, 3 * (size_t)e->N)))
+        return rc;
+    if (T && (rc = download(e, e->Tout, T, 3 * (size_t)e->N)))
+        return rc;
     unsigned int st;
     if ((rc = read_status(e, &st)))
         return rc;
@@ -2083,22 +1251,23 @@ template <class R> static int observables_t(mdx_engine *e, int replica, double *
         return rc;
     if ((rc = launch_gather<R, M_ENERGY>(e, nullptr)))
         return rc;
-    int first = replica < 0 ? 0 : e->first[replica];
-    int cnt = replica < 0 ? e->N : e->nper[replica];
-    const int nb = std::min(1024u, nblk(cnt));
+    const int first = replica < 0 ? 0 : e->first[replica];
+    const int cnt = replica < 0 ? e->N : e->nper[replica];
+    const int nb = (int)std::min(1024u, nblk(cnt));
     if ((rc = dalloc(e, e->partial, 16 * 8 * (size_t)nb)))
         return rc;
     hipLaunchKernelGGL(k_obs<R>, dim3(nb), dim3(256), 0, e->stream, first, cnt,
-                       (const R *)e->P.p, e->integ == MDX_INTEG_ROTOR ? (const R *)e->Lm.p : nullptr,
-                       (const double *)e->obs.p, 1, e->m, e->I[0], (double *)e->partial.p);
+                       (const R *)e->P[e->cur].p,
+                       e->integ == MDX_INTEG_ROTOR ? (const R *)e->Lm[e->cur].p : nullptr,
+                       (const double *)e->obs.p, e->m, e->I[0], (double *)e->partial.p);
     HIPCHK(hipGetLastError());
-    std::vector<double> part(16 * nb);
-    HIPCHK(hipMemcpyAsync(part.data(), e->partial.p, 8 * part.size(), hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(hipStreamSynchronize(e->stream));
+    std::vector<double> part(16 * (size_t)nb);
+    if ((rc = download(e, e->partial, part.data(), part.size())))
+        return rc;
     for (int k = 0; k < 16; k++) {
         double s = 0;
         for (int b = 0; b < nb; b++)
-            s += part[16 * b + k];
+            s += part[16 * (size_t)b + k];
         out[k] = s;
     }
     for (int k = 0; k < 6; k++)
@@ -2119,6 +1288,7 @@ int mdx_info(mdx_engine *e, double *out)
 {
     if (!e || !out)
         return set_err(MDX_ERR_ARG, "bad arguments");
+    HIPCHK(hipSetDevice(e->device));
     int rc = build_grid(e);
     if (rc)
         return rc;
@@ -2130,23 +1300,24 @@ int mdx_info(mdx_engine *e, double *out)
     return MDX_OK;
 }
 
+// per-kernel device times of nsteps steps launched eagerly with HIP events
+// on the engine's stream; the state advances by nsteps (+1 rotor) steps
 template <class R>
 static int profile_t(mdx_engine *e, int nsteps, int cap, const char **names, double *ms, int *nk)
 {
-    // kernels of one fused rotor/bead step, each bracketed by events
     static const char *kn[] = {"gather", "scan", "fill", "cellsort", "kick_drift"};
     const int K = 5;
     double acc[K] = {0, 0, 0, 0, 0};
     int cntk[K] = {0, 0, 0, 0, 0};
-    hipEvent_t ev[8];
-    for (int k = 0; k < 8; k++)
+    hipEvent_t ev[5];
+    for (int k = 0; k < 5; k++)
         HIPCHK(hipEventCreate(&ev[k]));
     int rc;
     if (e->integ == MDX_INTEG_ROTOR && !e->primed && (rc = do_prime<R>(e, nullptr)))
         return rc;
-    if (!e->binned && (rc = bin_current<R>(e)))
+    if (e->integ == MDX_INTEG_BEAD && !e->binned && (rc = bin_current<R>(e)))
         return rc;
-    hipLaunchKernelGGL(k_set_call_base, dim3(1), dim3(1), 0, e->stream, (StepCtl *)e->ctl.p);
+    launch_small(e, k_set_call_base);
     if (e->integ == MDX_INTEG_ROTOR) {
         KArgs<R> a = make_args<R>(e);
         HIPCHK(hipEventRecord(ev[0], e->stream));
@@ -2171,8 +1342,8 @@ static int profile_t(mdx_engine *e, int nsteps, int cap, const char **names, dou
                            (int *)e->start.p, (const int *)e->bsum.p, n, (const int *)e->total.p);
         HIPCHK(hipEventRecord(ev[1], e->stream));
         hipLaunchKernelGGL(k_fill, dim3(nblk(e->N)), dim3(256), 0, e->stream, e->N,
-                           (const int *)e->cell.p, (const int *)e->rank.p, (const int *)e->start.p,
-                           (int *)e->list.p);
+                           (const int *)e->cell[e->cur].p, (const int *)e->rank.p,
+                           (const int *)e->start.p, (int *)e->list.p);
         HIPCHK(hipEventRecord(ev[2], e->stream));
         hipLaunchKernelGGL(k_cellsort, dim3(nblk(n)), dim3(256), 0, e->stream, n,
                            (const int *)e->start.p, (int *)e->list.p);
@@ -2186,7 +1357,7 @@ static int profile_t(mdx_engine *e, int nsteps, int cap, const char **names, dou
         }
         HIPCHK(hipEventRecord(ev[4], e->stream));
         e->cur ^= 1;
-        hipLaunchKernelGGL(k_epoch_inc, dim3(1), dim3(1), 0, e->stream, (StepCtl *)e->ctl.p);
+        launch_small(e, k_epoch_inc);
         HIPCHK(hipEventSynchronize(ev[4]));
         float t;
         HIPCHK(hipEventElapsedTime(&t, ev[0], ev[1]));
@@ -2208,11 +1379,15 @@ static int profile_t(mdx_engine *e, int nsteps, int cap, const char **names, dou
     if (e->integ == MDX_INTEG_ROTOR) {
         if ((rc = launch_gather<R, M_STEP>(e, nullptr)))
             return rc;
-        hipLaunchKernelGGL(k_epoch_inc, dim3(1), dim3(1), 0, e->stream, (StepCtl *)e->ctl.p);
+        e->cur ^= 1;
+        launch_small(e, k_epoch_inc);
+        e->binned = false;
+    } else {
+        e->binned = true;
     }
     HIPCHK(hipStreamSynchronize(e->stream));
     e->steps_taken += nsteps + (e->integ == MDX_INTEG_ROTOR ? 1 : 0);
-    for (int k = 0; k < 8; k++)
+    for (int k = 0; k < 5; k++)
         (void)hipEventDestroy(ev[k]);
     int w = 0;
     for (int k = 0; k < K && w < cap; k++) {

@@ -1,0 +1,1087 @@
+// mdx_kernels.h -- gfx950 kernels of the Langevin step.
+//
+// Storage order.  Particles live in "slots".  Every kernel that produces the
+// next state is launched with one thread per slot of the *cell list* order:
+// thread s takes the particle in old slot list[s], and writes its next state
+// to slot s of the other buffer.  The state is therefore re-sorted into cell
+// order every step without an extra pass, so that the threads of a wavefront
+// handle particles of the same or adjacent cells (shared neighbour cells,
+// coalesced gathers).  Each particle carries its original index (id): noise
+// counters, the p1 < p2 roles of the patch loop, topology and the host
+// interface all use it.
+//
+// Reference paths in comments are relative to the reference's Code/ folder.
+#pragma once
+
+#include "mdx_device.h"
+#include "../../include/mdx.h"
+
+namespace mdx {
+
+// one position record: FP64 position + original index + (replica << 8 | type)
+struct __align__(16) PosRec {
+    double x, y, z;
+    uint32_t id;
+    uint32_t meta;
+};
+
+struct DevRep {
+    double l[3], l2s[3];
+    int nc[3];
+    int coff;
+    int first, count;
+    double Rt, Rr;   // sqrt(2 gamma kT / dt), sqrt(2 gamma_r kT / dt) (LangevinR.cpp:51-52)
+    double bead_r;   // sqrt(0.5 kT gamma m dt) (Langevin.cpp:44)
+    int prow;        // first patch row of this replica
+};
+
+struct StepCtl {
+    unsigned long long epoch;     // force evaluations so far (noise counter)
+    unsigned long long call_base; // epoch at the start of the current call
+    unsigned int status;          // bit0 force guard, bit1 non-finite
+    unsigned int pad;
+};
+
+enum { M_CONS = 0, M_PRIME = 1, M_STEP = 2, M_FUSED = 3, M_BEAD = 4, M_ENERGY = 5 };
+
+template <class R> struct KArgs {
+    int N;
+    int rotor;
+    int periodic;
+    const DevRep *reps;
+    // state: current (read) and next (written) buffers
+    const PosRec *X;
+    PosRec *Xn;
+    const R *Q;
+    R *Qn;
+    const R *p, *L, *F, *T;
+    R *pn, *Ln, *Fn, *Tn;
+    const int *slot; // slot of each original index
+    int *slotn;
+    // cell lists
+    const int *cell; // cell of each current slot
+    int *celln;      // cell of each next slot
+    int *rank, *count;
+    const int *start, *list;
+    // outputs by original index
+    double *Fout, *Tout, *obs;
+    // WCA (HSPotential)
+    int wca;
+    double wca_eps, wca_s6, wca_s12, wca_pl;
+    // patches
+    int has_patches;
+    int np[MAXT];
+    int pstart[MAXT];
+    int rstart[MAXT * MAXT];
+    unsigned long long active_mask; // bit ta * MAXT + tc: type pair with a nonzero row
+    const double *pdirs;
+    const PatchRow *prows;
+    double maxd2, rmax2, costm_screen;
+    // topology (indexed by original index)
+    const int *bond_off, *bond_j;
+    const double *bond_k, *bond_r0;
+    int bond_kind;
+    const int *ang_off, *ang_ref; // per particle: angle index * 4 + role
+    const int *ang_ijk;
+    const double *ang_k, *ang_t0;
+    int ang_form;
+    const int *exc_off, *exc_j;
+    const int *act_prev, *act_next;
+    double fa;
+    // integrator
+    double dt, hdt, c1, m, gamma, gammar, I[3];
+    double b_c5c2, b_c5c3q, b_c2;
+    int reortho_every;
+    // noise
+    uint32_t key0, key1;
+    const double *noise; // injected rows (ncomp per particle, original order) or NULL
+    StepCtl *ctl;
+    double fguard;
+};
+
+// ------------------------------------------------------------ small helpers
+// n = Q^T n_body in FP64 with the reference's operation order
+// (potentialtheta.h:559-565); Q promoted from R
+template <class R> __device__ __forceinline__ void lab_normal_exact(const R *q, const double *nb,
+                                                                    double n[3])
+{
+    n[0] = nb[0] * (double)q[0] + nb[1] * (double)q[3] + nb[2] * (double)q[6];
+    n[1] = nb[0] * (double)q[1] + nb[1] * (double)q[4] + nb[2] * (double)q[7];
+    n[2] = nb[0] * (double)q[2] + nb[1] * (double)q[5] + nb[2] * (double)q[8];
+}
+
+// LangevinNVTR::genfullmat (LangevinR.cpp:392-461)
+template <class R>
+__device__ __forceinline__ void genfullmat(R dt, const R I[3], const R L[3], R up[9])
+{
+    R tempx = dt * L[0] / I[0];
+    R tempy = dt * L[1] / I[1];
+    R tempz = dt * L[2] / I[2];
+    R qx = tempx / (R)4., qy = tempy / (R)4., hz = tempz / (R)2.;
+    R fac1 = ((R)1 - qx * qx) / ((R)1 + qx * qx);
+    R fac2 = tempx / ((R)2. * ((R)1 + qx * qx));
+    R fac3 = ((R)1 - qy * qy) / ((R)1 + qy * qy);
+    R fac4 = tempy / ((R)2. * ((R)1 + qy * qy));
+    R fac5 = ((R)1 - hz * hz) / ((R)1 + hz * hz);
+    R fac6 = (tempz) / ((R)1 + hz * hz);
+    up[0] = -fac4 * fac4 + fac3 * fac3 * fac5;
+    up[1] = fac3 * (fac2 * fac4 * ((R)1 + fac5) - fac1 * fac6);
+    up[2] = fac3 * (fac1 * fac4 * ((R)1 + fac5) + fac2 * fac6);
+    up[3] = fac3 * (fac2 * fac4 * ((R)1 + fac5) + fac1 * fac6);
+    up[4] = fac1 * fac1 * fac5 + fac2 * fac2 * (-fac3 * fac3 + fac4 * fac4 * fac5);
+    up[5] = -(fac1 * fac2 * (fac3 * fac3 + fac5 - fac4 * fac4 * fac5)) +
+            (fac1 * fac1 + fac2 * fac2) * fac4 * fac6;
+    up[6] = -(fac1 * fac3 * fac4 * ((R)1 + fac5)) + fac2 * fac3 * fac6;
+    up[7] = fac1 * fac2 * (fac3 * fac3 + fac5 - fac4 * fac4 * fac5) +
+            (fac1 * fac1 + fac2 * fac2) * fac4 * fac6;
+    up[8] = -(fac2 * fac2 * fac5) + fac1 * fac1 * (fac3 * fac3 - fac4 * fac4 * fac5);
+}
+
+// LangevinNVTR::rotate (LangevinR.cpp:595-685): L <- R L, Q <- R^T Q
+template <class R> __device__ __forceinline__ void rotate(R dt, const R I[3], R L[3], R q[9])
+{
+    R rr[9];
+    genfullmat(dt, I, L, rr);
+    R jx = L[0], jy = L[1], jz = L[2];
+    L[0] = jx * rr[0] + jy * rr[1] + jz * rr[2];
+    L[1] = jx * rr[3] + jy * rr[4] + jz * rr[5];
+    L[2] = jx * rr[6] + jy * rr[7] + jz * rr[8];
+    R o[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+        o[k] = q[k];
+    q[0] = o[0] * rr[0] + o[3] * rr[3] + o[6] * rr[6];
+    q[1] = o[1] * rr[0] + o[4] * rr[3] + o[7] * rr[6];
+    q[2] = o[2] * rr[0] + o[5] * rr[3] + o[8] * rr[6];
+    q[3] = o[0] * rr[1] + o[3] * rr[4] + o[6] * rr[7];
+    q[4] = o[1] * rr[1] + o[4] * rr[4] + o[7] * rr[7];
+    q[5] = o[2] * rr[1] + o[5] * rr[4] + o[8] * rr[7];
+    q[6] = o[0] * rr[2] + o[3] * rr[5] + o[6] * rr[8];
+    q[7] = o[1] * rr[2] + o[4] * rr[5] + o[7] * rr[8];
+    q[8] = o[2] * rr[2] + o[5] * rr[5] + o[8] * rr[8];
+}
+
+// symmetric re-orthonormalisation in FP64: Q <- 1.5 Q - 0.5 Q Q^T Q (one
+// Newton-Schulz step towards Q (Q^T Q)^(-1/2); FP32 drift is ~1e-7 per step)
+template <class R> __device__ __forceinline__ void reortho(R q[9])
+{
+    double a[9], m[9];
+    for (int k = 0; k < 9; k++)
+        a[k] = (double)q[k];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++)
+            m[3 * r + c] = a[3 * r] * a[3 * c] + a[3 * r + 1] * a[3 * c + 1] + a[3 * r + 2] * a[3 * c + 2];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            double s = m[3 * r] * a[c] + m[3 * r + 1] * a[3 + c] + m[3 * r + 2] * a[6 + c];
+            q[3 * r + c] = (R)(1.5 * a[3 * r + c] - 0.5 * s);
+        }
+}
+
+// cube::correct_position_and_momentum (geometry.h:403-445) for one coordinate
+template <class R>
+__device__ __forceinline__ void wrap1(double &x, R &p, double l, int periodic)
+{
+    if (periodic) {
+        if (x < 0) {
+            while (x < 0)
+                x += l;
+        } else if (x > l) {
+            while (x > l)
+                x -= l;
+        }
+    } else {
+        if (x < 0) {
+            x = -x;
+            p = -p;
+        } else if (x > l) {
+            x = l - (x - l);
+            p = -p;
+        }
+    }
+}
+
+template <class R> __device__ __forceinline__ void flag_force(const KArgs<R> &a, double f)
+{
+    if (a.fguard > 0 && fabs(f) > a.fguard)
+        atomicOr(&a.ctl->status, 1u);
+}
+
+// noise of particle (original index) o for the current force evaluation
+template <class R>
+__device__ __forceinline__ void load_noise(const KArgs<R> &a, uint32_t o, int ncomp, R out[6])
+{
+    unsigned long long ep = a.ctl->epoch;
+    if (a.noise) {
+        const double *row = a.noise + (size_t)(ep - a.ctl->call_base) * (size_t)a.N * ncomp +
+                            (size_t)o * ncomp;
+        for (int k = 0; k < ncomp; k++)
+            out[k] = (R)row[k];
+    } else {
+        U4 c0 = {o, (uint32_t)ep, (uint32_t)(ep >> 32), 0u};
+        U4 r0 = philox4x32_10(c0, a.key0, a.key1);
+        out[0] = unit_uniform<R>(r0.a);
+        out[1] = unit_uniform<R>(r0.b);
+        out[2] = unit_uniform<R>(r0.c);
+        if (ncomp > 3) {
+            out[3] = unit_uniform<R>(r0.d);
+            U4 c1 = {o, (uint32_t)ep, (uint32_t)(ep >> 32), 1u};
+            U4 r1 = philox4x32_10(c1, a.key0, a.key1);
+            out[4] = unit_uniform<R>(r1.a);
+            out[5] = unit_uniform<R>(r1.b);
+        }
+    }
+}
+
+// cube::assign_box with the engine's grid; count the particle into its cell
+template <class R>
+__device__ __forceinline__ void bin_particle(const KArgs<R> &a, int s, const DevRep &rp,
+                                             const double x[3], int *cellout)
+{
+    int cx = axis_cell(x[0], rp.nc[0], rp.l[0]);
+    int cy = axis_cell(x[1], rp.nc[1], rp.l[1]);
+    int cz = axis_cell(x[2], rp.nc[2], rp.l[2]);
+    int c = rp.coff + cx + rp.nc[0] * (cy + rp.nc[1] * cz);
+    cellout[s] = c;
+    a.rank[s] = atomicAdd(&a.count[c], 1);
+}
+
+template <class R>
+__device__ __forceinline__ void min_image(const KArgs<R> &a, const DevRep &rp, double x0,
+                                          double x1, double x2, const PosRec &b, double d[3])
+{
+    d[0] = mimage(x0 - b.x, rp.l[0], rp.l2s[0], a.periodic);
+    d[1] = mimage(x1 - b.y, rp.l[1], rp.l2s[1], a.periodic);
+    d[2] = mimage(x2 - b.z, rp.l[2], rp.l2s[2], a.periodic);
+}
+
+// ------------------------------------------------------------- gather kernel
+constexpr int PMAX = 16; // deferred patch candidates per particle held in LDS
+
+// neighbour slots of slot i: the 9 (y, z) rows of the 27-cell stencil; the x
+// neighbours of a row are contiguous in the cell list (one range, or two at
+// the periodic seam; fewer cells when a side has fewer than 3)
+template <class R, class Fn>
+__device__ __forceinline__ void for_each_neighbor(const KArgs<R> &a, const DevRep &rp, int i, Fn &&fn)
+{
+    const int c = a.cell[i] - rp.coff;
+    const int n0 = rp.nc[0], n1 = rp.nc[1], n2 = rp.nc[2];
+    const int cx = c % n0, cy = (c / n0) % n1, cz = c / (n0 * n1);
+    const int ylo = n1 >= 3 ? -1 : 0, yhi = n1 >= 2 ? 1 : 0;
+    const int zlo = n2 >= 3 ? -1 : 0, zhi = n2 >= 2 ? 1 : 0;
+    for (int oz = zlo; oz <= zhi; oz++) {
+        int z = cz + oz;
+        z = z < 0 ? z + n2 : (z >= n2 ? z - n2 : z);
+        for (int oy = ylo; oy <= yhi; oy++) {
+            int y = cy + oy;
+            y = y < 0 ? y + n1 : (y >= n1 ? y - n1 : y);
+            const int rb = rp.coff + n0 * (y + n1 * z);
+            int r0, r1, r2 = 0, r3 = 0;
+            if (n0 >= 3) {
+                if (cx == 0) {
+                    r0 = a.start[rb];
+                    r1 = a.start[rb + 2];
+                    r2 = a.start[rb + n0 - 1];
+                    r3 = a.start[rb + n0];
+                } else if (cx == n0 - 1) {
+                    r0 = a.start[rb + n0 - 2];
+                    r1 = a.start[rb + n0];
+                    r2 = a.start[rb];
+                    r3 = a.start[rb + 1];
+                } else {
+                    r0 = a.start[rb + cx - 1];
+                    r1 = a.start[rb + cx + 2];
+                }
+            } else {
+                r0 = a.start[rb];
+                r1 = a.start[rb + n0];
+            }
+            for (int k = r0; k < r1; k++) {
+                const int j = a.list[k];
+                if (j != i)
+                    fn(j);
+            }
+            for (int k = r2; k < r3; k++) {
+                const int j = a.list[k];
+                if (j != i)
+                    fn(j);
+            }
+        }
+    }
+}
+
+template <class R> struct Acc {
+    R F[3], T[3];
+    double epot, vir[9];
+};
+
+// roles of a patch pair: "a" = lower original index, exchanged when its type
+// is higher (GeneralPatch rows exist for ta <= tc; the potential is symmetric)
+__device__ __forceinline__ bool patch_roles(uint32_t oi, int ti, uint32_t oj, int tj, int &ta,
+                                            int &tc)
+{
+    bool i_is_a = oi < oj;
+    ta = i_is_a ? ti : tj;
+    tc = i_is_a ? tj : ti;
+    if (ta > tc) {
+        int t = ta;
+        ta = tc;
+        tc = t;
+        i_is_a = !i_is_a;
+    }
+    return i_is_a;
+}
+
+// all patch pairs between particle i and neighbour slot j
+// (LangevinR.cpp:923-1044, potentialtheta.h:517-750)
+template <class R, int MODE>
+__device__ __forceinline__ void patch_pair(const KArgs<R> &a, const DevRep &rp, const PosRec &xi,
+                                           int ti, const R *qi, int j, Acc<R> &acc)
+{
+    constexpr bool FP64 = sizeof(R) == 8;
+    const PosRec xj = a.X[j];
+    double d[3];
+    min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
+    const double r2s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const double rij = sqrt(r2s);
+    const int tj = xj.meta & 255;
+    int ta, tc;
+    const bool i_is_a = patch_roles(xi.id, ti, xj.id, tj, ta, tc);
+    const double sg = i_is_a ? 1.0 : -1.0;
+    const double u[3] = {sg * d[0] / rij, sg * d[1] / rij, sg * d[2] / rij};
+    R qj[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++)
+        qj[q] = a.Q[9 * (size_t)j + q];
+    const R *qa = i_is_a ? qi : qj;
+    const R *qc = i_is_a ? qj : qi;
+    // body-frame components of u screen the cone tests; the decisions are
+    // then taken on the reference's own FP64 expressions
+    const R ur[3] = {(R)u[0], (R)u[1], (R)u[2]};
+    const R va[3] = {qa[0] * ur[0] + qa[1] * ur[1] + qa[2] * ur[2],
+                     qa[3] * ur[0] + qa[4] * ur[1] + qa[5] * ur[2],
+                     qa[6] * ur[0] + qa[7] * ur[1] + qa[8] * ur[2]};
+    const R vc[3] = {qc[0] * ur[0] + qc[1] * ur[1] + qc[2] * ur[2],
+                     qc[3] * ur[0] + qc[4] * ur[1] + qc[5] * ur[2],
+                     qc[6] * ur[0] + qc[7] * ur[1] + qc[8] * ur[2]};
+    const int npa = a.np[ta], npc = a.np[tc];
+    const double *da = a.pdirs + 3 * a.pstart[ta];
+    const double *dc = a.pdirs + 3 * a.pstart[tc];
+    const int rowbase = rp.prow + a.rstart[ta * MAXT + tc];
+    const R scr = (R)a.costm_screen;
+    for (int ia = 0; ia < npa; ia++) {
+        const R s1 = -((R)da[3 * ia] * va[0] + (R)da[3 * ia + 1] * va[1] + (R)da[3 * ia + 2] * va[2]);
+        if (!(s1 > scr))
+            continue;
+        double n1[3];
+        lab_normal_exact(qa, da + 3 * ia, n1);
+        const double a1 = -(n1[0] * u[0] + n1[1] * u[1] + n1[2] * u[2]);
+        for (int ib = 0; ib < npc; ib++) {
+            const R s2 = (R)dc[3 * ib] * vc[0] + (R)dc[3 * ib + 1] * vc[1] + (R)dc[3 * ib + 2] * vc[2];
+            if (!(s2 > scr))
+                continue;
+            const PatchRow pr = a.prows[rowbase + ia * npc + ib];
+            if (rij > pr.idist || !(a1 > pr.costm))
+                continue;
+            double n2[3];
+            lab_normal_exact(qc, dc + 3 * ib, n2);
+            const double a2 = (n2[0] * u[0] + n2[1] * u[1] + n2[2] * u[2]);
+            if (!(a2 > pr.costm))
+                continue;
+            R f[3] = {0, 0, 0}, t1[3] = {0, 0, 0}, t2[3] = {0, 0, 0};
+            double e;
+            if constexpr (FP64) {
+                e = kf2_pair_fp64(u, rij, n1, n2, a1, a2, pr, f, t1, t2);
+            } else {
+                const float uf[3] = {(float)u[0], (float)u[1], (float)u[2]};
+                const float n1f[3] = {(float)n1[0], (float)n1[1], (float)n1[2]};
+                const float n2f[3] = {(float)n2[0], (float)n2[1], (float)n2[2]};
+                e = kf2_pair_fp32(uf, (float)rij, n1f, n2f, (float)a1, (float)a2,
+                                  fabs(a1 * a1 - 1) < 1E-5, fabs(a2 * a2 - 1) < 1E-5,
+                                  (float)pr.eps, (float)pr.d, (float)pr.kk, f, t1, t2);
+            }
+            if (i_is_a) {
+                for (int q = 0; q < 3; q++) {
+                    acc.F[q] += f[q];
+                    acc.T[q] += t1[q];
+                }
+            } else {
+                for (int q = 0; q < 3; q++) {
+                    acc.F[q] -= f[q];
+                    acc.T[q] += t2[q];
+                }
+            }
+            if (MODE == M_ENERGY) {
+                acc.epot += 0.5 * e;
+                for (int q = 0; q < 3; q++)
+                    for (int l = 0; l < 3; l++)
+                        acc.vir[3 * q + l] += 0.5 * d[l] * sg * (double)f[q];
+            }
+        }
+    }
+}
+
+// Force (lab) and patch torque (lab) of one particle, then the mode's
+// integrator epilogue.  Full neighbour list: every pair is evaluated from
+// both sides, each side keeps its own share; no atomics.  The neighbour loop
+// evaluates WCA and queues patch candidates (pairs within max_distance_check
+// whose types carry patch rows); the patch pairs are evaluated after the loop
+// (fewer live registers in the loop).
+template <class R, int MODE>
+__global__ void __launch_bounds__(256) k_gather(KArgs<R> a)
+{
+    constexpr bool FP64 = sizeof(R) == 8;
+    __shared__ int plist[PMAX * 256];
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.N)
+        return;
+    const int i = a.list[s];
+    const PosRec xi = a.X[i];
+    const uint32_t oi = xi.id;
+    const int ti = xi.meta & 255;
+    const DevRep rp = a.reps[xi.meta >> 8];
+    Acc<R> acc;
+    acc.F[0] = acc.F[1] = acc.F[2] = 0;
+    acc.T[0] = acc.T[1] = acc.T[2] = 0;
+    acc.epot = 0;
+    for (int k = 0; k < 9; k++)
+        acc.vir[k] = 0;
+
+    R qi[9];
+    const int npi = a.has_patches ? a.np[ti] : 0;
+    if (a.rotor) {
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+            qi[k] = a.Q[9 * (size_t)i + k];
+    }
+    int exc0 = 0, exc1 = 0;
+    if (a.exc_off) {
+        exc0 = a.exc_off[oi];
+        exc1 = a.exc_off[oi + 1];
+    }
+
+    if (a.wca || npi > 0) {
+        int npc = 0;
+        // candidate filter of the patch loop: within max_distance_check
+        // (LangevinR.cpp:938) and a type pair with patch rows
+        auto is_patch_candidate = [&](const PosRec &xj, double r2s) -> bool {
+            if (!(npi > 0 && r2s < a.maxd2))
+                return false;
+            const int tj = xj.meta & 255;
+            if (a.np[tj] == 0)
+                return false;
+            int ta, tc;
+            patch_roles(oi, ti, xj.id, tj, ta, tc);
+            return (a.active_mask >> (ta * MAXT + tc)) & 1ull;
+        };
+        for_each_neighbor(a, rp, i, [&](int j) {
+            const PosRec xj = a.X[j];
+            double d[3];
+            min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
+            const double r2s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (!(r2s < a.rmax2))
+                return;
+            // ---------------- WCA (MD.cpp:1700-1758, potential.h:112-125)
+            if (a.wca) {
+                const double rij = sqrt(r2s);
+                if (rij < a.wca_pl) {
+                    bool ex = false;
+                    for (int e = exc0; e < exc1; e++)
+                        ex |= (uint32_t)a.exc_j[e] == xj.id;
+                    if (!ex) {
+                        if constexpr (FP64) {
+                            const double f1 = hs_force(rij, a.wca_eps, a.wca_s6, a.wca_s12);
+                            flag_force(a, f1);
+                            for (int q = 0; q < 3; q++) {
+                                const double fac = f1 * d[q] / rij;
+                                acc.F[q] += fac;
+                                if (MODE == M_ENERGY)
+                                    for (int l = 0; l < 3; l++)
+                                        acc.vir[3 * q + l] += 0.5 * d[l] * fac;
+                            }
+                            if (MODE == M_ENERGY)
+                                acc.epot += 0.5 * wca_energy(rij, a.wca_eps, a.wca_s6, a.wca_s12);
+                        } else {
+                            const float rf = (float)rij;
+                            const float f1 =
+                                hs_force(rf, (float)a.wca_eps, (float)a.wca_s6, (float)a.wca_s12);
+                            flag_force(a, f1);
+                            const float fr = f1 / rf;
+                            for (int q = 0; q < 3; q++) {
+                                const float fac = fr * (float)d[q];
+                                acc.F[q] += fac;
+                                if (MODE == M_ENERGY)
+                                    for (int l = 0; l < 3; l++)
+                                        acc.vir[3 * q + l] += 0.5 * d[l] * fac;
+                            }
+                            if (MODE == M_ENERGY)
+                                acc.epot += 0.5 * (double)wca_energy(rf, (float)a.wca_eps,
+                                                                     (float)a.wca_s6,
+                                                                     (float)a.wca_s12);
+                        }
+                    }
+                }
+            }
+            if (is_patch_candidate(xj, r2s)) {
+                if (npc < PMAX)
+                    plist[npc * 256 + threadIdx.x] = j;
+                npc++;
+            }
+        });
+        // -------- patches (LangevinR.cpp:911-1047), in neighbour-loop order
+        const int nq = npc < PMAX ? npc : PMAX;
+        for (int q = 0; q < nq; q++)
+            patch_pair<R, MODE>(a, rp, xi, ti, qi, plist[q * 256 + threadIdx.x], acc);
+        if (npc > PMAX) {
+            // dense neighbourhoods: a second traversal takes the rest in order
+            int seen = 0;
+            for_each_neighbor(a, rp, i, [&](int j) {
+                const PosRec xj = a.X[j];
+                double d[3];
+                min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
+                const double r2s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                if (!(r2s < a.rmax2) || !is_patch_candidate(xj, r2s))
+                    return;
+                if (seen++ >= PMAX)
+                    patch_pair<R, MODE>(a, rp, xi, ti, qi, j, acc);
+            });
+        }
+    }
+    R *Fi = acc.F, *Ti = acc.T;
+    double &epot = acc.epot;
+    double *vir = acc.vir;
+
+    // ---------------------------------------- bonds (MD.cpp:1684-1764, 2037-2111)
+    if (a.bond_off) {
+        for (int b = a.bond_off[oi]; b < a.bond_off[oi + 1]; b++) {
+            const PosRec xj = a.X[a.slot[a.bond_j[b]]];
+            double d[3];
+            min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
+            const double dis = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            const double r = sqrt(dis);
+            const double k = a.bond_k[b], r0 = a.bond_r0[b];
+            double f1;
+            if (a.bond_kind == MDX_BOND_FENE)
+                f1 = -k * r / (1 - (r * r) / (r0 * r0)); // FENEPotential::force (potential.h:356)
+            else
+                f1 = -k * (r - r0);
+            flag_force(a, f1);
+            for (int c = 0; c < 3; c++) {
+                const double fac = f1 * d[c] / r;
+                Fi[c] += (R)fac;
+                if (MODE == M_ENERGY)
+                    for (int l = 0; l < 3; l++)
+                        vir[3 * c + l] += 0.5 * d[l] * fac;
+            }
+            if (MODE == M_ENERGY) {
+                if (a.bond_kind == MDX_BOND_FENE)
+                    epot += 0.5 * (-(k / 2.) * r0 * r0 * log(1.0 - r * r / (r0 * r0)));
+                else
+                    epot += 0.5 * 0.5 * k * (r - r0) * (r - r0);
+            }
+        }
+    }
+    // ------------------------------------ angles (MD.cpp:1925-1983, potential3.h)
+    if (a.ang_off) {
+        for (int q = a.ang_off[oi]; q < a.ang_off[oi + 1]; q++) {
+            const int ref = a.ang_ref[q];
+            const int t = ref >> 2, role = ref & 3;
+            const PosRec x1 = a.X[a.slot[a.ang_ijk[3 * t]]];
+            const PosRec x2 = a.X[a.slot[a.ang_ijk[3 * t + 1]]];
+            const PosRec x3 = a.X[a.slot[a.ang_ijk[3 * t + 2]]];
+            double ab[3], bc[3];
+            min_image(a, rp, x2.x, x2.y, x2.z, x1, ab);
+            min_image(a, rp, x3.x, x3.y, x3.z, x2, bc);
+            R abr[3] = {(R)ab[0], (R)ab[1], (R)ab[2]}, bcr[3] = {(R)bc[0], (R)bc[1], (R)bc[2]};
+            R out[3];
+            if (a.ang_form == MDX_ANGLE_HARMONIC)
+                bend_harmonic(abr, bcr, (R)a.ang_k[t], (R)a.ang_t0[t], role, out);
+            else
+                bend_cosine(abr, bcr, (R)a.ang_k[t], role, out);
+            for (int k = 0; k < 3; k++)
+                Fi[k] += out[k];
+            if (MODE == M_ENERGY && role == 1) {
+                double c = (ab[0] * bc[0] + ab[1] * bc[1] + ab[2] * bc[2]) /
+                           sqrt((ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2]) *
+                                (bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2]));
+                c = fmin(1.0, fmax(-1.0, c));
+                if (a.ang_form == MDX_ANGLE_HARMONIC)
+                    epot += 0.5 * a.ang_k[t] * (acos(c) - a.ang_t0[t]) * (acos(c) - a.ang_t0[t]);
+                else
+                    epot += a.ang_k[t] * (1 - c);
+            }
+        }
+    }
+    // -------------------------------------------------- tangential active force
+    if (a.act_prev) {
+        const int nx = a.act_next[oi] < 0 ? (int)oi : a.act_next[oi];
+        const int pv = a.act_prev[oi] < 0 ? (int)oi : a.act_prev[oi];
+        if (nx != pv) {
+            const PosRec xa = a.X[a.slot[nx]], xb = a.X[a.slot[pv]];
+            double t[3];
+            min_image(a, rp, xa.x, xa.y, xa.z, xb, t);
+            const double inv = 1.0 / sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+            for (int k = 0; k < 3; k++)
+                Fi[k] += (R)(a.fa * t[k] * inv);
+        }
+    }
+
+    if (!(isfinite((double)Fi[0]) && isfinite((double)Fi[1]) && isfinite((double)Fi[2]) &&
+          isfinite((double)Ti[0]) && isfinite((double)Ti[1]) && isfinite((double)Ti[2])))
+        atomicOr(&a.ctl->status, 2u);
+
+    // ------------------------------------------------------------ epilogues
+    if (MODE == M_CONS) {
+        for (int k = 0; k < 3; k++) {
+            a.Fout[3 * (size_t)oi + k] = (double)Fi[k];
+            a.Tout[3 * (size_t)oi + k] = (double)Ti[k];
+        }
+        return;
+    }
+    if (MODE == M_ENERGY) {
+        double *o = a.obs + 10 * (size_t)oi;
+        o[0] = epot;
+        for (int k = 0; k < 9; k++)
+            o[1 + k] = vir[k];
+        return;
+    }
+    a.slotn[oi] = s;
+    if (MODE == M_BEAD) {
+        // LangevinNVT::advance_mom (Langevin.cpp:254-269), advance_pos
+        // (Langevin.cpp:205-223), binning of the new position
+        R nz[6];
+        load_noise(a, oi, 3, nz);
+        const R rr = (R)rp.bead_r;
+        const R c5 = (R)1 - (R)(a.gamma * a.dt / 2.);
+        const R c4 = (R)a.b_c2 * rr;
+        double xn[3] = {xi.x, xi.y, xi.z};
+        R pk[3];
+        for (int k = 0; k < 3; k++) {
+            pk[k] = a.p[3 * (size_t)i + k];
+            pk[k] = (R)a.b_c5c2 * pk[k] + (R)a.b_c5c3q * Fi[k] + (c5 * c4 + rr) * nz[k];
+            if constexpr (FP64)
+                xn[k] = xn[k] + a.c1 * pk[k];
+            else
+                xn[k] = xn[k] + (double)((R)a.c1 * pk[k]);
+            wrap1(xn[k], pk[k], rp.l[k], a.periodic);
+            a.pn[3 * (size_t)s + k] = pk[k];
+        }
+        PosRec o = xi;
+        o.x = xn[0];
+        o.y = xn[1];
+        o.z = xn[2];
+        a.Xn[s] = o;
+        bin_particle(a, s, rp, xn, a.celln);
+        return;
+    }
+
+    // rotor: create_forces_and_torques_sphere (LangevinR.cpp:475-578)
+    R pl[3] = {a.p[3 * (size_t)i], a.p[3 * (size_t)i + 1], a.p[3 * (size_t)i + 2]};
+    R ll[3] = {a.L[3 * (size_t)i], a.L[3 * (size_t)i + 1], a.L[3 * (size_t)i + 2]};
+    R nz[6];
+    load_noise(a, oi, 6, nz);
+    {
+        const R *q = qi;
+        const R pbx = q[0] * pl[0] + q[1] * pl[1] + q[2] * pl[2];
+        const R pby = q[3] * pl[0] + q[4] * pl[1] + q[5] * pl[2];
+        const R pbz = q[6] * pl[0] + q[7] * pl[1] + q[8] * pl[2];
+        const R gm = (R)(-a.gamma / a.m);
+        const R fx = gm * pbx + (R)rp.Rt * nz[0];
+        const R fy = gm * pby + (R)rp.Rt * nz[1];
+        const R fz = gm * pbz + (R)rp.Rt * nz[2];
+        const R tbx = (R)(-a.gammar / a.I[0]) * ll[0] + (R)rp.Rr * nz[3];
+        const R tby = (R)(-a.gammar / a.I[1]) * ll[1] + (R)rp.Rr * nz[4];
+        const R tbz = (R)(-a.gammar / a.I[2]) * ll[2] + (R)rp.Rr * nz[5];
+        const R Fx = Fi[0] + fx * q[0] + fy * q[3] + fz * q[6];
+        const R Fy = Fi[1] + fx * q[1] + fy * q[4] + fz * q[7];
+        const R Fz = Fi[2] + fx * q[2] + fy * q[5] + fz * q[8];
+        const R Tx = tbx + Ti[0] * q[0] + Ti[1] * q[1] + Ti[2] * q[2];
+        const R Ty = tby + Ti[0] * q[3] + Ti[1] * q[4] + Ti[2] * q[5];
+        const R Tz = tbz + Ti[0] * q[6] + Ti[1] * q[7] + Ti[2] * q[8];
+        Fi[0] = Fx;
+        Fi[1] = Fy;
+        Fi[2] = Fz;
+        Ti[0] = Tx;
+        Ti[1] = Ty;
+        Ti[2] = Tz;
+    }
+    const R hdt = (R)a.hdt;
+    if (MODE == M_STEP || MODE == M_FUSED)
+        for (int k = 0; k < 3; k++) {
+            // advancemom_halfstep (LangevinR.cpp:327-347)
+            pl[k] += hdt * Fi[k];
+            ll[k] += hdt * Ti[k];
+        }
+    if (MODE == M_PRIME || MODE == M_STEP) {
+        a.Xn[s] = xi;
+        for (int k = 0; k < 9; k++)
+            a.Qn[9 * (size_t)s + k] = qi[k];
+        for (int k = 0; k < 3; k++) {
+            a.pn[3 * (size_t)s + k] = pl[k];
+            a.Ln[3 * (size_t)s + k] = ll[k];
+            a.Fn[3 * (size_t)s + k] = Fi[k];
+            a.Tn[3 * (size_t)s + k] = Ti[k];
+        }
+        return;
+    }
+    // M_FUSED: the next step's half kick with the same F, T, drift, rotation
+    for (int k = 0; k < 3; k++) {
+        pl[k] += hdt * Fi[k];
+        ll[k] += hdt * Ti[k];
+    }
+    double xn[3] = {xi.x, xi.y, xi.z};
+    for (int k = 0; k < 3; k++) {
+        if constexpr (FP64)
+            xn[k] = xn[k] + a.c1 * pl[k];
+        else
+            xn[k] = xn[k] + (double)((R)a.c1 * pl[k]);
+        wrap1(xn[k], pl[k], rp.l[k], a.periodic);
+    }
+    const R Ir[3] = {(R)a.I[0], (R)a.I[1], (R)a.I[2]};
+    rotate((R)a.dt, Ir, ll, qi);
+    if (!FP64 && a.reortho_every > 0 && ((a.ctl->epoch + 1) % a.reortho_every) == 0)
+        reortho(qi);
+    for (int k = 0; k < 3; k++) {
+        a.pn[3 * (size_t)s + k] = pl[k];
+        a.Ln[3 * (size_t)s + k] = ll[k];
+    }
+    for (int k = 0; k < 9; k++)
+        a.Qn[9 * (size_t)s + k] = qi[k];
+    PosRec o = xi;
+    o.x = xn[0];
+    o.y = xn[1];
+    o.z = xn[2];
+    a.Xn[s] = o;
+    bin_particle(a, s, rp, xn, a.celln);
+}
+
+// first half of a rotor step from the carried F, T (no reordering): half kick,
+// drift + wrap (Langevin.cpp:205-223), rotate (LangevinR.cpp:687-695), binning
+template <class R> __global__ void __launch_bounds__(256) k_kick_drift(KArgs<R> a)
+{
+    constexpr bool FP64 = sizeof(R) == 8;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.N)
+        return;
+    PosRec xr = a.X[i];
+    const DevRep rp = a.reps[xr.meta >> 8];
+    const R hdt = (R)a.hdt;
+    R pl[3], ll[3], q[9];
+    double xn[3] = {xr.x, xr.y, xr.z};
+    for (int k = 0; k < 3; k++) {
+        pl[k] = a.p[3 * (size_t)i + k] + hdt * a.F[3 * (size_t)i + k];
+        ll[k] = a.L[3 * (size_t)i + k] + hdt * a.T[3 * (size_t)i + k];
+    }
+    for (int k = 0; k < 9; k++)
+        q[k] = a.Q[9 * (size_t)i + k];
+    for (int k = 0; k < 3; k++) {
+        if constexpr (FP64)
+            xn[k] = xn[k] + a.c1 * pl[k];
+        else
+            xn[k] = xn[k] + (double)((R)a.c1 * pl[k]);
+        wrap1(xn[k], pl[k], rp.l[k], a.periodic);
+    }
+    const R Ir[3] = {(R)a.I[0], (R)a.I[1], (R)a.I[2]};
+    rotate((R)a.dt, Ir, ll, q);
+    if (!FP64 && a.reortho_every > 0 && ((a.ctl->epoch + 1) % a.reortho_every) == 0)
+        reortho(q);
+    for (int k = 0; k < 3; k++) {
+        a.pn[3 * (size_t)i + k] = pl[k];
+        a.Ln[3 * (size_t)i + k] = ll[k];
+    }
+    for (int k = 0; k < 9; k++)
+        a.Qn[9 * (size_t)i + k] = q[k];
+    xr.x = xn[0];
+    xr.y = xn[1];
+    xr.z = xn[2];
+    a.Xn[i] = xr;
+    a.slotn[xr.id] = i;
+    bin_particle(a, i, rp, xn, a.celln);
+}
+
+// binning of the current positions in place (after an upload or a reorder)
+template <class R> __global__ void __launch_bounds__(256) k_bin(KArgs<R> a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.N)
+        return;
+    const PosRec xr = a.X[i];
+    const DevRep rp = a.reps[xr.meta >> 8];
+    const double x[3] = {xr.x, xr.y, xr.z};
+    bin_particle(a, i, rp, x, a.celln);
+}
+
+// --------------------------------------------------------------- cell lists
+// exclusive scan of count[0..n) into start[0..n], counts zeroed after use.
+// phase 1: per block (1024 elements) local scan and block total
+__global__ void __launch_bounds__(256) k_scan1(const int *count, int *start, int *bsum, int n)
+{
+    __shared__ int sh[256];
+    const int base = blockIdx.x * 1024 + threadIdx.x * 4;
+    int v[4], s = 0;
+    for (int k = 0; k < 4; k++) {
+        v[k] = base + k < n ? count[base + k] : 0;
+        s += v[k];
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    int run = sh[threadIdx.x] - s;
+    for (int k = 0; k < 4; k++) {
+        if (base + k < n)
+            start[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 255)
+        bsum[blockIdx.x] = sh[255];
+}
+
+// phase 2: exclusive scan of the block totals (one block)
+__global__ void __launch_bounds__(1024) k_scan2(int *bsum, int nb, int *total)
+{
+    __shared__ int sh[1024];
+    const int per = (nb + 1023) / 1024;
+    const int b0 = threadIdx.x * per;
+    int s = 0;
+    for (int k = 0; k < per; k++)
+        if (b0 + k < nb)
+            s += bsum[b0 + k];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    int run = sh[threadIdx.x] - s;
+    for (int k = 0; k < per; k++)
+        if (b0 + k < nb) {
+            int v = bsum[b0 + k];
+            bsum[b0 + k] = run;
+            run += v;
+        }
+    if (threadIdx.x == 1023)
+        *total = sh[1023];
+}
+
+// phase 3: add block offsets, zero the counts, close the array
+__global__ void __launch_bounds__(256) k_scan3(int *count, int *start, const int *bsum, int n,
+                                               const int *total)
+{
+    const int base = blockIdx.x * 1024 + threadIdx.x * 4;
+    const int off = bsum[blockIdx.x];
+    for (int k = 0; k < 4; k++)
+        if (base + k < n) {
+            start[base + k] += off;
+            count[base + k] = 0;
+        }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        start[n] = *total;
+}
+
+__global__ void __launch_bounds__(256) k_fill(int N, const int *cell, const int *rank,
+                                              const int *start, int *list)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N)
+        list[start[cell[i]] + rank[i]] = i;
+}
+
+// ascending slot order inside each cell: makes the gather order, and so the
+// floating-point sums, independent of atomic arrival order
+__global__ void __launch_bounds__(256) k_cellsort(int ncells, const int *start, int *list)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells)
+        return;
+    int s0 = start[c], s1 = start[c + 1];
+    for (int s = s0 + 1; s < s1; s++) {
+        int v = list[s];
+        int t = s - 1;
+        while (t >= s0 && list[t] > v) {
+            list[t + 1] = list[t];
+            t--;
+        }
+        list[t + 1] = v;
+    }
+}
+
+__global__ void k_epoch_inc(StepCtl *ctl)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        ctl->epoch += 1;
+}
+
+__global__ void k_set_call_base(StepCtl *ctl)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        ctl->call_base = ctl->epoch;
+}
+
+// ---------------------------------------------------- parity entry points
+// positions of the particles with original index in [first, first+count),
+// in original order (x only; for the parity paths)
+__global__ void k_positions_by_id(const PosRec *X, int N, int first, int count, double *out)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N)
+        return;
+    PosRec r = X[s];
+    int o = (int)r.id - first;
+    if (o < 0 || o >= count)
+        return;
+    out[3 * (size_t)o] = r.x;
+    out[3 * (size_t)o + 1] = r.y;
+    out[3 * (size_t)o + 2] = r.z;
+}
+
+// cube::assign_box with a caller-given n (original order)
+__global__ void k_assign_custom(const double *X, int count, double l0, double l1, double l2,
+                                int n0, int n1, int n2, int *out)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count)
+        return;
+    const double *x = X + 3 * (size_t)k;
+    int cx = axis_cell(x[0], n0, l0), cy = axis_cell(x[1], n1, l1), cz = axis_cell(x[2], n2, l2);
+    out[k] = cx + n0 * (cy + n1 * cz);
+}
+
+// cube::distance_less_than (geometry.h:145-164)
+__device__ __forceinline__ bool dlt(const double *xi, const double *xj, const double l[3],
+                                    int periodic, double cut)
+{
+    double temp = 0.0;
+    const double c2 = cut * cut;
+    for (int k = 0; k < 3; k++) {
+        double dx = xi[k] - xj[k];
+        if (periodic) {
+            if (fabs(dx) > l[k] * 0.5)
+                dx = dx - (dx >= 0 ? fabs(l[k]) : -fabs(l[k]));
+        }
+        dx = dx * dx;
+        if (dx > c2)
+            return false;
+        temp += dx;
+    }
+    return temp < c2;
+}
+
+// pairs i < j (original order) of the reference's stencil and predicate:
+// count (write = 0) or write each particle's partners in ascending order
+__global__ void k_pairs(const double *X, int count, double l0, double l1, double l2, int periodic,
+                        int n0, int n1, int n2, double cut, const int *cellv, const int *cstart,
+                        const int *clist, int *cnt, const long long *off, int *ij, long long cap,
+                        int write)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count)
+        return;
+    const double l[3] = {l0, l1, l2};
+    const int nn[3] = {n0, n1, n2};
+    const double *xi = X + 3 * (size_t)k;
+    const int c = cellv[k];
+    const int cc[3] = {c % n0, (c / n0) % n1, c / (n0 * n1)};
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; a++) {
+        lo[a] = nn[a] >= 3 ? -1 : 0;
+        hi[a] = nn[a] >= 2 ? 1 : 0;
+    }
+    int found = 0;
+    long long o = write ? off[k] : 0;
+    for (int oz = lo[2]; oz <= hi[2]; oz++)
+        for (int oy = lo[1]; oy <= hi[1]; oy++)
+            for (int ox = lo[0]; ox <= hi[0]; ox++) {
+                int w[3] = {cc[0] + ox, cc[1] + oy, cc[2] + oz};
+                for (int a = 0; a < 3; a++)
+                    w[a] = w[a] < 0 ? w[a] + nn[a] : (w[a] >= nn[a] ? w[a] - nn[a] : w[a]);
+                int c2 = w[0] + n0 * (w[1] + n1 * w[2]);
+                for (int s = cstart[c2]; s < cstart[c2 + 1]; s++) {
+                    int j = clist[s];
+                    if (j <= k)
+                        continue;
+                    if (dlt(xi, X + 3 * (size_t)j, l, periodic, cut)) {
+                        if (write && o + found < cap) {
+                            ij[2 * (o + found)] = k;
+                            ij[2 * (o + found) + 1] = j;
+                        }
+                        found++;
+                    }
+                }
+            }
+    if (!write) {
+        cnt[k] = found;
+        return;
+    }
+    long long e = o + found < cap ? o + found : cap;
+    for (long long s = o + 1; s < e; s++) {
+        int v = ij[2 * s + 1];
+        long long t = s - 1;
+        while (t >= o && ij[2 * t + 1] > v) {
+            ij[2 * (t + 1) + 1] = ij[2 * t + 1];
+            t--;
+        }
+        ij[2 * (t + 1) + 1] = v;
+    }
+}
+
+// ------------------------------------------------------------ reductions
+// per-block partial sums (fixed order) of 16 observables: slot range
+// [first, first+count) for p, L; the same range of original indices for obs
+template <class R>
+__global__ void __launch_bounds__(256) k_obs(int first, int count, const R *p, const R *L,
+                                             const double *obs, double m, double I0,
+                                             double *partial)
+{
+    __shared__ double sh[256];
+    double acc[16];
+    for (int k = 0; k < 16; k++)
+        acc[k] = 0;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < count; s += gridDim.x * blockDim.x) {
+        int i = first + s;
+        for (int k = 0; k < 3; k++) {
+            double pk = (double)p[3 * (size_t)i + k], lk = L ? (double)L[3 * (size_t)i + k] : 0.0;
+            acc[k] += (pk * pk) / (2. * m);
+            acc[3 + k] += (lk * lk) / (2. * I0);
+        }
+        for (int k = 0; k < 10; k++)
+            acc[6 + k] += obs[10 * (size_t)i + k];
+    }
+    for (int k = 0; k < 16; k++) {
+        sh[threadIdx.x] = acc[k];
+        __syncthreads();
+        for (int off = 128; off > 0; off >>= 1) {
+            if (threadIdx.x < off)
+                sh[threadIdx.x] += sh[threadIdx.x + off];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            partial[16 * blockIdx.x + k] = sh[0];
+        __syncthreads();
+    }
+}
+
+// precision conversion (device <-> host FP64 layout)
+template <class A, class B> __global__ void k_convert(const A *in, B *out, size_t n)
+{
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n)
+        out[k] = (B)in[k];
+}
+
+// new (replica << 8 | type) words for the current slots from an array by
+// original index
+__global__ void k_set_meta(PosRec *X, int N, const uint32_t *meta_by_id)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < N)
+        X[s].meta = meta_by_id[X[s].id];
+}
+
+} // namespace mdx
