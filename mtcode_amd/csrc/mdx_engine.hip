@@ -20,6 +20,7 @@
 #include <cstdarg>
 #include <cstddef>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -89,6 +90,8 @@ struct mdx_engine {
     int has_active = 0;
     double fguard = 1e4;
     int reortho_every = 100;
+    int use_tiles = 0; // LDS-staged neighbour tiles (MDX_TILES=1 enables)
+    int use_split = 1; // walk / finish kernels when patches are active (MDX_SPLIT=0 disables)
     // grid
     int ncells = 0;
     std::vector<DevRep> reps;
@@ -108,6 +111,9 @@ struct mdx_engine {
     Buf ang_off, ang_ref, ang_ijk, ang_k, ang_t0;
     Buf exc_off, exc_j, act_prev, act_next;
     Buf noise, ctl, partial, tmpd, tmpi, tmpi2, tmpi3, tmpll, tmpx;
+    Buf Fw, ncand, cand; // split walk -> finish
+    hipEvent_t prof_mid = nullptr; // recorded between walk and finish while profiling
+    bool last_split = false;       // the last gather launch used the split form
 
     size_t rsz() const { return prec == MDX_PREC_FP64 ? 8 : 4; }
 };
@@ -255,6 +261,9 @@ template <class R> static KArgs<R> make_args(mdx_engine *e)
     a.Fout = (double *)e->Fout.p;
     a.Tout = (double *)e->Tout.p;
     a.obs = (double *)e->obs.p;
+    a.Fw = (R *)e->Fw.p;
+    a.ncand = (int *)e->ncand.p;
+    a.cand = (int *)e->cand.p;
     a.wca = e->wca;
     a.wca_eps = e->wca_eps;
     const double s3 = e->wca_sigma * e->wca_sigma * e->wca_sigma;
@@ -364,7 +373,29 @@ template <class R, int MODE> static int launch_gather(mdx_engine *e, const doubl
 {
     KArgs<R> a = make_args<R>(e);
     a.noise = noise_dev;
-    hipLaunchKernelGGL((k_gather<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
+    if (e->use_split && a.has_patches &&
+        (MODE == M_PRIME || MODE == M_STEP || MODE == M_FUSED || MODE == M_CONS)) {
+        int rc;
+        if ((rc = dalloc(e, e->Fw, 3 * e->rsz() * (size_t)e->N)) ||
+            (rc = dalloc(e, e->ncand, 4 * (size_t)e->N)) ||
+            (rc = dalloc(e, e->cand, 4 * (size_t)PMAXG * e->N)))
+            return rc;
+        a.Fw = (R *)e->Fw.p;
+        a.ncand = (int *)e->ncand.p;
+        a.cand = (int *)e->cand.p;
+        hipLaunchKernelGGL((k_walk<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
+        if (e->prof_mid)
+            HIPCHK(hipEventRecord(e->prof_mid, e->stream));
+        e->last_split = true;
+        hipLaunchKernelGGL((k_finish<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
+        HIPCHK(hipGetLastError());
+        return MDX_OK;
+    }
+    e->last_split = false;
+    if (e->use_tiles)
+        hipLaunchKernelGGL((k_tile<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
+    else
+        hipLaunchKernelGGL((k_gather<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
     HIPCHK(hipGetLastError());
     return MDX_OK;
 }
@@ -605,6 +636,10 @@ int mdx_create(const mdx_desc *d, mdx_engine **out)
     e->seed = d->seed;
     e->device = d->device;
     e->N = (int)N;
+    if (const char *t = std::getenv("MDX_TILES"))
+        e->use_tiles = std::atoi(t) != 0;
+    if (const char *t = std::getenv("MDX_SPLIT"))
+        e->use_split = std::atoi(t) != 0;
     long long f = 0;
     for (int r = 0; r < e->nrep; r++) {
         e->first.push_back((int)f);
@@ -659,7 +694,8 @@ void mdx_destroy(mdx_engine *e)
                    &e->bond_off, &e->bond_j, &e->bond_k, &e->bond_r0, &e->ang_off, &e->ang_ref,
                    &e->ang_ijk, &e->ang_k,   &e->ang_t0, &e->exc_off, &e->exc_j,  &e->act_prev,
                    &e->act_next, &e->noise,  &e->ctl,    &e->partial, &e->tmpd,   &e->tmpi,
-                   &e->tmpi2,   &e->tmpi3,   &e->tmpll,  &e->tmpx};
+                   &e->tmpi2,   &e->tmpi3,   &e->tmpll,  &e->tmpx,    &e->Fw,
+                   &e->ncand,   &e->cand};
     for (Buf *b : bufs)
         dfree(*b);
     for (int b = 0; b < 2; b++) {
@@ -1305,12 +1341,12 @@ int mdx_info(mdx_engine *e, double *out)
 template <class R>
 static int profile_t(mdx_engine *e, int nsteps, int cap, const char **names, double *ms, int *nk)
 {
-    static const char *kn[] = {"gather", "scan", "fill", "cellsort", "kick_drift"};
-    const int K = 5;
-    double acc[K] = {0, 0, 0, 0, 0};
-    int cntk[K] = {0, 0, 0, 0, 0};
-    hipEvent_t ev[5];
-    for (int k = 0; k < 5; k++)
+    static const char *kn[] = {"gather", "scan", "fill", "cellsort", "kick_drift", "walk", "finish"};
+    const int K = 7;
+    double acc[K] = {0, 0, 0, 0, 0, 0, 0};
+    int cntk[K] = {0, 0, 0, 0, 0, 0, 0};
+    hipEvent_t ev[6];
+    for (int k = 0; k < 6; k++)
         HIPCHK(hipEventCreate(&ev[k]));
     int rc;
     if (e->integ == MDX_INTEG_ROTOR && !e->primed && (rc = do_prime<R>(e, nullptr)))
@@ -1348,13 +1384,15 @@ static int profile_t(mdx_engine *e, int nsteps, int cap, const char **names, dou
         hipLaunchKernelGGL(k_cellsort, dim3(nblk(n)), dim3(256), 0, e->stream, n,
                            (const int *)e->start.p, (int *)e->list.p);
         HIPCHK(hipEventRecord(ev[3], e->stream));
-        if (e->integ == MDX_INTEG_ROTOR) {
-            if ((rc = launch_gather<R, M_FUSED>(e, nullptr)))
-                return rc;
-        } else {
-            if ((rc = launch_gather<R, M_BEAD>(e, nullptr)))
-                return rc;
-        }
+        // ev[5] is recorded between the walk and finish kernels of the split form
+        e->prof_mid = ev[5];
+        if (e->integ == MDX_INTEG_ROTOR)
+            rc = launch_gather<R, M_FUSED>(e, nullptr);
+        else
+            rc = launch_gather<R, M_BEAD>(e, nullptr);
+        e->prof_mid = nullptr;
+        if (rc)
+            return rc;
         HIPCHK(hipEventRecord(ev[4], e->stream));
         e->cur ^= 1;
         launch_small(e, k_epoch_inc);
@@ -1368,6 +1406,14 @@ static int profile_t(mdx_engine *e, int nsteps, int cap, const char **names, dou
         acc[3] += t;
         HIPCHK(hipEventElapsedTime(&t, ev[3], ev[4]));
         acc[0] += t;
+        if (e->last_split) {
+            float tw;
+            HIPCHK(hipEventElapsedTime(&tw, ev[3], ev[5]));
+            acc[5] += tw;
+            acc[6] += t - tw;
+            cntk[5]++;
+            cntk[6]++;
+        }
         cntk[0]++;
         cntk[1]++;
         cntk[2]++;
@@ -1387,7 +1433,7 @@ static int profile_t(mdx_engine *e, int nsteps, int cap, const char **names, dou
     }
     HIPCHK(hipStreamSynchronize(e->stream));
     e->steps_taken += nsteps + (e->integ == MDX_INTEG_ROTOR ? 1 : 0);
-    for (int k = 0; k < 5; k++)
+    for (int k = 0; k < 6; k++)
         (void)hipEventDestroy(ev[k]);
     int w = 0;
     for (int k = 0; k < K && w < cap; k++) {

@@ -65,6 +65,9 @@ template <class R> struct KArgs {
     const int *start, *list;
     // outputs by original index
     double *Fout, *Tout, *obs;
+    // split walk -> finish: WCA force and patch candidates per cell-list position
+    R *Fw;
+    int *ncand, *cand; // cand[q * N + s], q < PMAXG
     // WCA (HSPotential)
     int wca;
     double wca_eps, wca_s6, wca_s12, wca_pl;
@@ -256,7 +259,9 @@ __device__ __forceinline__ void min_image(const KArgs<R> &a, const DevRep &rp, d
 }
 
 // ------------------------------------------------------------- gather kernel
-constexpr int PMAX = 16; // deferred patch candidates per particle held in LDS
+constexpr int PMAX = 8;   // deferred patch candidates per particle held in LDS
+constexpr int PMAXG = 16; // patch candidates per particle handed from walk to finish
+enum { PH_ALL = 0, PH_WALK = 1, PH_FINISH = 2 };
 
 // neighbour slots of slot i: the 9 (y, z) rows of the 27-cell stencil; the x
 // neighbours of a row are contiguous in the cell list (one range, or two at
@@ -299,12 +304,12 @@ __device__ __forceinline__ void for_each_neighbor(const KArgs<R> &a, const DevRe
             for (int k = r0; k < r1; k++) {
                 const int j = a.list[k];
                 if (j != i)
-                    fn(j);
+                    fn(a.X[j], j);
             }
             for (int k = r2; k < r3; k++) {
                 const int j = a.list[k];
                 if (j != i)
-                    fn(j);
+                    fn(a.X[j], j);
             }
         }
     }
@@ -427,14 +432,14 @@ __device__ __forceinline__ void patch_pair(const KArgs<R> &a, const DevRep &rp, 
 // evaluates WCA and queues patch candidates (pairs within max_distance_check
 // whose types carry patch rows); the patch pairs are evaluated after the loop
 // (fewer live registers in the loop).
-template <class R, int MODE>
-__global__ void __launch_bounds__(256) k_gather(KArgs<R> a)
+// PH_ALL: walk, patches and integrator in one kernel; PH_WALK: the walk
+// only (WCA force and patch candidates to global memory); PH_FINISH: the
+// patches from those candidates, topology and the integrator.  The split
+// keeps the register-heavy patch code out of the neighbour walk.
+template <class R, int MODE, int PH, class Trav>
+__device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *plist, Trav &&trav)
 {
     constexpr bool FP64 = sizeof(R) == 8;
-    __shared__ int plist[PMAX * 256];
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= a.N)
-        return;
     const int i = a.list[s];
     const PosRec xi = a.X[i];
     const uint32_t oi = xi.id;
@@ -460,22 +465,41 @@ __global__ void __launch_bounds__(256) k_gather(KArgs<R> a)
         exc1 = a.exc_off[oi + 1];
     }
 
-    if (a.wca || npi > 0) {
+    // candidate filter of the patch loop: within max_distance_check
+    // (LangevinR.cpp:938) and a type pair with patch rows
+    auto is_patch_candidate = [&](const PosRec &xj, double r2s) -> bool {
+        if (!(npi > 0 && r2s < a.maxd2))
+            return false;
+        const int tj = xj.meta & 255;
+        if (a.np[tj] == 0)
+            return false;
+        int ta, tc;
+        patch_roles(oi, ti, xj.id, tj, ta, tc);
+        return (a.active_mask >> (ta * MAXT + tc)) & 1ull;
+    };
+    if (PH == PH_FINISH) {
+        for (int k = 0; k < 3; k++)
+            acc.F[k] = a.Fw[3 * (size_t)s + k];
+        const int npc = a.ncand[s];
+        const int nq = npc < PMAXG ? npc : PMAXG;
+        for (int q = 0; q < nq; q++)
+            patch_pair<R, MODE>(a, rp, xi, ti, qi, a.cand[(size_t)q * a.N + s], acc);
+        if (npc > PMAXG) {
+            int seen = 0;
+            trav(i, rp, [&](const PosRec &xj, int j) {
+                double d[3];
+                min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
+                const double r2s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                if (!(r2s < a.rmax2) || !is_patch_candidate(xj, r2s))
+                    return;
+                if (seen++ >= PMAXG)
+                    patch_pair<R, MODE>(a, rp, xi, ti, qi, j >= 0 ? j : a.slot[xj.id], acc);
+            });
+        }
+    }
+    if (PH != PH_FINISH && (a.wca || npi > 0)) {
         int npc = 0;
-        // candidate filter of the patch loop: within max_distance_check
-        // (LangevinR.cpp:938) and a type pair with patch rows
-        auto is_patch_candidate = [&](const PosRec &xj, double r2s) -> bool {
-            if (!(npi > 0 && r2s < a.maxd2))
-                return false;
-            const int tj = xj.meta & 255;
-            if (a.np[tj] == 0)
-                return false;
-            int ta, tc;
-            patch_roles(oi, ti, xj.id, tj, ta, tc);
-            return (a.active_mask >> (ta * MAXT + tc)) & 1ull;
-        };
-        for_each_neighbor(a, rp, i, [&](int j) {
-            const PosRec xj = a.X[j];
+        trav(i, rp, [&](const PosRec &xj, int j) {
             double d[3];
             min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
             const double r2s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
@@ -523,11 +547,22 @@ __global__ void __launch_bounds__(256) k_gather(KArgs<R> a)
                 }
             }
             if (is_patch_candidate(xj, r2s)) {
-                if (npc < PMAX)
-                    plist[npc * 256 + threadIdx.x] = j;
+                const int js = j >= 0 ? j : a.slot[xj.id];
+                if (PH == PH_WALK) {
+                    if (npc < PMAXG)
+                        a.cand[(size_t)npc * a.N + s] = js;
+                } else if (npc < PMAX) {
+                    plist[npc * 256 + threadIdx.x] = js;
+                }
                 npc++;
             }
         });
+        if (PH == PH_WALK) {
+            for (int k = 0; k < 3; k++)
+                a.Fw[3 * (size_t)s + k] = acc.F[k];
+            a.ncand[s] = npc;
+            return;
+        }
         // -------- patches (LangevinR.cpp:911-1047), in neighbour-loop order
         const int nq = npc < PMAX ? npc : PMAX;
         for (int q = 0; q < nq; q++)
@@ -535,17 +570,22 @@ __global__ void __launch_bounds__(256) k_gather(KArgs<R> a)
         if (npc > PMAX) {
             // dense neighbourhoods: a second traversal takes the rest in order
             int seen = 0;
-            for_each_neighbor(a, rp, i, [&](int j) {
-                const PosRec xj = a.X[j];
+            trav(i, rp, [&](const PosRec &xj, int j) {
                 double d[3];
                 min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
                 const double r2s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
                 if (!(r2s < a.rmax2) || !is_patch_candidate(xj, r2s))
                     return;
                 if (seen++ >= PMAX)
-                    patch_pair<R, MODE>(a, rp, xi, ti, qi, j, acc);
+                    patch_pair<R, MODE>(a, rp, xi, ti, qi, j >= 0 ? j : a.slot[xj.id], acc);
             });
         }
+    }
+    if (PH == PH_WALK) {
+        for (int k = 0; k < 3; k++)
+            a.Fw[3 * (size_t)s + k] = 0;
+        a.ncand[s] = 0;
+        return;
     }
     R *Fi = acc.F, *Ti = acc.T;
     double &epot = acc.epot;
@@ -753,6 +793,182 @@ __global__ void __launch_bounds__(256) k_gather(KArgs<R> a)
     o.z = xn[2];
     a.Xn[s] = o;
     bin_particle(a, s, rp, xn, a.celln);
+}
+
+// one thread per cell-list position, neighbours read from global memory
+template <class R, int MODE> __global__ void __launch_bounds__(256) k_gather(KArgs<R> a)
+{
+    __shared__ int plist[PMAX * 256];
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.N)
+        return;
+    process_particle<R, MODE, PH_ALL>(a, s, plist, [&](int i, const DevRep &rp, auto &&fn) {
+        for_each_neighbor(a, rp, i, fn);
+    });
+}
+
+// split form: the neighbour walk ...
+template <class R, int MODE> __global__ void __launch_bounds__(256) k_walk(KArgs<R> a)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.N)
+        return;
+    process_particle<R, MODE, PH_WALK>(a, s, nullptr, [&](int i, const DevRep &rp, auto &&fn) {
+        for_each_neighbor(a, rp, i, fn);
+    });
+}
+
+// ... then the patch pairs, topology and the integrator
+template <class R, int MODE> __global__ void __launch_bounds__(256) k_finish(KArgs<R> a)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.N)
+        return;
+    process_particle<R, MODE, PH_FINISH>(a, s, nullptr, [&](int i, const DevRep &rp, auto &&fn) {
+        for_each_neighbor(a, rp, i, fn);
+    });
+}
+
+// ------------------------------------------------------- LDS-staged tiles
+// A block owns 256 consecutive cell-list positions.  When they lie in one
+// replica and one z plane, in x-rows ya..yb, the block stages the 3 (nr + 2)
+// halo rows (z-1..z+1 x ya-1..yb+1) -- positions, slots and per-cell offsets --
+// into LDS and walks the neighbour cells there; otherwise (replica or plane
+// boundary, small grids, over-full tiles) it walks global memory.
+constexpr int TILE_CAP = 1536;   // staged particles
+constexpr int TILE_ROWCAP = 4096; // staged cell offsets
+constexpr int TILE_OWNROWS = 12; // own x-rows per tile
+constexpr int TILE_HROWS = 3 * (TILE_OWNROWS + 2);
+
+template <class R, int MODE> __global__ void __launch_bounds__(256) k_tile(KArgs<R> a)
+{
+    __shared__ int plist[PMAX * 256];
+    __shared__ PosRec lpos[TILE_CAP];
+    __shared__ int rowoff[TILE_ROWCAP];
+    __shared__ int hbase[TILE_HROWS + 1];
+    __shared__ int hcell[TILE_HROWS];
+    __shared__ int tinfo[8]; // use_lds, ya, nr, z, rep, nh
+    const int s0 = blockIdx.x * blockDim.x;
+    const int s1 = min(a.N, s0 + (int)blockDim.x);
+    if (threadIdx.x == 0) {
+        int use = 0;
+        const PosRec x0 = a.X[a.list[s0]], x1 = a.X[a.list[s1 - 1]];
+        const int rep = x0.meta >> 8;
+        if ((x1.meta >> 8) == (uint32_t)rep) {
+            const DevRep &rp = a.reps[rep];
+            const int n0 = rp.nc[0], n1 = rp.nc[1], n2 = rp.nc[2];
+            const int r0 = (a.cell[a.list[s0]] - rp.coff) / n0;
+            const int r1 = (a.cell[a.list[s1 - 1]] - rp.coff) / n0;
+            const int ya = r0 % n1, z = r0 / n1, yb = r1 % n1;
+            const int nr = r1 - r0 + 1;
+            if (n0 >= 3 && n1 >= 3 && n2 >= 3 && r1 / n1 == z && nr <= TILE_OWNROWS &&
+                nr + 2 <= n1 && 3 * (nr + 2) * (n0 + 1) <= TILE_ROWCAP) {
+                use = 1;
+                tinfo[1] = ya;
+                tinfo[2] = nr;
+                tinfo[3] = z;
+                tinfo[4] = rep;
+                tinfo[5] = 3 * (nr + 2);
+            }
+            (void)yb;
+        }
+        tinfo[0] = use;
+    }
+    __syncthreads();
+    if (tinfo[0]) {
+        const DevRep &rp = a.reps[tinfo[4]];
+        const int n0 = rp.nc[0], n1 = rp.nc[1], n2 = rp.nc[2];
+        const int ya = tinfo[1], nr = tinfo[2], z = tinfo[3], nh = tinfo[5];
+        // halo row h = (dz + 1) (nr + 2) + (y - ya + 1), first cell of its x-row
+        if ((int)threadIdx.x < nh) {
+            const int h = threadIdx.x;
+            const int dz = h / (nr + 2) - 1, yy = ya - 1 + h % (nr + 2);
+            const int zz = z + dz < 0 ? z + dz + n2 : (z + dz >= n2 ? z + dz - n2 : z + dz);
+            const int y2 = yy < 0 ? yy + n1 : (yy >= n1 ? yy - n1 : yy);
+            const int c0 = rp.coff + n0 * (y2 + n1 * zz);
+            hcell[h] = c0;
+            hbase[h + 1] = a.start[c0 + n0] - a.start[c0];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            hbase[0] = 0;
+            for (int h = 0; h < nh; h++)
+                hbase[h + 1] += hbase[h];
+            if (hbase[nh] > TILE_CAP)
+                tinfo[0] = 0;
+        }
+        __syncthreads();
+    }
+    if (tinfo[0]) {
+        const DevRep &rp = a.reps[tinfo[4]];
+        const int n0 = rp.nc[0];
+        const int nh = tinfo[5];
+        const int total = hbase[nh];
+        // per-cell offsets of the staged rows (LDS index of each cell's first particle)
+        for (int t = threadIdx.x; t < nh * (n0 + 1); t += blockDim.x) {
+            const int h = t / (n0 + 1), x = t % (n0 + 1);
+            rowoff[t] = hbase[h] + a.start[hcell[h] + x] - a.start[hcell[h]];
+        }
+        // staged particles, flattened over the rows
+        for (int f = threadIdx.x; f < total; f += blockDim.x) {
+            int lo = 0, hi = nh - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (hbase[mid] <= f)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const int k = a.start[hcell[lo]] + (f - hbase[lo]);
+            lpos[f] = a.X[a.list[k]];
+        }
+        __syncthreads();
+        const int s = s0 + threadIdx.x;
+        if (s >= s1)
+            return;
+        const int ya = tinfo[1], nr = tinfo[2];
+        process_particle<R, MODE, PH_ALL>(a, s, plist, [&](int i, const DevRep &rq, auto &&fn) {
+            const uint32_t self = a.X[i].id;
+            const int c = a.cell[i] - rq.coff;
+            const int cx = c % n0;
+            const int cy = (c / n0) % rq.nc[1];
+            for (int dz = 0; dz < 3; dz++) {
+                for (int dy = 0; dy < 3; dy++) {
+                    const int h = dz * (nr + 2) + (cy - ya + dy);
+                    const int *ro = rowoff + h * (n0 + 1);
+                    int q0, q1, q2 = 0, q3 = 0;
+                    if (cx == 0) {
+                        q0 = ro[0];
+                        q1 = ro[2];
+                        q2 = ro[n0 - 1];
+                        q3 = ro[n0];
+                    } else if (cx == n0 - 1) {
+                        q0 = ro[n0 - 2];
+                        q1 = ro[n0];
+                        q2 = ro[0];
+                        q3 = ro[1];
+                    } else {
+                        q0 = ro[cx - 1];
+                        q1 = ro[cx + 2];
+                    }
+                    // slots are recovered from the id only for patch candidates
+                    for (int q = q0; q < q1; q++)
+                        if (lpos[q].id != self)
+                            fn(lpos[q], -1);
+                    for (int q = q2; q < q3; q++)
+                        if (lpos[q].id != self)
+                            fn(lpos[q], -1);
+                }
+            }
+        });
+        return;
+    }
+    const int s = s0 + threadIdx.x;
+    if (s >= s1)
+        return;
+    process_particle<R, MODE, PH_ALL>(a, s, plist, [&](int i, const DevRep &rp, auto &&fn) {
+        for_each_neighbor(a, rp, i, fn);
+    });
 }
 
 // first half of a rotor step from the carried F, T (no reordering): half kick,

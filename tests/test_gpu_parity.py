@@ -256,3 +256,65 @@ def test_equipartition_with_philox_noise():
     say("<KE> per component", acc)
     # the explicit friction scheme of the reference at dt=0.005 gives kT/2 to a few %
     assert np.all(np.abs(acc - 0.5) < 0.05)
+
+
+# ----------------------------------------------------------- LDS tiles
+@pytest.mark.parametrize("precision", [mdx.PREC_MIXED, mdx.PREC_FP64])
+def test_lds_tiles_bitwise_equal_global_walk(monkeypatch, precision):
+    """The LDS-staged tile walk and the global-memory walk visit neighbours in
+    the same order, so the trajectories are bitwise identical."""
+    s = wl.condensate_ab(1 << 18)
+    say("tiles vs global", s["N"], precision)
+    outs = []
+    monkeypatch.setenv("MDX_SPLIT", "0")
+    for tiles in ("1", "0"):
+        monkeypatch.setenv("MDX_TILES", tiles)
+        with engine_for(s, precision, seed=4) as e:
+            e.prime()
+            e.step(20)
+            outs.append(e.download())
+            F, T = e.compute_forces()
+            outs[-1]["F"], outs[-1]["T"] = F, T
+    for k in ("x", "p", "Q", "L", "F", "T"):
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+# ------------------------------------------------- split walk / finish
+@pytest.mark.parametrize("precision", [mdx.PREC_MIXED, mdx.PREC_FP64])
+def test_split_walk_finish_bitwise_equal_fused(monkeypatch, precision):
+    """The two-kernel form (neighbour walk, then patches + integrator) adds the
+    same terms in the same order as the one-kernel form: bitwise identical
+    trajectories, forces and torques."""
+    s = wl.condensate_ab(1 << 18)
+    say("split vs fused", s["N"], precision)
+    outs = []
+    monkeypatch.setenv("MDX_TILES", "0")
+    for split in ("1", "0"):
+        monkeypatch.setenv("MDX_SPLIT", split)
+        with engine_for(s, precision, seed=5) as e:
+            e.prime()
+            e.step(20)
+            outs.append(e.download())
+            F, T = e.compute_forces()
+            outs[-1]["F"], outs[-1]["T"] = F, T
+    for k in ("x", "p", "Q", "L", "F", "T"):
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+def test_split_candidate_overflow_matches_oracle(monkeypatch):
+    """A simple-cubic cluster at spacing 0.95 gives interior particles 18 patch
+    candidates (6 at 0.95, 12 at 1.34, all inside max_distance_check), more
+    than the 16 the walk hands over, so the finish kernel walks the
+    neighbour cells again for the rest; forces and torques still match the
+    oracle."""
+    s = wl.patchy_tetra(512, phi=0.1)
+    g = np.arange(8) * 0.95
+    lat = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    s["x"] = lat + (s["box"] - 7 * 0.95) / 2
+    say("candidate overflow cluster", s["N"])
+    monkeypatch.setenv("MDX_SPLIT", "1")
+    with engine_for(s) as e:
+        F, T = e.compute_forces()
+    Fo, To = po.compute_forces(s, list_cut=3.5)
+    assert relmax(F, Fo) < 1e-12
+    assert relmax(T, To) < 1e-12
