@@ -108,6 +108,15 @@ int mdx_set_exclusions(mdx_engine *e, long long npairs, const int *ij);
 /* tangential active force fa * unit(x_next - x_prev), -1 = the particle itself
  * (no 3-D reference; cf. PositionForcesDueToAnglesGPUx, MicrotubuleGPU.cu:357-411) */
 int mdx_set_active(mdx_engine *e, double fa, const int *prev, const int *next);
+/* Neighbour list of mdx_step (the reference keeps a Verlet list of cut 3.5
+ * rebuilt every 20 steps: Condensate.cpp:4, 345).  skin > 0: pairs within
+ * reach + skin are listed at a build, and the list is rebuilt on the device
+ * as soon as twice the summed per-step maximum drift reaches the skin, so the
+ * pairs evaluated each step are exactly those within reach.  kmax list slots
+ * per particle (a particle with more walks its cells), state re-sorted into
+ * cell order every resort_every steps.  skin 0: every step bins and walks
+ * the 27 cells.  Defaults 0.5, 64, 200. */
+int mdx_set_neighbor_list(mdx_engine *e, double skin, int kmax, int resort_every);
 /* |pair force| guard; 0 disables.  The reference prints "force too big" and
  * pauses above 1e4 (MD.cpp:1711-1749); here mdx_step returns MDX_ERR_FORCE. */
 int mdx_set_force_guard(mdx_engine *e, double fmax);
@@ -146,8 +155,10 @@ int mdx_observables(mdx_engine *e, int replica, double *out);
  * Returns the kernel count in *nk (<= cap). */
 int mdx_profile_step(mdx_engine *e, int nsteps, int cap, const char **names, double *ms,
                      int *nk);
-/* engine facts: out[0] N, out[1] total cells, out[2] cell size (replica 0, x),
- * out[3] bytes of device memory held, out[4] steps taken */
+/* engine facts (out holds 8): out[0] N, out[1] total cells, out[2] cell size
+ * (replica 0, x), out[3] bytes of device memory held, out[4] steps taken,
+ * out[5] neighbour-list builds so far, out[6] skin, out[7] list slots per
+ * particle */
 int mdx_info(mdx_engine *e, double *out);
 
 #ifdef __cplusplus

@@ -92,6 +92,15 @@ struct mdx_engine {
     int reortho_every = 100;
     int use_tiles = 0; // LDS-staged neighbour tiles (MDX_TILES=1 enables)
     int use_split = 1; // walk / finish kernels when patches are active (MDX_SPLIT=0 disables)
+    // neighbour list: skin > 0 enables it for mdx_step (MDX_SKIN), kmax slots per
+    // particle (MDX_KMAX), state re-sorted into cell order every resort_every
+    // steps (MDX_RESORT)
+    double skin = 0.5;
+    int kmax = 64;
+    int resort_every = 200;
+    bool list_valid = false;
+    bool counted = false; // cell counts of the current positions are held (not yet scanned)
+    int since_resort = 0;
     // grid
     int ncells = 0;
     std::vector<DevRep> reps;
@@ -112,6 +121,8 @@ struct mdx_engine {
     Buf exc_off, exc_j, act_prev, act_next;
     Buf noise, ctl, partial, tmpd, tmpi, tmpi2, tmpi3, tmpll, tmpx;
     Buf Fw, ncand, cand; // split walk -> finish
+    Buf nbr, nn, bcell;  // neighbour list
+    Buf bmax;            // per-block drift maxima
     hipEvent_t prof_mid = nullptr; // recorded between walk and finish while profiling
     bool last_split = false;       // the last gather launch used the split form
 
@@ -185,7 +196,9 @@ static int build_grid(mdx_engine *e)
 {
     if (!e->grid_dirty)
         return MDX_OK;
-    const double rc = reach(e);
+    // with the neighbour list the cells hold the list reach (interaction + skin)
+    const double r0 = reach(e);
+    const double rc = r0 > 0 && e->skin > 0 ? r0 + e->skin : r0;
     e->reps.resize(e->nrep);
     long long coff = 0;
     for (int r = 0; r < e->nrep; r++) {
@@ -226,6 +239,8 @@ static int build_grid(mdx_engine *e)
     e->grid_dirty = false;
     e->binned = false;
     e->primed = false;
+    e->counted = false;
+    e->list_valid = false;
     return MDX_OK;
 }
 
@@ -329,26 +344,49 @@ template <class R> static KArgs<R> make_args(mdx_engine *e)
     a.noise = nullptr;
     a.ctl = (StepCtl *)e->ctl.p;
     a.fguard = e->fguard;
+    a.lmode = 0;
+    a.nbr = (const int *)e->nbr.p;
+    a.nbrw = (int *)e->nbr.p;
+    a.nn = (int *)e->nn.p;
+    a.bcell = (int *)e->bcell.p;
+    a.kmax = e->kmax;
+    const double rl = reach(e) + e->skin;
+    a.rl2 = rl * rl;
+    a.skin = e->skin;
+    a.bmax = (float *)e->bmax.p;
     return a;
 }
 
 // scan + fill + per-cell ordering of the cell list built by the binning of
-// the current slots
-static int finish_bins(mdx_engine *e)
+// the current slots (cells: the binning's cell of each slot; gate: run only
+// when a list rebuild was requested)
+static int finish_bins(mdx_engine *e, const StepCtl *gate = nullptr, const int *cells = nullptr)
 {
     const int n = e->ncells;
     const int nb = (n + 1023) / 1024;
+    if (!cells)
+        cells = (const int *)e->cell[e->cur].p;
     hipLaunchKernelGGL(k_scan1, dim3(nb), dim3(256), 0, e->stream, (const int *)e->count.p,
-                       (int *)e->start.p, (int *)e->bsum.p, n);
+                       (int *)e->start.p, (int *)e->bsum.p, n, gate);
     hipLaunchKernelGGL(k_scan2, dim3(1), dim3(1024), 0, e->stream, (int *)e->bsum.p, nb,
-                       (int *)e->total.p);
+                       (int *)e->total.p, gate);
     hipLaunchKernelGGL(k_scan3, dim3(nb), dim3(256), 0, e->stream, (int *)e->count.p,
-                       (int *)e->start.p, (const int *)e->bsum.p, n, (const int *)e->total.p);
-    hipLaunchKernelGGL(k_fill, dim3(nblk(e->N)), dim3(256), 0, e->stream, e->N,
-                       (const int *)e->cell[e->cur].p, (const int *)e->rank.p,
-                       (const int *)e->start.p, (int *)e->list.p);
+                       (int *)e->start.p, (const int *)e->bsum.p, n, (const int *)e->total.p,
+                       gate);
+    hipLaunchKernelGGL(k_fill, dim3(nblk(e->N)), dim3(256), 0, e->stream, e->N, cells,
+                       (const int *)e->rank.p, (const int *)e->start.p, (int *)e->list.p, gate);
     hipLaunchKernelGGL(k_cellsort, dim3(nblk(n)), dim3(256), 0, e->stream, n,
-                       (const int *)e->start.p, (int *)e->list.p);
+                       (const int *)e->start.p, (int *)e->list.p, gate);
+    HIPCHK(hipGetLastError());
+    return MDX_OK;
+}
+
+// cell counts of the current slots into cells (gate as above)
+template <class R> static int count_current(mdx_engine *e, int *cells, const StepCtl *gate)
+{
+    KArgs<R> a = make_args<R>(e);
+    a.celln = cells;
+    hipLaunchKernelGGL(k_bin<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, a, gate);
     HIPCHK(hipGetLastError());
     return MDX_OK;
 }
@@ -359,20 +397,23 @@ template <class R> static int bin_current(mdx_engine *e)
     int rc = build_grid(e);
     if (rc)
         return rc;
-    KArgs<R> a = make_args<R>(e);
-    a.celln = (int *)e->cell[e->cur].p;
-    hipLaunchKernelGGL(k_bin<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
-    HIPCHK(hipGetLastError());
+    if (!e->counted && (rc = count_current<R>(e, (int *)e->cell[e->cur].p, nullptr)))
+        return rc;
     if ((rc = finish_bins(e)))
         return rc;
+    e->counted = false;
     e->binned = true;
+    // the cell list no longer describes the positions of the neighbour-list build
+    e->list_valid = false;
     return MDX_OK;
 }
 
-template <class R, int MODE> static int launch_gather(mdx_engine *e, const double *noise_dev)
+template <class R, int MODE>
+static int launch_gather(mdx_engine *e, const double *noise_dev, int lmode = 0)
 {
     KArgs<R> a = make_args<R>(e);
     a.noise = noise_dev;
+    a.lmode = lmode;
     if (e->use_split && a.has_patches &&
         (MODE == M_PRIME || MODE == M_STEP || MODE == M_FUSED || MODE == M_CONS)) {
         int rc;
@@ -392,7 +433,7 @@ template <class R, int MODE> static int launch_gather(mdx_engine *e, const doubl
         return MDX_OK;
     }
     e->last_split = false;
-    if (e->use_tiles)
+    if (e->use_tiles && !lmode)
         hipLaunchKernelGGL((k_tile<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
     else
         hipLaunchKernelGGL((k_gather<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
@@ -456,6 +497,8 @@ template <class R> static int do_prime(mdx_engine *e, const double *noise0)
     e->cur ^= 1;
     launch_small(e, k_epoch_inc);
     e->binned = false; // the state moved to cell order; its list is stale
+    e->counted = false;
+    e->list_valid = false;
     unsigned int st;
     if ((rc = read_status(e, &st)))
         return rc;
@@ -463,29 +506,156 @@ template <class R> static int do_prime(mdx_engine *e, const double *noise0)
     return status_to_rc(st);
 }
 
-// enqueue nsteps rotor steps (requires primed F, T)
-template <class R> static int enqueue_rotor(mdx_engine *e, int nsteps, const double *nd)
+static int nbr_alloc(mdx_engine *e)
+{
+    int rc;
+    if ((rc = dalloc(e, e->nbr, 4 * (size_t)e->kmax * e->N)) ||
+        (rc = dalloc(e, e->nn, 4 * (size_t)e->N)) || (rc = dalloc(e, e->bcell, 4 * (size_t)e->N)) ||
+        (rc = dalloc(e, e->bmax, 4 * (size_t)nblk(e->N))))
+        return rc;
+    return MDX_OK;
+}
+
+// after a kernel that moved the particles in list mode (its blocks wrote bmax)
+static int decide(mdx_engine *e)
+{
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(256), 0, e->stream, (StepCtl *)e->ctl.p,
+                       (const float *)e->bmax.p, (int)nblk(e->N), e->skin);
+    HIPCHK(hipGetLastError());
+    return MDX_OK;
+}
+
+// neighbour list of the current slots from the cell list just built
+template <class R> static int build_list(mdx_engine *e, const int *cells, int gated)
+{
+    KArgs<R> a = make_args<R>(e);
+    a.lmode = 1;
+    hipLaunchKernelGGL(k_nbuild<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, a, cells, gated);
+    HIPCHK(hipGetLastError());
+    return MDX_OK;
+}
+
+// optional per-step events: ev[4 s + 0] step start, + 1 after the cell / list
+// work, + 2 after the walk (split form), + 3 step end
+static inline void mark(mdx_engine *e, hipEvent_t *ev, int s, int k)
+{
+    if (ev)
+        (void)hipEventRecord(ev[4 * s + k], e->stream);
+}
+
+// one step of the loop.  MODE: M_FUSED / M_STEP (rotor), M_BEAD.
+// Without the list (skin 0) every step scans the counts its predecessor took
+// and walks the 27 cells; the gather re-sorts the state into cell order.
+// With the list, a re-sort step does the same and then builds the list at
+// the new positions; the steps in between walk the list in place, and a
+// rebuild runs (gated kernels) whenever the displacement bound asks for it.
+template <class R, int MODE>
+static int enqueue_one(mdx_engine *e, const double *nd, hipEvent_t *ev, int s, bool last)
+{
+    int rc;
+    const bool lm = e->skin > 0;
+    mark(e, ev, s, 0);
+    if (!lm) {
+        if (!e->counted && (rc = count_current<R>(e, (int *)e->cell[e->cur].p, nullptr)))
+            return rc;
+        if ((rc = finish_bins(e)))
+            return rc;
+        mark(e, ev, s, 1);
+        e->prof_mid = ev ? ev[4 * s + 2] : nullptr;
+        rc = launch_gather<R, MODE>(e, nd, 0);
+        e->prof_mid = nullptr;
+        if (rc)
+            return rc;
+        e->cur ^= 1;
+        launch_small(e, k_epoch_inc);
+        e->counted = MODE != M_STEP; // FUSED and BEAD bin the new positions
+        e->binned = false;
+    } else if (!e->list_valid || e->since_resort >= e->resort_every) {
+        if ((rc = nbr_alloc(e)))
+            return rc;
+        if (!e->counted && (rc = count_current<R>(e, (int *)e->cell[e->cur].p, nullptr)))
+            return rc;
+        if ((rc = finish_bins(e)))
+            return rc;
+        mark(e, ev, s, 1);
+        e->prof_mid = ev ? ev[4 * s + 2] : nullptr;
+        rc = launch_gather<R, MODE>(e, nd, 0);
+        e->prof_mid = nullptr;
+        if (rc)
+            return rc;
+        e->cur ^= 1;
+        launch_small(e, k_epoch_inc);
+        if (MODE != M_STEP) {
+            // the gather binned the new positions: cell list, then the list
+            if ((rc = finish_bins(e)))
+                return rc;
+            if ((rc = build_list<R>(e, (const int *)e->cell[e->cur].p, 0)))
+                return rc;
+            launch_small(e, k_list_reset);
+            e->list_valid = true;
+            e->since_resort = 0;
+            e->binned = MODE == M_BEAD;
+        } else {
+            e->list_valid = false; // re-sorted, list not rebuilt
+        }
+        e->counted = false;
+    } else {
+        const StepCtl *gate = (const StepCtl *)e->ctl.p;
+        if ((rc = count_current<R>(e, (int *)e->bcell.p, gate)))
+            return rc;
+        if ((rc = finish_bins(e, gate, (const int *)e->bcell.p)))
+            return rc;
+        if ((rc = build_list<R>(e, (const int *)e->bcell.p, 1)))
+            return rc;
+        mark(e, ev, s, 1);
+        e->prof_mid = ev ? ev[4 * s + 2] : nullptr;
+        rc = launch_gather<R, MODE>(e, nd, 1);
+        e->prof_mid = nullptr;
+        if (rc)
+            return rc;
+        if (MODE != M_STEP && (rc = decide(e)))
+            return rc;
+        e->cur ^= 1;
+        launch_small(e, k_epoch_inc);
+        e->since_resort++;
+        e->binned = false;
+        e->counted = false;
+    }
+    if (ev && !e->last_split)
+        mark(e, ev, s, 2);
+    mark(e, ev, s, 3);
+    (void)last;
+    return MDX_OK;
+}
+
+// enqueue nsteps rotor steps (requires primed F, T); ev: see mark()
+template <class R> static int enqueue_rotor(mdx_engine *e, int nsteps, const double *nd,
+                                            hipEvent_t *ev = nullptr, hipEvent_t *kev = nullptr)
 {
     int rc;
     launch_small(e, k_set_call_base);
     {
         KArgs<R> a = make_args<R>(e);
+        // with a live list the drift counts toward its displacement bound;
+        // otherwise the new positions are binned for the next step
+        a.lmode = e->skin > 0 && e->list_valid;
+        if (kev)
+            HIPCHK(hipEventRecord(kev[0], e->stream));
         hipLaunchKernelGGL(k_kick_drift<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
         HIPCHK(hipGetLastError());
+        if (a.lmode && (rc = decide(e)))
+            return rc;
+        if (kev)
+            HIPCHK(hipEventRecord(kev[1], e->stream));
         e->cur ^= 1;
+        e->counted = !a.lmode;
     }
     for (int s = 0; s < nsteps; s++) {
-        if ((rc = finish_bins(e)))
+        const bool last = s + 1 == nsteps;
+        rc = last ? enqueue_one<R, M_STEP>(e, nd, ev, s, last)
+                  : enqueue_one<R, M_FUSED>(e, nd, ev, s, last);
+        if (rc)
             return rc;
-        if (s + 1 < nsteps) {
-            if ((rc = launch_gather<R, M_FUSED>(e, nd)))
-                return rc;
-        } else {
-            if ((rc = launch_gather<R, M_STEP>(e, nd)))
-                return rc;
-        }
-        e->cur ^= 1;
-        launch_small(e, k_epoch_inc);
     }
     e->binned = false;
     e->steps_taken += nsteps;
@@ -508,25 +678,28 @@ template <class R> static int do_step_rotor(mdx_engine *e, int nsteps, const dou
     return status_to_rc(st);
 }
 
+template <class R> static int enqueue_bead(mdx_engine *e, int nsteps, const double *nd,
+                                           hipEvent_t *ev = nullptr)
+{
+    int rc;
+    if ((rc = build_grid(e)))
+        return rc;
+    launch_small(e, k_set_call_base);
+    for (int s = 0; s < nsteps; s++)
+        if ((rc = enqueue_one<R, M_BEAD>(e, nd, ev, s, s + 1 == nsteps)))
+            return rc;
+    e->steps_taken += nsteps;
+    return MDX_OK;
+}
+
 template <class R> static int do_step_bead(mdx_engine *e, int nsteps, const double *noise)
 {
     int rc;
-    if (!e->binned && (rc = bin_current<R>(e)))
-        return rc;
     const double *nd;
     if ((rc = noise_upload(e, noise, (size_t)nsteps * e->N * 3, &nd)))
         return rc;
-    launch_small(e, k_set_call_base);
-    for (int s = 0; s < nsteps; s++) {
-        if ((rc = launch_gather<R, M_BEAD>(e, nd)))
-            return rc;
-        e->cur ^= 1;
-        launch_small(e, k_epoch_inc);
-        if ((rc = finish_bins(e)))
-            return rc;
-    }
-    e->binned = true;
-    e->steps_taken += nsteps;
+    if ((rc = enqueue_bead<R>(e, nsteps, nd)))
+        return rc;
     unsigned int st;
     if ((rc = read_status(e, &st)))
         return rc;
@@ -640,6 +813,12 @@ int mdx_create(const mdx_desc *d, mdx_engine **out)
         e->use_tiles = std::atoi(t) != 0;
     if (const char *t = std::getenv("MDX_SPLIT"))
         e->use_split = std::atoi(t) != 0;
+    if (const char *t = std::getenv("MDX_SKIN"))
+        e->skin = std::max(0.0, std::atof(t));
+    if (const char *t = std::getenv("MDX_KMAX"))
+        e->kmax = std::max(1, std::atoi(t));
+    if (const char *t = std::getenv("MDX_RESORT"))
+        e->resort_every = std::max(1, std::atoi(t));
     long long f = 0;
     for (int r = 0; r < e->nrep; r++) {
         e->first.push_back((int)f);
@@ -695,7 +874,8 @@ void mdx_destroy(mdx_engine *e)
                    &e->ang_ijk, &e->ang_k,   &e->ang_t0, &e->exc_off, &e->exc_j,  &e->act_prev,
                    &e->act_next, &e->noise,  &e->ctl,    &e->partial, &e->tmpd,   &e->tmpi,
                    &e->tmpi2,   &e->tmpi3,   &e->tmpll,  &e->tmpx,    &e->Fw,
-                   &e->ncand,   &e->cand};
+                   &e->ncand,   &e->cand,    &e->nbr,    &e->nn,     &e->bcell,
+                   &e->bmax};
     for (Buf *b : bufs)
         dfree(*b);
     for (int b = 0; b < 2; b++) {
@@ -765,6 +945,8 @@ int mdx_upload_state(mdx_engine *e, const double *x, const double *p, const doub
                                       : upload_state_t<float>(e, x, p, Q, L);
     e->binned = false;
     e->primed = false;
+    e->counted = false;
+    e->list_valid = false;
     return rc;
 }
 
@@ -1333,115 +1515,83 @@ int mdx_info(mdx_engine *e, double *out)
     out[2] = e->reps[0].l[0] / e->reps[0].nc[0];
     out[3] = (double)e->bytes_held;
     out[4] = (double)e->steps_taken;
+    StepCtl c;
+    HIPCHK(hipMemcpyAsync(&c, e->ctl.p, sizeof c, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    out[5] = (double)c.nbuilds;
+    out[6] = e->skin;
+    out[7] = e->kmax;
     return MDX_OK;
 }
 
-// per-kernel device times of nsteps steps launched eagerly with HIP events
-// on the engine's stream; the state advances by nsteps (+1 rotor) steps
+int mdx_set_neighbor_list(mdx_engine *e, double skin, int kmax, int resort_every)
+{
+    if (!e || !(skin >= 0) || kmax < 1 || resort_every < 1)
+        return set_err(MDX_ERR_ARG, "skin >= 0, kmax >= 1 and resort_every >= 1 required");
+    HIPCHK(hipSetDevice(e->device));
+    if (skin != e->skin)
+        e->grid_dirty = true; // cells hold the interaction reach plus the skin
+    e->skin = skin;
+    e->kmax = kmax;
+    e->resort_every = resort_every;
+    e->list_valid = false;
+    return MDX_OK;
+}
+
+// per-phase device times of nsteps steps launched eagerly with HIP events on
+// the engine's stream (the same launches as mdx_step); the state advances by
+// nsteps steps
 template <class R>
 static int profile_t(mdx_engine *e, int nsteps, int cap, const char **names, double *ms, int *nk)
 {
-    static const char *kn[] = {"gather", "scan", "fill", "cellsort", "kick_drift", "walk", "finish"};
-    const int K = 7;
-    double acc[K] = {0, 0, 0, 0, 0, 0, 0};
-    int cntk[K] = {0, 0, 0, 0, 0, 0, 0};
-    hipEvent_t ev[6];
-    for (int k = 0; k < 6; k++)
-        HIPCHK(hipEventCreate(&ev[k]));
     int rc;
     if (e->integ == MDX_INTEG_ROTOR && !e->primed && (rc = do_prime<R>(e, nullptr)))
         return rc;
-    if (e->integ == MDX_INTEG_BEAD && !e->binned && (rc = bin_current<R>(e)))
+    std::vector<hipEvent_t> ev(4 * (size_t)nsteps + 2);
+    for (auto &x : ev)
+        HIPCHK(hipEventCreate(&x));
+    if (e->integ == MDX_INTEG_ROTOR)
+        rc = enqueue_rotor<R>(e, nsteps, nullptr, ev.data() + 2, ev.data());
+    else
+        rc = enqueue_bead<R>(e, nsteps, nullptr, ev.data() + 2);
+    if (rc)
         return rc;
-    launch_small(e, k_set_call_base);
-    if (e->integ == MDX_INTEG_ROTOR) {
-        KArgs<R> a = make_args<R>(e);
-        HIPCHK(hipEventRecord(ev[0], e->stream));
-        hipLaunchKernelGGL(k_kick_drift<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
-        HIPCHK(hipEventRecord(ev[1], e->stream));
-        HIPCHK(hipEventSynchronize(ev[1]));
-        float t;
-        HIPCHK(hipEventElapsedTime(&t, ev[0], ev[1]));
-        acc[4] += t;
-        cntk[4]++;
-        e->cur ^= 1;
-    }
-    const int n = e->ncells;
-    const int nb = (n + 1023) / 1024;
-    for (int s = 0; s < nsteps; s++) {
-        HIPCHK(hipEventRecord(ev[0], e->stream));
-        hipLaunchKernelGGL(k_scan1, dim3(nb), dim3(256), 0, e->stream, (const int *)e->count.p,
-                           (int *)e->start.p, (int *)e->bsum.p, n);
-        hipLaunchKernelGGL(k_scan2, dim3(1), dim3(1024), 0, e->stream, (int *)e->bsum.p, nb,
-                           (int *)e->total.p);
-        hipLaunchKernelGGL(k_scan3, dim3(nb), dim3(256), 0, e->stream, (int *)e->count.p,
-                           (int *)e->start.p, (const int *)e->bsum.p, n, (const int *)e->total.p);
-        HIPCHK(hipEventRecord(ev[1], e->stream));
-        hipLaunchKernelGGL(k_fill, dim3(nblk(e->N)), dim3(256), 0, e->stream, e->N,
-                           (const int *)e->cell[e->cur].p, (const int *)e->rank.p,
-                           (const int *)e->start.p, (int *)e->list.p);
-        HIPCHK(hipEventRecord(ev[2], e->stream));
-        hipLaunchKernelGGL(k_cellsort, dim3(nblk(n)), dim3(256), 0, e->stream, n,
-                           (const int *)e->start.p, (int *)e->list.p);
-        HIPCHK(hipEventRecord(ev[3], e->stream));
-        // ev[5] is recorded between the walk and finish kernels of the split form
-        e->prof_mid = ev[5];
-        if (e->integ == MDX_INTEG_ROTOR)
-            rc = launch_gather<R, M_FUSED>(e, nullptr);
-        else
-            rc = launch_gather<R, M_BEAD>(e, nullptr);
-        e->prof_mid = nullptr;
-        if (rc)
-            return rc;
-        HIPCHK(hipEventRecord(ev[4], e->stream));
-        e->cur ^= 1;
-        launch_small(e, k_epoch_inc);
-        HIPCHK(hipEventSynchronize(ev[4]));
-        float t;
-        HIPCHK(hipEventElapsedTime(&t, ev[0], ev[1]));
-        acc[1] += t;
-        HIPCHK(hipEventElapsedTime(&t, ev[1], ev[2]));
-        acc[2] += t;
-        HIPCHK(hipEventElapsedTime(&t, ev[2], ev[3]));
-        acc[3] += t;
-        HIPCHK(hipEventElapsedTime(&t, ev[3], ev[4]));
-        acc[0] += t;
-        if (e->last_split) {
-            float tw;
-            HIPCHK(hipEventElapsedTime(&tw, ev[3], ev[5]));
-            acc[5] += tw;
-            acc[6] += t - tw;
-            cntk[5]++;
-            cntk[6]++;
-        }
-        cntk[0]++;
-        cntk[1]++;
-        cntk[2]++;
-        cntk[3]++;
-    }
-    // leave the state at a step boundary: bin, then a plain (non-fused) step
-    if ((rc = finish_bins(e)))
-        return rc;
-    if (e->integ == MDX_INTEG_ROTOR) {
-        if ((rc = launch_gather<R, M_STEP>(e, nullptr)))
-            return rc;
-        e->cur ^= 1;
-        launch_small(e, k_epoch_inc);
-        e->binned = false;
-    } else {
-        e->binned = true;
-    }
     HIPCHK(hipStreamSynchronize(e->stream));
-    e->steps_taken += nsteps + (e->integ == MDX_INTEG_ROTOR ? 1 : 0);
-    for (int k = 0; k < 6; k++)
-        (void)hipEventDestroy(ev[k]);
+    static const char *kn[] = {"bins", "walk", "finish", "gather", "step", "kick_drift"};
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    const hipEvent_t *se = ev.data() + 2;
+    for (int s = 0; s < nsteps; s++) {
+        float t01, t12, t23, t03;
+        HIPCHK(hipEventElapsedTime(&t01, se[4 * s], se[4 * s + 1]));
+        HIPCHK(hipEventElapsedTime(&t12, se[4 * s + 1], se[4 * s + 2]));
+        HIPCHK(hipEventElapsedTime(&t23, se[4 * s + 2], se[4 * s + 3]));
+        HIPCHK(hipEventElapsedTime(&t03, se[4 * s], se[4 * s + 3]));
+        acc[0] += t01;
+        acc[1] += t12;
+        acc[2] += t23;
+        acc[3] += t12 + t23;
+        acc[4] += t03;
+    }
+    int K = 5;
+    if (e->integ == MDX_INTEG_ROTOR) {
+        float t;
+        HIPCHK(hipEventElapsedTime(&t, ev[0], ev[1]));
+        acc[5] = t * nsteps; // reported once, not averaged
+        K = 6;
+    }
+    for (auto &x : ev)
+        (void)hipEventDestroy(x);
     int w = 0;
     for (int k = 0; k < K && w < cap; k++) {
-        if (!cntk[k])
+        if (k == 2 && !e->last_split)
+            continue; // no separate finish kernel
+        if (k == 1 && !e->last_split) {
+            names[w] = "gather";
+            ms[w++] = acc[3] / nsteps;
             continue;
+        }
         names[w] = kn[k];
-        ms[w] = acc[k] / cntk[k];
-        w++;
+        ms[w++] = acc[k] / nsteps;
     }
     *nk = w;
     unsigned int st;

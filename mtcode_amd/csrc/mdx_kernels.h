@@ -40,6 +40,10 @@ struct StepCtl {
     unsigned long long call_base; // epoch at the start of the current call
     unsigned int status;          // bit0 force guard, bit1 non-finite
     unsigned int pad;
+    // neighbour list: rebuild flag, displacement bound since the last build
+    unsigned int rebuild; // 1: the next step rebuilds the list before its walk
+    unsigned int nbuilds; // list builds so far
+    double D;             // sum over steps of the max drift since the last build
 };
 
 enum { M_CONS = 0, M_PRIME = 1, M_STEP = 2, M_FUSED = 3, M_BEAD = 4, M_ENERGY = 5 };
@@ -68,6 +72,14 @@ template <class R> struct KArgs {
     // split walk -> finish: WCA force and patch candidates per cell-list position
     R *Fw;
     int *ncand, *cand; // cand[q * N + s], q < PMAXG
+    // neighbour list (lmode): slot s walks nbr[k * N + s], k < nn[s] (<= kmax),
+    // else the 27 cells of its cell at the last build, bcell[s]
+    int lmode;
+    const int *nbr;
+    int *nbrw, *nn, *bcell;
+    int kmax;
+    double rl2, skin;
+    float *bmax; // per block: max drift of the block's particles
     // WCA (HSPotential)
     int wca;
     double wca_eps, wca_s6, wca_s12, wca_pl;
@@ -267,9 +279,10 @@ enum { PH_ALL = 0, PH_WALK = 1, PH_FINISH = 2 };
 // neighbours of a row are contiguous in the cell list (one range, or two at
 // the periodic seam; fewer cells when a side has fewer than 3)
 template <class R, class Fn>
-__device__ __forceinline__ void for_each_neighbor(const KArgs<R> &a, const DevRep &rp, int i, Fn &&fn)
+__device__ __forceinline__ void for_each_in_cells(const KArgs<R> &a, const DevRep &rp, int cabs, int i,
+                                                  Fn &&fn)
 {
-    const int c = a.cell[i] - rp.coff;
+    const int c = cabs - rp.coff;
     const int n0 = rp.nc[0], n1 = rp.nc[1], n2 = rp.nc[2];
     const int cx = c % n0, cy = (c / n0) % n1, cz = c / (n0 * n1);
     const int ylo = n1 >= 3 ? -1 : 0, yhi = n1 >= 2 ? 1 : 0;
@@ -312,6 +325,55 @@ __device__ __forceinline__ void for_each_neighbor(const KArgs<R> &a, const DevRe
                     fn(a.X[j], j);
             }
         }
+    }
+}
+
+template <class R, class Fn>
+__device__ __forceinline__ void for_each_neighbor(const KArgs<R> &a, const DevRep &rp, int i, Fn &&fn)
+{
+    for_each_in_cells(a, rp, a.cell[i], i, fn);
+}
+
+// neighbours of slot i from the list (order of the build's cell walk), or,
+// for a particle with more than kmax listed, the 27 cells of the build.  Both
+// are supersets of the pairs within reach while the displacement bound holds.
+template <class R, class Fn>
+__device__ __forceinline__ void for_each_listed(const KArgs<R> &a, const DevRep &rp, int i, Fn &&fn)
+{
+    const int n = a.nn[i];
+    if (n <= a.kmax) {
+        for (int k = 0; k < n; k++) {
+            const int j = a.nbr[(size_t)k * a.N + i];
+            fn(a.X[j], j);
+        }
+    } else {
+        for_each_in_cells(a, rp, a.bcell[i], i, fn);
+    }
+}
+
+template <class R, class Fn>
+__device__ __forceinline__ void for_each_near(const KArgs<R> &a, const DevRep &rp, int i, Fn &&fn)
+{
+    if (a.lmode)
+        for_each_listed(a, rp, i, fn);
+    else
+        for_each_neighbor(a, rp, i, fn);
+}
+
+// block max of the drift into bmax[block]; k_decide reduces them
+template <class R> __device__ __forceinline__ void drift_bound(const KArgs<R> &a, float d)
+{
+    __shared__ float wmax[4];
+    for (int o = 32; o > 0; o >>= 1)
+        d = fmaxf(d, __shfl_xor(d, o));
+    if ((threadIdx.x & 63) == 0)
+        wmax[threadIdx.x >> 6] = d;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = 0.f;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++)
+            m = fmaxf(m, wmax[w]);
+        a.bmax[blockIdx.x] = m;
     }
 }
 
@@ -437,10 +499,11 @@ __device__ __forceinline__ void patch_pair(const KArgs<R> &a, const DevRep &rp, 
 // patches from those candidates, topology and the integrator.  The split
 // keeps the register-heavy patch code out of the neighbour walk.
 template <class R, int MODE, int PH, class Trav>
-__device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *plist, Trav &&trav)
+__device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *plist, Trav &&trav,
+                                                 float &drift)
 {
     constexpr bool FP64 = sizeof(R) == 8;
-    const int i = a.list[s];
+    const int i = a.lmode ? s : a.list[s];
     const PosRec xi = a.X[i];
     const uint32_t oi = xi.id;
     const int ti = xi.meta & 255;
@@ -453,8 +516,9 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
         acc.vir[k] = 0;
 
     R qi[9];
-    const int npi = a.has_patches ? a.np[ti] : 0;
-    if (a.rotor) {
+    // the bead integrator carries no orientations: no patch code in its kernels
+    const int npi = (MODE != M_BEAD && a.has_patches) ? a.np[ti] : 0;
+    if (MODE != M_BEAD && a.rotor) {
 #pragma unroll
         for (int k = 0; k < 9; k++)
             qi[k] = a.Q[9 * (size_t)i + k];
@@ -696,13 +760,17 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
         const R c4 = (R)a.b_c2 * rr;
         double xn[3] = {xi.x, xi.y, xi.z};
         R pk[3];
+        double dd = 0;
         for (int k = 0; k < 3; k++) {
             pk[k] = a.p[3 * (size_t)i + k];
             pk[k] = (R)a.b_c5c2 * pk[k] + (R)a.b_c5c3q * Fi[k] + (c5 * c4 + rr) * nz[k];
+            double step;
             if constexpr (FP64)
-                xn[k] = xn[k] + a.c1 * pk[k];
+                step = a.c1 * pk[k];
             else
-                xn[k] = xn[k] + (double)((R)a.c1 * pk[k]);
+                step = (double)((R)a.c1 * pk[k]);
+            xn[k] = xn[k] + step;
+            dd += step * step;
             wrap1(xn[k], pk[k], rp.l[k], a.periodic);
             a.pn[3 * (size_t)s + k] = pk[k];
         }
@@ -711,7 +779,10 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
         o.y = xn[1];
         o.z = xn[2];
         a.Xn[s] = o;
-        bin_particle(a, s, rp, xn, a.celln);
+        if (a.lmode)
+            drift = (float)sqrt(dd);
+        else
+            bin_particle(a, s, rp, xn, a.celln);
         return;
     }
 
@@ -770,11 +841,15 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
         ll[k] += hdt * Ti[k];
     }
     double xn[3] = {xi.x, xi.y, xi.z};
+    double dd = 0;
     for (int k = 0; k < 3; k++) {
+        double step;
         if constexpr (FP64)
-            xn[k] = xn[k] + a.c1 * pl[k];
+            step = a.c1 * pl[k];
         else
-            xn[k] = xn[k] + (double)((R)a.c1 * pl[k]);
+            step = (double)((R)a.c1 * pl[k]);
+        xn[k] = xn[k] + step;
+        dd += step * step;
         wrap1(xn[k], pl[k], rp.l[k], a.periodic);
     }
     const R Ir[3] = {(R)a.I[0], (R)a.I[1], (R)a.I[2]};
@@ -792,41 +867,90 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
     o.y = xn[1];
     o.z = xn[2];
     a.Xn[s] = o;
-    bin_particle(a, s, rp, xn, a.celln);
+    if (a.lmode)
+        drift = (float)sqrt(dd);
+    else
+        bin_particle(a, s, rp, xn, a.celln);
 }
 
 // one thread per cell-list position, neighbours read from global memory
+// the first kernel after a list build clears the request and the bound
+template <class R> __device__ __forceinline__ void list_built(const KArgs<R> &a)
+{
+    if (a.lmode && blockIdx.x == 0 && threadIdx.x == 0 && a.ctl->rebuild) {
+        a.ctl->rebuild = 0u;
+        a.ctl->D = 0.0;
+    }
+}
+
+// one thread per cell-list position (list mode: per slot)
 template <class R, int MODE> __global__ void __launch_bounds__(256) k_gather(KArgs<R> a)
 {
     __shared__ int plist[PMAX * 256];
+    list_built(a);
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= a.N)
-        return;
-    process_particle<R, MODE, PH_ALL>(a, s, plist, [&](int i, const DevRep &rp, auto &&fn) {
-        for_each_neighbor(a, rp, i, fn);
-    });
+    float drift = 0.f;
+    if (s < a.N)
+        process_particle<R, MODE, PH_ALL>(
+            a, s, plist, [&](int i, const DevRep &rp, auto &&fn) { for_each_near(a, rp, i, fn); },
+            drift);
+    if ((MODE == M_FUSED || MODE == M_BEAD) && a.lmode)
+        drift_bound(a, drift);
 }
 
 // split form: the neighbour walk ...
 template <class R, int MODE> __global__ void __launch_bounds__(256) k_walk(KArgs<R> a)
 {
+    list_built(a);
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.N)
         return;
-    process_particle<R, MODE, PH_WALK>(a, s, nullptr, [&](int i, const DevRep &rp, auto &&fn) {
-        for_each_neighbor(a, rp, i, fn);
-    });
+    float drift = 0.f;
+    process_particle<R, MODE, PH_WALK>(
+        a, s, nullptr, [&](int i, const DevRep &rp, auto &&fn) { for_each_near(a, rp, i, fn); },
+        drift);
 }
 
 // ... then the patch pairs, topology and the integrator
 template <class R, int MODE> __global__ void __launch_bounds__(256) k_finish(KArgs<R> a)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    float drift = 0.f;
+    if (s < a.N)
+        process_particle<R, MODE, PH_FINISH>(
+            a, s, nullptr, [&](int i, const DevRep &rp, auto &&fn) { for_each_near(a, rp, i, fn); },
+            drift);
+    if ((MODE == M_FUSED || MODE == M_BEAD) && a.lmode)
+        drift_bound(a, drift);
+}
+
+// neighbour list of every slot from the cell list of the current positions:
+// the slots within sqrt(rl2) in the order of the 27-cell walk.  gated: only
+// when a rebuild was requested.
+template <class R> __global__ void __launch_bounds__(256) k_nbuild(KArgs<R> a, const int *cells, int gated)
+{
+    if (gated && !a.ctl->rebuild)
+        return;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        atomicAdd(&a.ctl->nbuilds, 1u);
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.N)
         return;
-    process_particle<R, MODE, PH_FINISH>(a, s, nullptr, [&](int i, const DevRep &rp, auto &&fn) {
-        for_each_neighbor(a, rp, i, fn);
+    const PosRec xi = a.X[s];
+    const DevRep rp = a.reps[xi.meta >> 8];
+    const int c = cells[s];
+    a.bcell[s] = c;
+    int n = 0;
+    for_each_in_cells(a, rp, c, s, [&](const PosRec &xj, int j) {
+        double d[3];
+        min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
+        if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < a.rl2) {
+            if (n < a.kmax)
+                a.nbrw[(size_t)n * a.N + s] = j;
+            n++;
+        }
     });
+    a.nn[s] = n;
 }
 
 // ------------------------------------------------------- LDS-staged tiles
@@ -927,6 +1051,7 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_tile(KArgs
         if (s >= s1)
             return;
         const int ya = tinfo[1], nr = tinfo[2];
+        float drift_unused = 0.f;
         process_particle<R, MODE, PH_ALL>(a, s, plist, [&](int i, const DevRep &rq, auto &&fn) {
             const uint32_t self = a.X[i].id;
             const int c = a.cell[i] - rq.coff;
@@ -960,25 +1085,23 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_tile(KArgs
                             fn(lpos[q], -1);
                 }
             }
-        });
+        }, drift_unused);
         return;
     }
     const int s = s0 + threadIdx.x;
     if (s >= s1)
         return;
+    float drift_unused = 0.f;
     process_particle<R, MODE, PH_ALL>(a, s, plist, [&](int i, const DevRep &rp, auto &&fn) {
         for_each_neighbor(a, rp, i, fn);
-    });
+    }, drift_unused);
 }
 
 // first half of a rotor step from the carried F, T (no reordering): half kick,
 // drift + wrap (Langevin.cpp:205-223), rotate (LangevinR.cpp:687-695), binning
-template <class R> __global__ void __launch_bounds__(256) k_kick_drift(KArgs<R> a)
+template <class R> __device__ __forceinline__ float kick_drift_one(const KArgs<R> &a, int i)
 {
     constexpr bool FP64 = sizeof(R) == 8;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.N)
-        return;
     PosRec xr = a.X[i];
     const DevRep rp = a.reps[xr.meta >> 8];
     const R hdt = (R)a.hdt;
@@ -990,11 +1113,15 @@ template <class R> __global__ void __launch_bounds__(256) k_kick_drift(KArgs<R> 
     }
     for (int k = 0; k < 9; k++)
         q[k] = a.Q[9 * (size_t)i + k];
+    double dd = 0;
     for (int k = 0; k < 3; k++) {
+        double step;
         if constexpr (FP64)
-            xn[k] = xn[k] + a.c1 * pl[k];
+            step = a.c1 * pl[k];
         else
-            xn[k] = xn[k] + (double)((R)a.c1 * pl[k]);
+            step = (double)((R)a.c1 * pl[k]);
+        xn[k] = xn[k] + step;
+        dd += step * step;
         wrap1(xn[k], pl[k], rp.l[k], a.periodic);
     }
     const R Ir[3] = {(R)a.I[0], (R)a.I[1], (R)a.I[2]};
@@ -1012,12 +1139,26 @@ template <class R> __global__ void __launch_bounds__(256) k_kick_drift(KArgs<R> 
     xr.z = xn[2];
     a.Xn[i] = xr;
     a.slotn[xr.id] = i;
-    bin_particle(a, i, rp, xn, a.celln);
+    if (!a.lmode)
+        bin_particle(a, i, rp, xn, a.celln);
+    return (float)sqrt(dd);
+}
+
+template <class R> __global__ void __launch_bounds__(256) k_kick_drift(KArgs<R> a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float drift = 0.f;
+    if (i < a.N)
+        drift = kick_drift_one(a, i);
+    if (a.lmode)
+        drift_bound(a, drift);
 }
 
 // binning of the current positions in place (after an upload or a reorder)
-template <class R> __global__ void __launch_bounds__(256) k_bin(KArgs<R> a)
+template <class R> __global__ void __launch_bounds__(256) k_bin(KArgs<R> a, const StepCtl *gate)
 {
+    if (gate && !gate->rebuild)
+        return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.N)
         return;
@@ -1030,8 +1171,12 @@ template <class R> __global__ void __launch_bounds__(256) k_bin(KArgs<R> a)
 // --------------------------------------------------------------- cell lists
 // exclusive scan of count[0..n) into start[0..n], counts zeroed after use.
 // phase 1: per block (1024 elements) local scan and block total
-__global__ void __launch_bounds__(256) k_scan1(const int *count, int *start, int *bsum, int n)
+// gate: when non-null, each kernel runs only if a list rebuild was requested
+__global__ void __launch_bounds__(256) k_scan1(const int *count, int *start, int *bsum, int n,
+                                               const StepCtl *gate)
 {
+    if (gate && !gate->rebuild)
+        return;
     __shared__ int sh[256];
     const int base = blockIdx.x * 1024 + threadIdx.x * 4;
     int v[4], s = 0;
@@ -1058,8 +1203,10 @@ __global__ void __launch_bounds__(256) k_scan1(const int *count, int *start, int
 }
 
 // phase 2: exclusive scan of the block totals (one block)
-__global__ void __launch_bounds__(1024) k_scan2(int *bsum, int nb, int *total)
+__global__ void __launch_bounds__(1024) k_scan2(int *bsum, int nb, int *total, const StepCtl *gate)
 {
+    if (gate && !gate->rebuild)
+        return;
     __shared__ int sh[1024];
     const int per = (nb + 1023) / 1024;
     const int b0 = threadIdx.x * per;
@@ -1088,8 +1235,10 @@ __global__ void __launch_bounds__(1024) k_scan2(int *bsum, int nb, int *total)
 
 // phase 3: add block offsets, zero the counts, close the array
 __global__ void __launch_bounds__(256) k_scan3(int *count, int *start, const int *bsum, int n,
-                                               const int *total)
+                                               const int *total, const StepCtl *gate)
 {
+    if (gate && !gate->rebuild)
+        return;
     const int base = blockIdx.x * 1024 + threadIdx.x * 4;
     const int off = bsum[blockIdx.x];
     for (int k = 0; k < 4; k++)
@@ -1102,8 +1251,10 @@ __global__ void __launch_bounds__(256) k_scan3(int *count, int *start, const int
 }
 
 __global__ void __launch_bounds__(256) k_fill(int N, const int *cell, const int *rank,
-                                              const int *start, int *list)
+                                              const int *start, int *list, const StepCtl *gate)
 {
+    if (gate && !gate->rebuild)
+        return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < N)
         list[start[cell[i]] + rank[i]] = i;
@@ -1111,8 +1262,11 @@ __global__ void __launch_bounds__(256) k_fill(int N, const int *cell, const int 
 
 // ascending slot order inside each cell: makes the gather order, and so the
 // floating-point sums, independent of atomic arrival order
-__global__ void __launch_bounds__(256) k_cellsort(int ncells, const int *start, int *list)
+__global__ void __launch_bounds__(256) k_cellsort(int ncells, const int *start, int *list,
+                                                  const StepCtl *gate)
 {
+    if (gate && !gate->rebuild)
+        return;
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncells)
         return;
@@ -1132,6 +1286,39 @@ __global__ void k_epoch_inc(StepCtl *ctl)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0)
         ctl->epoch += 1;
+}
+
+// a list was just built at the current positions: no rebuild pending, bound 0
+__global__ void k_list_reset(StepCtl *ctl)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctl->rebuild = 0u;
+        ctl->D = 0.0;
+    }
+}
+
+// the grid max of the block drifts of the kernel before is added to the
+// displacement bound D; a list rebuild is requested once 2 D reaches the skin
+// (a pair within reach now was within reach + skin at the build)
+__global__ void __launch_bounds__(256) k_decide(StepCtl *ctl, const float *bmax, int nb, double skin)
+{
+    __shared__ float wmax[4];
+    float m = 0.f;
+    for (int b = threadIdx.x; b < nb; b += 256)
+        m = fmaxf(m, bmax[b]);
+    for (int o = 32; o > 0; o >>= 1)
+        m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0)
+        wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+        // margin for the rounding of the drift and of the FP64 position update
+        const double D = ctl->D + (double)m * (1 + 1e-6) + 1e-9;
+        ctl->D = D;
+        if (2 * D >= skin)
+            ctl->rebuild = 1u;
+    }
 }
 
 __global__ void k_set_call_base(StepCtl *ctl)

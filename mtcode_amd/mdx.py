@@ -72,6 +72,7 @@ def lib():
     L.mdx_set_exclusions.argtypes = [C.c_void_p, C.c_longlong, _ip]
     L.mdx_set_active.argtypes = [C.c_void_p, C.c_double, _ip, _ip]
     L.mdx_set_force_guard.argtypes = [C.c_void_p, C.c_double]
+    L.mdx_set_neighbor_list.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int]
     L.mdx_cell_assign.argtypes = [C.c_void_p, C.c_int, _ip, _ip]
     L.mdx_neighbor_pairs.argtypes = [C.c_void_p, C.c_int, _ip, C.c_double, _ip, C.c_longlong,
                                      C.POINTER(C.c_longlong)]
@@ -268,10 +269,14 @@ class Engine:
         return {names[k].decode(): float(ms[k]) for k in range(nk.value)}
 
     def info(self):
-        out = np.zeros(5)
+        out = np.zeros(8)
         _check(lib().mdx_info(self.h, out.ctypes.data_as(_dp)))
         return {"N": int(out[0]), "cells": int(out[1]), "cell_size": out[2],
-                "device_bytes": int(out[3]), "steps": int(out[4])}
+                "device_bytes": int(out[3]), "steps": int(out[4]), "list_builds": int(out[5]),
+                "skin": float(out[6]), "kmax": int(out[7])}
+
+    def set_neighbor_list(self, skin=0.5, kmax=64, resort_every=200):
+        _check(lib().mdx_set_neighbor_list(self.h, C.c_double(skin), int(kmax), int(resort_every)))
 
 
 def engine_from_system(s, precision=PREC_FP64, seed=0, device=0, replicas=None):

@@ -337,36 +337,118 @@ def active_polymer(nchains: int = 8192, nbeads: int = 256, phi: float = 0.15, se
     return s
 
 
+class _BeadGrid:
+    """Spatial hash of placed beads (cubic cells of edge >= rmin, K slots per
+    cell) for the distance test of chain growth in a periodic cube."""
+
+    K = 8  # points >= rmin apart in a cell of edge < 2 rmin: at most 8
+
+    def __init__(self, L: float, rmin: float, cap: int):
+        self.L, self.rmin = float(L), float(rmin)
+        self.nc = max(3, min(int(L // rmin), 512))
+        if L / self.nc >= 2 * rmin:
+            self.nc = int(np.ceil(L / (1.999 * rmin)))
+        self.slots = np.full((self.nc ** 3, self.K), -1, dtype=np.int32)
+        self.fill = np.zeros(self.nc ** 3, dtype=np.int8)
+        self.pts = np.zeros((cap, 3))
+        self.n = 0
+
+    def _cell3(self, p):
+        return np.minimum((p / (self.L / self.nc)).astype(np.int64), self.nc - 1)
+
+    def clear_of(self, p):
+        """True where p (m x 3, in [0, L)) is at least rmin from every placed bead"""
+        nc, L = self.nc, self.L
+        c3 = self._cell3(p)
+        off = np.array([(i, j, k) for i in (-1, 0, 1) for j in (-1, 0, 1) for k in (-1, 0, 1)])
+        nb = (c3[:, None, :] + off[None]) % nc
+        cid = (nb[..., 0] * nc + nb[..., 1]) * nc + nb[..., 2]
+        ids = self.slots[cid].reshape(len(p), -1)
+        valid = ids >= 0
+        q = self.pts[np.where(valid, ids, 0)]
+        d = p[:, None, :] - q
+        d -= L * np.round(d / L)
+        r2 = np.einsum("mkj,mkj->mk", d, d)
+        return ~np.any(valid & (r2 < self.rmin ** 2), axis=1)
+
+    def add(self, p):
+        """insert points; returns their ids"""
+        nc = self.nc
+        c3 = self._cell3(p)
+        cid = (c3[:, 0] * nc + c3[:, 1]) * nc + c3[:, 2]
+        ids = np.arange(self.n, self.n + len(p), dtype=np.int32)
+        self.pts[self.n:self.n + len(p)] = p
+        self.n += len(p)
+        order = np.argsort(cid, kind="stable")
+        cs = cid[order]
+        first = np.r_[0, np.flatnonzero(cs[1:] != cs[:-1]) + 1]
+        rank = np.arange(len(cs)) - np.repeat(first, np.diff(np.r_[first, len(cs)]))
+        slot = self.fill[cs].astype(np.int64) + rank
+        if np.any(slot >= self.K):
+            raise RuntimeError("bead grid cell overflow")
+        self.slots[cs, slot] = ids[order]
+        np.add.at(self.fill, cs, 1)
+        return ids
+
+    def remove(self, ids):
+        nc = self.nc
+        for i in np.asarray(ids).ravel():
+            c3 = self._cell3(self.pts[i:i + 1])[0]
+            c = (c3[0] * nc + c3[1]) * nc + c3[2]
+            f = int(self.fill[c])
+            k = int(np.flatnonzero(self.slots[c, :f] == i)[0])
+            self.slots[c, k] = self.slots[c, f - 1]
+            self.slots[c, f - 1] = -1
+            self.fill[c] = f - 1
+
+
 def _grow_chains(nchains: int, nbeads: int, L: float, rng: np.random.Generator,
-                 rmin: float = 0.9) -> np.ndarray:
+                 rmin: float = 0.9, tries: int = 50, back: int = 4) -> np.ndarray:
     """Off-lattice self-avoiding walks grown in parallel, one bead per chain
-    per round, rejecting trial steps closer than rmin to any placed bead."""
+    per round: unit steps in random directions, a trial rejected when closer
+    than rmin (minimum image) to any placed bead or to another accepted trial
+    of the same round.  A chain that fails `tries` times in a row drops its
+    last `back` beads and grows again from there."""
     from scipy.spatial import cKDTree
 
     n = nchains * nbeads
     x = np.zeros((nchains, nbeads, 3))
+    gid = np.full((nchains, nbeads), -1, dtype=np.int64)
+    grid = _BeadGrid(L, rmin, n + 64 * nchains * back)
     x[:, 0] = nonoverlapping_uniform(nchains, L, rng, rmin=2.0)
-    placed = [x[:, 0].copy()]
-    for k in range(1, nbeads):
-        todo = np.arange(nchains)
-        cand = np.zeros((nchains, 3))
-        for _ in range(200):
-            v = rng.normal(size=(len(todo), 3))
-            v /= np.linalg.norm(v, axis=1, keepdims=True)
-            cand[todo] = np.mod(x[todo, k - 1] + v, L)
-            tree = cKDTree(np.mod(np.concatenate(placed), L), boxsize=L)
-            d, _ = tree.query(cand[todo], k=1)
-            ok = d >= rmin
-            # trial beads must not clash with each other either
-            t2 = cKDTree(cand[todo], boxsize=L)
-            clash = t2.query_pairs(rmin, output_type="ndarray")
-            if len(clash):
-                ok[np.unique(clash[:, 1])] = False
-            todo = todo[~ok]
-            if len(todo) == 0:
-                break
-            cand_ok = np.setdiff1d(np.arange(nchains), todo)
-            _ = cand_ok
-        x[:, k] = cand
-        placed.append(cand.copy())
+    gid[:, 0] = grid.add(x[:, 0])
+    length = np.ones(nchains, dtype=np.int64)
+    fails = np.zeros(nchains, dtype=np.int64)
+    rounds = 0
+    while True:
+        act = np.flatnonzero(length < nbeads)
+        if len(act) == 0:
+            break
+        rounds += 1
+        if rounds > 50 * nbeads + 1000:
+            raise RuntimeError("chain growth did not finish")
+        v = rng.normal(size=(len(act), 3))
+        v /= np.linalg.norm(v, axis=1, keepdims=True)
+        cand = np.mod(x[act, length[act] - 1] + v, L)
+        ok = grid.clear_of(cand)
+        idx = np.flatnonzero(ok)
+        if len(idx) > 1:
+            pairs = cKDTree(cand[idx], boxsize=L).query_pairs(rmin, output_type="ndarray")
+            if len(pairs):
+                ok[idx[np.unique(pairs[:, 1])]] = False
+        acc = act[ok]
+        x[acc, length[acc]] = cand[ok]
+        gid[acc, length[acc]] = grid.add(cand[ok])
+        length[acc] += 1
+        fails[acc] = 0
+        rej = act[~ok]
+        fails[rej] += 1
+        for c in rej[fails[rej] >= tries]:
+            nb = int(min(back, length[c] - 1))
+            grid.remove(gid[c, length[c] - nb:length[c]])
+            gid[c, length[c] - nb:length[c]] = -1
+            length[c] -= nb
+            fails[c] = 0
+    if grid.n > len(grid.pts):
+        raise RuntimeError("bead grid capacity exceeded")
     return np.mod(x.reshape(n, 3), L)

@@ -267,6 +267,7 @@ def test_lds_tiles_bitwise_equal_global_walk(monkeypatch, precision):
     say("tiles vs global", s["N"], precision)
     outs = []
     monkeypatch.setenv("MDX_SPLIT", "0")
+    monkeypatch.setenv("MDX_SKIN", "0")
     for tiles in ("1", "0"):
         monkeypatch.setenv("MDX_TILES", tiles)
         with engine_for(s, precision, seed=4) as e:
@@ -289,6 +290,7 @@ def test_split_walk_finish_bitwise_equal_fused(monkeypatch, precision):
     say("split vs fused", s["N"], precision)
     outs = []
     monkeypatch.setenv("MDX_TILES", "0")
+    monkeypatch.setenv("MDX_SKIN", "0")
     for split in ("1", "0"):
         monkeypatch.setenv("MDX_SPLIT", split)
         with engine_for(s, precision, seed=5) as e:
@@ -318,3 +320,62 @@ def test_split_candidate_overflow_matches_oracle(monkeypatch):
     Fo, To = po.compute_forces(s, list_cut=3.5)
     assert relmax(F, Fo) < 1e-12
     assert relmax(T, To) < 1e-12
+
+
+# ------------------------------------------------------------- neighbour list
+def _run(s, precision, nsteps, skin, kmax=64, resort=200, seed=7):
+    with engine_for(s, precision, seed=seed) as e:
+        e.set_neighbor_list(skin, kmax, resort)
+        if s["integrator"] == "rotor":
+            e.prime()
+        e.step(nsteps)
+        out = e.download()
+        out["builds"] = e.info()["list_builds"]
+    return out
+
+
+@pytest.mark.parametrize("kmax", [64, 3])
+def test_neighbor_list_rotor_matches_cell_walk(kmax):
+    """Steps that walk the neighbour list (rebuilt on the device from the
+    displacement bound, re-sorted every 25 steps; kmax=3 sends most particles
+    to the cell fallback) evaluate the same pairs as the exhaustive cell walk;
+    only the summation order differs, so FP64 trajectories agree to rounding."""
+    s = wl.condensate_ab(1 << 15, phi=0.2)
+    say("list vs cell walk (rotor)", s["N"], "kmax", kmax)
+    ref = _run(s, mdx.PREC_FP64, 80, 0.0)
+    lst = _run(s, mdx.PREC_FP64, 80, 0.3, kmax=kmax, resort=25)
+    say("list builds", lst["builds"])
+    assert lst["builds"] >= 4
+    for k in ("x", "p", "Q", "L"):
+        err = relmax(lst[k], ref[k])
+        say(k, err)
+        assert err < 1e-9, k
+
+
+@pytest.mark.parametrize("builder", [
+    lambda: wl.mt_bundle(2, 13, 60, spacing=6.0, box=(12.0, 12.0, 60.0)),
+    lambda: wl.active_polymer(nchains=64, nbeads=64, phi=0.15, seed=3),
+])
+def test_neighbor_list_topology_matches_cell_walk(builder):
+    """Bead integrator (microtubule bundle: bonds, bending, exclusions) and a
+    rotor system with topology (active polymer with patchy end beads)."""
+    s = builder()
+    say("list vs cell walk", s["name"], s["integrator"], s["N"])
+    ref = _run(s, mdx.PREC_FP64, 80, 0.0)
+    lst = _run(s, mdx.PREC_FP64, 80, 0.3, kmax=8, resort=30)
+    say("list builds", lst["builds"])
+    assert lst["builds"] >= 3
+    for k in ("x", "p"):
+        err = relmax(lst[k], ref[k])
+        say(k, err)
+        assert err < 1e-9, k
+
+
+def test_neighbor_list_mixed_close_to_cell_walk():
+    s = wl.condensate_ab(1 << 15, phi=0.2)
+    ref = _run(s, mdx.PREC_MIXED, 40, 0.0)
+    lst = _run(s, mdx.PREC_MIXED, 40, 0.4)
+    for k in ("x", "p", "Q", "L"):
+        err = relmax(lst[k], ref[k])
+        say("mixed", k, err)
+        assert err < 1e-3, k
