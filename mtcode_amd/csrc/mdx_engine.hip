@@ -120,10 +120,12 @@ struct mdx_engine {
     Buf ang_off, ang_ref, ang_ijk, ang_k, ang_t0;
     Buf exc_off, exc_j, act_prev, act_next;
     Buf noise, ctl, partial, tmpd, tmpi, tmpi2, tmpi3, tmpll, tmpx;
-    Buf Fw, ncand, cand; // split walk -> finish
+    Buf Fw, ncand, cand, Tp; // split walk -> (patch ->) finish
+    int use_compact = 1;     // compacted patch kernel + integrate kernel (MDX_COMPACT=0: one finish kernel)
     Buf nbr, nn, bcell;  // neighbour list
     Buf bmax;            // per-block drift maxima
-    hipEvent_t prof_mid = nullptr; // recorded between walk and finish while profiling
+    hipEvent_t prof_mid = nullptr;  // recorded between walk and finish while profiling
+    hipEvent_t prof_mid2 = nullptr; // recorded between patch and integrate while profiling
     bool last_split = false;       // the last gather launch used the split form
 
     size_t rsz() const { return prec == MDX_PREC_FP64 ? 8 : 4; }
@@ -421,14 +423,27 @@ static int launch_gather(mdx_engine *e, const double *noise_dev, int lmode = 0)
             (rc = dalloc(e, e->ncand, 4 * (size_t)e->N)) ||
             (rc = dalloc(e, e->cand, 4 * (size_t)PMAXG * e->N)))
             return rc;
+        if (e->use_compact && (rc = dalloc(e, e->Tp, 3 * e->rsz() * (size_t)e->N)))
+            return rc;
         a.Fw = (R *)e->Fw.p;
         a.ncand = (int *)e->ncand.p;
         a.cand = (int *)e->cand.p;
+        a.Tp = (R *)e->Tp.p;
         hipLaunchKernelGGL((k_walk<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
         if (e->prof_mid)
             HIPCHK(hipEventRecord(e->prof_mid, e->stream));
         e->last_split = true;
-        hipLaunchKernelGGL((k_finish<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
+        if (e->use_compact) {
+            hipLaunchKernelGGL((k_patch<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
+            if (e->prof_mid2)
+                HIPCHK(hipEventRecord(e->prof_mid2, e->stream));
+            hipLaunchKernelGGL((k_integrate<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream,
+                               a);
+        } else {
+            if (e->prof_mid2)
+                HIPCHK(hipEventRecord(e->prof_mid2, e->stream));
+            hipLaunchKernelGGL((k_finish<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
+        }
         HIPCHK(hipGetLastError());
         return MDX_OK;
     }
@@ -535,12 +550,13 @@ template <class R> static int build_list(mdx_engine *e, const int *cells, int ga
     return MDX_OK;
 }
 
-// optional per-step events: ev[4 s + 0] step start, + 1 after the cell / list
-// work, + 2 after the walk (split form), + 3 step end
+// optional per-step events: ev[5 s + 0] step start, + 1 after the cell / list
+// work, + 2 after the walk, + 3 after the patch kernel (split form), + 4 step end
+constexpr int NEV = 5;
 static inline void mark(mdx_engine *e, hipEvent_t *ev, int s, int k)
 {
     if (ev)
-        (void)hipEventRecord(ev[4 * s + k], e->stream);
+        (void)hipEventRecord(ev[NEV * s + k], e->stream);
 }
 
 // one step of the loop.  MODE: M_FUSED / M_STEP (rotor), M_BEAD.
@@ -561,9 +577,10 @@ static int enqueue_one(mdx_engine *e, const double *nd, hipEvent_t *ev, int s, b
         if ((rc = finish_bins(e)))
             return rc;
         mark(e, ev, s, 1);
-        e->prof_mid = ev ? ev[4 * s + 2] : nullptr;
+        e->prof_mid = ev ? ev[NEV * s + 2] : nullptr;
+        e->prof_mid2 = ev ? ev[NEV * s + 3] : nullptr;
         rc = launch_gather<R, MODE>(e, nd, 0);
-        e->prof_mid = nullptr;
+        e->prof_mid = e->prof_mid2 = nullptr;
         if (rc)
             return rc;
         e->cur ^= 1;
@@ -578,9 +595,10 @@ static int enqueue_one(mdx_engine *e, const double *nd, hipEvent_t *ev, int s, b
         if ((rc = finish_bins(e)))
             return rc;
         mark(e, ev, s, 1);
-        e->prof_mid = ev ? ev[4 * s + 2] : nullptr;
+        e->prof_mid = ev ? ev[NEV * s + 2] : nullptr;
+        e->prof_mid2 = ev ? ev[NEV * s + 3] : nullptr;
         rc = launch_gather<R, MODE>(e, nd, 0);
-        e->prof_mid = nullptr;
+        e->prof_mid = e->prof_mid2 = nullptr;
         if (rc)
             return rc;
         e->cur ^= 1;
@@ -592,6 +610,9 @@ static int enqueue_one(mdx_engine *e, const double *nd, hipEvent_t *ev, int s, b
             if ((rc = build_list<R>(e, (const int *)e->cell[e->cur].p, 0)))
                 return rc;
             launch_small(e, k_list_reset);
+            // list steps keep the slots and do not rewrite the slot map
+            HIPCHK(hipMemcpyAsync(e->slot[e->cur ^ 1].p, e->slot[e->cur].p, 4 * (size_t)e->N,
+                                  hipMemcpyDeviceToDevice, e->stream));
             e->list_valid = true;
             e->since_resort = 0;
             e->binned = MODE == M_BEAD;
@@ -608,9 +629,10 @@ static int enqueue_one(mdx_engine *e, const double *nd, hipEvent_t *ev, int s, b
         if ((rc = build_list<R>(e, (const int *)e->bcell.p, 1)))
             return rc;
         mark(e, ev, s, 1);
-        e->prof_mid = ev ? ev[4 * s + 2] : nullptr;
+        e->prof_mid = ev ? ev[NEV * s + 2] : nullptr;
+        e->prof_mid2 = ev ? ev[NEV * s + 3] : nullptr;
         rc = launch_gather<R, MODE>(e, nd, 1);
-        e->prof_mid = nullptr;
+        e->prof_mid = e->prof_mid2 = nullptr;
         if (rc)
             return rc;
         if (MODE != M_STEP && (rc = decide(e)))
@@ -621,9 +643,11 @@ static int enqueue_one(mdx_engine *e, const double *nd, hipEvent_t *ev, int s, b
         e->binned = false;
         e->counted = false;
     }
-    if (ev && !e->last_split)
+    if (ev && !e->last_split) {
         mark(e, ev, s, 2);
-    mark(e, ev, s, 3);
+        mark(e, ev, s, 3);
+    }
+    mark(e, ev, s, 4);
     (void)last;
     return MDX_OK;
 }
@@ -813,6 +837,8 @@ int mdx_create(const mdx_desc *d, mdx_engine **out)
         e->use_tiles = std::atoi(t) != 0;
     if (const char *t = std::getenv("MDX_SPLIT"))
         e->use_split = std::atoi(t) != 0;
+    if (const char *t = std::getenv("MDX_COMPACT"))
+        e->use_compact = std::atoi(t) != 0;
     if (const char *t = std::getenv("MDX_SKIN"))
         e->skin = std::max(0.0, std::atof(t));
     if (const char *t = std::getenv("MDX_KMAX"))
@@ -875,7 +901,7 @@ void mdx_destroy(mdx_engine *e)
                    &e->act_next, &e->noise,  &e->ctl,    &e->partial, &e->tmpd,   &e->tmpi,
                    &e->tmpi2,   &e->tmpi3,   &e->tmpll,  &e->tmpx,    &e->Fw,
                    &e->ncand,   &e->cand,    &e->nbr,    &e->nn,     &e->bcell,
-                   &e->bmax};
+                   &e->bmax,    &e->Tp};
     for (Buf *b : bufs)
         dfree(*b);
     for (int b = 0; b < 2; b++) {
@@ -1547,7 +1573,7 @@ static int profile_t(mdx_engine *e, int nsteps, int cap, const char **names, dou
     int rc;
     if (e->integ == MDX_INTEG_ROTOR && !e->primed && (rc = do_prime<R>(e, nullptr)))
         return rc;
-    std::vector<hipEvent_t> ev(4 * (size_t)nsteps + 2);
+    std::vector<hipEvent_t> ev(NEV * (size_t)nsteps + 2);
     for (auto &x : ev)
         HIPCHK(hipEventCreate(&x));
     if (e->integ == MDX_INTEG_ROTOR)
@@ -1557,39 +1583,42 @@ static int profile_t(mdx_engine *e, int nsteps, int cap, const char **names, dou
     if (rc)
         return rc;
     HIPCHK(hipStreamSynchronize(e->stream));
-    static const char *kn[] = {"bins", "walk", "finish", "gather", "step", "kick_drift"};
-    double acc[6] = {0, 0, 0, 0, 0, 0};
+    // bins: cell/list work; walk; patch; finish (integrator, or patch +
+    // integrator in one kernel); gather: walk..end; step: all
+    static const char *kn[] = {"bins", "walk", "patch", "finish", "gather", "step",
+                               "kick_drift", "bins_min", "bins_max"};
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 1e30, 0};
     const hipEvent_t *se = ev.data() + 2;
     for (int s = 0; s < nsteps; s++) {
-        float t01, t12, t23, t03;
-        HIPCHK(hipEventElapsedTime(&t01, se[4 * s], se[4 * s + 1]));
-        HIPCHK(hipEventElapsedTime(&t12, se[4 * s + 1], se[4 * s + 2]));
-        HIPCHK(hipEventElapsedTime(&t23, se[4 * s + 2], se[4 * s + 3]));
-        HIPCHK(hipEventElapsedTime(&t03, se[4 * s], se[4 * s + 3]));
-        acc[0] += t01;
-        acc[1] += t12;
-        acc[2] += t23;
-        acc[3] += t12 + t23;
-        acc[4] += t03;
+        float t[4], tall;
+        for (int k = 0; k < 4; k++)
+            HIPCHK(hipEventElapsedTime(&t[k], se[NEV * s + k], se[NEV * s + k + 1]));
+        HIPCHK(hipEventElapsedTime(&tall, se[NEV * s], se[NEV * s + 4]));
+        acc[0] += t[0];
+        acc[1] += t[1];
+        acc[2] += t[2];
+        acc[3] += t[3];
+        acc[4] += t[1] + t[2] + t[3];
+        acc[5] += tall;
+        acc[7] = std::min(acc[7], (double)t[0] * nsteps);
+        acc[8] = std::max(acc[8], (double)t[0] * nsteps);
     }
-    int K = 5;
     if (e->integ == MDX_INTEG_ROTOR) {
         float t;
         HIPCHK(hipEventElapsedTime(&t, ev[0], ev[1]));
-        acc[5] = t * nsteps; // reported once, not averaged
-        K = 6;
+        acc[6] = t * nsteps; // reported once, not averaged
     }
     for (auto &x : ev)
         (void)hipEventDestroy(x);
+    const bool compact = e->last_split && e->use_compact;
     int w = 0;
-    for (int k = 0; k < K && w < cap; k++) {
-        if (k == 2 && !e->last_split)
-            continue; // no separate finish kernel
-        if (k == 1 && !e->last_split) {
-            names[w] = "gather";
-            ms[w++] = acc[3] / nsteps;
+    for (int k = 0; k < 9 && w < cap; k++) {
+        if (k == 6 && e->integ != MDX_INTEG_ROTOR)
             continue;
-        }
+        if (k == 2 && !compact)
+            continue; // no separate patch kernel
+        if ((k == 1 || k == 3) && !e->last_split)
+            continue; // one gather kernel
         names[w] = kn[k];
         ms[w++] = acc[k] / nsteps;
     }

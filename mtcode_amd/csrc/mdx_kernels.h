@@ -72,6 +72,7 @@ template <class R> struct KArgs {
     // split walk -> finish: WCA force and patch candidates per cell-list position
     R *Fw;
     int *ncand, *cand; // cand[q * N + s], q < PMAXG
+    R *Tp;             // patch torque per position (compacted patch kernel)
     // neighbour list (lmode): slot s walks nbr[k * N + s], k < nn[s] (<= kmax),
     // else the 27 cells of its cell at the last build, bcell[s]
     int lmode;
@@ -273,7 +274,7 @@ __device__ __forceinline__ void min_image(const KArgs<R> &a, const DevRep &rp, d
 // ------------------------------------------------------------- gather kernel
 constexpr int PMAX = 8;   // deferred patch candidates per particle held in LDS
 constexpr int PMAXG = 16; // patch candidates per particle handed from walk to finish
-enum { PH_ALL = 0, PH_WALK = 1, PH_FINISH = 2 };
+enum { PH_ALL = 0, PH_WALK = 1, PH_FINISH = 2, PH_INTEGRATE = 3 };
 
 // neighbour slots of slot i: the 9 (y, z) rows of the 27-cell stencil; the x
 // neighbours of a row are contiguous in the cell list (one range, or two at
@@ -342,9 +343,20 @@ __device__ __forceinline__ void for_each_listed(const KArgs<R> &a, const DevRep 
 {
     const int n = a.nn[i];
     if (n <= a.kmax) {
-        for (int k = 0; k < n; k++) {
-            const int j = a.nbr[(size_t)k * a.N + i];
-            fn(a.X[j], j);
+        // four entries per round: their index and position loads overlap
+        for (int k0 = 0; k0 < n; k0 += 4) {
+            int j[4];
+            PosRec x[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                j[u] = k0 + u < n ? a.nbr[(size_t)(k0 + u) * a.N + i] : i;
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                x[u] = a.X[j[u]];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (k0 + u < n)
+                    fn(x[u], j[u]);
         }
     } else {
         for_each_in_cells(a, rp, a.bcell[i], i, fn);
@@ -517,7 +529,7 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
 
     R qi[9];
     // the bead integrator carries no orientations: no patch code in its kernels
-    const int npi = (MODE != M_BEAD && a.has_patches) ? a.np[ti] : 0;
+    const int npi = (MODE != M_BEAD && PH != PH_INTEGRATE && a.has_patches) ? a.np[ti] : 0;
     if (MODE != M_BEAD && a.rotor) {
 #pragma unroll
         for (int k = 0; k < 9; k++)
@@ -561,7 +573,14 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
             });
         }
     }
-    if (PH != PH_FINISH && (a.wca || npi > 0)) {
+    if (PH == PH_INTEGRATE) {
+        // force with patches and patch torque from the compacted patch kernel
+        for (int k = 0; k < 3; k++) {
+            acc.F[k] = a.Fw[3 * (size_t)s + k];
+            acc.T[k] = a.Tp[3 * (size_t)s + k];
+        }
+    }
+    if (PH != PH_FINISH && PH != PH_INTEGRATE && (a.wca || npi > 0)) {
         int npc = 0;
         trav(i, rp, [&](const PosRec &xj, int j) {
             double d[3];
@@ -749,7 +768,9 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
             o[1 + k] = vir[k];
         return;
     }
-    a.slotn[oi] = s;
+    // list mode keeps the slots: both slot maps already hold them
+    if (!a.lmode)
+        a.slotn[oi] = s;
     if (MODE == M_BEAD) {
         // LangevinNVT::advance_mom (Langevin.cpp:254-269), advance_pos
         // (Langevin.cpp:205-223), binning of the new position
@@ -918,6 +939,141 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_finish(KAr
     float drift = 0.f;
     if (s < a.N)
         process_particle<R, MODE, PH_FINISH>(
+            a, s, nullptr, [&](int i, const DevRep &rp, auto &&fn) { for_each_near(a, rp, i, fn); },
+            drift);
+    if ((MODE == M_FUSED || MODE == M_BEAD) && a.lmode)
+        drift_bound(a, drift);
+}
+
+// split form, compacted: the patch candidates of a block's 256 positions are
+// laid out back to back and evaluated one per thread (no idle lanes on
+// positions without candidates); each position then adds its candidates'
+// terms in candidate order to its walk force (the same order as the one-kernel
+// form; bitwise equal to it while at most one patch pair of a particle pair
+// passes the cone tests, as for cos(theta_m) >= 0.5 with these patch sets)
+template <class R, int MODE> __global__ void __launch_bounds__(256) k_patch(KArgs<R> a)
+{
+    __shared__ int off[256];
+    __shared__ unsigned char owner[256 * PMAXG];
+    __shared__ PosRec sx[256];
+    __shared__ R sq[256 * 9];
+    __shared__ R ent[256 * 6];
+    __shared__ int wsum[4];
+    const int tid = threadIdx.x;
+    const int s = blockIdx.x * 256 + tid;
+    const bool live = s < a.N;
+    const int nc = live ? a.ncand[s] : 0;
+    const int nq = nc < PMAXG ? nc : PMAXG;
+    R F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
+    if (live) {
+        const int i = a.lmode ? s : a.list[s];
+        sx[tid] = a.X[i];
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+            sq[9 * tid + k] = a.Q[9 * (size_t)i + k];
+        for (int k = 0; k < 3; k++)
+            F[k] = a.Fw[3 * (size_t)s + k];
+    }
+    // exclusive scan of nq over the block
+    int v = nq;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(v, o);
+        if ((tid & 63) >= o)
+            v += t;
+    }
+    if ((tid & 63) == 63)
+        wsum[tid >> 6] = v;
+    __syncthreads();
+    int wbase = 0;
+    for (int w = 0; w < (tid >> 6); w++)
+        wbase += wsum[w];
+    const int total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    off[tid] = wbase + v - nq;
+    for (int k = 0; k < nq; k++)
+        owner[wbase + v - nq + k] = (unsigned char)tid;
+    __syncthreads();
+    for (int base = 0; base < total; base += 256) {
+        const int e = base + tid;
+        if (e < total) {
+            const int o = owner[e];
+            const int q = e - off[o];
+            const PosRec xo = sx[o];
+            const DevRep rp = a.reps[xo.meta >> 8];
+            R qo[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++)
+                qo[k] = sq[9 * o + k];
+            Acc<R> ea;
+            for (int k = 0; k < 3; k++)
+                ea.F[k] = ea.T[k] = 0;
+            patch_pair<R, MODE>(a, rp, xo, xo.meta & 255, qo,
+                                a.cand[(size_t)q * a.N + blockIdx.x * 256 + o], ea);
+            for (int k = 0; k < 3; k++) {
+                ent[6 * tid + k] = ea.F[k];
+                ent[6 * tid + 3 + k] = ea.T[k];
+            }
+        }
+        __syncthreads();
+        const int lo = max(off[tid], base), hi = min(off[tid] + nq, base + 256);
+        for (int e2 = lo; e2 < hi; e2++)
+            for (int k = 0; k < 3; k++) {
+                F[k] += ent[6 * (e2 - base) + k];
+                T[k] += ent[6 * (e2 - base) + 3 + k];
+            }
+        __syncthreads();
+    }
+    if (!live)
+        return;
+    if (nc > PMAXG) {
+        // candidates beyond the handed-over ones: walk again, in walk order
+        const int i = a.lmode ? s : a.list[s];
+        const PosRec xi = sx[tid];
+        const int ti = xi.meta & 255;
+        const DevRep rp = a.reps[xi.meta >> 8];
+        R qi[9];
+        for (int k = 0; k < 9; k++)
+            qi[k] = sq[9 * tid + k];
+        Acc<R> acc;
+        for (int k = 0; k < 3; k++) {
+            acc.F[k] = F[k];
+            acc.T[k] = T[k];
+        }
+        const int npi = a.np[ti];
+        int seen = 0;
+        for_each_near(a, rp, i, [&](const PosRec &xj, int j) {
+            double d[3];
+            min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
+            const double r2s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (!(r2s < a.rmax2) || !(npi > 0 && r2s < a.maxd2))
+                return;
+            const int tj = xj.meta & 255;
+            if (a.np[tj] == 0)
+                return;
+            int ta, tc;
+            patch_roles(xi.id, ti, xj.id, tj, ta, tc);
+            if (!((a.active_mask >> (ta * MAXT + tc)) & 1ull))
+                return;
+            if (seen++ >= PMAXG)
+                patch_pair<R, MODE>(a, rp, xi, ti, qi, j >= 0 ? j : a.slot[xj.id], acc);
+        });
+        for (int k = 0; k < 3; k++) {
+            F[k] = acc.F[k];
+            T[k] = acc.T[k];
+        }
+    }
+    for (int k = 0; k < 3; k++) {
+        a.Fw[3 * (size_t)s + k] = F[k];
+        a.Tp[3 * (size_t)s + k] = T[k];
+    }
+}
+
+// ... and the topology terms and the integrator
+template <class R, int MODE> __global__ void __launch_bounds__(256) k_integrate(KArgs<R> a)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    float drift = 0.f;
+    if (s < a.N)
+        process_particle<R, MODE, PH_INTEGRATE>(
             a, s, nullptr, [&](int i, const DevRep &rp, auto &&fn) { for_each_near(a, rp, i, fn); },
             drift);
     if ((MODE == M_FUSED || MODE == M_BEAD) && a.lmode)
@@ -1138,9 +1294,10 @@ template <class R> __device__ __forceinline__ float kick_drift_one(const KArgs<R
     xr.y = xn[1];
     xr.z = xn[2];
     a.Xn[i] = xr;
-    a.slotn[xr.id] = i;
-    if (!a.lmode)
+    if (!a.lmode) {
+        a.slotn[xr.id] = i;
         bin_particle(a, i, rp, xn, a.celln);
+    }
     return (float)sqrt(dd);
 }
 
