@@ -117,6 +117,16 @@ int mdx_set_active(mdx_engine *e, double fa, const int *prev, const int *next);
  * cell order every resort_every steps.  skin 0: every step bins and walks
  * the 27 cells.  Defaults 0.5, 64, 200. */
 int mdx_set_neighbor_list(mdx_engine *e, double skin, int kmax, int resort_every);
+/* switch interaction terms off again (MDX_TERM_* bits); set them again with
+ * the setters above */
+#define MDX_TERM_WCA 1
+#define MDX_TERM_PATCHES 2
+#define MDX_TERM_BONDS 4
+#define MDX_TERM_ANGLES 8
+#define MDX_TERM_EXCLUSIONS 16
+#define MDX_TERM_ACTIVE 32
+#define MDX_TERM_ALL 63
+int mdx_clear_interactions(mdx_engine *e, int terms);
 /* |pair force| guard; 0 disables.  The reference prints "force too big" and
  * pauses above 1e4 (MD.cpp:1711-1749); here mdx_step returns MDX_ERR_FORCE. */
 int mdx_set_force_guard(mdx_engine *e, double fmax);
@@ -148,6 +158,23 @@ int mdx_step(mdx_engine *e, int nsteps, const double *noise);
  * out[3..5] <L_k^2/2I_0> (LangevinR.cpp:170-193), out[6] potential energy,
  * out[7..15] virial sum_pairs r_ij (x) f_ij (calculatestress, MD.cpp:2381-2424) */
 int mdx_observables(mdx_engine *e, int replica, double *out);
+
+/* Single stages of the step, one reference call each, on the engine's state
+ * (host arrays by original index, the reference's layouts).  For drivers that
+ * compose the step themselves; afterwards mdx_step needs mdx_prime again. */
+/* LangevinNVTR::advancemom_halfstep (LangevinR.cpp:327-347): p += dt/2 F,
+ * L += dt/2 T (T ignored for the bead integrator) */
+int mdx_stage_halfkick(mdx_engine *e, const double *F, const double *T);
+/* LangevinNVT::advance_pos (Langevin.cpp:205-223) with
+ * cube::correct_position_and_momentum (geometry.h:403-445) */
+int mdx_stage_advance_pos(mdx_engine *e);
+/* LangevinNVTR::rotate (LangevinR.cpp:595-695) */
+int mdx_stage_rotate(mdx_engine *e);
+/* LangevinNVTR::create_forces_and_torques_sphere (LangevinR.cpp:475-578):
+ * F (lab) in/out; T lab torque in, body torque out; RT N x 6 */
+int mdx_stage_sphere(mdx_engine *e, double *F, double *T, const double *RT);
+/* LangevinNVT::advance_mom (Langevin.cpp:254-269): R N x 3 */
+int mdx_stage_advance_mom(mdx_engine *e, const double *F, const double *R);
 
 /* Diagnostics. */
 /* per-kernel average device time (ms) over nsteps eagerly launched steps,

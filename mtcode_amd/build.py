@@ -57,5 +57,28 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+HOST_HEADER = os.path.join(HERE, "host", "mtcode_amd.hpp")
+DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "dropin_driver.cpp")
+DRIVER = os.path.join(ROOT, "tests", "cpp", "dropin_driver")
+
+
+def build_driver(verbose: bool = False) -> str:
+    """Compile the C++ drop-in test driver (host classes over libmdx.so) with
+    the host compiler; no HIP headers are involved."""
+    cxx = os.environ.get("CXX", "g++")
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}",
+           f"-I{os.path.join(HERE, 'host')}", DRIVER_SRC, f"-L{HERE}", "-lmdx",
+           "-Wl,-rpath,$ORIGIN/../../mtcode_amd", "-o", DRIVER + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("failed to build the drop-in driver")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_driver(verbose=True))

@@ -1317,6 +1317,28 @@ int mdx_set_active(mdx_engine *e, double fa, const int *prev, const int *next)
     return MDX_OK;
 }
 
+int mdx_clear_interactions(mdx_engine *e, int terms)
+{
+    if (!e || terms < 0)
+        return set_err(MDX_ERR_ARG, "bad arguments");
+    if (terms & MDX_TERM_WCA)
+        e->wca = 0;
+    if (terms & MDX_TERM_PATCHES) {
+        e->ntypes = 0;
+        e->active_mask = 0;
+    }
+    if (terms & MDX_TERM_BONDS)
+        e->nbonds = 0;
+    if (terms & MDX_TERM_ANGLES)
+        e->nangles = 0;
+    if (terms & MDX_TERM_EXCLUSIONS)
+        e->nexcl = 0;
+    if (terms & MDX_TERM_ACTIVE)
+        e->has_active = 0;
+    e->grid_dirty = true;
+    return MDX_OK;
+}
+
 int mdx_set_force_guard(mdx_engine *e, double fmax)
 {
     if (!e || !(fmax >= 0))
@@ -1548,6 +1570,161 @@ int mdx_info(mdx_engine *e, double *out)
     out[6] = e->skin;
     out[7] = e->kmax;
     return MDX_OK;
+}
+
+// ------------------------------------------------ single-stage entry points
+// Each changes the engine's state in place (slot order kept); afterwards the
+// carried forces of mdx_step are stale (mdx_prime again) and, if positions
+// moved, so are the cell and neighbour lists.
+static void stage_done(mdx_engine *e, bool moved)
+{
+    e->primed = false;
+    if (moved) {
+        e->binned = false;
+        e->counted = false;
+        e->list_valid = false;
+    }
+}
+
+static int stage_upload(mdx_engine *e, Buf &b, const double *h, size_t n)
+{
+    int rc = dalloc(e, b, n * sizeof(double));
+    if (rc)
+        return rc;
+    HIPCHK(hipMemcpyAsync(b.p, h, n * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    return MDX_OK;
+}
+
+template <class R> static int stage_halfkick_t(mdx_engine *e, const double *F, const double *T)
+{
+    int rc;
+    const size_t n3 = 3 * (size_t)e->N;
+    if ((rc = stage_upload(e, e->tmpd, F, n3)))
+        return rc;
+    const bool rot = e->integ == MDX_INTEG_ROTOR;
+    if (rot && (rc = stage_upload(e, e->tmpx, T, n3)))
+        return rc;
+    hipLaunchKernelGGL(k_stage_halfkick<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, e->N,
+                       (const PosRec *)e->X[e->cur].p, (R *)e->P[e->cur].p,
+                       rot ? (R *)e->Lm[e->cur].p : nullptr, (const double *)e->tmpd.p,
+                       rot ? (const double *)e->tmpx.p : nullptr, e->dt / 2.);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(e->stream));
+    stage_done(e, false);
+    return MDX_OK;
+}
+
+int mdx_stage_halfkick(mdx_engine *e, const double *F, const double *T)
+{
+    if (!e || !F || (e->integ == MDX_INTEG_ROTOR && !T))
+        return set_err(MDX_ERR_ARG, "engine, F (and T for rotors) required");
+    HIPCHK(hipSetDevice(e->device));
+    return e->prec == MDX_PREC_FP64 ? stage_halfkick_t<double>(e, F, T)
+                                    : stage_halfkick_t<float>(e, F, T);
+}
+
+template <class R> static int stage_advance_pos_t(mdx_engine *e)
+{
+    int rc = build_grid(e);
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(k_stage_advance_pos<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, e->N,
+                       (PosRec *)e->X[e->cur].p, (R *)e->P[e->cur].p, (const DevRep *)e->dreps.p,
+                       e->dt / e->m, e->periodic);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(e->stream));
+    stage_done(e, true);
+    return MDX_OK;
+}
+
+int mdx_stage_advance_pos(mdx_engine *e)
+{
+    if (!e)
+        return set_err(MDX_ERR_ARG, "null engine");
+    HIPCHK(hipSetDevice(e->device));
+    return e->prec == MDX_PREC_FP64 ? stage_advance_pos_t<double>(e) : stage_advance_pos_t<float>(e);
+}
+
+template <class R> static int stage_rotate_t(mdx_engine *e)
+{
+    hipLaunchKernelGGL(k_stage_rotate<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, e->N,
+                       (R *)e->Lm[e->cur].p, (R *)e->Q[e->cur].p, e->dt, e->I[0], e->I[1], e->I[2]);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(e->stream));
+    stage_done(e, false);
+    return MDX_OK;
+}
+
+int mdx_stage_rotate(mdx_engine *e)
+{
+    if (!e)
+        return set_err(MDX_ERR_ARG, "null engine");
+    if (e->integ != MDX_INTEG_ROTOR)
+        return set_err(MDX_ERR_STATE, "rotate applies to the rotor integrator");
+    HIPCHK(hipSetDevice(e->device));
+    return e->prec == MDX_PREC_FP64 ? stage_rotate_t<double>(e) : stage_rotate_t<float>(e);
+}
+
+template <class R> static int stage_sphere_t(mdx_engine *e, double *F, double *T, const double *RT)
+{
+    int rc = build_grid(e);
+    if (rc)
+        return rc;
+    const size_t n3 = 3 * (size_t)e->N;
+    if ((rc = stage_upload(e, e->tmpd, F, n3)) || (rc = stage_upload(e, e->tmpx, T, n3)) ||
+        (rc = stage_upload(e, e->noise, RT, 2 * n3)))
+        return rc;
+    hipLaunchKernelGGL(k_stage_sphere<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, e->N,
+                       (const PosRec *)e->X[e->cur].p, (const R *)e->P[e->cur].p,
+                       (const R *)e->Lm[e->cur].p, (const R *)e->Q[e->cur].p,
+                       (const DevRep *)e->dreps.p, (double *)e->tmpd.p, (double *)e->tmpx.p,
+                       (const double *)e->noise.p, -e->gamma / e->m, -e->gammar / e->I[0],
+                       -e->gammar / e->I[1], -e->gammar / e->I[2]);
+    HIPCHK(hipGetLastError());
+    if ((rc = download(e, e->tmpd, F, n3)) || (rc = download(e, e->tmpx, T, n3)))
+        return rc;
+    stage_done(e, false);
+    return MDX_OK;
+}
+
+int mdx_stage_sphere(mdx_engine *e, double *F, double *T, const double *RT)
+{
+    if (!e || !F || !T || !RT)
+        return set_err(MDX_ERR_ARG, "engine, F, T and RT required");
+    if (e->integ != MDX_INTEG_ROTOR)
+        return set_err(MDX_ERR_STATE, "the sphere update applies to the rotor integrator");
+    HIPCHK(hipSetDevice(e->device));
+    return e->prec == MDX_PREC_FP64 ? stage_sphere_t<double>(e, F, T, RT)
+                                    : stage_sphere_t<float>(e, F, T, RT);
+}
+
+template <class R> static int stage_advance_mom_t(mdx_engine *e, const double *F, const double *Rn)
+{
+    int rc = build_grid(e);
+    if (rc)
+        return rc;
+    const size_t n3 = 3 * (size_t)e->N;
+    if ((rc = stage_upload(e, e->tmpd, F, n3)) || (rc = stage_upload(e, e->tmpx, Rn, n3)))
+        return rc;
+    const double d = (e->gamma * e->dt / 2.), q = (e->dt) / 2.;
+    const double c2 = (1.0 / (1.0 + (d))), c3 = (1.0 / (1.0 + (d))) * q, c5 = (1 - (d));
+    hipLaunchKernelGGL(k_stage_advance_mom<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, e->N,
+                       (const PosRec *)e->X[e->cur].p, (R *)e->P[e->cur].p,
+                       (const DevRep *)e->dreps.p, (const double *)e->tmpd.p,
+                       (const double *)e->tmpx.p, c5 * c2, (c5 * (c3) + q), c2, c5);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(e->stream));
+    stage_done(e, false);
+    return MDX_OK;
+}
+
+int mdx_stage_advance_mom(mdx_engine *e, const double *F, const double *R)
+{
+    if (!e || !F || !R)
+        return set_err(MDX_ERR_ARG, "engine, F and R required");
+    HIPCHK(hipSetDevice(e->device));
+    return e->prec == MDX_PREC_FP64 ? stage_advance_mom_t<double>(e, F, R)
+                                    : stage_advance_mom_t<float>(e, F, R);
 }
 
 int mdx_set_neighbor_list(mdx_engine *e, double skin, int kmax, int resort_every)

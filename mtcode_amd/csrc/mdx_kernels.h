@@ -1593,6 +1593,133 @@ __global__ void k_pairs(const double *X, int count, double l0, double l1, double
 
 // ------------------------------------------------------------ reductions
 // per-block partial sums (fixed order) of 16 observables: slot range
+// ------------------------------------------------ single-stage entry points
+// One reference call each, on the current buffer in place (slot order); host
+// arrays are staged by original index.
+
+// LangevinNVTR::advancemom_halfstep (LangevinR.cpp:327-347); L may be null
+template <class R>
+__global__ void __launch_bounds__(256) k_stage_halfkick(int N, const PosRec *X, R *p, R *L,
+                                                        const double *F, const double *T, double hdt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    const size_t o = X[i].id;
+    for (int k = 0; k < 3; k++) {
+        p[3 * (size_t)i + k] += (R)hdt * (R)F[3 * o + k];
+        if (L)
+            L[3 * (size_t)i + k] += (R)hdt * (R)T[3 * o + k];
+    }
+}
+
+// LangevinNVT::advance_pos (Langevin.cpp:205-223) + cube::correct_position_and_momentum
+// (geometry.h:403-445)
+template <class R>
+__global__ void __launch_bounds__(256) k_stage_advance_pos(int N, PosRec *X, R *p, const DevRep *reps,
+                                                           double c1, int periodic)
+{
+    constexpr bool FP64 = sizeof(R) == 8;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    PosRec xr = X[i];
+    const DevRep rp = reps[xr.meta >> 8];
+    double x[3] = {xr.x, xr.y, xr.z};
+    for (int k = 0; k < 3; k++) {
+        R pk = p[3 * (size_t)i + k];
+        if constexpr (FP64)
+            x[k] = x[k] + c1 * pk;
+        else
+            x[k] = x[k] + (double)((R)c1 * pk);
+        wrap1(x[k], pk, rp.l[k], periodic);
+        p[3 * (size_t)i + k] = pk;
+    }
+    xr.x = x[0];
+    xr.y = x[1];
+    xr.z = x[2];
+    X[i] = xr;
+}
+
+// LangevinNVTR::rotate (LangevinR.cpp:595-695)
+template <class R>
+__global__ void __launch_bounds__(256) k_stage_rotate(int N, R *L, R *Q, double dt, double I0,
+                                                      double I1, double I2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    R l[3], q[9];
+    for (int k = 0; k < 3; k++)
+        l[k] = L[3 * (size_t)i + k];
+    for (int k = 0; k < 9; k++)
+        q[k] = Q[9 * (size_t)i + k];
+    const R Ir[3] = {(R)I0, (R)I1, (R)I2};
+    rotate((R)dt, Ir, l, q);
+    for (int k = 0; k < 3; k++)
+        L[3 * (size_t)i + k] = l[k];
+    for (int k = 0; k < 9; k++)
+        Q[9 * (size_t)i + k] = q[k];
+}
+
+// LangevinNVTR::create_forces_and_torques_sphere (LangevinR.cpp:475-578):
+// F (lab) in/out, T lab in -> body out, RT N x 6; all by original index
+template <class R>
+__global__ void __launch_bounds__(256) k_stage_sphere(int N, const PosRec *X, const R *p, const R *L,
+                                                      const R *Q, const DevRep *reps, double *F,
+                                                      double *T, const double *RT, double gm,
+                                                      double gr0, double gr1, double gr2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    const PosRec xr = X[i];
+    const size_t o = xr.id;
+    const DevRep rp = reps[xr.meta >> 8];
+    const R *q = Q + 9 * (size_t)i;
+    const R pl[3] = {p[3 * (size_t)i], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2]};
+    const R ll[3] = {L[3 * (size_t)i], L[3 * (size_t)i + 1], L[3 * (size_t)i + 2]};
+    const R pbx = q[0] * pl[0] + q[1] * pl[1] + q[2] * pl[2];
+    const R pby = q[3] * pl[0] + q[4] * pl[1] + q[5] * pl[2];
+    const R pbz = q[6] * pl[0] + q[7] * pl[1] + q[8] * pl[2];
+    const double *nz = RT + 6 * o;
+    const R fx = (R)gm * pbx + (R)rp.Rt * (R)nz[0];
+    const R fy = (R)gm * pby + (R)rp.Rt * (R)nz[1];
+    const R fz = (R)gm * pbz + (R)rp.Rt * (R)nz[2];
+    const R tbx = (R)gr0 * ll[0] + (R)rp.Rr * (R)nz[3];
+    const R tby = (R)gr1 * ll[1] + (R)rp.Rr * (R)nz[4];
+    const R tbz = (R)gr2 * ll[2] + (R)rp.Rr * (R)nz[5];
+    const R F0 = (R)F[3 * o], F1 = (R)F[3 * o + 1], F2 = (R)F[3 * o + 2];
+    const R T0 = (R)T[3 * o], T1 = (R)T[3 * o + 1], T2 = (R)T[3 * o + 2];
+    F[3 * o] = (double)(F0 + fx * q[0] + fy * q[3] + fz * q[6]);
+    F[3 * o + 1] = (double)(F1 + fx * q[1] + fy * q[4] + fz * q[7]);
+    F[3 * o + 2] = (double)(F2 + fx * q[2] + fy * q[5] + fz * q[8]);
+    T[3 * o] = (double)(tbx + T0 * q[0] + T1 * q[1] + T2 * q[2]);
+    T[3 * o + 1] = (double)(tby + T0 * q[3] + T1 * q[4] + T2 * q[5]);
+    T[3 * o + 2] = (double)(tbz + T0 * q[6] + T1 * q[7] + T2 * q[8]);
+}
+
+// LangevinNVT::advance_mom (Langevin.cpp:254-269), coefficients Langevin.cpp:29-53
+// (r per replica through its kT)
+template <class R>
+__global__ void __launch_bounds__(256) k_stage_advance_mom(int N, const PosRec *X, R *p,
+                                                           const DevRep *reps, const double *F,
+                                                           const double *Rn, double c5c2,
+                                                           double c5c3q, double c2, double c5)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    const PosRec xr = X[i];
+    const size_t o = xr.id;
+    const DevRep rp = reps[xr.meta >> 8];
+    const R rr = (R)rp.bead_r;
+    const R c4 = (R)c2 * rr;
+    for (int k = 0; k < 3; k++)
+        p[3 * (size_t)i + k] = (R)c5c2 * p[3 * (size_t)i + k] + (R)c5c3q * (R)F[3 * o + k] +
+                               ((R)c5 * c4 + rr) * (R)Rn[3 * o + k];
+}
+
 // [first, first+count) for p, L; the same range of original indices for obs
 template <class R>
 __global__ void __launch_bounds__(256) k_obs(int first, int count, const R *p, const R *L,

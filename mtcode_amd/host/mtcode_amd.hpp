@@ -1,0 +1,687 @@
+// mtcode_amd.hpp -- C++ host classes over the engine's C ABI (include/mdx.h)
+// with the API of the reference's system, potential and integrator classes,
+// so a driver written against MDBase (MD, LangevinNVT, LangevinNVTR, cube,
+// HSPotential, ComboPatch, ...) runs its step on the GPU.
+//
+// Header-only; needs only mdx.h (no HIP headers).  Link with -lmdx.
+//
+// State lives on the device between calls; the classes keep host mirrors of
+// the reference's matrices and copy in either direction only when a getter or
+// a setter needs it.  Each force or stage call corresponds to one reference
+// call (the stage entry points of the ABI); run() is the fast path: the whole
+// loop of Condensate::run (Condensate.cpp:318-377) or of the bead drivers
+// (mainActivePolymer.cpp:381-426) as device-resident steps.
+//
+// Differences from the reference, by design:
+//  - errors throw mtamd::mdx_error instead of error() printing, waiting on
+//    stdin and exiting (Basic/basic.h:61-71);
+//  - the pair list handed to the force calls is accepted for API
+//    compatibility: the engine evaluates every pair within the interaction
+//    reach, which equals the reference's result whenever the list holds all
+//    pairs within the cut-off (as calculatepairs() with cut >= reach builds);
+//  - matrix / vector1 are minimal row-major stand-ins holding what the
+//    reference's DataStructures/matrix2.h and vector1.h hold.
+//
+// Reference paths are relative to the reference's Code/ folder.
+#ifndef MTCODE_AMD_HPP
+#define MTCODE_AMD_HPP
+
+#include "mdx.h"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+namespace mtamd {
+
+class mdx_error : public std::runtime_error {
+  public:
+    int code;
+    mdx_error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(int rc)
+{
+    if (rc != MDX_OK)
+        throw mdx_error(rc, mdx_last_error());
+}
+
+// ------------------------------------------------------------ containers
+template <class T> class vector1 {
+    // bytes for bool (std::vector<bool> hands out proxies, not references)
+    using S = typename std::conditional<std::is_same<T, bool>::value, unsigned char, T>::type;
+    std::vector<S> v_;
+
+  public:
+    vector1() {}
+    explicit vector1(int n, T x = T()) : v_((size_t)n, (S)x) {}
+    int getsize() const { return (int)v_.size(); }
+    S &operator[](int i) { return v_[(size_t)i]; }
+    const S &operator[](int i) const { return v_[(size_t)i]; }
+};
+
+// row-major n x m (DataStructures/matrix2.h:24-37)
+template <class T> class matrix {
+    int n_ = 0, m_ = 0;
+    std::vector<T> v_;
+
+  public:
+    matrix() {}
+    matrix(int n, int m, T x = T()) : n_(n), m_(m), v_((size_t)n * m, x) {}
+    int getNsafe() const { return n_; }
+    int getnrows() const { return n_; }
+    int getncols() const { return m_; }
+    T &operator()(int i, int j) { return v_[(size_t)i * m_ + j]; }
+    const T &operator()(int i, int j) const { return v_[(size_t)i * m_ + j]; }
+    T *getdata() { return v_.data(); }
+    const T *getdata() const { return v_.data(); }
+    void reset(T x) { std::fill(v_.begin(), v_.end(), x); }
+    matrix &operator+=(const matrix &o)
+    {
+        if (o.n_ != n_ || o.m_ != m_)
+            throw mdx_error(MDX_ERR_ARG, "matrix sizes differ in +=");
+        for (size_t k = 0; k < v_.size(); k++)
+            v_[k] += o.v_[k];
+        return *this;
+    }
+};
+
+// ------------------------------------------------------------ geometry
+// cube (MDBase/geometry.h:43-66): edge l, periodic flag per dimension
+struct cube {
+    int dimension;
+    double l;
+    vector1<bool> pb;
+    bool periodic;
+    double l2s;
+
+    cube(double ll, vector1<bool> pbb, int dim) : dimension(dim), l(ll), pb(pbb)
+    {
+        if (pbb.getsize() != dim)
+            throw mdx_error(MDX_ERR_ARG, "boundary conditions must have the system's dimension");
+        periodic = pbb[0];
+        l2s = (ll / 2.) * (ll / 2.);
+    }
+
+    // generate_boxes_relationships (geometry.h:523-609): the 3^3 stencil of
+    // each of num^3 cells (index i + num j + num^2 k), periodic, sorted and
+    // without repeats; tb receives the row length
+    matrix<int> generate_boxes_relationships(int num, int &tb) const
+    {
+        if (num < 1)
+            throw mdx_error(MDX_ERR_ARG, "cells per side must be >= 1");
+        const int nc = num * num * num;
+        std::vector<std::vector<int>> rows((size_t)nc);
+        size_t width = 0;
+        for (int k = 0; k < num; k++)
+            for (int j = 0; j < num; j++)
+                for (int i = 0; i < num; i++) {
+                    std::vector<int> r;
+                    for (int c = -1; c <= 1; c++)
+                        for (int b = -1; b <= 1; b++)
+                            for (int a = -1; a <= 1; a++) {
+                                const int ii = ((i + a) % num + num) % num;
+                                const int jj = ((j + b) % num + num) % num;
+                                const int kk = ((k + c) % num + num) % num;
+                                r.push_back(ii + num * jj + num * num * kk);
+                            }
+                    std::sort(r.begin(), r.end());
+                    r.erase(std::unique(r.begin(), r.end()), r.end());
+                    width = std::max(width, r.size());
+                    rows[(size_t)i + num * j + (size_t)num * num * k] = r;
+                }
+        matrix<int> out(nc, (int)width);
+        for (int c = 0; c < nc; c++)
+            for (size_t w = 0; w < rows[(size_t)c].size(); w++)
+                out(c, (int)w) = rows[(size_t)c][w];
+        tb = (int)width;
+        return out;
+    }
+};
+
+// ------------------------------------------------------------ potentials
+// HSPotential (Potentials/potential.h:79-125): WCA with the literal 1.12246
+// sigma cut-off
+struct HSPotential {
+    double epsilon, sigma;
+    HSPotential(double a, double b) : epsilon(a), sigma(b) {}
+};
+// HarmonicPotential (potential.h:265-297): 0.5 k (r - x0)^2
+struct HarmonicPotential {
+    double k, x0;
+    HarmonicPotential(double a, double b) : k(a), x0(b) {}
+};
+// FENEPotential (potential.h:345-376)
+struct FENEPotential {
+    double kbond, R_0;
+    FENEPotential(double a, double b) : kbond(a), R_0(b) {}
+};
+// BendingPotential (Potentials/potential3.h:23-130): 0.5 k (theta - theta0)^2
+struct BendingPotential {
+    double k0, theta0;
+    BendingPotential(double a, double b) : k0(a), theta0(b) {}
+};
+// k (1 - cos theta) on consecutive bonds (no reference counterpart)
+struct CosineBendingPotential {
+    double k;
+    explicit CosineBendingPotential(double a) : k(a) {}
+};
+
+// ComboPatch (Potentials/ComboPatch/combopatch.h:8-29): patch directions per
+// type (body frame), KernFrenkelOnePatch2 rows (eps, d, theta_m) in
+// GeneralPatch order, and the particle index ranges of the types
+struct ComboPatch {
+    std::vector<int> npatch;   // per type
+    std::vector<double> dirs;  // sum(npatch) x 3, type-major
+    std::vector<double> rows;  // rows x 3
+    std::vector<int> num_per_type; // cumulative end index of each type; empty: all type 0
+    virtual ~ComboPatch() {}
+    int ntypes() const { return (int)npatch.size(); }
+    std::vector<int> types(int N) const
+    {
+        std::vector<int> t((size_t)N, 0);
+        if (num_per_type.empty())
+            return t;
+        int ty = 0;
+        for (int i = 0; i < N; i++) {
+            while (ty < ntypes() && i >= num_per_type[(size_t)ty])
+                ty++;
+            if (ty >= ntypes())
+                throw mdx_error(MDX_ERR_ARG, "particle index beyond the last type range");
+            t[(size_t)i] = ty;
+        }
+        return t;
+    }
+};
+
+// TetrahedralPatch(strength, distance, angle) (TetrahedralPatch.cpp:4-82):
+// one type, 4 tetrahedral patches, 16 equal rows
+struct TetrahedralPatch : ComboPatch {
+    TetrahedralPatch(double strr, double disss, double angg)
+    {
+        npatch = {4};
+        const double a = std::sqrt(8.0 / 9.0), b = std::sqrt(2.0 / 9.0), c = std::sqrt(2.0 / 3.0);
+        dirs = {a, 0.0, -1.0 / 3.0, -b, c, -1.0 / 3.0, -b, -c, -1.0 / 3.0, 0.0, 0.0, 1.0};
+        for (int k = 0; k < 16; k++)
+            rows.insert(rows.end(), {strr, disss, angg});
+    }
+};
+
+// BivalentPatch(strength, distance, angle) (BivalentPatch.h:6-12): one type,
+// patches along -x and +x, 4 equal rows
+struct BivalentPatch : ComboPatch {
+    BivalentPatch(double strr, double disss, double angg)
+    {
+        npatch = {2};
+        dirs = {-1.0, 0.0, 0.0, 1.0, 0.0, 0.0};
+        for (int k = 0; k < 4; k++)
+            rows.insert(rows.end(), {strr, disss, angg});
+    }
+};
+
+// GeneralPatch(patches per type, cumulative particle count per type, rows,
+// directions) (GeneralPatch.cpp:7-121)
+struct GeneralPatch : ComboPatch {
+    GeneralPatch(vector1<int> no_patches_per_type, vector1<int> num_per_typee,
+                 matrix<double> &params, matrix<double> &orient, bool = true)
+    {
+        const int nt = no_patches_per_type.getsize();
+        if (num_per_typee.getsize() != nt)
+            throw mdx_error(MDX_ERR_ARG, "one particle count per type required");
+        int nd = 0, nr = 0;
+        for (int t = 0; t < nt; t++) {
+            npatch.push_back(no_patches_per_type[t]);
+            num_per_type.push_back(num_per_typee[t]);
+            nd += no_patches_per_type[t];
+        }
+        for (int i = 0; i < nt; i++)
+            for (int j = i; j < nt; j++)
+                nr += no_patches_per_type[i] * no_patches_per_type[j];
+        if (orient.getNsafe() != nd || orient.getncols() != 3 || params.getNsafe() != nr ||
+            params.getncols() < 3)
+            throw mdx_error(MDX_ERR_ARG, "GeneralPatch: direction or parameter table size");
+        dirs.assign(orient.getdata(), orient.getdata() + 3 * (size_t)nd);
+        for (int r = 0; r < nr; r++)
+            rows.insert(rows.end(), {params(r, 0), params(r, 1), params(r, 2)});
+    }
+};
+
+// ------------------------------------------------------------ integrators
+// Interaction terms of a whole step (run()).
+struct StepTerms {
+    const HSPotential *wca = nullptr;
+    const ComboPatch *patches = nullptr;
+    const matrix<int> *bonds = nullptr;          // B x 2
+    const HarmonicPotential *harmonic = nullptr; // one k, r0 for all bonds, or
+    const FENEPotential *fene = nullptr;
+    const matrix<int> *triplets = nullptr;       // A x 3
+    const BendingPotential *bending = nullptr;
+    const CosineBendingPotential *cosine = nullptr;
+    const matrix<int> *exclusions = nullptr;     // pairs without WCA
+};
+
+// LangevinNVT (MDBase/Langevin.h, Langevin.cpp): translational Langevin
+// dynamics of point particles in a cube
+class LangevinNVT {
+  protected:
+    cube geo_;
+    double kT_ = 1, gamma_ = 1, dt_ = 0.005, m_ = 1;
+    double gammar_ = 1, im_[3] = {0.1, 0.1, 0.1}, maxd2_ = 2.4;
+    int integ_;
+    int prec_ = MDX_PREC_FP64;
+    unsigned long long seed_ = 1;
+    int device_ = 0;
+    matrix<double> dat_, mom_, orient_, angmom_;
+    mdx_engine *e_ = nullptr;
+    bool host_stale_ = false; // the device holds a newer state
+    bool dev_stale_ = true;   // the host mirrors hold a newer state
+
+    bool rotor() const { return integ_ == MDX_INTEG_ROTOR; }
+    int N() const { return dat_.getNsafe(); }
+
+    void pull()
+    {
+        if (!host_stale_ || !e_)
+            return;
+        check(mdx_download_state(e_, dat_.getdata(), mom_.getdata(),
+                                 rotor() ? orient_.getdata() : nullptr,
+                                 rotor() ? angmom_.getdata() : nullptr));
+        host_stale_ = false;
+    }
+    // parameters held by the engine descriptor: a change rebuilds the engine
+    void drop_engine()
+    {
+        if (!e_)
+            return;
+        pull();
+        mdx_destroy(e_);
+        e_ = nullptr;
+        dev_stale_ = true;
+    }
+    void touch_state()
+    {
+        pull();
+        dev_stale_ = true;
+    }
+    mdx_engine *engine()
+    {
+        if (N() == 0)
+            throw mdx_error(MDX_ERR_STATE, "no particles (setdat first)");
+        if (!e_) {
+            const int n = N();
+            const double box[3] = {geo_.l, geo_.l, geo_.l};
+            mdx_desc d = {};
+            d.nreplicas = 1;
+            d.n_per_replica = &n;
+            d.box = box;
+            d.periodic = geo_.periodic ? 1 : 0;
+            d.kT = &kT_;
+            d.dt = dt_;
+            d.m = m_;
+            d.gamma = gamma_;
+            d.gammar = gammar_;
+            for (int k = 0; k < 3; k++)
+                d.inertia[k] = im_[k];
+            d.precision = prec_;
+            d.integrator = integ_;
+            d.seed = seed_;
+            d.device = device_;
+            check(mdx_create(&d, &e_));
+            dev_stale_ = true;
+        }
+        if (dev_stale_) {
+            check(mdx_upload_state(e_, dat_.getdata(), mom_.getdata(),
+                                   rotor() ? orient_.getdata() : nullptr,
+                                   rotor() ? angmom_.getdata() : nullptr));
+            dev_stale_ = false;
+            host_stale_ = false;
+        }
+        return e_;
+    }
+    void set_param(double &field, double v)
+    {
+        if (field != v) {
+            drop_engine();
+            field = v;
+        }
+    }
+    void apply_terms(const StepTerms &t)
+    {
+        mdx_engine *e = engine();
+        check(mdx_clear_interactions(e, MDX_TERM_ALL));
+        const int n = N();
+        if (t.wca)
+            check(mdx_set_pair_wca(e, t.wca->epsilon, t.wca->sigma, 0.0));
+        if (t.patches) {
+            const std::vector<int> ty = t.patches->types(n);
+            check(mdx_set_patches(e, t.patches->ntypes(), ty.data(), t.patches->npatch.data(),
+                                  t.patches->dirs.data(), t.patches->rows.data(), maxd2_));
+        }
+        if (t.bonds && (t.harmonic || t.fene)) {
+            const long long nb = t.bonds->getNsafe();
+            std::vector<double> k((size_t)nb, t.harmonic ? t.harmonic->k : t.fene->kbond);
+            std::vector<double> r((size_t)nb, t.harmonic ? t.harmonic->x0 : t.fene->R_0);
+            check(mdx_set_bonds(e, nb, t.bonds->getdata(), k.data(), r.data(),
+                                t.harmonic ? MDX_BOND_HARMONIC : MDX_BOND_FENE));
+        }
+        if (t.triplets && (t.bending || t.cosine)) {
+            const long long na = t.triplets->getNsafe();
+            std::vector<double> k((size_t)na, t.bending ? t.bending->k0 : t.cosine->k);
+            std::vector<double> th((size_t)na, t.bending ? t.bending->theta0 : 0.0);
+            check(mdx_set_angles(e, na, t.triplets->getdata(), k.data(), th.data(),
+                                 t.bending ? MDX_ANGLE_HARMONIC : MDX_ANGLE_COSINE));
+        }
+        if (t.exclusions)
+            check(mdx_set_exclusions(e, t.exclusions->getNsafe(), t.exclusions->getdata()));
+    }
+    // conservative forces (and patch torques) of the given terms, lab frame
+    void forces(const StepTerms &t, matrix<double> &F, matrix<double> *T)
+    {
+        apply_terms(t);
+        F = matrix<double>(N(), 3);
+        matrix<double> Tt(N(), 3);
+        check(mdx_compute_forces(e_, F.getdata(), rotor() ? Tt.getdata() : nullptr));
+        if (T)
+            *T = Tt;
+    }
+
+  public:
+    explicit LangevinNVT(cube &a) : geo_(a), integ_(MDX_INTEG_BEAD) {}
+    LangevinNVT(const LangevinNVT &) = delete;
+    LangevinNVT &operator=(const LangevinNVT &) = delete;
+    virtual ~LangevinNVT()
+    {
+        if (e_)
+            mdx_destroy(e_);
+    }
+
+    // ---- MD (MDBase/MD.h)
+    virtual void setdat(const matrix<double> &x)
+    {
+        if (x.getncols() != 3)
+            throw mdx_error(MDX_ERR_ARG, "positions must be N x 3");
+        pull();
+        if (x.getNsafe() != N())
+            drop_engine();
+        if (mom_.getNsafe() != x.getNsafe())
+            mom_ = matrix<double>(x.getNsafe(), 3);
+        dat_ = x;
+        dev_stale_ = true;
+    }
+    matrix<double> getdat()
+    {
+        pull();
+        return dat_;
+    }
+    int getN() const { return N(); }
+    cube getgeo() const { return geo_; }
+
+    // ---- LangevinNVT (MDBase/Langevin.h)
+    virtual void setgamma(double g) { set_param(gamma_, g); }
+    virtual void setdt(double t) { set_param(dt_, t); }
+    virtual void setkT(double t) { set_param(kT_, t); }
+    void setm(double v) { set_param(m_, v); }
+    void setmom(matrix<double> &p)
+    {
+        if (p.getNsafe() != N() || p.getncols() != 3)
+            throw mdx_error(MDX_ERR_ARG, "momenta must be N x 3");
+        touch_state();
+        mom_ = p;
+    }
+    matrix<double> getmom()
+    {
+        pull();
+        return mom_;
+    }
+    double getkT() const { return kT_; }
+    double getm() const { return m_; }
+    double getgamma() const { return gamma_; }
+    double getdt() const { return dt_; }
+
+    // ---- engine options (no reference counterpart)
+    void setprecision(int p) { set_param_i(prec_, p); }
+    void setdevice(int d) { set_param_i(device_, d); }
+    void setseed(unsigned long long s)
+    {
+        if (s != seed_) {
+            drop_engine();
+            seed_ = s;
+        }
+    }
+    mdx_engine *handle() { return engine(); }
+
+    // ---- pair search (MD::calculatepairs, MD.cpp:1035-1098): every i < j
+    // with distance below cut (cube::distance_less_than, geometry.h:145-164),
+    // cell side from the stencil table's row count; caller deletes
+    matrix<int> *calculatepairs(matrix<int> &boxes, double cut)
+    {
+        int n = (int)std::lround(std::cbrt((double)boxes.getNsafe()));
+        if (n < 1 || n * n * n != boxes.getNsafe())
+            throw mdx_error(MDX_ERR_ARG, "stencil table must have n^3 rows");
+        const int nn[3] = {n, n, n};
+        mdx_engine *e = engine();
+        long long np = 0;
+        int rc = mdx_neighbor_pairs(e, 0, nn, cut, nullptr, 0, &np);
+        if (rc != MDX_OK && rc != MDX_ERR_CAPACITY)
+            check(rc);
+        matrix<int> *P = new matrix<int>((int)np, 2);
+        rc = mdx_neighbor_pairs(e, 0, nn, cut, P->getdata(), np, &np);
+        if (rc != MDX_OK) {
+            delete P;
+            check(rc);
+        }
+        return P;
+    }
+
+    // ---- forces (MD::calculateforces, MD.cpp:1684-1764, and friends); the
+    // pair/bond lists are the terms' support (see the header comment)
+    matrix<double> calculateforces(matrix<int> &, HSPotential &p)
+    {
+        StepTerms t;
+        t.wca = &p;
+        matrix<double> F;
+        forces(t, F, nullptr);
+        return F;
+    }
+    matrix<double> calculateforces(matrix<int> &bonds, HarmonicPotential &p)
+    {
+        StepTerms t;
+        t.bonds = &bonds;
+        t.harmonic = &p;
+        matrix<double> F;
+        forces(t, F, nullptr);
+        return F;
+    }
+    matrix<double> calculateforces(matrix<int> &bonds, FENEPotential &p)
+    {
+        StepTerms t;
+        t.bonds = &bonds;
+        t.fene = &p;
+        matrix<double> F;
+        forces(t, F, nullptr);
+        return F;
+    }
+    // per-bond rest lengths, one k (MD.cpp:2037-2111)
+    matrix<double> calculateforcesharmonic(matrix<int> &bonds, matrix<double> &bondlengths, double k)
+    {
+        mdx_engine *e = engine();
+        check(mdx_clear_interactions(e, MDX_TERM_ALL));
+        const long long nb = bonds.getNsafe();
+        std::vector<double> kk((size_t)nb, k), r0((size_t)nb);
+        for (long long b = 0; b < nb; b++)
+            r0[(size_t)b] = bondlengths((int)b, 0);
+        check(mdx_set_bonds(e, nb, bonds.getdata(), kk.data(), r0.data(), MDX_BOND_HARMONIC));
+        matrix<double> F(N(), 3), T(N(), 3);
+        check(mdx_compute_forces(e, F.getdata(), rotor() ? T.getdata() : nullptr));
+        return F;
+    }
+    matrix<double> calculateforces_threebody(matrix<int> &triplets, BendingPotential &p)
+    {
+        StepTerms t;
+        t.triplets = &triplets;
+        t.bending = &p;
+        matrix<double> F;
+        forces(t, F, nullptr);
+        return F;
+    }
+    matrix<double> calculateforces_threebody(matrix<int> &triplets, CosineBendingPotential &p)
+    {
+        StepTerms t;
+        t.triplets = &triplets;
+        t.cosine = &p;
+        matrix<double> F;
+        forces(t, F, nullptr);
+        return F;
+    }
+
+    // ---- stages (Langevin.cpp:205-269)
+    void advance_pos()
+    {
+        check(mdx_stage_advance_pos(engine()));
+        host_stale_ = true;
+    }
+    virtual void advance_mom(matrix<double> &F, matrix<double> &R)
+    {
+        check(mdx_stage_advance_mom(engine(), F.getdata(), R.getdata()));
+        host_stale_ = true;
+    }
+
+    // ---- whole steps on the device (bead loop, mainActivePolymer.cpp:390-425):
+    // noise null (engine Philox) or nsteps x N x 3
+    void run(int nsteps, const StepTerms &t, const double *noise = nullptr)
+    {
+        if (rotor())
+            throw mdx_error(MDX_ERR_STATE, "use LangevinNVTR::run for rotors");
+        apply_terms(t);
+        check(mdx_step(e_, nsteps, noise));
+        host_stale_ = true;
+    }
+
+  protected:
+    void set_param_i(int &field, int v)
+    {
+        if (field != v) {
+            drop_engine();
+            field = v;
+        }
+    }
+};
+
+// LangevinNVTR (MDBase/Rotational/LangevinR.h, LangevinR.cpp): rotors with
+// orientation (lab -> body) and body-frame angular momentum
+class LangevinNVTR : public LangevinNVT {
+  public:
+    explicit LangevinNVTR(cube &a) : LangevinNVT(a) { integ_ = MDX_INTEG_ROTOR; }
+
+    void setdat(const matrix<double> &x) override
+    {
+        LangevinNVT::setdat(x);
+        if (orient_.getNsafe() != N()) {
+            orient_ = matrix<double>(N(), 9);
+            for (int i = 0; i < N(); i++)
+                orient_(i, 0) = orient_(i, 4) = orient_(i, 8) = 1.0;
+        }
+        if (angmom_.getNsafe() != N())
+            angmom_ = matrix<double>(N(), 3);
+    }
+    // initialize(x, p, Q, L) (LangevinR.cpp:308-325)
+    void initialize(matrix<double> &x, matrix<double> &p, matrix<double> &Q, matrix<double> &L)
+    {
+        setdat(x);
+        setmom(p);
+        setorientation(Q);
+        setangularmomenta(L);
+    }
+    void setgammar(double b) { set_param(gammar_, b); }
+    void setmaxdistance(double distance) { maxd2_ = distance; }
+    void setIM(const vector1<double> &im)
+    {
+        if (im.getsize() < 9)
+            throw mdx_error(MDX_ERR_ARG, "moment of inertia tensor has 9 entries");
+        set_param(im_[0], im[0]);
+        set_param(im_[1], im[4]);
+        set_param(im_[2], im[8]);
+    }
+    void setorientation(matrix<double> &ort)
+    {
+        if (ort.getNsafe() != N() || ort.getncols() != 9)
+            throw mdx_error(MDX_ERR_ARG, "set orient dimensions must match in MD");
+        touch_state();
+        orient_ = ort;
+    }
+    void setangularmomenta(matrix<double> &amom)
+    {
+        if (amom.getNsafe() != N() || amom.getncols() != 3)
+            throw mdx_error(MDX_ERR_ARG, "angular momenta must be N x 3");
+        touch_state();
+        angmom_ = amom;
+    }
+    matrix<double> &getorientation()
+    {
+        pull();
+        return orient_;
+    }
+    matrix<double> &getangmom()
+    {
+        pull();
+        return angmom_;
+    }
+
+    // calculate_forces_and_torques3D(pairs, ComboPatch&, F, T)
+    // (LangevinR.cpp:911-1047): adds the patch forces and lab torques
+    void calculate_forces_and_torques3D(matrix<int> &, ComboPatch &pots, matrix<double> &F,
+                                        matrix<double> &T)
+    {
+        StepTerms t;
+        t.patches = &pots;
+        matrix<double> Fp, Tp;
+        forces(t, Fp, &Tp);
+        F += Fp;
+        T += Tp;
+    }
+    // create_forces_and_torques_sphere(F, T, RT) (LangevinR.cpp:475-578)
+    void create_forces_and_torques_sphere(matrix<double> &F, matrix<double> &T, matrix<double> &RT)
+    {
+        check(mdx_stage_sphere(engine(), F.getdata(), T.getdata(), RT.getdata()));
+    }
+    // advancemom_halfstep(F, T) (LangevinR.cpp:327-347)
+    void advancemom_halfstep(matrix<double> &F, matrix<double> &T)
+    {
+        check(mdx_stage_halfkick(engine(), F.getdata(), T.getdata()));
+        host_stale_ = true;
+    }
+    // rotate() (LangevinR.cpp:595-695)
+    void rotate()
+    {
+        check(mdx_stage_rotate(engine()));
+        host_stale_ = true;
+    }
+    // measured_temperature(out) (LangevinR.cpp:170-193): <p_k^2/2m>, <L_k^2/2I>
+    void measured_temperature(vector1<double> &out)
+    {
+        double o[16];
+        check(mdx_observables(engine(), -1, o));
+        out = vector1<double>(6);
+        for (int k = 0; k < 6; k++)
+            out[k] = o[k];
+    }
+
+    // ---- whole steps on the device: the loop of Condensate::run
+    // (Condensate.cpp:318-377), set-up force evaluation included.  noise
+    // null (engine Philox) or (nsteps + 1) x N x 6, row 0 for the set-up.
+    void run(int nsteps, const StepTerms &t, const double *noise = nullptr)
+    {
+        apply_terms(t);
+        check(mdx_prime(e_, noise));
+        if (nsteps > 0)
+            check(mdx_step(e_, nsteps, noise ? noise + 6 * (size_t)N() : nullptr));
+        host_stale_ = true;
+    }
+};
+
+} // namespace mtamd
+
+#endif

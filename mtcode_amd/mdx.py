@@ -73,6 +73,12 @@ def lib():
     L.mdx_set_active.argtypes = [C.c_void_p, C.c_double, _ip, _ip]
     L.mdx_set_force_guard.argtypes = [C.c_void_p, C.c_double]
     L.mdx_set_neighbor_list.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int]
+    L.mdx_stage_halfkick.argtypes = [C.c_void_p, _dp, _dp]
+    L.mdx_clear_interactions.argtypes = [C.c_void_p, C.c_int]
+    L.mdx_stage_advance_pos.argtypes = [C.c_void_p]
+    L.mdx_stage_rotate.argtypes = [C.c_void_p]
+    L.mdx_stage_sphere.argtypes = [C.c_void_p, _dp, _dp, _dp]
+    L.mdx_stage_advance_mom.argtypes = [C.c_void_p, _dp, _dp]
     L.mdx_cell_assign.argtypes = [C.c_void_p, C.c_int, _ip, _ip]
     L.mdx_neighbor_pairs.argtypes = [C.c_void_p, C.c_int, _ip, C.c_double, _ip, C.c_longlong,
                                      C.POINTER(C.c_longlong)]
@@ -274,6 +280,30 @@ class Engine:
         return {"N": int(out[0]), "cells": int(out[1]), "cell_size": out[2],
                 "device_bytes": int(out[3]), "steps": int(out[4]), "list_builds": int(out[5]),
                 "skin": float(out[6]), "kmax": int(out[7])}
+
+    # ------------------------------------------------- single stages
+    def stage_halfkick(self, F, T=None):
+        F, fp = _d(F)
+        T, tp = _d(T) if T is not None else (None, None)
+        _check(lib().mdx_stage_halfkick(self.h, fp, tp))
+
+    def stage_advance_pos(self):
+        _check(lib().mdx_stage_advance_pos(self.h))
+
+    def stage_rotate(self):
+        _check(lib().mdx_stage_rotate(self.h))
+
+    def stage_sphere(self, F, T, RT):
+        F = np.array(F, dtype=np.float64, order="C")
+        T = np.array(T, dtype=np.float64, order="C")
+        RT, rp = _d(RT)
+        _check(lib().mdx_stage_sphere(self.h, F.ctypes.data_as(_dp), T.ctypes.data_as(_dp), rp))
+        return F, T
+
+    def stage_advance_mom(self, F, R):
+        F, fp = _d(F)
+        R, rp = _d(R)
+        _check(lib().mdx_stage_advance_mom(self.h, fp, rp))
 
     def set_neighbor_list(self, skin=0.5, kmax=64, resort_every=200):
         _check(lib().mdx_set_neighbor_list(self.h, C.c_double(skin), int(kmax), int(resort_every)))

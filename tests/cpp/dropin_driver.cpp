@@ -1,0 +1,201 @@
+// dropin_driver.cpp -- drives the C++ host classes (mtcode_amd/host/mtcode_amd.hpp)
+// the way the reference's drivers drive MDBase, for tests/test_dropin.py:
+//
+//   rotor: the loop of Condensate::run (Condensate.cpp:318-377) with WCA and
+//          TetrahedralPatch, either call by call (mode "stages": calculatepairs,
+//          calculateforces, calculate_forces_and_torques3D,
+//          create_forces_and_torques_sphere, advancemom_halfstep, advance_pos,
+//          rotate) or as device-resident steps (mode "run")
+//   bead:  the bead loop (forces, advance_mom, advance_pos;
+//          mainActivePolymer.cpp:390-425) with WCA, harmonic bonds and cosine
+//          bending, call by call or through run()
+//
+// usage: dropin_driver <dir>
+//   dir/job.txt  key value lines: integ mode N L nsteps dt gamma gammar kT
+//                precision eps_p d thetam nbonds nangles kbond r0 kappa
+//   dir/x.bin (N x 3), Q.bin (N x 9, rotor), noise.bin, bonds.bin, angles.bin
+//   writes dir/out_x.bin, out_p.bin (+ out_Q.bin, out_L.bin for rotors)
+//   integ "stencil" with key n: writes the stencil table out_stencil.bin (no GPU)
+#include "mtcode_amd.hpp"
+
+#include <cstdio>
+#include <fstream>
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace mtamd;
+
+static std::string g_dir;
+static std::map<std::string, std::string> g_job;
+
+template <class T> static std::vector<T> rd(const std::string &name, size_t n)
+{
+    std::vector<T> v(n);
+    std::ifstream f(g_dir + "/" + name, std::ios::binary);
+    if (!f.read(reinterpret_cast<char *>(v.data()), (std::streamsize)(n * sizeof(T))))
+        throw mdx_error(MDX_ERR_ARG, "cannot read " + name);
+    return v;
+}
+
+template <class T> static void wr(const std::string &name, const T *p, size_t n)
+{
+    std::ofstream f(g_dir + "/" + name, std::ios::binary);
+    f.write(reinterpret_cast<const char *>(p), (std::streamsize)(n * sizeof(T)));
+}
+
+static double jd(const std::string &k) { return std::stod(g_job.at(k)); }
+static int ji(const std::string &k) { return std::stoi(g_job.at(k)); }
+
+template <class T> static matrix<T> to_mat(const std::vector<T> &v, int n, int m)
+{
+    matrix<T> a(n, m);
+    std::copy(v.begin(), v.end(), a.getdata());
+    return a;
+}
+
+static matrix<double> noise_row(const std::vector<double> &nz, int row, int N, int w)
+{
+    matrix<double> R(N, w);
+    std::copy(nz.begin() + (size_t)row * N * w, nz.begin() + (size_t)(row + 1) * N * w, R.getdata());
+    return R;
+}
+
+static int rotor_main()
+{
+    const int N = ji("N"), nsteps = ji("nsteps");
+    vector1<bool> pb(3, true);
+    cube geo(jd("L"), pb, 3);
+    LangevinNVTR b(geo);
+    b.setprecision(ji("precision"));
+    b.setdt(jd("dt"));
+    b.setgamma(jd("gamma"));
+    b.setgammar(jd("gammar"));
+    b.setkT(jd("kT"));
+    matrix<double> x = to_mat(rd<double>("x.bin", (size_t)N * 3), N, 3);
+    matrix<double> Q = to_mat(rd<double>("Q.bin", (size_t)N * 9), N, 9);
+    matrix<double> p(N, 3), L(N, 3);
+    b.initialize(x, p, Q, L);
+    HSPotential wsa(1.0, 1.0);
+    TetrahedralPatch pots(jd("eps_p"), jd("d"), jd("thetam"));
+    const std::vector<double> nz = rd<double>("noise.bin", (size_t)(nsteps + 1) * N * 6);
+    if (g_job.at("mode") == "run") {
+        StepTerms t;
+        t.wca = &wsa;
+        t.patches = &pots;
+        b.run(nsteps, t, nz.data());
+    } else {
+        // Condensate::run, call by call
+        int tb;
+        const int n = std::max(1, (int)(geo.l / 4.0));
+        matrix<int> boxes = geo.generate_boxes_relationships(n, tb);
+        matrix<int> *pairs = b.calculatepairs(boxes, 3.5);
+        matrix<double> F = b.calculateforces(*pairs, wsa);
+        matrix<double> T(N, 3);
+        b.calculate_forces_and_torques3D(*pairs, pots, F, T);
+        matrix<double> RT = noise_row(nz, 0, N, 6);
+        b.create_forces_and_torques_sphere(F, T, RT);
+        for (int i = 0; i < nsteps; i++) {
+            b.advancemom_halfstep(F, T);
+            b.advance_pos();
+            b.rotate();
+            if (i > 0 && i % 20 == 0) {
+                delete pairs;
+                pairs = b.calculatepairs(boxes, 3.5);
+            }
+            F = b.calculateforces(*pairs, wsa);
+            T.reset(0.0);
+            b.calculate_forces_and_torques3D(*pairs, pots, F, T);
+            RT = noise_row(nz, i + 1, N, 6);
+            b.create_forces_and_torques_sphere(F, T, RT);
+            b.advancemom_halfstep(F, T);
+        }
+        delete pairs;
+    }
+    matrix<double> xo = b.getdat(), po = b.getmom();
+    wr("out_x.bin", xo.getdata(), (size_t)N * 3);
+    wr("out_p.bin", po.getdata(), (size_t)N * 3);
+    wr("out_Q.bin", b.getorientation().getdata(), (size_t)N * 9);
+    wr("out_L.bin", b.getangmom().getdata(), (size_t)N * 3);
+    return 0;
+}
+
+static int bead_main()
+{
+    const int N = ji("N"), nsteps = ji("nsteps");
+    vector1<bool> pb(3, true);
+    cube geo(jd("L"), pb, 3);
+    LangevinNVT b(geo);
+    b.setprecision(ji("precision"));
+    b.setdt(jd("dt"));
+    b.setgamma(jd("gamma"));
+    b.setkT(jd("kT"));
+    matrix<double> x = to_mat(rd<double>("x.bin", (size_t)N * 3), N, 3);
+    b.setdat(x);
+    const int nb = ji("nbonds"), na = ji("nangles");
+    matrix<int> bonds = to_mat(rd<int>("bonds.bin", (size_t)nb * 2), nb, 2);
+    matrix<int> angles = to_mat(rd<int>("angles.bin", (size_t)na * 3), na, 3);
+    HSPotential wsa(1.0, 1.0);
+    HarmonicPotential harm(jd("kbond"), jd("r0"));
+    CosineBendingPotential bend(jd("kappa"));
+    const std::vector<double> nz = rd<double>("noise.bin", (size_t)nsteps * N * 3);
+    if (g_job.at("mode") == "run") {
+        StepTerms t;
+        t.wca = &wsa;
+        t.bonds = &bonds;
+        t.harmonic = &harm;
+        t.triplets = &angles;
+        t.cosine = &bend;
+        b.run(nsteps, t, nz.data());
+    } else {
+        int tb;
+        const int n = std::max(1, (int)(geo.l / 4.0));
+        matrix<int> boxes = geo.generate_boxes_relationships(n, tb);
+        matrix<int> *pairs = b.calculatepairs(boxes, 3.5);
+        for (int i = 0; i < nsteps; i++) {
+            if (i > 0 && i % 20 == 0) {
+                delete pairs;
+                pairs = b.calculatepairs(boxes, 3.5);
+            }
+            matrix<double> F = b.calculateforces(*pairs, wsa);
+            F += b.calculateforces(bonds, harm);
+            F += b.calculateforces_threebody(angles, bend);
+            matrix<double> R = noise_row(nz, i, N, 3);
+            b.advance_mom(F, R);
+            b.advance_pos();
+        }
+        delete pairs;
+    }
+    matrix<double> xo = b.getdat(), po = b.getmom();
+    wr("out_x.bin", xo.getdata(), (size_t)N * 3);
+    wr("out_p.bin", po.getdata(), (size_t)N * 3);
+    return 0;
+}
+
+int main(int argc, char **argv)
+{
+    if (argc != 2) {
+        fprintf(stderr, "usage: dropin_driver <dir>\n");
+        return 2;
+    }
+    g_dir = argv[1];
+    std::ifstream f(g_dir + "/job.txt");
+    std::string k, v;
+    while (f >> k >> v)
+        g_job[k] = v;
+    try {
+        if (g_job.at("integ") == "stencil") {
+            // cube::generate_boxes_relationships only (no GPU)
+            vector1<bool> pb(3, true);
+            cube geo(1.0, pb, 3);
+            int tb;
+            matrix<int> st = geo.generate_boxes_relationships(ji("n"), tb);
+            wr("out_stencil.bin", st.getdata(), (size_t)st.getNsafe() * tb);
+            return 0;
+        }
+        return g_job.at("integ") == "rotor" ? rotor_main() : bead_main();
+    } catch (const std::exception &e) {
+        fprintf(stderr, "dropin_driver: %s\n", e.what());
+        return 1;
+    }
+}

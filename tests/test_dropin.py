@@ -1,0 +1,105 @@
+"""The C++ host classes (mtcode_amd/host/mtcode_amd.hpp) used the way the
+reference's drivers use MDBase: tests/cpp/dropin_driver runs the loop of
+Condensate::run call by call (calculatepairs, calculateforces,
+calculate_forces_and_torques3D, create_forces_and_torques_sphere,
+advancemom_halfstep, advance_pos, rotate) or as device-resident steps
+(run()), and the bead loop likewise; final states are compared with the
+oracle driven by the same noise."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from mtcode_amd import workloads as wl
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "tests", "cpp", "dropin_driver")
+
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def run_driver(d, job, arrays):
+    with open(os.path.join(d, "job.txt"), "w") as f:
+        for k, v in job.items():
+            f.write(f"{k} {v}\n")
+    for k, v in arrays.items():
+        v.tofile(os.path.join(d, k + ".bin"))
+    if not os.path.exists(DRIVER):
+        raise RuntimeError("tests/cpp/dropin_driver is not built (run __graft_entry__.build())")
+    r = subprocess.run([DRIVER, d], capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        raise RuntimeError(f"dropin_driver failed: {r.stderr}")
+
+
+def rd(d, name, shape):
+    return np.fromfile(os.path.join(d, name), dtype=np.float64).reshape(shape)
+
+
+def relmax(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 6, 16])
+def test_stencil_matches_reference_fixture(tmp_path, golden, n):
+    """cube::generate_boxes_relationships of the C++ classes (CPU only)"""
+    g = golden("cells_patchy4096.npz")
+    run_driver(str(tmp_path), {"integ": "stencil", "n": n}, {})
+    out = np.fromfile(os.path.join(str(tmp_path), "out_stencil.bin"), dtype=np.int32)
+    boxes = out.reshape(n ** 3, -1)
+    if f"boxes_n{n}" in g:
+        assert np.array_equal(boxes, g[f"boxes_n{n}"])
+    else:
+        h = (boxes.astype(np.int64) * np.arange(1, boxes.shape[1] + 1)).sum(1)
+        assert np.array_equal(h, g[f"boxes_n{n}_row_hash"])
+        assert boxes.sum() == g[f"boxes_n{n}_sum"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["stages", "run"])
+def test_rotor_condensate_loop_vs_oracle(tmp_path, mode):
+    s = wl.patchy_tetra(512)
+    N, nsteps, seed = s["N"], 30, 9
+    say("drop-in rotor", mode, N, nsteps)
+    nz = po.noise(seed, 0, (nsteps + 1) * N * 6)
+    e, d, th = s["patches"]["params"][0]
+    job = {"integ": "rotor", "mode": mode, "N": N, "L": repr(float(s["box"][0])),
+           "nsteps": nsteps, "dt": s["dt"], "gamma": s["gamma"], "gammar": repr(float(s["gammar"])),
+           "kT": s["kT"], "precision": 0, "eps_p": repr(float(e)), "d": repr(float(d)),
+           "thetam": repr(float(th))}
+    run_driver(str(tmp_path), job, {"x": s["x"], "Q": s["Q"], "noise": nz})
+    ref = po.run_rotors(s, nsteps, seed, 0, list_cut=3.5, rebuild_every=20)
+    for k, shape in (("x", (N, 3)), ("p", (N, 3)), ("Q", (N, 9)), ("L", (N, 3))):
+        err = relmax(rd(str(tmp_path), f"out_{k}.bin", shape), ref[k])
+        say(k, err)
+        assert err < 1e-10, k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["stages", "run"])
+def test_bead_loop_vs_oracle(tmp_path, mode):
+    s = wl.mt_bundle(2, 13, 40, spacing=6.0, box=(40.0, 40.0, 40.0))
+    s = dict(s)
+    s["exclusions"] = None  # the reference's bead loops have no exclusion list
+    N, nsteps, seed = s["N"], 30, 4
+    say("drop-in bead", mode, N, nsteps)
+    nz = po.noise(seed, 0, nsteps * N * 3)
+    assert np.all(s["bond_k"] == s["bond_k"][0]) and np.all(s["bond_r0"] == s["bond_r0"][0])
+    job = {"integ": "bead", "mode": mode, "N": N, "L": repr(float(s["box"][0])),
+           "nsteps": nsteps, "dt": s["dt"], "gamma": s["gamma"], "kT": s["kT"], "precision": 0,
+           "nbonds": len(s["bonds"]), "nangles": len(s["angles"]),
+           "kbond": repr(float(s["bond_k"][0])), "r0": repr(float(s["bond_r0"][0])),
+           "kappa": repr(float(s["angle_k"][0]))}
+    assert s["angle_form"] == 1
+    run_driver(str(tmp_path), job, {"x": s["x"], "noise": nz,
+                                    "bonds": np.asarray(s["bonds"], dtype=np.int32),
+                                    "angles": np.asarray(s["angles"], dtype=np.int32)})
+    ref = po.run_beads(s, nsteps, seed, 0, list_cut=3.5, rebuild_every=20)
+    for k in ("x", "p"):
+        err = relmax(rd(str(tmp_path), f"out_{k}.bin", (N, 3)), ref[k])
+        say(k, err)
+        assert err < 1e-10, k
