@@ -434,7 +434,8 @@ static int launch_gather(mdx_engine *e, const double *noise_dev, int lmode = 0)
             HIPCHK(hipEventRecord(e->prof_mid, e->stream));
         e->last_split = true;
         if (e->use_compact) {
-            hipLaunchKernelGGL((k_patch<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
+            hipLaunchKernelGGL((k_patch<R, MODE>), dim3(nblk(e->N, PBLK)), dim3(256), 0, e->stream,
+                               a);
             if (e->prof_mid2)
                 HIPCHK(hipEventRecord(e->prof_mid2, e->stream));
             hipLaunchKernelGGL((k_integrate<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream,

@@ -272,9 +272,22 @@ __device__ __forceinline__ void min_image(const KArgs<R> &a, const DevRep &rp, d
 }
 
 // ------------------------------------------------------------- gather kernel
+// the walk screens patch candidates (cone screens) before handing them over
+#ifndef MDX_WALK_SCREEN
+#define MDX_WALK_SCREEN 1
+#endif
 constexpr int PMAX = 8;   // deferred patch candidates per particle held in LDS
 constexpr int PMAXG = 16; // patch candidates per particle handed from walk to finish
 enum { PH_ALL = 0, PH_WALK = 1, PH_FINISH = 2, PH_INTEGRATE = 3 };
+
+// Workgroups go to the 8 XCDs round-robin; this gives each XCD one contiguous
+// range of blocks (of cell-ordered slots), so the neighbours a block gathers
+// are mostly in its own XCD's L2.  A bijection on [0, G).
+__device__ __forceinline__ int xcd_block(int b, int G)
+{
+    const int q = G >> 3, r = G & 7, x = b & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
 
 // neighbour slots of slot i: the 9 (y, z) rows of the 27-cell stencil; the x
 // neighbours of a row are contiguous in the cell list (one range, or two at
@@ -500,6 +513,40 @@ __device__ __forceinline__ void patch_pair(const KArgs<R> &a, const DevRep &rp, 
     }
 }
 
+// The cone screen of patch_pair on the particle's own patches alone (no
+// load of the partner): false when none of its patches passes, in which case
+// patch_pair adds exactly nothing for the pair.  Evaluated in R with a
+// reciprocal square root: its error (~1e-7 in FP32, ~1e-16 in FP64) is far
+// inside the screen's margin below cos(theta_m) (1e-5 FP32, 1e-12 FP64), so a
+// pair it rejects fails patch_pair's exact FP64 cone test as well.  The walk
+// uses it to hand over only candidates that can bind.
+template <class R>
+__device__ __forceinline__ bool own_screen(const KArgs<R> &a, uint32_t oi, int ti, const R *qi,
+                                           const PosRec &xj, const double d[3], double r2s)
+{
+    const int tj = xj.meta & 255;
+    int ta, tc;
+    const bool i_is_a = patch_roles(oi, ti, xj.id, tj, ta, tc);
+    // u = (d / r), sign folded into the test below
+    const R ri = (R)1 / sqrt((R)r2s);
+    const R ur[3] = {(R)d[0] * ri, (R)d[1] * ri, (R)d[2] * ri};
+    const R v[3] = {qi[0] * ur[0] + qi[1] * ur[1] + qi[2] * ur[2],
+                    qi[3] * ur[0] + qi[4] * ur[1] + qi[5] * ur[2],
+                    qi[6] * ur[0] + qi[7] * ur[1] + qi[8] * ur[2]};
+    const int t = i_is_a ? ta : tc;
+    const int np = a.np[t];
+    const double *dd = a.pdirs + 3 * a.pstart[t];
+    const R scr = (R)a.costm_screen;
+    bool pass = false;
+    for (int k = 0; k < np; k++) {
+        const R sd = (R)dd[3 * k] * v[0] + (R)dd[3 * k + 1] * v[1] + (R)dd[3 * k + 2] * v[2];
+        // u = +d/r for role a, whose test is -(n . u); u = -d/r for role c,
+        // whose test is n . u: both are -(n . d/r)
+        pass |= -sd > scr;
+    }
+    return pass;
+}
+
 // Force (lab) and patch torque (lab) of one particle, then the mode's
 // integrator epilogue.  Full neighbour list: every pair is evaluated from
 // both sides, each side keeps its own share; no atomics.  The neighbour loop
@@ -553,6 +600,11 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
         patch_roles(oi, ti, xj.id, tj, ta, tc);
         return (a.active_mask >> (ta * MAXT + tc)) & 1ull;
     };
+    // the candidates the split form hands over: those that pass the screen
+    auto is_listed_candidate = [&](const PosRec &xj, int js, const double d[3], double r2s) -> bool {
+        (void)js;
+        return is_patch_candidate(xj, r2s) && own_screen(a, oi, ti, qi, xj, d, r2s);
+    };
     if (PH == PH_FINISH) {
         for (int k = 0; k < 3; k++)
             acc.F[k] = a.Fw[3 * (size_t)s + k];
@@ -566,18 +618,22 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
                 double d[3];
                 min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
                 const double r2s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-                if (!(r2s < a.rmax2) || !is_patch_candidate(xj, r2s))
+                const int js = j >= 0 ? j : a.slot[xj.id];
+                if (!(r2s < a.rmax2) ||
+                    !(MDX_WALK_SCREEN ? is_listed_candidate(xj, js, d, r2s) : is_patch_candidate(xj, r2s)))
                     return;
                 if (seen++ >= PMAXG)
-                    patch_pair<R, MODE>(a, rp, xi, ti, qi, j >= 0 ? j : a.slot[xj.id], acc);
+                    patch_pair<R, MODE>(a, rp, xi, ti, qi, js, acc);
             });
         }
     }
     if (PH == PH_INTEGRATE) {
         // force with patches and patch torque from the compacted patch kernel
+        // (which leaves positions without candidates untouched)
+        const bool pt = a.has_patches && a.ncand[s] > 0;
         for (int k = 0; k < 3; k++) {
             acc.F[k] = a.Fw[3 * (size_t)s + k];
-            acc.T[k] = a.Tp[3 * (size_t)s + k];
+            acc.T[k] = pt ? a.Tp[3 * (size_t)s + k] : (R)0;
         }
     }
     if (PH != PH_FINISH && PH != PH_INTEGRATE && (a.wca || npi > 0)) {
@@ -629,14 +685,16 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
                     }
                 }
             }
-            if (is_patch_candidate(xj, r2s)) {
+            if (PH == PH_WALK) {
                 const int js = j >= 0 ? j : a.slot[xj.id];
-                if (PH == PH_WALK) {
+                if (MDX_WALK_SCREEN ? is_listed_candidate(xj, js, d, r2s) : is_patch_candidate(xj, r2s)) {
                     if (npc < PMAXG)
                         a.cand[(size_t)npc * a.N + s] = js;
-                } else if (npc < PMAX) {
-                    plist[npc * 256 + threadIdx.x] = js;
+                    npc++;
                 }
+            } else if (is_patch_candidate(xj, r2s)) {
+                if (npc < PMAX)
+                    plist[npc * 256 + threadIdx.x] = j >= 0 ? j : a.slot[xj.id];
                 npc++;
             }
         });
@@ -923,7 +981,7 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_gather(KAr
 template <class R, int MODE> __global__ void __launch_bounds__(256) k_walk(KArgs<R> a)
 {
     list_built(a);
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (s >= a.N)
         return;
     float drift = 0.f;
@@ -945,98 +1003,125 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_finish(KAr
         drift_bound(a, drift);
 }
 
-// split form, compacted: the patch candidates of a block's 256 positions are
-// laid out back to back and evaluated one per thread (no idle lanes on
-// positions without candidates); each position then adds its candidates'
-// terms in candidate order to its walk force (the same order as the one-kernel
-// form; bitwise equal to it while at most one patch pair of a particle pair
-// passes the cone tests, as for cos(theta_m) >= 0.5 with these patch sets)
+// split form, compacted: the patch candidates of a block's PBLK positions
+// are laid out back to back and evaluated one per thread (no idle lanes on
+// positions without candidates; one block per PBLK positions keeps the whole
+// grid resident).  The thread holding a position's first candidate of a
+// round then adds that round's terms in candidate order to the position's
+// force (the walk force in Fw) and patch torque (Tp): the same order as the
+// one-kernel form, and bitwise equal to it while at most one patch pair of a
+// particle pair passes the cone tests (true for cos(theta_m) >= 0.5 with
+// these patch sets).  Positions without candidates are not touched (the
+// integrator reads ncand).
+constexpr int PBLK = 2048;
 template <class R, int MODE> __global__ void __launch_bounds__(256) k_patch(KArgs<R> a)
 {
-    __shared__ int off[256];
-    __shared__ unsigned char owner[256 * PMAXG];
-    __shared__ PosRec sx[256];
-    __shared__ R sq[256 * 9];
+    constexpr int PER = PBLK / 256;
+    __shared__ int off[PBLK + 1];
     __shared__ R ent[256 * 6];
     __shared__ int wsum[4];
     const int tid = threadIdx.x;
-    const int s = blockIdx.x * 256 + tid;
-    const bool live = s < a.N;
-    const int nc = live ? a.ncand[s] : 0;
-    const int nq = nc < PMAXG ? nc : PMAXG;
-    R F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
-    if (live) {
-        const int i = a.lmode ? s : a.list[s];
-        sx[tid] = a.X[i];
-#pragma unroll
-        for (int k = 0; k < 9; k++)
-            sq[9 * tid + k] = a.Q[9 * (size_t)i + k];
-        for (int k = 0; k < 3; k++)
-            F[k] = a.Fw[3 * (size_t)s + k];
+    const int base = xcd_block(blockIdx.x, gridDim.x) * PBLK;
+    // exclusive scan of min(ncand, PMAXG) in position order (position
+    // k * 256 + tid of the block, k < PER)
+    int carry = 0;
+    for (int k = 0; k < PER; k++) {
+        const int p = base + k * 256 + tid;
+        const int nc = p < a.N ? a.ncand[p] : 0;
+        const int nq = nc < PMAXG ? nc : PMAXG;
+        int v = nq;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(v, o);
+            if ((tid & 63) >= o)
+                v += t;
+        }
+        if ((tid & 63) == 63)
+            wsum[tid >> 6] = v;
+        __syncthreads();
+        int wb = 0;
+        for (int w = 0; w < (tid >> 6); w++)
+            wb += wsum[w];
+        off[k * 256 + tid] = carry + wb + v - nq;
+        carry += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
     }
-    // exclusive scan of nq over the block
-    int v = nq;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int t = __shfl_up(v, o);
-        if ((tid & 63) >= o)
-            v += t;
-    }
-    if ((tid & 63) == 63)
-        wsum[tid >> 6] = v;
+    if (tid == 0)
+        off[PBLK] = carry;
     __syncthreads();
-    int wbase = 0;
-    for (int w = 0; w < (tid >> 6); w++)
-        wbase += wsum[w];
-    const int total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-    off[tid] = wbase + v - nq;
-    for (int k = 0; k < nq; k++)
-        owner[wbase + v - nq + k] = (unsigned char)tid;
-    __syncthreads();
-    for (int base = 0; base < total; base += 256) {
-        const int e = base + tid;
+    const int total = carry;
+    for (int r0 = 0; r0 < total; r0 += 256) {
+        const int e = r0 + tid;
+        int o = 0;
         if (e < total) {
-            const int o = owner[e];
+            // owner: the last position whose offset is <= e
+            int lo = 0, hi = PBLK - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (off[mid] <= e)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            o = lo;
+            const int so = base + o;
             const int q = e - off[o];
-            const PosRec xo = sx[o];
+            const int i = a.lmode ? so : a.list[so];
+            const PosRec xo = a.X[i];
             const DevRep rp = a.reps[xo.meta >> 8];
             R qo[9];
 #pragma unroll
             for (int k = 0; k < 9; k++)
-                qo[k] = sq[9 * o + k];
+                qo[k] = a.Q[9 * (size_t)i + k];
             Acc<R> ea;
             for (int k = 0; k < 3; k++)
                 ea.F[k] = ea.T[k] = 0;
-            patch_pair<R, MODE>(a, rp, xo, xo.meta & 255, qo,
-                                a.cand[(size_t)q * a.N + blockIdx.x * 256 + o], ea);
+            patch_pair<R, MODE>(a, rp, xo, xo.meta & 255, qo, a.cand[(size_t)q * a.N + so], ea);
             for (int k = 0; k < 3; k++) {
                 ent[6 * tid + k] = ea.F[k];
                 ent[6 * tid + 3 + k] = ea.T[k];
             }
         }
         __syncthreads();
-        const int lo = max(off[tid], base), hi = min(off[tid] + nq, base + 256);
-        for (int e2 = lo; e2 < hi; e2++)
-            for (int k = 0; k < 3; k++) {
-                F[k] += ent[6 * (e2 - base) + k];
-                T[k] += ent[6 * (e2 - base) + 3 + k];
+        if (e < total) {
+            const int first = max(off[o], r0);
+            if (e == first) {
+                const int last = min(off[o + 1], r0 + 256);
+                const size_t so = (size_t)base + o;
+                R F[3], T[3];
+                for (int k = 0; k < 3; k++) {
+                    F[k] = a.Fw[3 * so + k];
+                    T[k] = first == off[o] ? (R)0 : a.Tp[3 * so + k];
+                }
+                for (int e2 = first; e2 < last; e2++)
+                    for (int k = 0; k < 3; k++) {
+                        F[k] += ent[6 * (e2 - r0) + k];
+                        T[k] += ent[6 * (e2 - r0) + 3 + k];
+                    }
+                for (int k = 0; k < 3; k++) {
+                    a.Fw[3 * so + k] = F[k];
+                    a.Tp[3 * so + k] = T[k];
+                }
             }
+        }
+        __threadfence_block();
         __syncthreads();
     }
-    if (!live)
-        return;
-    if (nc > PMAXG) {
-        // candidates beyond the handed-over ones: walk again, in walk order
-        const int i = a.lmode ? s : a.list[s];
-        const PosRec xi = sx[tid];
+    // candidates beyond the handed-over ones: walk again, in walk order
+    for (int k = 0; k < PER; k++) {
+        const int so = base + k * 256 + tid;
+        if (so >= a.N || a.ncand[so] <= PMAXG)
+            continue;
+        const int i = a.lmode ? so : a.list[so];
+        const PosRec xi = a.X[i];
         const int ti = xi.meta & 255;
         const DevRep rp = a.reps[xi.meta >> 8];
         R qi[9];
-        for (int k = 0; k < 9; k++)
-            qi[k] = sq[9 * tid + k];
+        for (int k2 = 0; k2 < 9; k2++)
+            qi[k2] = a.Q[9 * (size_t)i + k2];
         Acc<R> acc;
-        for (int k = 0; k < 3; k++) {
-            acc.F[k] = F[k];
-            acc.T[k] = T[k];
+        for (int k2 = 0; k2 < 3; k2++) {
+            acc.F[k2] = a.Fw[3 * (size_t)so + k2];
+            acc.T[k2] = a.Tp[3 * (size_t)so + k2];
         }
         const int npi = a.np[ti];
         int seen = 0;
@@ -1053,24 +1138,23 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_patch(KArg
             patch_roles(xi.id, ti, xj.id, tj, ta, tc);
             if (!((a.active_mask >> (ta * MAXT + tc)) & 1ull))
                 return;
+            if (MDX_WALK_SCREEN && !own_screen(a, xi.id, ti, qi, xj, d, r2s))
+                return;
+            const int js = j >= 0 ? j : a.slot[xj.id];
             if (seen++ >= PMAXG)
-                patch_pair<R, MODE>(a, rp, xi, ti, qi, j >= 0 ? j : a.slot[xj.id], acc);
+                patch_pair<R, MODE>(a, rp, xi, ti, qi, js, acc);
         });
-        for (int k = 0; k < 3; k++) {
-            F[k] = acc.F[k];
-            T[k] = acc.T[k];
+        for (int k2 = 0; k2 < 3; k2++) {
+            a.Fw[3 * (size_t)so + k2] = acc.F[k2];
+            a.Tp[3 * (size_t)so + k2] = acc.T[k2];
         }
-    }
-    for (int k = 0; k < 3; k++) {
-        a.Fw[3 * (size_t)s + k] = F[k];
-        a.Tp[3 * (size_t)s + k] = T[k];
     }
 }
 
 // ... and the topology terms and the integrator
 template <class R, int MODE> __global__ void __launch_bounds__(256) k_integrate(KArgs<R> a)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     float drift = 0.f;
     if (s < a.N)
         process_particle<R, MODE, PH_INTEGRATE>(
@@ -1089,7 +1173,7 @@ template <class R> __global__ void __launch_bounds__(256) k_nbuild(KArgs<R> a, c
         return;
     if (blockIdx.x == 0 && threadIdx.x == 0)
         atomicAdd(&a.ctl->nbuilds, 1u);
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (s >= a.N)
         return;
     const PosRec xi = a.X[s];

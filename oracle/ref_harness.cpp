@@ -22,6 +22,7 @@
  *              with the injected noise of orc_noise_uniform
  *   traj_bead  x,p -> final state, mainActivePolymer step order, injected noise
  *   time_rotor x,Q -> wall-clock of the Condensate::run loop (OpenMP, rand())
+ *   long_rotor x,Q -> snapshots of a long Condensate::run loop (rand() noise)
  */
 #include <stdlib.h>
 #include <iostream>
@@ -516,6 +517,78 @@ static int cmd_time_rotor()
     return 0;
 }
 
+static int cmd_long_rotor()
+{
+    /* The Condensate::run loop (Condensate.cpp:288-377) with the reference's
+     * own rand() noise (srand(seed)) for nsteps steps; x, p, Q, L appended to
+     * snap_*.bin after every `every` steps. */
+    int N = (int)ji("N"), nsteps = (int)ji("nsteps"), n = (int)ji("n"), every = (int)ji("every");
+    cube geo = make_cube();
+    LangevinNVTR b(geo);
+    matrix<double> x = to_mat(rd<double>("x", (size_t)N * 3), N, 3);
+    matrix<double> Q = to_mat(rd<double>("Q", (size_t)N * 9), N, 9);
+    matrix<double> p(N, 3), L(N, 3);
+    b.initialize(x, p, Q, L);
+    set_rotor_params(b);
+    ComboPatch *pots = make_patches();
+    double cut = jd("cut");
+    srand((unsigned)ji("seed"));
+    int tb;
+    matrix<int> boxes = geo.generate_boxes_relationships(n, tb);
+    matrix<int> *pairs = b.calculatepairs(boxes, cut);
+    HSPotential wsa(jd("wca_eps"), jd("wca_sigma"));
+    matrix<double> F(N, 3), T(N, 3), RT(N, 6);
+    F = b.calculateforces(*pairs, wsa);
+    b.calculate_forces_and_torques3D(*pairs, *pots, F, T);
+    generate_uniform_random_matrix(RT);
+    b.create_forces_and_torques_sphere(F, T, RT);
+    FILE *fx = fopen((g_dir + "/snap_x.bin").c_str(), "wb");
+    FILE *fp = fopen((g_dir + "/snap_p.bin").c_str(), "wb");
+    FILE *fq = fopen((g_dir + "/snap_Q.bin").c_str(), "wb");
+    FILE *fl = fopen((g_dir + "/snap_L.bin").c_str(), "wb");
+    if (!fx || !fp || !fq || !fl)
+        die("cannot open snapshot files");
+    for (int i = 0; i < nsteps; i++) {
+        if (i > 0 && i % 20 == 0) {
+            delete pairs;
+            pairs = b.calculatepairs(boxes, cut);
+        }
+        b.advancemom_halfstep(F, T);
+        b.advance_pos();
+        b.rotate();
+        F = b.calculateforces(*pairs, wsa);
+        T.reset_parallel(0.0);
+        b.calculate_forces_and_torques3D(*pairs, *pots, F, T);
+        generate_uniform_random_matrix(RT);
+        b.create_forces_and_torques_sphere(F, T, RT);
+        b.advancemom_halfstep(F, T);
+        if ((i + 1) % every == 0) {
+            matrix<double> xs = b.getdat(), ps = b.getmom();
+            matrix<double> &qs = b.getorientation(), &ls = b.getangmom();
+            for (int k = 0; k < N; k++) {
+                for (int c = 0; c < 3; c++) {
+                    double v = xs(k, c);
+                    fwrite(&v, 8, 1, fx);
+                    v = ps(k, c);
+                    fwrite(&v, 8, 1, fp);
+                    v = ls(k, c);
+                    fwrite(&v, 8, 1, fl);
+                }
+                for (int c = 0; c < 9; c++) {
+                    double v = qs(k, c);
+                    fwrite(&v, 8, 1, fq);
+                }
+            }
+        }
+    }
+    fclose(fx);
+    fclose(fp);
+    fclose(fq);
+    fclose(fl);
+    delete pairs;
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc != 3)
@@ -537,6 +610,8 @@ int main(int argc, char **argv)
         return cmd_traj_bead();
     if (cmd == "time_rotor")
         return cmd_time_rotor();
+    if (cmd == "long_rotor")
+        return cmd_long_rotor();
     die("unknown command " + cmd);
     return 2;
 }
