@@ -113,9 +113,9 @@ int mdx_set_active(mdx_engine *e, double fa, const int *prev, const int *next);
  * reach + skin are listed at a build, and the list is rebuilt on the device
  * as soon as twice the summed per-step maximum drift reaches the skin, so the
  * pairs evaluated each step are exactly those within reach.  kmax list slots
- * per particle (a particle with more walks its cells), state re-sorted into
+ * per particle (at least 4; a particle with more walks its cells), state re-sorted into
  * cell order every resort_every steps.  skin 0: every step bins and walks
- * the 27 cells.  Defaults 0.5, 64, 200. */
+ * the 27 cells.  Defaults 0.4, 64, 200. */
 int mdx_set_neighbor_list(mdx_engine *e, double skin, int kmax, int resort_every);
 /* switch interaction terms off again (MDX_TERM_* bits); set them again with
  * the setters above */

@@ -95,7 +95,7 @@ struct mdx_engine {
     // neighbour list: skin > 0 enables it for mdx_step (MDX_SKIN), kmax slots per
     // particle (MDX_KMAX), state re-sorted into cell order every resort_every
     // steps (MDX_RESORT)
-    double skin = 0.5;
+    double skin = 0.4;
     int kmax = 64;
     int resort_every = 200;
     bool list_valid = false;
@@ -114,14 +114,15 @@ struct mdx_engine {
     // device buffers (double-buffered state in slot order)
     Buf X[2], Q[2], P[2], Lm[2], Fc[2], Tc[2], slot[2], cell[2];
     Buf Fout, Tout, obs;
-    Buf dreps, prows, pdirs_d;
-    Buf rank, count, start, list, bsum, total;
+    Buf dreps, prows, pdirs_d, pdirs_f;
+    Buf rank, count, start, list, bsum;
     Buf bond_off, bond_j, bond_k, bond_r0;
     Buf ang_off, ang_ref, ang_ijk, ang_k, ang_t0;
     Buf exc_off, exc_j, act_prev, act_next;
     Buf noise, ctl, partial, tmpd, tmpi, tmpi2, tmpi3, tmpll, tmpx;
     Buf Fw, ncand, cand, Tp; // split walk -> (patch ->) finish
     int use_compact = 1;     // compacted patch kernel + integrate kernel (MDX_COMPACT=0: one finish kernel)
+    int use_walkc = 1;       // compacted list walk (MDX_WALKC=0: one lane per position)
     Buf nbr, nn, bcell;  // neighbour list
     Buf bmax;            // per-block drift maxima
     hipEvent_t prof_mid = nullptr;  // recorded between walk and finish while profiling
@@ -287,6 +288,8 @@ template <class R> static KArgs<R> make_args(mdx_engine *e)
     a.wca_s6 = s3 * s3;
     a.wca_s12 = a.wca_s6 * a.wca_s6;
     a.wca_pl = e->wca_pl;
+    a.wca_pl2 = e->wca_pl * e->wca_pl * (1 + 1e-12);
+    a.nrep = e->nrep;
     a.has_patches = e->ntypes > 0 && e->integ == MDX_INTEG_ROTOR && e->active_mask != 0;
     for (int t = 0; t < MAXT; t++) {
         a.np[t] = t < e->ntypes ? e->np[t] : 0;
@@ -296,6 +299,9 @@ template <class R> static KArgs<R> make_args(mdx_engine *e)
         a.rstart[k] = k < (int)e->rstart.size() ? e->rstart[k] : 0;
     a.active_mask = e->active_mask;
     a.pdirs = (const double *)e->pdirs_d.p;
+    // the walk's own-patch screen reads the directions in R
+    a.pdirs_r = sizeof(R) == 8 ? (const R *)e->pdirs_d.p : (const R *)e->pdirs_f.p;
+    a.wca_pl2lo = e->wca_pl * e->wca_pl * (1 - 1e-12);
     a.prows = (const PatchRow *)e->prows.p;
     a.maxd2 = e->maxd2;
     a.rmax2 = std::max(e->wca ? e->wca_pl * e->wca_pl * (1 + 1e-12) : 0.0,
@@ -370,11 +376,8 @@ static int finish_bins(mdx_engine *e, const StepCtl *gate = nullptr, const int *
         cells = (const int *)e->cell[e->cur].p;
     hipLaunchKernelGGL(k_scan1, dim3(nb), dim3(256), 0, e->stream, (const int *)e->count.p,
                        (int *)e->start.p, (int *)e->bsum.p, n, gate);
-    hipLaunchKernelGGL(k_scan2, dim3(1), dim3(1024), 0, e->stream, (int *)e->bsum.p, nb,
-                       (int *)e->total.p, gate);
     hipLaunchKernelGGL(k_scan3, dim3(nb), dim3(256), 0, e->stream, (int *)e->count.p,
-                       (int *)e->start.p, (const int *)e->bsum.p, n, (const int *)e->total.p,
-                       gate);
+                       (int *)e->start.p, (const int *)e->bsum.p, n, gate);
     hipLaunchKernelGGL(k_fill, dim3(nblk(e->N)), dim3(256), 0, e->stream, e->N, cells,
                        (const int *)e->rank.p, (const int *)e->start.p, (int *)e->list.p, gate);
     hipLaunchKernelGGL(k_cellsort, dim3(nblk(n)), dim3(256), 0, e->stream, n,
@@ -429,7 +432,10 @@ static int launch_gather(mdx_engine *e, const double *noise_dev, int lmode = 0)
         a.ncand = (int *)e->ncand.p;
         a.cand = (int *)e->cand.p;
         a.Tp = (R *)e->Tp.p;
-        hipLaunchKernelGGL((k_walk<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
+        if (lmode && e->use_walkc)
+            hipLaunchKernelGGL((k_walkc<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
+        else
+            hipLaunchKernelGGL((k_walk<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
         if (e->prof_mid)
             HIPCHK(hipEventRecord(e->prof_mid, e->stream));
         e->last_split = true;
@@ -532,11 +538,12 @@ static int nbr_alloc(mdx_engine *e)
     return MDX_OK;
 }
 
-// after a kernel that moved the particles in list mode (its blocks wrote bmax)
-static int decide(mdx_engine *e)
+// after a kernel that moved the particles in list mode (its blocks wrote
+// bmax); epoch_inc: also count the step's force evaluation
+static int decide(mdx_engine *e, int epoch_inc)
 {
     hipLaunchKernelGGL(k_decide, dim3(1), dim3(256), 0, e->stream, (StepCtl *)e->ctl.p,
-                       (const float *)e->bmax.p, (int)nblk(e->N), e->skin);
+                       (const float *)e->bmax.p, (int)nblk(e->N), e->skin, epoch_inc);
     HIPCHK(hipGetLastError());
     return MDX_OK;
 }
@@ -636,10 +643,13 @@ static int enqueue_one(mdx_engine *e, const double *nd, hipEvent_t *ev, int s, b
         e->prof_mid = e->prof_mid2 = nullptr;
         if (rc)
             return rc;
-        if (MODE != M_STEP && (rc = decide(e)))
-            return rc;
+        if (MODE != M_STEP) {
+            if ((rc = decide(e, 1)))
+                return rc;
+        } else {
+            launch_small(e, k_epoch_inc);
+        }
         e->cur ^= 1;
-        launch_small(e, k_epoch_inc);
         e->since_resort++;
         e->binned = false;
         e->counted = false;
@@ -668,7 +678,7 @@ template <class R> static int enqueue_rotor(mdx_engine *e, int nsteps, const dou
             HIPCHK(hipEventRecord(kev[0], e->stream));
         hipLaunchKernelGGL(k_kick_drift<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
         HIPCHK(hipGetLastError());
-        if (a.lmode && (rc = decide(e)))
+        if (a.lmode && (rc = decide(e, 0)))
             return rc;
         if (kev)
             HIPCHK(hipEventRecord(kev[1], e->stream));
@@ -838,12 +848,14 @@ int mdx_create(const mdx_desc *d, mdx_engine **out)
         e->use_tiles = std::atoi(t) != 0;
     if (const char *t = std::getenv("MDX_SPLIT"))
         e->use_split = std::atoi(t) != 0;
+    if (const char *t = std::getenv("MDX_WALKC"))
+        e->use_walkc = std::atoi(t) != 0;
     if (const char *t = std::getenv("MDX_COMPACT"))
         e->use_compact = std::atoi(t) != 0;
     if (const char *t = std::getenv("MDX_SKIN"))
         e->skin = std::max(0.0, std::atof(t));
     if (const char *t = std::getenv("MDX_KMAX"))
-        e->kmax = std::max(1, std::atoi(t));
+        e->kmax = std::max(MDX_LBATCH, std::atoi(t));
     if (const char *t = std::getenv("MDX_RESORT"))
         e->resort_every = std::max(1, std::atoi(t));
     long long f = 0;
@@ -878,7 +890,6 @@ int mdx_create(const mdx_desc *d, mdx_engine **out)
     rc |= dalloc(e, e->rank, 4 * N);
     rc |= dalloc(e, e->list, 4 * N);
     rc |= dalloc(e, e->ctl, sizeof(StepCtl));
-    rc |= dalloc(e, e->total, 16);
     if (rc) {
         std::string msg = g_err;
         mdx_destroy(e);
@@ -896,7 +907,8 @@ void mdx_destroy(mdx_engine *e)
     if (e->stream)
         (void)hipStreamSynchronize(e->stream);
     Buf *bufs[] = {&e->Fout,    &e->Tout,    &e->obs,    &e->dreps,  &e->prows,   &e->pdirs_d,
-                   &e->rank,    &e->count,   &e->start,  &e->list,   &e->bsum,    &e->total,
+                   &e->pdirs_f,
+                   &e->rank,    &e->count,   &e->start,  &e->list,   &e->bsum,
                    &e->bond_off, &e->bond_j, &e->bond_k, &e->bond_r0, &e->ang_off, &e->ang_ref,
                    &e->ang_ijk, &e->ang_k,   &e->ang_t0, &e->exc_off, &e->exc_j,  &e->act_prev,
                    &e->act_next, &e->noise,  &e->ctl,    &e->partial, &e->tmpd,   &e->tmpi,
@@ -1151,6 +1163,9 @@ int mdx_set_patches(mdx_engine *e, int ntypes, const int *type_of_particle,
     HIPCHK(hipStreamSynchronize(e->stream));
     std::vector<double> pd = e->pdirs;
     pd.resize(std::max<size_t>(pd.size(), 3), 0.0);
+    std::vector<float> pf(pd.begin(), pd.end());
+    if ((rc = upload(e, e->pdirs_f, pf.data(), pf.size())))
+        return rc;
     if ((rc = upload(e, e->pdirs_d, pd.data(), pd.size())))
         return rc;
     return patch_rows_upload(e);
@@ -1730,8 +1745,9 @@ int mdx_stage_advance_mom(mdx_engine *e, const double *F, const double *R)
 
 int mdx_set_neighbor_list(mdx_engine *e, double skin, int kmax, int resort_every)
 {
-    if (!e || !(skin >= 0) || kmax < 1 || resort_every < 1)
-        return set_err(MDX_ERR_ARG, "skin >= 0, kmax >= 1 and resort_every >= 1 required");
+    if (!e || !(skin >= 0) || kmax < MDX_LBATCH || resort_every < 1)
+        return set_err(MDX_ERR_ARG, "skin >= 0, kmax >= %d and resort_every >= 1 required",
+                       MDX_LBATCH);
     HIPCHK(hipSetDevice(e->device));
     if (skin != e->skin)
         e->grid_dirty = true; // cells hold the interaction reach plus the skin

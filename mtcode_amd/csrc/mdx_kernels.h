@@ -73,6 +73,7 @@ template <class R> struct KArgs {
     R *Fw;
     int *ncand, *cand; // cand[q * N + s], q < PMAXG
     R *Tp;             // patch torque per position (compacted patch kernel)
+    int nrep; // replicas (1: the replica record is loaded uniformly)
     // neighbour list (lmode): slot s walks nbr[k * N + s], k < nn[s] (<= kmax),
     // else the 27 cells of its cell at the last build, bcell[s]
     int lmode;
@@ -83,7 +84,7 @@ template <class R> struct KArgs {
     float *bmax; // per block: max drift of the block's particles
     // WCA (HSPotential)
     int wca;
-    double wca_eps, wca_s6, wca_s12, wca_pl;
+    double wca_eps, wca_s6, wca_s12, wca_pl, wca_pl2, wca_pl2lo;
     // patches
     int has_patches;
     int np[MAXT];
@@ -91,6 +92,7 @@ template <class R> struct KArgs {
     int rstart[MAXT * MAXT];
     unsigned long long active_mask; // bit ta * MAXT + tc: type pair with a nonzero row
     const double *pdirs;
+    const R *pdirs_r; // the same directions in R
     const PatchRow *prows;
     double maxd2, rmax2, costm_screen;
     // topology (indexed by original index)
@@ -276,6 +278,10 @@ __device__ __forceinline__ void min_image(const KArgs<R> &a, const DevRep &rp, d
 #ifndef MDX_WALK_SCREEN
 #define MDX_WALK_SCREEN 1
 #endif
+// neighbour-list entries loaded together per round of the list walk
+#ifndef MDX_LBATCH
+#define MDX_LBATCH 4
+#endif
 constexpr int PMAX = 8;   // deferred patch candidates per particle held in LDS
 constexpr int PMAXG = 16; // patch candidates per particle handed from walk to finish
 enum { PH_ALL = 0, PH_WALK = 1, PH_FINISH = 2, PH_INTEGRATE = 3 };
@@ -287,6 +293,12 @@ __device__ __forceinline__ int xcd_block(int b, int G)
 {
     const int q = G >> 3, r = G & 7, x = b & 7;
     return x * q + min(x, r) + (b >> 3);
+}
+
+// the replica record of a particle; one replica: a uniform load
+template <class R> __device__ __forceinline__ DevRep rep_rec(const KArgs<R> &a, uint32_t meta)
+{
+    return a.nrep == 1 ? a.reps[0] : a.reps[meta >> 8];
 }
 
 // neighbour slots of slot i: the 9 (y, z) rows of the 27-cell stencil; the x
@@ -354,20 +366,27 @@ __device__ __forceinline__ void for_each_neighbor(const KArgs<R> &a, const DevRe
 template <class R, class Fn>
 __device__ __forceinline__ void for_each_listed(const KArgs<R> &a, const DevRep &rp, int i, Fn &&fn)
 {
+    // the first MDX_LBATCH entries are loaded together with the count (the
+    // list has at least that many slots per particle; entries beyond the
+    // count are stale slot numbers of an earlier build, never used)
+    int j0[MDX_LBATCH];
+#pragma unroll
+    for (int u = 0; u < MDX_LBATCH; u++)
+        j0[u] = a.nbr[(size_t)u * a.N + i];
     const int n = a.nn[i];
     if (n <= a.kmax) {
-        // four entries per round: their index and position loads overlap
-        for (int k0 = 0; k0 < n; k0 += 4) {
-            int j[4];
-            PosRec x[4];
+        // MDX_LBATCH entries per round: their index and position loads overlap
+        for (int k0 = 0; k0 < n; k0 += MDX_LBATCH) {
+            int j[MDX_LBATCH];
+            PosRec x[MDX_LBATCH];
 #pragma unroll
-            for (int u = 0; u < 4; u++)
-                j[u] = k0 + u < n ? a.nbr[(size_t)(k0 + u) * a.N + i] : i;
+            for (int u = 0; u < MDX_LBATCH; u++)
+                j[u] = k0 + u < n ? (k0 == 0 ? j0[u] : a.nbr[(size_t)(k0 + u) * a.N + i]) : i;
 #pragma unroll
-            for (int u = 0; u < 4; u++)
+            for (int u = 0; u < MDX_LBATCH; u++)
                 x[u] = a.X[j[u]];
 #pragma unroll
-            for (int u = 0; u < 4; u++)
+            for (int u = 0; u < MDX_LBATCH; u++)
                 if (k0 + u < n)
                     fn(x[u], j[u]);
         }
@@ -513,6 +532,73 @@ __device__ __forceinline__ void patch_pair(const KArgs<R> &a, const DevRep &rp, 
     }
 }
 
+// WCA term of one pair (MD.cpp:1700-1758, potential.h:112-125) into acc:
+// d = x_i - x_j minimum-imaged, r2s = |d|^2, exclusions [exc0, exc1) of i
+template <class R, int MODE>
+__device__ __forceinline__ void wca_term(const KArgs<R> &a, const double d[3], double r2s,
+                                         uint32_t idj, int exc0, int exc1, Acc<R> &acc)
+{
+    constexpr bool FP64 = sizeof(R) == 8;
+    // ---------------- WCA (MD.cpp:1700-1758, potential.h:112-125)
+    // (r^2 against the squared cut-off with a margin first; the
+    // decision itself is the reference's sqrt(r^2) < cut-off)
+    if (a.wca && r2s < a.wca_pl2) {
+        // FP64: the reference's sqrt(r^2) < cut-off; mixed: the same
+        // decision taken on r^2 outside a 1e-12 band around the
+        // squared cut-off (where the two cannot differ), sqrt inside it
+        double rij = 0;
+        bool inside;
+        if constexpr (FP64) {
+            rij = sqrt(r2s);
+            inside = rij < a.wca_pl;
+        } else {
+            inside = r2s < a.wca_pl2lo || sqrt(r2s) < a.wca_pl;
+        }
+        if (inside) {
+            bool ex = false;
+            for (int e = exc0; e < exc1; e++)
+                ex |= (uint32_t)a.exc_j[e] == idj;
+            if (!ex) {
+                if constexpr (FP64) {
+                    const double f1 = hs_force(rij, a.wca_eps, a.wca_s6, a.wca_s12);
+                    flag_force(a, f1);
+                    for (int q = 0; q < 3; q++) {
+                        const double fac = f1 * d[q] / rij;
+                        acc.F[q] += fac;
+                        if (MODE == M_ENERGY)
+                            for (int l = 0; l < 3; l++)
+                                acc.vir[3 * q + l] += 0.5 * d[l] * fac;
+                    }
+                    if (MODE == M_ENERGY)
+                        acc.epot += 0.5 * wca_energy(rij, a.wca_eps, a.wca_s6, a.wca_s12);
+                } else {
+                    // 4 eps (12 s12 / r^13 - 6 s6 / r^7) from 1/r
+                    const float r2f = (float)r2s;
+                    const float ir = rsqrtf(r2f);
+                    const float rf = r2f * ir;
+                    const float ir2 = ir * ir;
+                    const float ir6 = ir2 * ir2 * ir2;
+                    const float f1 = 4.f * (float)a.wca_eps * ir * ir6 *
+                                     (12.f * (float)a.wca_s12 * ir6 - 6.f * (float)a.wca_s6);
+                    flag_force(a, f1);
+                    const float fr = f1 * ir;
+                    for (int q = 0; q < 3; q++) {
+                        const float fac = fr * (float)d[q];
+                        acc.F[q] += fac;
+                        if (MODE == M_ENERGY)
+                            for (int l = 0; l < 3; l++)
+                                acc.vir[3 * q + l] += 0.5 * d[l] * fac;
+                    }
+                    if (MODE == M_ENERGY)
+                        acc.epot += 0.5 * (double)wca_energy(rf, (float)a.wca_eps,
+                                                             (float)a.wca_s6,
+                                                             (float)a.wca_s12);
+                }
+            }
+        }
+    }
+}
+
 // The cone screen of patch_pair on the particle's own patches alone (no
 // load of the partner): false when none of its patches passes, in which case
 // patch_pair adds exactly nothing for the pair.  Evaluated in R with a
@@ -528,18 +614,22 @@ __device__ __forceinline__ bool own_screen(const KArgs<R> &a, uint32_t oi, int t
     int ta, tc;
     const bool i_is_a = patch_roles(oi, ti, xj.id, tj, ta, tc);
     // u = (d / r), sign folded into the test below
-    const R ri = (R)1 / sqrt((R)r2s);
+    R ri;
+    if constexpr (sizeof(R) == 8)
+        ri = (R)1 / sqrt((R)r2s);
+    else
+        ri = rsqrtf((float)r2s);
     const R ur[3] = {(R)d[0] * ri, (R)d[1] * ri, (R)d[2] * ri};
     const R v[3] = {qi[0] * ur[0] + qi[1] * ur[1] + qi[2] * ur[2],
                     qi[3] * ur[0] + qi[4] * ur[1] + qi[5] * ur[2],
                     qi[6] * ur[0] + qi[7] * ur[1] + qi[8] * ur[2]};
     const int t = i_is_a ? ta : tc;
     const int np = a.np[t];
-    const double *dd = a.pdirs + 3 * a.pstart[t];
+    const R *dd = a.pdirs_r + 3 * a.pstart[t];
     const R scr = (R)a.costm_screen;
     bool pass = false;
     for (int k = 0; k < np; k++) {
-        const R sd = (R)dd[3 * k] * v[0] + (R)dd[3 * k + 1] * v[1] + (R)dd[3 * k + 2] * v[2];
+        const R sd = dd[3 * k] * v[0] + dd[3 * k + 1] * v[1] + dd[3 * k + 2] * v[2];
         // u = +d/r for role a, whose test is -(n . u); u = -d/r for role c,
         // whose test is n . u: both are -(n . d/r)
         pass |= -sd > scr;
@@ -566,7 +656,7 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
     const PosRec xi = a.X[i];
     const uint32_t oi = xi.id;
     const int ti = xi.meta & 255;
-    const DevRep rp = a.reps[xi.meta >> 8];
+    const DevRep rp = rep_rec(a, xi.meta);
     Acc<R> acc;
     acc.F[0] = acc.F[1] = acc.F[2] = 0;
     acc.T[0] = acc.T[1] = acc.T[2] = 0;
@@ -644,47 +734,7 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
             const double r2s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
             if (!(r2s < a.rmax2))
                 return;
-            // ---------------- WCA (MD.cpp:1700-1758, potential.h:112-125)
-            if (a.wca) {
-                const double rij = sqrt(r2s);
-                if (rij < a.wca_pl) {
-                    bool ex = false;
-                    for (int e = exc0; e < exc1; e++)
-                        ex |= (uint32_t)a.exc_j[e] == xj.id;
-                    if (!ex) {
-                        if constexpr (FP64) {
-                            const double f1 = hs_force(rij, a.wca_eps, a.wca_s6, a.wca_s12);
-                            flag_force(a, f1);
-                            for (int q = 0; q < 3; q++) {
-                                const double fac = f1 * d[q] / rij;
-                                acc.F[q] += fac;
-                                if (MODE == M_ENERGY)
-                                    for (int l = 0; l < 3; l++)
-                                        acc.vir[3 * q + l] += 0.5 * d[l] * fac;
-                            }
-                            if (MODE == M_ENERGY)
-                                acc.epot += 0.5 * wca_energy(rij, a.wca_eps, a.wca_s6, a.wca_s12);
-                        } else {
-                            const float rf = (float)rij;
-                            const float f1 =
-                                hs_force(rf, (float)a.wca_eps, (float)a.wca_s6, (float)a.wca_s12);
-                            flag_force(a, f1);
-                            const float fr = f1 / rf;
-                            for (int q = 0; q < 3; q++) {
-                                const float fac = fr * (float)d[q];
-                                acc.F[q] += fac;
-                                if (MODE == M_ENERGY)
-                                    for (int l = 0; l < 3; l++)
-                                        acc.vir[3 * q + l] += 0.5 * d[l] * fac;
-                            }
-                            if (MODE == M_ENERGY)
-                                acc.epot += 0.5 * (double)wca_energy(rf, (float)a.wca_eps,
-                                                                     (float)a.wca_s6,
-                                                                     (float)a.wca_s12);
-                        }
-                    }
-                }
-            }
+            wca_term<R, MODE>(a, d, r2s, xj.id, exc0, exc1, acc);
             if (PH == PH_WALK) {
                 const int js = j >= 0 ? j : a.slot[xj.id];
                 if (MDX_WALK_SCREEN ? is_listed_candidate(xj, js, d, r2s) : is_patch_candidate(xj, r2s)) {
@@ -990,6 +1040,144 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_walk(KArgs
         drift);
 }
 
+// List-mode walk, compacted: the list entries of a block's 256 positions are
+// laid out back to back and evaluated one per thread (every lane busy, instead
+// of each lane looping over its own list with the wave waiting for its longest
+// list); each position then adds its entries' WCA forces and appends its
+// patch candidates in list order -- the same sums in the same order as the
+// one-lane walk.  Positions with more than kmax entries walk the build's
+// cells in their own lane.
+template <class R, int MODE> __global__ void __launch_bounds__(256) k_walkc(KArgs<R> a)
+{
+    __shared__ int off[257];
+    __shared__ PosRec sx[256];
+    __shared__ R sq[256 * 9];
+    __shared__ int sexc[256 * 2];
+    __shared__ R pf[256 * 3];
+    __shared__ int pj[256];
+    __shared__ int wsum[4];
+    list_built(a);
+    const int tid = threadIdx.x;
+    const int base = xcd_block(blockIdx.x, gridDim.x) * 256;
+    const int s = base + tid;
+    const bool live = s < a.N;
+    const int n = live ? a.nn[s] : 0;
+    const bool fb = n > a.kmax;
+    const int cnt = fb ? 0 : n;
+    PosRec xi;
+    if (live) {
+        xi = a.X[s];
+        sx[tid] = xi;
+        if (a.has_patches && a.np[xi.meta & 255] > 0) {
+#pragma unroll
+            for (int k = 0; k < 9; k++)
+                sq[9 * tid + k] = a.Q[9 * (size_t)s + k];
+        }
+        sexc[2 * tid] = a.exc_off ? a.exc_off[xi.id] : 0;
+        sexc[2 * tid + 1] = a.exc_off ? a.exc_off[xi.id + 1] : 0;
+    }
+    // exclusive scan of the entry counts
+    int v = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(v, o);
+        if ((tid & 63) >= o)
+            v += t;
+    }
+    if ((tid & 63) == 63)
+        wsum[tid >> 6] = v;
+    __syncthreads();
+    int wb = 0;
+    for (int w = 0; w < (tid >> 6); w++)
+        wb += wsum[w];
+    const int total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    off[tid] = wb + v - cnt;
+    if (tid == 255)
+        off[256] = total;
+    __syncthreads();
+
+    // WCA of a pair into acc; true for a patch candidate that passes the
+    // own-patch screen (the one-lane walk's rule)
+    auto visit = [&](const PosRec &xo, const R *qo, int e0, int e1, const PosRec &xj,
+                     Acc<R> &acc) -> bool {
+        const DevRep rp = rep_rec(a, xo.meta);
+        double d[3];
+        min_image(a, rp, xo.x, xo.y, xo.z, xj, d);
+        const double r2s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (!(r2s < a.rmax2))
+            return false;
+        wca_term<R, MODE>(a, d, r2s, xj.id, e0, e1, acc);
+        const int to = xo.meta & 255;
+        if (!(a.has_patches && a.np[to] > 0 && r2s < a.maxd2))
+            return false;
+        const int tj = xj.meta & 255;
+        if (a.np[tj] == 0)
+            return false;
+        int ta, tc;
+        patch_roles(xo.id, to, xj.id, tj, ta, tc);
+        if (!((a.active_mask >> (ta * MAXT + tc)) & 1ull))
+            return false;
+        return !MDX_WALK_SCREEN || own_screen(a, xo.id, to, qo, xj, d, r2s);
+    };
+
+    R F[3] = {0, 0, 0};
+    int npc = 0;
+    for (int r0 = 0; r0 < total; r0 += 256) {
+        const int e = r0 + tid;
+        if (e < total) {
+            int lo = 0, hi = 255;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (off[mid] <= e)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const int o = lo;
+            const int so = base + o;
+            const int j = a.nbr[(size_t)(e - off[o]) * a.N + so];
+            const PosRec xo = sx[o];
+            Acc<R> pa;
+            pa.F[0] = pa.F[1] = pa.F[2] = 0;
+            const bool c = visit(xo, sq + 9 * o, sexc[2 * o], sexc[2 * o + 1], a.X[j], pa);
+            for (int k = 0; k < 3; k++)
+                pf[3 * tid + k] = pa.F[k];
+            pj[tid] = c ? j : -1;
+        }
+        __syncthreads();
+        const int lo = max(off[tid], r0), hi = min(off[tid] + cnt, r0 + 256);
+        for (int e2 = lo; e2 < hi; e2++) {
+            for (int k = 0; k < 3; k++)
+                F[k] += pf[3 * (e2 - r0) + k];
+            const int jc = pj[e2 - r0];
+            if (jc >= 0) {
+                if (npc < PMAXG)
+                    a.cand[(size_t)npc * a.N + s] = jc;
+                npc++;
+            }
+        }
+        __syncthreads();
+    }
+    if (!live)
+        return;
+    if (fb) {
+        const DevRep rp = rep_rec(a, xi.meta);
+        Acc<R> pa;
+        pa.F[0] = pa.F[1] = pa.F[2] = 0;
+        for_each_in_cells(a, rp, a.bcell[s], s, [&](const PosRec &xj, int j) {
+            if (visit(xi, sq + 9 * tid, sexc[2 * tid], sexc[2 * tid + 1], xj, pa)) {
+                if (npc < PMAXG)
+                    a.cand[(size_t)npc * a.N + s] = j;
+                npc++;
+            }
+        });
+        for (int k = 0; k < 3; k++)
+            F[k] = pa.F[k];
+    }
+    for (int k = 0; k < 3; k++)
+        a.Fw[3 * (size_t)s + k] = F[k];
+    a.ncand[s] = npc;
+}
+
 // ... then the patch pairs, topology and the integrator
 template <class R, int MODE> __global__ void __launch_bounds__(256) k_finish(KArgs<R> a)
 {
@@ -1067,7 +1255,7 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_patch(KArg
             const int q = e - off[o];
             const int i = a.lmode ? so : a.list[so];
             const PosRec xo = a.X[i];
-            const DevRep rp = a.reps[xo.meta >> 8];
+            const DevRep rp = rep_rec(a, xo.meta);
             R qo[9];
 #pragma unroll
             for (int k = 0; k < 9; k++)
@@ -1114,7 +1302,7 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_patch(KArg
         const int i = a.lmode ? so : a.list[so];
         const PosRec xi = a.X[i];
         const int ti = xi.meta & 255;
-        const DevRep rp = a.reps[xi.meta >> 8];
+        const DevRep rp = rep_rec(a, xi.meta);
         R qi[9];
         for (int k2 = 0; k2 < 9; k2++)
             qi[k2] = a.Q[9 * (size_t)i + k2];
@@ -1177,7 +1365,7 @@ template <class R> __global__ void __launch_bounds__(256) k_nbuild(KArgs<R> a, c
     if (s >= a.N)
         return;
     const PosRec xi = a.X[s];
-    const DevRep rp = a.reps[xi.meta >> 8];
+    const DevRep rp = rep_rec(a, xi.meta);
     const int c = cells[s];
     a.bcell[s] = c;
     int n = 0;
@@ -1343,7 +1531,7 @@ template <class R> __device__ __forceinline__ float kick_drift_one(const KArgs<R
 {
     constexpr bool FP64 = sizeof(R) == 8;
     PosRec xr = a.X[i];
-    const DevRep rp = a.reps[xr.meta >> 8];
+    const DevRep rp = rep_rec(a, xr.meta);
     const R hdt = (R)a.hdt;
     R pl[3], ll[3], q[9];
     double xn[3] = {xr.x, xr.y, xr.z};
@@ -1404,7 +1592,7 @@ template <class R> __global__ void __launch_bounds__(256) k_bin(KArgs<R> a, cons
     if (i >= a.N)
         return;
     const PosRec xr = a.X[i];
-    const DevRep rp = a.reps[xr.meta >> 8];
+    const DevRep rp = rep_rec(a, xr.meta);
     const double x[3] = {xr.x, xr.y, xr.z};
     bin_particle(a, i, rp, x, a.celln);
 }
@@ -1443,52 +1631,33 @@ __global__ void __launch_bounds__(256) k_scan1(const int *count, int *start, int
         bsum[blockIdx.x] = sh[255];
 }
 
-// phase 2: exclusive scan of the block totals (one block)
-__global__ void __launch_bounds__(1024) k_scan2(int *bsum, int nb, int *total, const StepCtl *gate)
+// phase 3: add block offsets, zero the counts, close the array
+// (bsum holds the block totals of phase 1; each block sums those before it)
+__global__ void __launch_bounds__(256) k_scan3(int *count, int *start, const int *bsum, int n,
+                                               const StepCtl *gate)
 {
     if (gate && !gate->rebuild)
         return;
-    __shared__ int sh[1024];
-    const int per = (nb + 1023) / 1024;
-    const int b0 = threadIdx.x * per;
-    int s = 0;
-    for (int k = 0; k < per; k++)
-        if (b0 + k < nb)
-            s += bsum[b0 + k];
-    sh[threadIdx.x] = s;
+    __shared__ int sh[256];
+    int part = 0;
+    for (int b = threadIdx.x; b < (int)blockIdx.x; b += 256)
+        part += bsum[b];
+    sh[threadIdx.x] = part;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-        __syncthreads();
-        sh[threadIdx.x] += t;
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+            sh[threadIdx.x] += sh[threadIdx.x + o];
         __syncthreads();
     }
-    int run = sh[threadIdx.x] - s;
-    for (int k = 0; k < per; k++)
-        if (b0 + k < nb) {
-            int v = bsum[b0 + k];
-            bsum[b0 + k] = run;
-            run += v;
-        }
-    if (threadIdx.x == 1023)
-        *total = sh[1023];
-}
-
-// phase 3: add block offsets, zero the counts, close the array
-__global__ void __launch_bounds__(256) k_scan3(int *count, int *start, const int *bsum, int n,
-                                               const int *total, const StepCtl *gate)
-{
-    if (gate && !gate->rebuild)
-        return;
     const int base = blockIdx.x * 1024 + threadIdx.x * 4;
-    const int off = bsum[blockIdx.x];
+    const int off = sh[0];
     for (int k = 0; k < 4; k++)
         if (base + k < n) {
             start[base + k] += off;
             count[base + k] = 0;
         }
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        start[n] = *total;
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+        start[n] = off + bsum[blockIdx.x];
 }
 
 __global__ void __launch_bounds__(256) k_fill(int N, const int *cell, const int *rank,
@@ -1541,7 +1710,8 @@ __global__ void k_list_reset(StepCtl *ctl)
 // the grid max of the block drifts of the kernel before is added to the
 // displacement bound D; a list rebuild is requested once 2 D reaches the skin
 // (a pair within reach now was within reach + skin at the build)
-__global__ void __launch_bounds__(256) k_decide(StepCtl *ctl, const float *bmax, int nb, double skin)
+__global__ void __launch_bounds__(256) k_decide(StepCtl *ctl, const float *bmax, int nb, double skin,
+                                                int epoch_inc)
 {
     __shared__ float wmax[4];
     float m = 0.f;
@@ -1559,6 +1729,8 @@ __global__ void __launch_bounds__(256) k_decide(StepCtl *ctl, const float *bmax,
         ctl->D = D;
         if (2 * D >= skin)
             ctl->rebuild = 1u;
+        if (epoch_inc)
+            ctl->epoch += 1; // the step's force evaluation is done (k_epoch_inc)
     }
 }
 

@@ -305,7 +305,7 @@ class Engine:
         R, rp = _d(R)
         _check(lib().mdx_stage_advance_mom(self.h, fp, rp))
 
-    def set_neighbor_list(self, skin=0.5, kmax=64, resort_every=200):
+    def set_neighbor_list(self, skin=0.4, kmax=64, resort_every=200):
         _check(lib().mdx_set_neighbor_list(self.h, C.c_double(skin), int(kmax), int(resort_every)))
 
 

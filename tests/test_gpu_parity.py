@@ -334,10 +334,10 @@ def _run(s, precision, nsteps, skin, kmax=64, resort=200, seed=7):
     return out
 
 
-@pytest.mark.parametrize("kmax", [64, 3])
+@pytest.mark.parametrize("kmax", [64, 4])
 def test_neighbor_list_rotor_matches_cell_walk(kmax):
     """Steps that walk the neighbour list (rebuilt on the device from the
-    displacement bound, re-sorted every 25 steps; kmax=3 sends most particles
+    displacement bound, re-sorted every 25 steps; kmax=4 sends most particles
     to the cell fallback) evaluate the same pairs as the exhaustive cell walk;
     only the summation order differs, so FP64 trajectories agree to rounding."""
     s = wl.condensate_ab(1 << 15, phi=0.2)
