@@ -57,6 +57,11 @@ struct Buf {
     size_t bytes = 0;
 };
 
+struct GraphSlot {
+    hipGraphExec_t exec = nullptr;
+    std::vector<char> sig; // launch arguments it was captured with
+};
+
 } // namespace
 
 struct mdx_engine {
@@ -123,6 +128,9 @@ struct mdx_engine {
     Buf Fw, ncand, cand, Tp; // split walk -> (patch ->) finish
     int use_compact = 1;     // compacted patch kernel + integrate kernel (MDX_COMPACT=0: one finish kernel)
     int use_walkc = 1;       // compacted list walk (MDX_WALKC=0: one lane per position)
+    int use_graph = 1;       // list steps replayed from captured graphs (MDX_GRAPH=0: eager)
+    bool list_eager = false; // an eager list step ran since the last list build
+    GraphSlot graphs[2];     // per current state buffer
     Buf nbr, nn, bcell;  // neighbour list
     Buf bmax;            // per-block drift maxima
     hipEvent_t prof_mid = nullptr;  // recorded between walk and finish while profiling
@@ -567,6 +575,72 @@ static inline void mark(mdx_engine *e, hipEvent_t *ev, int s, int k)
         (void)hipEventRecord(ev[NEV * s + k], e->stream);
 }
 
+// the kernels of one list step (gated list upkeep, walk / patch / integrate,
+// displacement decision); no host state changes, so it can be captured
+template <class R, int MODE>
+static int launch_list_step(mdx_engine *e, const double *nd, hipEvent_t *ev, int s)
+{
+    int rc;
+    const StepCtl *gate = (const StepCtl *)e->ctl.p;
+    if ((rc = count_current<R>(e, (int *)e->bcell.p, gate)))
+        return rc;
+    if ((rc = finish_bins(e, gate, (const int *)e->bcell.p)))
+        return rc;
+    if ((rc = build_list<R>(e, (const int *)e->bcell.p, 1)))
+        return rc;
+    mark(e, ev, s, 1);
+    e->prof_mid = ev ? ev[NEV * s + 2] : nullptr;
+    e->prof_mid2 = ev ? ev[NEV * s + 3] : nullptr;
+    rc = launch_gather<R, MODE>(e, nd, 1);
+    e->prof_mid = e->prof_mid2 = nullptr;
+    if (rc)
+        return rc;
+    if (MODE != M_STEP) {
+        if ((rc = decide(e, 1)))
+            return rc;
+    } else {
+        launch_small(e, k_epoch_inc);
+    }
+    return MDX_OK;
+}
+
+// a list step from a graph captured per state buffer (the kernels' arguments
+// differ only in which buffer is current); recaptured when the arguments
+// change.  The graph holds kernel nodes only (no memset or copy nodes).
+template <class R, int MODE> static int list_step_graph(mdx_engine *e)
+{
+    KArgs<R> a = make_args<R>(e);
+    a.lmode = 1;
+    std::vector<char> sig(sizeof a + 2 * sizeof(int));
+    memcpy(sig.data(), &a, sizeof a);
+    memcpy(sig.data() + sizeof a, &e->ncells, sizeof(int));
+    memcpy(sig.data() + sizeof a + sizeof(int), &e->N, sizeof(int));
+    GraphSlot &g = e->graphs[e->cur];
+    if (!g.exec || g.sig != sig) {
+        if (g.exec) {
+            HIPCHK(hipGraphExecDestroy(g.exec));
+            g.exec = nullptr;
+        }
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
+        int rc = launch_list_step<R, MODE>(e, nullptr, nullptr, 0);
+        hipError_t ce = hipStreamEndCapture(e->stream, &graph);
+        if (rc)
+            return rc;
+        if (ce != hipSuccess)
+            return set_err(MDX_ERR_HIP, "stream capture failed: %s", hipGetErrorString(ce));
+        hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) {
+            g.exec = nullptr;
+            return set_err(MDX_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ie));
+        }
+        g.sig = sig;
+    }
+    HIPCHK(hipGraphLaunch(g.exec, e->stream));
+    return MDX_OK;
+}
+
 // one step of the loop.  MODE: M_FUSED / M_STEP (rotor), M_BEAD.
 // Without the list (skin 0) every step scans the counts its predecessor took
 // and walks the 27 cells; the gather re-sorts the state into cell order.
@@ -618,6 +692,7 @@ static int enqueue_one(mdx_engine *e, const double *nd, hipEvent_t *ev, int s, b
             if ((rc = build_list<R>(e, (const int *)e->cell[e->cur].p, 0)))
                 return rc;
             launch_small(e, k_list_reset);
+            e->list_eager = false;
             // list steps keep the slots and do not rewrite the slot map
             HIPCHK(hipMemcpyAsync(e->slot[e->cur ^ 1].p, e->slot[e->cur].p, 4 * (size_t)e->N,
                                   hipMemcpyDeviceToDevice, e->stream));
@@ -629,26 +704,16 @@ static int enqueue_one(mdx_engine *e, const double *nd, hipEvent_t *ev, int s, b
         }
         e->counted = false;
     } else {
-        const StepCtl *gate = (const StepCtl *)e->ctl.p;
-        if ((rc = count_current<R>(e, (int *)e->bcell.p, gate)))
-            return rc;
-        if ((rc = finish_bins(e, gate, (const int *)e->bcell.p)))
-            return rc;
-        if ((rc = build_list<R>(e, (const int *)e->bcell.p, 1)))
-            return rc;
-        mark(e, ev, s, 1);
-        e->prof_mid = ev ? ev[NEV * s + 2] : nullptr;
-        e->prof_mid2 = ev ? ev[NEV * s + 3] : nullptr;
-        rc = launch_gather<R, MODE>(e, nd, 1);
-        e->prof_mid = e->prof_mid2 = nullptr;
+        // replayed from a captured graph once an eager list step has run
+        // with the same launch arguments (engine noise, not profiling)
+        const bool graph = e->use_graph && !ev && !nd && MODE != M_STEP && e->list_eager;
+        if (graph)
+            rc = list_step_graph<R, MODE>(e);
+        else
+            rc = launch_list_step<R, MODE>(e, nd, ev, s);
         if (rc)
             return rc;
-        if (MODE != M_STEP) {
-            if ((rc = decide(e, 1)))
-                return rc;
-        } else {
-            launch_small(e, k_epoch_inc);
-        }
+        e->list_eager = true;
         e->cur ^= 1;
         e->since_resort++;
         e->binned = false;
@@ -848,6 +913,8 @@ int mdx_create(const mdx_desc *d, mdx_engine **out)
         e->use_tiles = std::atoi(t) != 0;
     if (const char *t = std::getenv("MDX_SPLIT"))
         e->use_split = std::atoi(t) != 0;
+    if (const char *t = std::getenv("MDX_GRAPH"))
+        e->use_graph = std::atoi(t) != 0;
     if (const char *t = std::getenv("MDX_WALKC"))
         e->use_walkc = std::atoi(t) != 0;
     if (const char *t = std::getenv("MDX_COMPACT"))
@@ -917,6 +984,9 @@ void mdx_destroy(mdx_engine *e)
                    &e->bmax,    &e->Tp};
     for (Buf *b : bufs)
         dfree(*b);
+    for (GraphSlot &g : e->graphs)
+        if (g.exec)
+            (void)hipGraphExecDestroy(g.exec);
     for (int b = 0; b < 2; b++) {
         dfree(e->X[b]);
         dfree(e->Q[b]);
