@@ -42,6 +42,9 @@ HBM_PEAK_GBS = 8000.0
 #   mixed: x FP64 24 + p FP32 12 + Q FP32 36 + L FP32 12 = 84 B state -> 2*84 + 4
 #   fp64 : x 24 + p 24 + Q 72 + L 24 = 144 B state -> 2*144 + 4
 ALGO_BYTES = {"mixed": 2 * 84 + 4, "fp64": 2 * 144 + 4}
+# beads: x 24 + p 12 (mixed) or 24 (fp64), plus the static type word and the
+# bead's bond (4 x 4 B) and angle (2 x 4 B) partners
+BEAD_BYTES = {"mixed": 2 * 36 + 28, "fp64": 2 * 48 + 28}
 # kernels of one step and the phases of mdx_profile_step that time them
 STEP_PHASES = ("bins", "walk", "patch", "finish")
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_condensate_ab_1M_mixed.json")
@@ -61,33 +64,67 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+WORKLOADS = {
+    "condensate_ab": ("1,048,576 two-species patchy condensate: A 3 planar patches / B 2 "
+                      "patches, A-B eps_p=8, d=1.2, cos(theta_m)=0.95, WCA eps=sigma=1 "
+                      "rc=1.12246, packing fraction 0.05, kT=1, gamma=1, gamma_r=1/3, dt=0.005"),
+    "patchy_tetra": ("4096 tetrahedral patchy particles, eps_p=6, d=1.2, cos(theta_m)=0.95, WCA, "
+                     "packing fraction 0.1, kT=1, gamma=1, gamma_r=1/3, dt=0.005"),
+    "replicas": ("256 independent tetrahedral patchy replicas, N log-uniform in [512, 16384] "
+                 "(1.17M total), kT ~ U[0.8,1.2], eps_p ~ U[4,10], packing fraction 0.1, "
+                 "one engine"),
+    "mt_bundle": ("416,000-bead microtubule bundle: 64 tubes x 13 protofilaments x 500 beads, "
+                  "harmonic bonds k=100 r0=1, cosine bending kappa=50, WCA between non-bonded "
+                  "beads, box 200x200x600, dt=0.002"),
+    "active_polymer": ("2,097,152-bead active polymer melt: 8192 chains x 256 beads, harmonic "
+                       "k=200, cosine bending kappa=20, tangential f_a=1, patchy rotor end "
+                       "beads (2 patches, eps_p=5), packing fraction 0.15"),
+}
+
+
 def build_workload(name):
+    """(system dict or None, list of replica dicts or None)"""
     if name == "condensate_ab":
-        return wl.condensate_ab()
+        return wl.condensate_ab(), None
     if name == "patchy_tetra":
-        return wl.patchy_tetra()
+        return wl.patchy_tetra(), None
+    if name == "replicas":
+        reps = wl.replicas(progress=log)
+        return reps[0], reps
+    if name == "mt_bundle":
+        return wl.mt_bundle(), None
+    if name == "active_polymer":
+        return wl.active_polymer(progress=log), None
     raise SystemExit(f"unknown workload {name}")
 
 
-def force_error(s):
+def force_error(s, reps=None):
     """mixed vs FP64 conservative forces/torques on the same frozen
     configuration (orientations rounded to FP32, as the mixed engine holds them)"""
-    s = dict(s)
-    s["Q"] = s["Q"].astype(np.float32).astype(np.float64)
-    with mdx.engine_from_system(s, precision=mdx.PREC_FP64) as e:
+    def rounded(x):
+        x = dict(x)
+        if x.get("Q") is not None:
+            x["Q"] = x["Q"].astype(np.float32).astype(np.float64)
+        return x
+    s = rounded(s)
+    reps = [rounded(r) for r in reps] if reps else None
+    with mdx.engine_from_system(s, precision=mdx.PREC_FP64, replicas=reps) as e:
         F64, T64 = e.compute_forces()
-    with mdx.engine_from_system(s, precision=mdx.PREC_MIXED) as e:
+    with mdx.engine_from_system(s, precision=mdx.PREC_MIXED, replicas=reps) as e:
         F32, T32 = e.compute_forces()
-    return {
+    out = {
         "force_rel_err_max": float(np.abs(F32 - F64).max() / np.abs(F64).max()),
         "force_rel_err_rms": float(np.linalg.norm(F32 - F64) / np.linalg.norm(F64)),
-        "torque_rel_err_max": float(np.abs(T32 - T64).max() / max(np.abs(T64).max(), 1e-300)),
     }
+    if np.abs(T64).max() > 0:
+        out["torque_rel_err_max"] = float(np.abs(T32 - T64).max() / np.abs(T64).max())
+    return out
 
 
-def time_engine(s, precision, steps, warmup, seed=1):
-    with mdx.engine_from_system(s, precision=precision, seed=seed) as e:
-        e.prime()
+def time_engine(s, precision, steps, warmup, seed=1, reps=None):
+    with mdx.engine_from_system(s, precision=precision, seed=seed, replicas=reps) as e:
+        if s["integrator"] == "rotor":
+            e.prime()
         if warmup:
             e.step(warmup)
         t0 = time.perf_counter()
@@ -110,9 +147,18 @@ def cpu_baseline(s, nsteps, threads):
     job = {"N": s["N"], "L": repr(float(s["box"][0])), "periodic": 1,
            "n": int(s["box"][0] // 4), "cut": 3.5, "wca_eps": 1.0, "wca_sigma": 1.0,
            "dt": s["dt"], "gamma": s["gamma"], "gammar": repr(float(s["gammar"])),
-           "kT": s["kT"], "nsteps": nsteps, "maxd2": 2.4, "patch_kind": "general",
-           "ntypes": 2, "npatch_0": 3, "npatch_1": 2,
-           "numpertype_0": int(np.sum(ps["types"] == 0)), "numpertype_1": s["N"]}
+           "kT": s["kT"], "nsteps": nsteps, "maxd2": 2.4}
+    if list(ps["npatch"]) == [4]:
+        e, d, th = ps["params"][0]
+        job.update({"patch_kind": "tetra", "eps_p": repr(float(e)), "d": repr(float(d)),
+                    "thetam": repr(float(th))})
+    else:
+        job.update({"patch_kind": "general", "ntypes": len(ps["npatch"])})
+        cum = 0
+        for t, npt in enumerate(ps["npatch"]):
+            cum += int(np.sum(ps["types"] == t))
+            job[f"npatch_{t}"] = int(npt)
+            job[f"numpertype_{t}"] = cum
     with tempfile.TemporaryDirectory() as d:
         out = po.run_ref("time_rotor", d, job, {"x": s["x"], "Q": s["Q"],
                                                 "params": ps["params"], "dirs": ps["dirs"]},
@@ -162,18 +208,21 @@ def main():
         copy_probe()
 
     log("building workload", args.workload)
-    s = build_workload(args.workload)
-    N = s["N"]
+    s, reps = build_workload(args.workload)
+    N = sum(r["N"] for r in reps) if reps else s["N"]
+    rotor = s["integrator"] == "rotor"
+    algo = ALGO_BYTES if rotor else BEAD_BYTES
     log("timing mixed precision", args.steps, "steps after", args.warmup, "warm-up")
-    t_mixed, prof, info = time_engine(s, mdx.PREC_MIXED, args.steps, args.warmup)
+    t_mixed, prof, info = time_engine(s, mdx.PREC_MIXED, args.steps, args.warmup, reps=reps)
     log("mixed ms/step", 1e3 * t_mixed / args.steps, "kernels", prof)
     value = N * args.steps / t_mixed
-    step_ms = sum(prof.get(k, 0.0) for k in STEP_PHASES)
-    achieved = ALGO_BYTES["mixed"] * N / (step_ms * 1e-3) / 1e9
+    step_ms = sum(prof.get(k, 0.0) for k in STEP_PHASES + ("gather",) if k in prof
+                  and not (k == "gather" and "walk" in prof))
+    achieved = algo["mixed"] * N / (step_ms * 1e-3) / 1e9
     tr = measured_traffic()
     traffic = None
     per_kernel = {k: {"ms": prof[k]} for k in STEP_PHASES if k in prof}
-    if tr and tr.get("N") == N:
+    if tr and tr.get("N") == N and args.workload == "condensate_ab":
         traffic = tr["bytes_per_step"]
         for k, v in tr.get("per_phase_bytes", {}).items():
             if k in per_kernel:
@@ -193,9 +242,7 @@ def main():
         "dtype": "f32 (f64 positions and decisions)",
         "data": "synthetic (seeded uniform positions with overlap rejection, Arvo random rotations)",
         "config": {
-            "workload": f"{args.workload}: N={N}, box={float(s['box'][0]):.3f}, "
-                        "A 3 planar patches / B 2 patches, A-B eps_p=8, d=1.2, cos(theta_m)=0.95, "
-                        "WCA eps=sigma=1 rc=1.12246, kT=1, gamma=1, gamma_r=1/3, dt=0.005",
+            "workload": f"{args.workload}: N={N}, " + WORKLOADS[args.workload],
             "precision": "mixed",
             "cells": info["cells"],
             "cell_size": round(info["cell_size"], 4),
@@ -207,9 +254,11 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "one step: k_walk + k_patch + k_integrate + list upkeep (k_bin, k_scan1-3, "
-                      "k_fill, k_cellsort, k_nbuild, gated; amortised)",
-            "algorithmic_bytes_per_particle_step": ALGO_BYTES["mixed"],
+            "kernel": ("one step: k_walkc + k_patch + k_integrate + list upkeep (k_bin, k_scan1, "
+                       "k_scan3, k_fill, k_cellsort, k_nbuild, gated; amortised) + k_decide"
+                       if "walk" in prof else
+                       "one step: k_gather + list upkeep (gated; amortised) + k_decide"),
+            "algorithmic_bytes_per_particle_step": algo["mixed"],
             "step_kernel_ms": step_ms,
             "per_kernel": per_kernel,
         },
@@ -218,15 +267,15 @@ def main():
     if not args.no_fp64:
         fsteps = max(10, args.steps // 10)
         log("timing fp64", fsteps, "steps")
-        t64, prof64, _ = time_engine(s, mdx.PREC_FP64, fsteps, 2)
+        t64, prof64, _ = time_engine(s, mdx.PREC_FP64, fsteps, 2, reps=reps)
         log("fp64 ms/step", 1e3 * t64 / fsteps)
         v64 = N * fsteps / t64
         out["fp64_gpu_value"] = v64
         out["mixed_over_fp64"] = value / v64
         out["fp64_kernel_ms"] = prof64
-        out.update(force_error(s))
+        out.update(force_error(s, reps))
         log("force error", {k: out[k] for k in out if "err" in k})
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload in ("condensate_ab", "patchy_tetra"):
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         log("cpu baseline (reference harness),", threads, "threads")
         cb = cpu_baseline(s, args.cpu_steps, threads)

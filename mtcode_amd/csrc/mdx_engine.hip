@@ -448,8 +448,12 @@ static int launch_gather(mdx_engine *e, const double *noise_dev, int lmode = 0)
             HIPCHK(hipEventRecord(e->prof_mid, e->stream));
         e->last_split = true;
         if (e->use_compact) {
-            hipLaunchKernelGGL((k_patch<R, MODE>), dim3(nblk(e->N, PBLK)), dim3(256), 0, e->stream,
-                               a);
+            // positions per block: as many as keep about 512 blocks
+            int pblk = 256;
+            while (pblk < PBLK && (long long)e->N / (2 * pblk) >= 512)
+                pblk *= 2;
+            hipLaunchKernelGGL((k_patch<R, MODE>), dim3(nblk(e->N, pblk)), dim3(256), 0, e->stream,
+                               a, pblk);
             if (e->prof_mid2)
                 HIPCHK(hipEventRecord(e->prof_mid2, e->stream));
             hipLaunchKernelGGL((k_integrate<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream,

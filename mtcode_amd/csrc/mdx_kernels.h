@@ -360,6 +360,25 @@ __device__ __forceinline__ void for_each_neighbor(const KArgs<R> &a, const DevRe
     for_each_in_cells(a, rp, a.cell[i], i, fn);
 }
 
+// nn[i]: entry count, and LIST_EXC when the list holds a pair without WCA
+// (an exclusion kept for its patch interaction); lists without that flag hold
+// no excluded pair, so their walk needs no exclusion test
+constexpr int LIST_EXC = 1 << 30;
+template <class R> __device__ __forceinline__ int list_count(const KArgs<R> &a, int i)
+{
+    return a.nn[i] & (LIST_EXC - 1);
+}
+// exclusions must be tested for slot i's pairs in this walk
+template <class R> __device__ __forceinline__ bool needs_exclusions(const KArgs<R> &a, int i)
+{
+    if (!a.exc_off)
+        return false;
+    if (!a.lmode)
+        return true;
+    const int v = a.nn[i];
+    return (v & LIST_EXC) || (v & (LIST_EXC - 1)) > a.kmax;
+}
+
 // neighbours of slot i from the list (order of the build's cell walk), or,
 // for a particle with more than kmax listed, the 27 cells of the build.  Both
 // are supersets of the pairs within reach while the displacement bound holds.
@@ -373,7 +392,7 @@ __device__ __forceinline__ void for_each_listed(const KArgs<R> &a, const DevRep 
 #pragma unroll
     for (int u = 0; u < MDX_LBATCH; u++)
         j0[u] = a.nbr[(size_t)u * a.N + i];
-    const int n = a.nn[i];
+    const int n = list_count(a, i);
     if (n <= a.kmax) {
         // MDX_LBATCH entries per round: their index and position loads overlap
         for (int k0 = 0; k0 < n; k0 += MDX_LBATCH) {
@@ -673,7 +692,7 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
             qi[k] = a.Q[9 * (size_t)i + k];
     }
     int exc0 = 0, exc1 = 0;
-    if (a.exc_off) {
+    if (needs_exclusions(a, i)) {
         exc0 = a.exc_off[oi];
         exc1 = a.exc_off[oi + 1];
     }
@@ -1061,7 +1080,7 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_walkc(KArg
     const int base = xcd_block(blockIdx.x, gridDim.x) * 256;
     const int s = base + tid;
     const bool live = s < a.N;
-    const int n = live ? a.nn[s] : 0;
+    const int n = live ? list_count(a, s) : 0;
     const bool fb = n > a.kmax;
     const int cnt = fb ? 0 : n;
     PosRec xi;
@@ -1073,8 +1092,9 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_walkc(KArg
             for (int k = 0; k < 9; k++)
                 sq[9 * tid + k] = a.Q[9 * (size_t)s + k];
         }
-        sexc[2 * tid] = a.exc_off ? a.exc_off[xi.id] : 0;
-        sexc[2 * tid + 1] = a.exc_off ? a.exc_off[xi.id + 1] : 0;
+        const bool ne = needs_exclusions(a, s);
+        sexc[2 * tid] = ne ? a.exc_off[xi.id] : 0;
+        sexc[2 * tid + 1] = ne ? a.exc_off[xi.id + 1] : 0;
     }
     // exclusive scan of the entry counts
     int v = cnt;
@@ -1191,9 +1211,9 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_finish(KAr
         drift_bound(a, drift);
 }
 
-// split form, compacted: the patch candidates of a block's PBLK positions
+// split form, compacted: the patch candidates of a block of pblk positions
 // are laid out back to back and evaluated one per thread (no idle lanes on
-// positions without candidates; one block per PBLK positions keeps the whole
+// positions without candidates; large blocks keep the whole
 // grid resident).  The thread holding a position's first candidate of a
 // round then adds that round's terms in candidate order to the position's
 // force (the walk force in Fw) and patch torque (Tp): the same order as the
@@ -1201,15 +1221,15 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_finish(KAr
 // particle pair passes the cone tests (true for cos(theta_m) >= 0.5 with
 // these patch sets).  Positions without candidates are not touched (the
 // integrator reads ncand).
-constexpr int PBLK = 2048;
-template <class R, int MODE> __global__ void __launch_bounds__(256) k_patch(KArgs<R> a)
+constexpr int PBLK = 2048; // largest block of positions (pblk: 256 .. PBLK, a multiple of 256)
+template <class R, int MODE> __global__ void __launch_bounds__(256) k_patch(KArgs<R> a, int pblk)
 {
-    constexpr int PER = PBLK / 256;
+    const int PER = pblk / 256;
     __shared__ int off[PBLK + 1];
     __shared__ R ent[256 * 6];
     __shared__ int wsum[4];
     const int tid = threadIdx.x;
-    const int base = xcd_block(blockIdx.x, gridDim.x) * PBLK;
+    const int base = xcd_block(blockIdx.x, gridDim.x) * pblk;
     // exclusive scan of min(ncand, PMAXG) in position order (position
     // k * 256 + tid of the block, k < PER)
     int carry = 0;
@@ -1234,7 +1254,7 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_patch(KArg
         __syncthreads();
     }
     if (tid == 0)
-        off[PBLK] = carry;
+        off[pblk] = carry;
     __syncthreads();
     const int total = carry;
     for (int r0 = 0; r0 < total; r0 += 256) {
@@ -1242,7 +1262,7 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_patch(KArg
         int o = 0;
         if (e < total) {
             // owner: the last position whose offset is <= e
-            int lo = 0, hi = PBLK - 1;
+            int lo = 0, hi = pblk - 1;
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
                 if (off[mid] <= e)
@@ -1368,17 +1388,41 @@ template <class R> __global__ void __launch_bounds__(256) k_nbuild(KArgs<R> a, c
     const DevRep rp = rep_rec(a, xi.meta);
     const int c = cells[s];
     a.bcell[s] = c;
+    int exc0 = 0, exc1 = 0;
+    if (a.exc_off) {
+        exc0 = a.exc_off[xi.id];
+        exc1 = a.exc_off[xi.id + 1];
+    }
+    const int ti = xi.meta & 255;
     int n = 0;
+    bool kept_excluded = false;
     for_each_in_cells(a, rp, c, s, [&](const PosRec &xj, int j) {
         double d[3];
         min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
-        if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < a.rl2) {
-            if (n < a.kmax)
-                a.nbrw[(size_t)n * a.N + s] = j;
-            n++;
+        if (!(d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < a.rl2))
+            return;
+        // a pair without WCA (exclusion) is listed only if it can still
+        // interact through patches
+        bool ex = false;
+        for (int e = exc0; e < exc1; e++)
+            ex |= (uint32_t)a.exc_j[e] == xj.id;
+        if (ex) {
+            const int tj = xj.meta & 255;
+            bool patchy = a.has_patches && a.np[ti] > 0 && a.np[tj] > 0;
+            if (patchy) {
+                int ta, tc;
+                patch_roles(xi.id, ti, xj.id, tj, ta, tc);
+                patchy = (a.active_mask >> (ta * MAXT + tc)) & 1ull;
+            }
+            if (!patchy)
+                return;
+            kept_excluded = true;
         }
+        if (n < a.kmax)
+            a.nbrw[(size_t)n * a.N + s] = j;
+        n++;
     });
-    a.nn[s] = n;
+    a.nn[s] = n | (kept_excluded ? LIST_EXC : 0);
 }
 
 // ------------------------------------------------------- LDS-staged tiles

@@ -204,7 +204,7 @@ def condensate_ab(n: int = 1 << 20, phi: float = 0.05, seed_pos: int = 3, seed_q
 
 
 def replicas(count: int = 256, seed: int = 5, nmin: int = 512, nmax: int = 16384,
-             phi: float = 0.1, scale: float = 1.0) -> list:
+             phi: float = 0.1, scale: float = 1.0, progress=None) -> list:
     """Independent tetrahedral systems with N log-uniform in [nmin, nmax],
     kT ~ U[0.8, 1.2], eps_p ~ U[4, 10], each in its own box at packing
     fraction phi.  ``scale`` shrinks every N (for tests)."""
@@ -219,6 +219,8 @@ def replicas(count: int = 256, seed: int = 5, nmin: int = 512, nmax: int = 16384
                          kT=float(kts[r]))
         s["name"] = f"replica{r}"
         out.append(s)
+        if progress and (r + 1) % 32 == 0:
+            progress(f"replica {r + 1}/{count}")
     return out
 
 
@@ -290,7 +292,7 @@ def mt_bundle(ntubes_side: int = 8, nproto: int = 13, nbeads: int = 500, spacing
 
 def active_polymer(nchains: int = 8192, nbeads: int = 256, phi: float = 0.15, seed: int = 6,
                    kbond: float = 200.0, kappa: float = 20.0, fa: float = 1.0,
-                   eps_p: float = 5.0) -> dict:
+                   eps_p: float = 5.0, progress=None) -> dict:
     """Chains grown as off-lattice self-avoiding random walks (unit steps, no
     bead closer than 0.9 to an earlier bead of any chain), periodic box at
     volume fraction phi.  Harmonic bonds, cosine bending, tangential active
@@ -300,7 +302,7 @@ def active_polymer(nchains: int = 8192, nbeads: int = 256, phi: float = 0.15, se
     L = box_for_packing(n, phi)
     rng = np.random.default_rng(seed)
     s = _base("active_polymer", n, L, 0.005, 1.0, 1.0, 1.0 / 3.0, "rotor")
-    x = _grow_chains(nchains, nbeads, L, rng)
+    x = _grow_chains(nchains, nbeads, L, rng, progress=progress)
     s["x"] = x
     idx = np.arange(n).reshape(nchains, nbeads)
     b = np.stack([idx[:, :-1].ravel(), idx[:, 1:].ravel()], 1).astype(np.int32)
@@ -403,7 +405,7 @@ class _BeadGrid:
 
 
 def _grow_chains(nchains: int, nbeads: int, L: float, rng: np.random.Generator,
-                 rmin: float = 0.9, tries: int = 50, back: int = 4) -> np.ndarray:
+                 rmin: float = 0.9, tries: int = 50, back: int = 4, progress=None) -> np.ndarray:
     """Off-lattice self-avoiding walks grown in parallel, one bead per chain
     per round: unit steps in random directions, a trial rejected when closer
     than rmin (minimum image) to any placed bead or to another accepted trial
@@ -425,6 +427,8 @@ def _grow_chains(nchains: int, nbeads: int, L: float, rng: np.random.Generator,
         if len(act) == 0:
             break
         rounds += 1
+        if progress and rounds % 64 == 0:
+            progress(f"chain growth round {rounds}, {int(length.sum())}/{n} beads")
         if rounds > 50 * nbeads + 1000:
             raise RuntimeError("chain growth did not finish")
         v = rng.normal(size=(len(act), 3))
