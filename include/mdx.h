@@ -145,6 +145,21 @@ int mdx_neighbor_pairs(mdx_engine *e, int replica, const int n[3], double cut, i
  * calculate_forces_and_torques3D(pairs, ComboPatch, F, T) (LangevinR.cpp:911-1047) */
 int mdx_compute_forces(mdx_engine *e, double *F, double *T);
 
+/* Analysis of the current state (the scope's next row: observables). */
+/* calcg1 (Analysis/PairCorrelation.cpp:9-155) for replica r: counts[k] =
+ * number of particle pairs with floor(dis / dx) = k, k < floor(max / dx)
+ * (*nbins; at most 4096), dis the minimum-image distance of cube::distance
+ * (geometry.h:107-123), on floor(l / max) cells per side as the reference
+ * builds them.  Bit-exact integer counts. */
+int mdx_pair_histogram(mdx_engine *e, int replica, double max, double dx, long long *counts,
+                       int cap, int *nbins);
+/* patch bonds per particle (N ints, original order): partners with a patch
+ * pair whose Kern-Frenkel term is non-zero (eps != 0, r <= 1.4 d, both cone
+ * tests, potentialtheta.h:517-750, within max_distance_check), decided with
+ * the force loop's own FP64 expressions.  No reference counterpart (the
+ * reference's analysis counts bonds from its binding store). */
+int mdx_patch_bonds(mdx_engine *e, int *bonds);
+
 /* Time stepping. */
 /* Rotor integrator: computes the carried F, T at the current state
  * (Condensate.cpp:318-331).  noise0 (N x 6, nullable: Philox) */

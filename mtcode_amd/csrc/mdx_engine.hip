@@ -427,8 +427,12 @@ static int launch_gather(mdx_engine *e, const double *noise_dev, int lmode = 0)
     KArgs<R> a = make_args<R>(e);
     a.noise = noise_dev;
     a.lmode = lmode;
-    if (e->use_split && a.has_patches &&
-        (MODE == M_PRIME || MODE == M_STEP || MODE == M_FUSED || MODE == M_CONS)) {
+    // split form: patch systems, and list steps of the bead integrator (walk,
+    // then the integrator kernel; no patch kernel)
+    const bool bead_list = MODE == M_BEAD && lmode && e->use_walkc;
+    if (e->use_split && ((a.has_patches && (MODE == M_PRIME || MODE == M_STEP ||
+                                           MODE == M_FUSED || MODE == M_CONS)) ||
+                         bead_list)) {
         int rc;
         if ((rc = dalloc(e, e->Fw, 3 * e->rsz() * (size_t)e->N)) ||
             (rc = dalloc(e, e->ncand, 4 * (size_t)e->N)) ||
@@ -447,7 +451,12 @@ static int launch_gather(mdx_engine *e, const double *noise_dev, int lmode = 0)
         if (e->prof_mid)
             HIPCHK(hipEventRecord(e->prof_mid, e->stream));
         e->last_split = true;
-        if (e->use_compact) {
+        if (bead_list) {
+            if (e->prof_mid2)
+                HIPCHK(hipEventRecord(e->prof_mid2, e->stream));
+            hipLaunchKernelGGL((k_integrate<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream,
+                               a);
+        } else if (e->use_compact) {
             // positions per block: as many as keep about 512 blocks
             int pblk = 256;
             while (pblk < PBLK && (long long)e->N / (2 * pblk) >= 512)
@@ -1470,14 +1479,11 @@ int mdx_cell_assign(mdx_engine *e, int replica, const int n[3], int *cell_of)
     return download(e, e->tmpi, cell_of, (size_t)cnt);
 }
 
-int mdx_neighbor_pairs(mdx_engine *e, int replica, const int n[3], double cut, int *ij,
-                       long long cap, long long *npairs)
+// positions of replica r in original order (tmpx), their cells on an
+// n[0] x n[1] x n[2] grid (tmpi) and that grid's cell list (tmpi3: starts,
+// tmpi2: members in original order); a parity / analysis path, not the step
+static int custom_cells(mdx_engine *e, int replica, const int n[3])
 {
-    if (!e || replica < 0 || replica >= e->nrep || !n || !npairs || cap < 0 || (cap > 0 && !ij))
-        return set_err(MDX_ERR_ARG, "bad arguments");
-    if (n[0] < 1 || n[1] < 1 || n[2] < 1 || !(cut > 0))
-        return set_err(MDX_ERR_ARG, "n >= 1 and cut > 0 required");
-    HIPCHK(hipSetDevice(e->device));
     const long long nc = (long long)n[0] * n[1] * n[2];
     if (nc > (1ll << 28))
         return set_err(MDX_ERR_ARG, "too many cells");
@@ -1485,14 +1491,12 @@ int mdx_neighbor_pairs(mdx_engine *e, int replica, const int n[3], double cut, i
     const double *b = &e->box[3 * replica];
     int rc;
     if ((rc = replica_positions(e, replica)) || (rc = dalloc(e, e->tmpi, 4 * (size_t)cnt)) ||
-        (rc = dalloc(e, e->tmpi2, 4 * (size_t)cnt)) || (rc = dalloc(e, e->tmpi3, 4 * (size_t)(nc + 1))) ||
-        (rc = dalloc(e, e->tmpll, 8 * (size_t)(cnt + 1))) || (rc = dalloc(e, e->partial, 4 * (size_t)cnt)))
+        (rc = dalloc(e, e->tmpi2, 4 * (size_t)cnt)) || (rc = dalloc(e, e->tmpi3, 4 * (size_t)(nc + 1))))
         return rc;
-    const double *X = (const double *)e->tmpx.p;
-    hipLaunchKernelGGL(k_assign_custom, dim3(nblk(cnt)), dim3(256), 0, e->stream, X, cnt, b[0],
-                       b[1], b[2], n[0], n[1], n[2], (int *)e->tmpi.p);
+    hipLaunchKernelGGL(k_assign_custom, dim3(nblk(cnt)), dim3(256), 0, e->stream,
+                       (const double *)e->tmpx.p, cnt, b[0], b[1], b[2], n[0], n[1], n[2],
+                       (int *)e->tmpi.p);
     HIPCHK(hipGetLastError());
-    // counting sort of the cell ids on the host (parity path, not the step)
     std::vector<int> cellv(cnt), cstart(nc + 1, 0), clist(cnt);
     if ((rc = download(e, e->tmpi, cellv.data(), (size_t)cnt)))
         return rc;
@@ -1508,6 +1512,90 @@ int mdx_neighbor_pairs(mdx_engine *e, int replica, const int n[3], double cut, i
     if ((rc = upload(e, e->tmpi3, cstart.data(), cstart.size())) ||
         (rc = upload(e, e->tmpi2, clist.data(), clist.size())))
         return rc;
+    return MDX_OK;
+}
+
+int mdx_pair_histogram(mdx_engine *e, int replica, double max, double dx, long long *counts,
+                       int cap, int *nbins)
+{
+    if (!e || replica < 0 || replica >= e->nrep || !nbins || cap < 0 || (cap > 0 && !counts))
+        return set_err(MDX_ERR_ARG, "bad arguments");
+    if (!(max > 0) || !(dx > 0))
+        return set_err(MDX_ERR_ARG, "max and dx must be > 0");
+    const double gsd = std::floor(max / dx);
+    if (gsd > GMAXB)
+        return set_err(MDX_ERR_ARG, "at most %d bins", GMAXB);
+    const int gs = (int)gsd;
+    HIPCHK(hipSetDevice(e->device));
+    // calcg1: floor(l / max) cells per side (PairCorrelation.cpp:14)
+    const double *b = &e->box[3 * replica];
+    int n[3];
+    for (int k = 0; k < 3; k++) {
+        n[k] = (int)std::floor(b[k] / max);
+        if (n[k] < 1)
+            return set_err(MDX_ERR_ARG, "max larger than the box");
+    }
+    int rc;
+    if ((rc = custom_cells(e, replica, n)) || (rc = dalloc(e, e->tmpll, 8 * (size_t)std::max(gs, 1))))
+        return rc;
+    HIPCHK(hipMemsetAsync(e->tmpll.p, 0, 8 * (size_t)std::max(gs, 1), e->stream));
+    const int cnt = e->nper[replica];
+    hipLaunchKernelGGL(k_gcount, dim3(nblk(cnt)), dim3(256), 0, e->stream,
+                       (const double *)e->tmpx.p, cnt, b[0], b[1], b[2], e->periodic, n[0], n[1],
+                       n[2], (const int *)e->tmpi.p, (const int *)e->tmpi3.p,
+                       (const int *)e->tmpi2.p, dx, gs, (unsigned long long *)e->tmpll.p);
+    HIPCHK(hipGetLastError());
+    *nbins = gs;
+    std::vector<long long> h((size_t)std::max(gs, 1));
+    if ((rc = download(e, e->tmpll, h.data(), h.size())))
+        return rc;
+    for (int k = 0; k < std::min(gs, cap); k++)
+        counts[k] = h[(size_t)k];
+    if (cap < gs)
+        return set_err(MDX_ERR_CAPACITY, "%d bins, room for %d", gs, cap);
+    return MDX_OK;
+}
+
+template <class R> static int patch_bonds_t(mdx_engine *e, int *bonds)
+{
+    int rc;
+    if (!e->binned && (rc = bin_current<R>(e)))
+        return rc;
+    if ((rc = dalloc(e, e->tmpi, 4 * (size_t)e->N)))
+        return rc;
+    KArgs<R> a = make_args<R>(e);
+    hipLaunchKernelGGL(k_patch_bonds<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, a,
+                       (int *)e->tmpi.p);
+    HIPCHK(hipGetLastError());
+    return download(e, e->tmpi, bonds, (size_t)e->N);
+}
+
+int mdx_patch_bonds(mdx_engine *e, int *bonds)
+{
+    if (!e || !bonds)
+        return set_err(MDX_ERR_ARG, "bad arguments");
+    if (e->integ != MDX_INTEG_ROTOR)
+        return set_err(MDX_ERR_STATE, "patch bonds need the rotor integrator");
+    HIPCHK(hipSetDevice(e->device));
+    return e->prec == MDX_PREC_FP64 ? patch_bonds_t<double>(e, bonds)
+                                    : patch_bonds_t<float>(e, bonds);
+}
+
+int mdx_neighbor_pairs(mdx_engine *e, int replica, const int n[3], double cut, int *ij,
+                       long long cap, long long *npairs)
+{
+    if (!e || replica < 0 || replica >= e->nrep || !n || !npairs || cap < 0 || (cap > 0 && !ij))
+        return set_err(MDX_ERR_ARG, "bad arguments");
+    if (n[0] < 1 || n[1] < 1 || n[2] < 1 || !(cut > 0))
+        return set_err(MDX_ERR_ARG, "n >= 1 and cut > 0 required");
+    HIPCHK(hipSetDevice(e->device));
+    const int cnt = e->nper[replica];
+    const double *b = &e->box[3 * replica];
+    int rc;
+    if ((rc = custom_cells(e, replica, n)) || (rc = dalloc(e, e->tmpll, 8 * (size_t)(cnt + 1))) ||
+        (rc = dalloc(e, e->partial, 4 * (size_t)cnt)))
+        return rc;
+    const double *X = (const double *)e->tmpx.p;
     hipLaunchKernelGGL(k_pairs, dim3(nblk(cnt)), dim3(256), 0, e->stream, X, cnt, b[0], b[1],
                        b[2], e->periodic, n[0], n[1], n[2], cut, (const int *)e->tmpi.p,
                        (const int *)e->tmpi3.p, (const int *)e->tmpi2.p, (int *)e->partial.p,

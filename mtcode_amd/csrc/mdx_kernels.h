@@ -1891,6 +1891,128 @@ __global__ void k_pairs(const double *X, int count, double l0, double l1, double
     }
 }
 
+// calcg1 (Analysis/PairCorrelation.cpp:9-155): every unordered pair of
+// stencil-adjacent cells once (j > k), the distance of cube::distance
+// (geometry.h:107-123), bin floor(dis / dx) counted when below gs.  Counts are
+// integers: the atomic order does not change them.
+constexpr int GMAXB = 4096;
+__global__ void __launch_bounds__(256) k_gcount(const double *X, int count, double l0, double l1,
+                                                double l2, int periodic, int n0, int n1, int n2,
+                                                const int *cellv, const int *cstart,
+                                                const int *clist, double dxb, int gs,
+                                                unsigned long long *hist)
+{
+    __shared__ unsigned int sh[GMAXB];
+    for (int b = threadIdx.x; b < gs; b += blockDim.x)
+        sh[b] = 0u;
+    __syncthreads();
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) {
+        const double l[3] = {l0, l1, l2};
+        const int nn[3] = {n0, n1, n2};
+        const double *xi = X + 3 * (size_t)k;
+        const int c = cellv[k];
+        const int cc[3] = {c % n0, (c / n0) % n1, c / (n0 * n1)};
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; a++) {
+            lo[a] = nn[a] >= 3 ? -1 : 0;
+            hi[a] = nn[a] >= 2 ? 1 : 0;
+        }
+        for (int oz = lo[2]; oz <= hi[2]; oz++)
+            for (int oy = lo[1]; oy <= hi[1]; oy++)
+                for (int ox = lo[0]; ox <= hi[0]; ox++) {
+                    int w[3] = {cc[0] + ox, cc[1] + oy, cc[2] + oz};
+                    for (int a = 0; a < 3; a++)
+                        w[a] = w[a] < 0 ? w[a] + nn[a] : (w[a] >= nn[a] ? w[a] - nn[a] : w[a]);
+                    const int c2 = w[0] + n0 * (w[1] + n1 * w[2]);
+                    for (int s = cstart[c2]; s < cstart[c2 + 1]; s++) {
+                        const int j = clist[s];
+                        if (j <= k)
+                            continue;
+                        const double *xj = X + 3 * (size_t)j;
+                        double temp = 0.0;
+                        for (int a = 0; a < 3; a++) {
+                            double d = xi[a] - xj[a];
+                            if (periodic && fabs(d) > l[a] * 0.5)
+                                d = d - (d >= 0 ? fabs(l[a]) : -fabs(l[a]));
+                            temp += d * d;
+                        }
+                        const int in = (int)floor(sqrt(temp) / dxb);
+                        if (in < gs)
+                            atomicAdd(&sh[in], 1u);
+                    }
+                }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < gs; b += blockDim.x)
+        if (sh[b])
+            atomicAdd(&hist[b], (unsigned long long)sh[b]);
+}
+
+// patch bonds per particle: partners with a patch pair whose Kern-Frenkel
+// term is non-zero (eps != 0, r <= 1.4 d and both cone tests passing,
+// potentialtheta.h:517-750), decided with patch_pair's FP64 expressions
+template <class R>
+__global__ void __launch_bounds__(256) k_patch_bonds(KArgs<R> a, int *bonds)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.N)
+        return;
+    const int i = a.list[s];
+    const PosRec xi = a.X[i];
+    const int ti = xi.meta & 255;
+    const DevRep rp = rep_rec(a, xi.meta);
+    int nb = 0;
+    if (a.has_patches && a.np[ti] > 0) {
+        R qi[9];
+        for (int k = 0; k < 9; k++)
+            qi[k] = a.Q[9 * (size_t)i + k];
+        for_each_neighbor(a, rp, i, [&](const PosRec &xj, int j) {
+            double d[3];
+            min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
+            const double r2s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (!(r2s < a.maxd2))
+                return;
+            const int tj = xj.meta & 255;
+            if (a.np[tj] == 0)
+                return;
+            int ta, tc;
+            const bool i_is_a = patch_roles(xi.id, ti, xj.id, tj, ta, tc);
+            if (!((a.active_mask >> (ta * MAXT + tc)) & 1ull))
+                return;
+            const double rij = sqrt(r2s);
+            const double sg = i_is_a ? 1.0 : -1.0;
+            const double u[3] = {sg * d[0] / rij, sg * d[1] / rij, sg * d[2] / rij};
+            R qj[9];
+            for (int k = 0; k < 9; k++)
+                qj[k] = a.Q[9 * (size_t)j + k];
+            const R *qa = i_is_a ? qi : qj;
+            const R *qc = i_is_a ? qj : qi;
+            const int npa = a.np[ta], npc = a.np[tc];
+            const double *da = a.pdirs + 3 * a.pstart[ta];
+            const double *dc = a.pdirs + 3 * a.pstart[tc];
+            const int rowbase = rp.prow + a.rstart[ta * MAXT + tc];
+            bool bonded = false;
+            for (int ia = 0; ia < npa && !bonded; ia++) {
+                double n1[3];
+                lab_normal_exact(qa, da + 3 * ia, n1);
+                const double a1 = -(n1[0] * u[0] + n1[1] * u[1] + n1[2] * u[2]);
+                for (int ib = 0; ib < npc && !bonded; ib++) {
+                    const PatchRow pr = a.prows[rowbase + ia * npc + ib];
+                    if (pr.eps == 0.0 || rij > pr.idist || !(a1 > pr.costm))
+                        continue;
+                    double n2[3];
+                    lab_normal_exact(qc, dc + 3 * ib, n2);
+                    const double a2 = (n2[0] * u[0] + n2[1] * u[1] + n2[2] * u[2]);
+                    bonded = a2 > pr.costm;
+                }
+            }
+            nb += bonded ? 1 : 0;
+        });
+    }
+    bonds[xi.id] = nb;
+}
+
 // ------------------------------------------------------------ reductions
 // per-block partial sums (fixed order) of 16 observables: slot range
 // ------------------------------------------------ single-stage entry points

@@ -75,6 +75,9 @@ def lib():
     L.mdx_set_neighbor_list.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int]
     L.mdx_stage_halfkick.argtypes = [C.c_void_p, _dp, _dp]
     L.mdx_clear_interactions.argtypes = [C.c_void_p, C.c_int]
+    L.mdx_pair_histogram.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double,
+                                     C.POINTER(C.c_longlong), C.c_int, C.POINTER(C.c_int)]
+    L.mdx_patch_bonds.argtypes = [C.c_void_p, _ip]
     L.mdx_stage_advance_pos.argtypes = [C.c_void_p]
     L.mdx_stage_rotate.argtypes = [C.c_void_p]
     L.mdx_stage_sphere.argtypes = [C.c_void_p, _dp, _dp, _dp]
@@ -280,6 +283,21 @@ class Engine:
         return {"N": int(out[0]), "cells": int(out[1]), "cell_size": out[2],
                 "device_bytes": int(out[3]), "steps": int(out[4]), "list_builds": int(out[5]),
                 "skin": float(out[6]), "kmax": int(out[7])}
+
+    # ------------------------------------------------------- analysis
+    def pair_histogram(self, max_, dx, replica=0):
+        cap = int(np.floor(max_ / dx)) + 1
+        out = np.zeros(cap, dtype=np.int64)
+        nb = C.c_int(0)
+        _check(lib().mdx_pair_histogram(self.h, int(replica), C.c_double(max_), C.c_double(dx),
+                                         out.ctypes.data_as(C.POINTER(C.c_longlong)), cap,
+                                         C.byref(nb)))
+        return out[:nb.value]
+
+    def patch_bonds(self):
+        out = np.zeros(self.N, dtype=np.int32)
+        _check(lib().mdx_patch_bonds(self.h, out.ctypes.data_as(_ip)))
+        return out
 
     # ------------------------------------------------- single stages
     def stage_halfkick(self, F, T=None):

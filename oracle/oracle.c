@@ -707,6 +707,30 @@ void orc_advance_mom(const orc_params *P, int N, double *p, const double *F, con
                 c5 * c2 * p[3 * i + k] + (c5 * (c3) + q) * F[3 * i + k] + (c5 * (c4) + r) * R[3 * i + k];
 }
 
+int orc_calcg1(const orc_box *b, int N, const double *x, double max, double dx,
+               long long *counts, int cap)
+{
+    /* PairCorrelation.cpp:140-150 binning; geometry.h:107-123 distance */
+    int gs = (int)floor(max / dx);
+    for (int k = 0; k < gs && k < cap; k++)
+        counts[k] = 0;
+    for (int i = 0; i < N; i++)
+        for (int j = i + 1; j < N; j++) {
+            double temp = 0.0;
+            for (int k = 0; k < 3; k++) {
+                double d = x[3 * i + k] - x[3 * j + k];
+                if (b->periodic && fabs(d) > b->l[k] * 0.5)
+                    d = d - (d >= 0 ? fabs(b->l[k]) : -fabs(b->l[k]));
+                temp += d * d;
+            }
+            double dis = sqrt(temp);
+            int in = (int)floor(dis / dx);
+            if (in < gs && in < cap)
+                counts[in] += 1;
+        }
+    return gs;
+}
+
 void orc_measured_temperature(const orc_params *P, int N, const double *p, const double *L,
                               double out[6])
 {

@@ -31,6 +31,7 @@
  *                                  noise checks
  *   orc_philox4x32_10            : the Random123 known-answer vectors
  *   orc_noise_uniform            : moments / support (its own definition)
+ *   orc_calcg1                   : golden fixture from the compiled reference
  */
 #ifndef MTCODE_ORACLE_H
 #define MTCODE_ORACLE_H
@@ -220,6 +221,14 @@ int orc_run_rotors(const orc_params *P, const orc_box *b, int N, double *x, doub
 int orc_run_beads(const orc_params *P, const orc_box *b, int N, double *x, double *p,
                   const orc_interactions *it, int nsteps, const double *noise,
                   int snap_every, double *snaps);
+
+/* calcg1 (Analysis/PairCorrelation.cpp:9-155): counts[k] = number of pairs
+ * i < j with floor(dis / dx) = k < floor(max / dx), dis = cube::distance
+ * (geometry.h:107-123).  All pairs are visited (the reference visits the
+ * pairs of stencil-adjacent cells of size >= max; the others have
+ * dis >= max and fall outside the histogram).  Returns the bin count. */
+int orc_calcg1(const orc_box *b, int N, const double *x, double max, double dx,
+               long long *counts, int cap);
 
 /* measured_temperature (LangevinR.cpp:170-193): out[0..2] <p^2/2m>, out[3..5] <L^2/2I0> */
 void orc_measured_temperature(const orc_params *P, int N, const double *p, const double *L,

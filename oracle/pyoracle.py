@@ -64,6 +64,8 @@ def lib():
         L = C.CDLL(LIB_PATH)
         L.orc_fasterfloor.restype = C.c_int
         L.orc_fasterfloor.argtypes = [C.c_double]
+        L.orc_calcg1.argtypes = [C.POINTER(Box), C.c_int, _dp, C.c_double, C.c_double,
+                                 C.POINTER(C.c_longlong), C.c_int]
         L.orc_assign_all.argtypes = [C.POINTER(Box), C.c_int, _dp, _ip, _ip]
         L.orc_boxes_relationships.restype = C.c_int
         L.orc_boxes_relationships.argtypes = [_ip, _ip]
@@ -293,6 +295,16 @@ def kf2_energy(un, rij, Q1, Q2, nb1, nb2, att, dis, thetam):
     keep = _Keep()
     return lib().orc_kf2_energy(keep.d(un), float(rij), keep.d(Q1), keep.d(Q2), keep.d(nb1),
                                 keep.d(nb2), float(att), float(dis), float(thetam))
+
+
+def calcg1(box, x, max_, dx, periodic=True):
+    b = make_box(box, periodic)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    cap = int(np.floor(max_ / dx)) + 1
+    out = np.zeros(cap, dtype=np.int64)
+    gs = lib().orc_calcg1(C.byref(b), len(x), x.ctypes.data_as(_dp), C.c_double(max_),
+                          C.c_double(dx), out.ctypes.data_as(C.POINTER(C.c_longlong)), cap)
+    return out[:gs]
 
 
 def noise(seed, first, count):

@@ -23,6 +23,7 @@
  *   traj_bead  x,p -> final state, mainActivePolymer step order, injected noise
  *   time_rotor x,Q -> wall-clock of the Condensate::run loop (OpenMP, rand())
  *   long_rotor x,Q -> snapshots of a long Condensate::run loop (rand() noise)
+ *   calcg1    x    -> calcg1 pair-distance histogram (Analysis/PairCorrelation.cpp)
  */
 #include <stdlib.h>
 #include <iostream>
@@ -61,6 +62,8 @@ using namespace std::chrono;
 #include "MDBase/MD.h"
 #include "MDBase/Langevin.h"
 #include "MDBase/Rotational/LangevinR.h"
+
+#include "MDBase/Analysis/PairCorrelation.cpp"
 
 #include "oracle.h"
 
@@ -589,6 +592,20 @@ static int cmd_long_rotor()
     return 0;
 }
 
+static int cmd_calcg1()
+{
+    /* calcg1 (Analysis/PairCorrelation.cpp:9-155): pair-distance histogram */
+    int N = (int)ji("N");
+    cube geo = make_cube();
+    matrix<double> x = to_mat(rd<double>("x", (size_t)N * 3), N, 3);
+    vector1<int> g = calcg1(x, geo, jd("max"), jd("dx"));
+    vector<long long> out(g.getsize());
+    for (int i = 0; i < g.getsize(); i++)
+        out[i] = g[i];
+    wr("counts", out.data(), out.size());
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc != 3)
@@ -612,6 +629,8 @@ int main(int argc, char **argv)
         return cmd_time_rotor();
     if (cmd == "long_rotor")
         return cmd_long_rotor();
+    if (cmd == "calcg1")
+        return cmd_calcg1();
     die("unknown command " + cmd);
     return 2;
 }

@@ -290,6 +290,26 @@ def fixture_bonded():
     np.savez_compressed(os.path.join(OUT, "bonded.npz"), **out)
 
 
+def fixture_calcg1():
+    """calcg1 histograms of the reference (Analysis/PairCorrelation.cpp) at
+    4, 2 and 1 cells per side"""
+    out = {}
+    cases = [("c4096", wl.patchy_tetra(4096), 4.0, 0.05),
+             ("c512", wl.patchy_tetra(512), 6.0, 0.1),
+             ("c512n1", wl.patchy_tetra(512), 13.0, 0.25)]
+    for name, s, mx, dx in cases:
+        d = job_dir(f"calcg1_{name}")
+        L = float(s["box"][0])
+        po.run_ref("calcg1", d, {"N": s["N"], "L": repr(L), "periodic": 1, "max": repr(mx),
+                                 "dx": repr(dx)}, {"x": s["x"]})
+        out[f"{name}_x"] = s["x"]
+        out[f"{name}_L"] = L
+        out[f"{name}_max"] = mx
+        out[f"{name}_dx"] = dx
+        out[f"{name}_counts"] = po.read_ref(d, "counts", dtype=np.int64)
+    np.savez_compressed(os.path.join(OUT, "calcg1.npz"), **out)
+
+
 def fixture_traj_rotor(N, nsteps=1000, snap=100, seed=77):
     s = wl.patchy_tetra(N)
     L = float(s["box"][0])
@@ -343,8 +363,14 @@ def main():
     fixture_traj_rotor(64)
     fixture_traj_rotor(512)
     fixture_traj_bead()
+    fixture_calcg1()
     print("fixtures written to", OUT)
 
 
 if __name__ == "__main__":
-    main()
+    # python tests/golden/make_golden.py [fixture ...]: only the named ones
+    if len(sys.argv) > 1:
+        for name in sys.argv[1:]:
+            globals()[f"fixture_{name}"]()
+    else:
+        main()

@@ -24,11 +24,12 @@ def gr(x, L):
     return 2.0 * h / (N * rho * shell)
 
 
-def bonds(x, Q, L, dirs, d=1.2, costm=0.95):
-    """bonds per particle"""
+def bonds(x, Q, L, dirs, d=1.2, costm=0.95, rmax=None):
+    """bonds per particle; rmax: also require r < rmax (the force loop's
+    max_distance_check, LangevinR.cpp:938)"""
     N = len(x)
     t = cKDTree(np.mod(x, L), boxsize=L)
-    pr = t.query_pairs(1.4 * d, output_type="ndarray")
+    pr = t.query_pairs(1.4 * d if rmax is None else min(1.4 * d, rmax), output_type="ndarray")
     i, j = pr[:, 0], pr[:, 1]
     dv = x[i] - x[j]
     dv -= L * np.round(dv / L)

@@ -411,3 +411,17 @@ def test_graph_replay_bitwise_equal_eager(monkeypatch, builder):
     keys = ("x", "p", "Q", "L") if s["integrator"] == "rotor" else ("x", "p")
     for k in keys:
         assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+# ---------------------------------------------------------------- analysis
+@pytest.mark.parametrize("case", ["c4096", "c512", "c512n1"])
+def test_pair_histogram_bit_exact(golden, case):
+    """mdx_pair_histogram against the reference's calcg1 counts"""
+    g = golden("calcg1.npz")
+    x = g[f"{case}_x"]
+    s = wl.patchy_tetra(len(x))
+    s["x"] = x
+    assert float(s["box"][0]) == float(g[f"{case}_L"])
+    with engine_for(s) as e:
+        c = e.pair_histogram(float(g[f"{case}_max"]), float(g[f"{case}_dx"]))
+    assert np.array_equal(c, g[f"{case}_counts"])

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import longrun_stats as ls
+import pyoracle as po
 from mtcode_amd import mdx
 from mtcode_amd import workloads as wl
 
@@ -38,6 +39,16 @@ def test_long_run_statistics_match_reference(golden, precision):
                 st = e.download()
                 frames.append((st["x"], st["p"], st["Q"], st["L"]))
         say("builds", e.info()["list_builds"])
+        # the engine's own observables on the last frame: patch bonds equal
+        # the cone-test count within max_distance_check, the pair histogram
+        # equals calcg1's
+        x, Q = frames[-1][0], frames[-1][2]
+        gb = e.patch_bonds()
+        hist = e.pair_histogram(4.0, 0.05)
+    nb = ls.bonds(x, Q, L, dirs, rmax=np.sqrt(s["patches"]["maxd2"]))
+    say("bonds", int(gb.sum()), int(nb.sum()), "mismatched particles", int((gb != nb).sum()))
+    assert np.array_equal(gb, nb)
+    assert np.array_equal(hist, po.calcg1(s["box"], x, 4.0, 0.05))
     ser = ls.series(frames, L, dirs)
     for k in ("bond_mean", "ke_t", "ke_r", "bond_hist"):
         a = ls.mean_err(ser[k])

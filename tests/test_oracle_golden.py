@@ -115,3 +115,12 @@ def test_bead_trajectory_1000_steps(golden):
                      int(g["n"]))
     assert np.max(np.abs(r["x"] - g["x_out"])) < 1e-10
     assert relmax(r["p"], g["p_out"]) < 1e-10
+
+
+@pytest.mark.parametrize("case", ["c4096", "c512", "c512n1"])
+def test_calcg1_histogram_bit_exact(golden, case):
+    """calcg1 (Analysis/PairCorrelation.cpp) at 6, 2 and 1 cells per side"""
+    g = golden("calcg1.npz")
+    L = float(g[f"{case}_L"])
+    c = po.calcg1(np.array([L] * 3), g[f"{case}_x"], float(g[f"{case}_max"]), float(g[f"{case}_dx"]))
+    assert np.array_equal(c, g[f"{case}_counts"])
