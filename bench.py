@@ -121,10 +121,12 @@ def force_error(s, reps=None):
     return out
 
 
-def time_engine(s, precision, steps, warmup, seed=1, reps=None):
+def time_engine(s, precision, steps, warmup, seed=1, reps=None, autotune=False):
     with mdx.engine_from_system(s, precision=precision, seed=seed, replicas=reps) as e:
         if s["integrator"] == "rotor":
             e.prime()
+        if autotune:
+            log("autotune", e.autotune(100))
         if warmup:
             e.step(warmup)
         t0 = time.perf_counter()
@@ -199,6 +201,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="let the engine's autotuner pick skin and walk form before timing")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the engine has no multi-GPU path in this build "
@@ -213,7 +217,8 @@ def main():
     rotor = s["integrator"] == "rotor"
     algo = ALGO_BYTES if rotor else BEAD_BYTES
     log("timing mixed precision", args.steps, "steps after", args.warmup, "warm-up")
-    t_mixed, prof, info = time_engine(s, mdx.PREC_MIXED, args.steps, args.warmup, reps=reps)
+    t_mixed, prof, info = time_engine(s, mdx.PREC_MIXED, args.steps, args.warmup, reps=reps,
+                                      autotune=args.autotune)
     log("mixed ms/step", 1e3 * t_mixed / args.steps, "kernels", prof)
     value = N * args.steps / t_mixed
     step_ms = sum(prof.get(k, 0.0) for k in STEP_PHASES + ("gather",) if k in prof
@@ -244,6 +249,7 @@ def main():
         "config": {
             "workload": f"{args.workload}: N={N}, " + WORKLOADS[args.workload],
             "precision": "mixed",
+            "neighbour_list_skin": info["skin"],
             "cells": info["cells"],
             "cell_size": round(info["cell_size"], 4),
         },

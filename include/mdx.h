@@ -127,6 +127,14 @@ int mdx_set_neighbor_list(mdx_engine *e, double skin, int kmax, int resort_every
 #define MDX_TERM_ACTIVE 32
 #define MDX_TERM_ALL 63
 int mdx_clear_interactions(mdx_engine *e, int terms);
+/* Autotuner (no reference counterpart): runs 2 x trial_steps steps with each
+ * candidate neighbour-list skin (0.25, 0.4, 0.6, 0.9) and walk form
+ * (compacted lanes, one lane per particle), timing the second half with HIP
+ * events, and keeps the fastest.  The steps are real steps (the state
+ * advances; rotors are primed again after each change).  report: 3 doubles
+ * per candidate (skin, walk form 1/0, ms per step), cap candidates;
+ * *nconfigs receives the number tried.  Rotors: after mdx_prime. */
+int mdx_autotune(mdx_engine *e, int trial_steps, int cap, double *report, int *nconfigs);
 /* |pair force| guard; 0 disables.  The reference prints "force too big" and
  * pauses above 1e4 (MD.cpp:1711-1749); here mdx_step returns MDX_ERR_FORCE. */
 int mdx_set_force_guard(mdx_engine *e, double fmax);

@@ -1905,6 +1905,80 @@ int mdx_stage_advance_mom(mdx_engine *e, const double *F, const double *R)
                                     : stage_advance_mom_t<float>(e, F, R);
 }
 
+// Autotuner: time trial steps of each candidate (skin x walk form) on this
+// engine's own system and keep the fastest.  The trial steps are real steps
+// (the state advances); rotors are primed again after each change.
+template <class R> static int autotune_t(mdx_engine *e, int trial, int cap, double *report, int *ncfg)
+{
+    static const double skins[] = {0.25, 0.4, 0.6, 0.9};
+    static const int walks[] = {1, 0};
+    hipEvent_t t0, t1;
+    HIPCHK(hipEventCreate(&t0));
+    HIPCHK(hipEventCreate(&t1));
+    int k = 0, best = -1, rc = MDX_OK;
+    double best_ms = 1e300;
+    const bool rotor = e->integ == MDX_INTEG_ROTOR;
+    for (double sk : skins)
+        for (int wv : walks) {
+            if (sk != e->skin)
+                e->grid_dirty = true;
+            e->skin = sk;
+            e->use_walkc = wv;
+            e->list_valid = false;
+            if (rotor && (rc = do_prime<R>(e, nullptr)))
+                break;
+            // one call to settle the list and the graphs, then the timed call
+            if ((rc = rotor ? enqueue_rotor<R>(e, trial, nullptr) : enqueue_bead<R>(e, trial, nullptr)))
+                break;
+            HIPCHK(hipEventRecord(t0, e->stream));
+            if ((rc = rotor ? enqueue_rotor<R>(e, trial, nullptr) : enqueue_bead<R>(e, trial, nullptr)))
+                break;
+            HIPCHK(hipEventRecord(t1, e->stream));
+            HIPCHK(hipEventSynchronize(t1));
+            float ms;
+            HIPCHK(hipEventElapsedTime(&ms, t0, t1));
+            const double per = ms / trial;
+            if (k < cap) {
+                report[3 * k] = sk;
+                report[3 * k + 1] = wv;
+                report[3 * k + 2] = per;
+            }
+            if (per < best_ms) {
+                best_ms = per;
+                best = k;
+            }
+            k++;
+        }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (rc)
+        return rc;
+    const double bsk = skins[best / 2];
+    if (bsk != e->skin)
+        e->grid_dirty = true;
+    e->skin = bsk;
+    e->use_walkc = walks[best % 2];
+    e->list_valid = false;
+    if (rotor && (rc = do_prime<R>(e, nullptr)))
+        return rc;
+    *ncfg = k;
+    unsigned int st;
+    if ((rc = read_status(e, &st)))
+        return rc;
+    return status_to_rc(st);
+}
+
+int mdx_autotune(mdx_engine *e, int trial_steps, int cap, double *report, int *nconfigs)
+{
+    if (!e || trial_steps < 1 || cap < 0 || (cap > 0 && !report) || !nconfigs)
+        return set_err(MDX_ERR_ARG, "bad arguments");
+    if (e->integ == MDX_INTEG_ROTOR && !e->primed)
+        return set_err(MDX_ERR_STATE, "mdx_autotune before mdx_prime");
+    HIPCHK(hipSetDevice(e->device));
+    return e->prec == MDX_PREC_FP64 ? autotune_t<double>(e, trial_steps, cap, report, nconfigs)
+                                    : autotune_t<float>(e, trial_steps, cap, report, nconfigs);
+}
+
 int mdx_set_neighbor_list(mdx_engine *e, double skin, int kmax, int resort_every)
 {
     if (!e || !(skin >= 0) || kmax < MDX_LBATCH || resort_every < 1)

@@ -78,6 +78,7 @@ def lib():
     L.mdx_pair_histogram.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double,
                                      C.POINTER(C.c_longlong), C.c_int, C.POINTER(C.c_int)]
     L.mdx_patch_bonds.argtypes = [C.c_void_p, _ip]
+    L.mdx_autotune.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.POINTER(C.c_int)]
     L.mdx_stage_advance_pos.argtypes = [C.c_void_p]
     L.mdx_stage_rotate.argtypes = [C.c_void_p]
     L.mdx_stage_sphere.argtypes = [C.c_void_p, _dp, _dp, _dp]
@@ -283,6 +284,14 @@ class Engine:
         return {"N": int(out[0]), "cells": int(out[1]), "cell_size": out[2],
                 "device_bytes": int(out[3]), "steps": int(out[4]), "list_builds": int(out[5]),
                 "skin": float(out[6]), "kmax": int(out[7])}
+
+    def autotune(self, trial_steps=100):
+        rep = np.zeros(3 * 16)
+        n = C.c_int(0)
+        _check(lib().mdx_autotune(self.h, int(trial_steps), 16, rep.ctypes.data_as(_dp),
+                                  C.byref(n)))
+        return [{"skin": rep[3 * k], "walkc": int(rep[3 * k + 1]), "ms_per_step": rep[3 * k + 2]}
+                for k in range(n.value)]
 
     # ------------------------------------------------------- analysis
     def pair_histogram(self, max_, dx, replica=0):

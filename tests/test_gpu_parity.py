@@ -425,3 +425,16 @@ def test_pair_histogram_bit_exact(golden, case):
     with engine_for(s) as e:
         c = e.pair_histogram(float(g[f"{case}_max"]), float(g[f"{case}_dx"]))
     assert np.array_equal(c, g[f"{case}_counts"])
+
+
+def test_autotune_tries_every_candidate_and_keeps_the_fastest():
+    s = wl.condensate_ab(1 << 16, phi=0.1)
+    with engine_for(s, mdx.PREC_MIXED, seed=2) as e:
+        e.prime()
+        rep = e.autotune(10)
+        say("autotune", rep)
+        assert len(rep) == 8 and all(r["ms_per_step"] > 0 for r in rep)
+        best = min(rep, key=lambda r: r["ms_per_step"])
+        assert e.info()["skin"] == best["skin"]
+        e.step(20)
+        assert np.isfinite(e.download()["x"]).all()
