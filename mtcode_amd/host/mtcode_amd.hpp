@@ -30,6 +30,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <fstream>
+#include <iomanip>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -88,6 +92,63 @@ template <class T> class matrix {
         return *this;
     }
 };
+
+// ------------------------------------------------------------ CSV files
+// the reference's matrix output (operator<<=, DataStructures/matrix2.cpp:898-912):
+// rows of comma-separated values in the stream's default format
+template <class T> std::ostream &write_csv(std::ostream &s, const matrix<T> &a)
+{
+    const int nr = a.getNsafe(), nc = a.getncols();
+    for (int i = 0; i < nr; ++i)
+        for (int j = 0; j < nc; ++j) {
+            if (j == nc - 1)
+                s << a(i, j) << std::endl;
+            else
+                s << a(i, j) << ",";
+        }
+    return s;
+}
+
+// outfunc (matrix2.cpp:1557-1565): writes <s>.csv
+template <class T> void outfunc(const matrix<T> &a, std::string s)
+{
+    s = s + ".csv";
+    std::ofstream f(s.c_str());
+    if (!f.is_open())
+        throw mdx_error(MDX_ERR_ARG, "failed to open file " + s);
+    write_csv(f, a);
+}
+
+// importcsv (matrix2.cpp:1596-1640): rows of comma-separated numbers (atof),
+// short rows padded with zeros to the longest; error set when the file is
+// missing
+template <class T> matrix<T> importcsv(std::string filename, T, bool &error)
+{
+    std::ifstream data(filename.c_str());
+    error = false;
+    if (!data.is_open()) {
+        error = true;
+        return matrix<T>();
+    }
+    std::string line;
+    std::vector<std::vector<T>> a;
+    while (std::getline(data, line)) {
+        std::stringstream ls(line);
+        std::string cell;
+        std::vector<T> b;
+        while (std::getline(ls, cell, ','))
+            b.push_back((T)std::atof(cell.c_str()));
+        a.push_back(b);
+    }
+    size_t nz = 0;
+    for (auto &r : a)
+        nz = std::max(nz, r.size());
+    matrix<T> q((int)a.size(), (int)nz);
+    for (size_t i = 0; i < a.size(); i++)
+        for (size_t j = 0; j < a[i].size(); j++)
+            q((int)i, (int)j) = a[i][j];
+    return q;
+}
 
 // ------------------------------------------------------------ geometry
 // cube (MDBase/geometry.h:43-66): edge l, periodic flag per dimension
@@ -679,6 +740,43 @@ class LangevinNVTR : public LangevinNVT {
         if (nsteps > 0)
             check(mdx_step(e_, nsteps, noise ? noise + 6 * (size_t)N() : nullptr));
         host_stale_ = true;
+    }
+
+    // Condensate::run(runtime, every, strbase) (Condensate.cpp:288-419) with
+    // its output: after step i with i % every == 0, pos<strbase>_i=<k>.csv and
+    // orientation<strbase>_i=<k>.csv (k = i / every, zero-padded to the digits
+    // of ceil(runtime / every)), in the reference's CSV format
+    void run(int runtime, int every, const std::string &strbase, const StepTerms &t)
+    {
+        if (every < 1)
+            throw mdx_error(MDX_ERR_ARG, "every must be >= 1");
+        int tf = (int)std::ceil((double)runtime / (double)every);
+        int digits = 0;
+        do {
+            ++digits;
+            tf /= 10;
+        } while (tf);
+        apply_terms(t);
+        check(mdx_prime(e_, nullptr));
+        int done = 0; // steps taken
+        for (int i = 0; i < runtime; i += every) {
+            // advance to the end of step i
+            check(mdx_step(e_, i + 1 - done, nullptr));
+            done = i + 1;
+            host_stale_ = true;
+            std::stringstream ss;
+            ss << std::setw(digits) << std::setfill('0') << (i / every);
+            std::ofstream fp(("pos" + strbase + "_i=" + ss.str() + ".csv").c_str());
+            std::ofstream fo(("orientation" + strbase + "_i=" + ss.str() + ".csv").c_str());
+            if (!fp.is_open() || !fo.is_open())
+                throw mdx_error(MDX_ERR_ARG, "cannot open output files");
+            write_csv(fp, getdat());
+            write_csv(fo, getorientation());
+        }
+        if (runtime > done) {
+            check(mdx_step(e_, runtime - done, nullptr));
+            host_stale_ = true;
+        }
     }
 };
 

@@ -24,6 +24,7 @@
  *   time_rotor x,Q -> wall-clock of the Condensate::run loop (OpenMP, rand())
  *   long_rotor x,Q -> snapshots of a long Condensate::run loop (rand() noise)
  *   calcg1    x    -> calcg1 pair-distance histogram (Analysis/PairCorrelation.cpp)
+ *   csv       x    -> the matrix written by the reference's CSV operator
  */
 #include <stdlib.h>
 #include <iostream>
@@ -606,6 +607,16 @@ static int cmd_calcg1()
     return 0;
 }
 
+static int cmd_csv()
+{
+    /* the reference's matrix CSV output (operator<<=, matrix2.cpp:898-912) */
+    int N = (int)ji("N"), M = (int)ji("M");
+    matrix<double> x = to_mat(rd<double>("x", (size_t)N * M), N, M);
+    ofstream f((g_dir + "/ref.csv").c_str());
+    f <<= x;
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc != 3)
@@ -631,6 +642,8 @@ int main(int argc, char **argv)
         return cmd_long_rotor();
     if (cmd == "calcg1")
         return cmd_calcg1();
+    if (cmd == "csv")
+        return cmd_csv();
     die("unknown command " + cmd);
     return 2;
 }

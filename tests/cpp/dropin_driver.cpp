@@ -16,10 +16,13 @@
 //   dir/x.bin (N x 3), Q.bin (N x 9, rotor), noise.bin, bonds.bin, angles.bin
 //   writes dir/out_x.bin, out_p.bin (+ out_Q.bin, out_L.bin for rotors)
 //   integ "stencil" with key n: writes the stencil table out_stencil.bin (no GPU)
+//   integ "csv" with N, M: outfunc / importcsv of x.bin (no GPU)
+//   rotor mode "output": Condensate::run(nsteps, every) with its CSV files
 #include "mtcode_amd.hpp"
 
 #include <cstdio>
 #include <fstream>
+#include <unistd.h>
 #include <map>
 #include <string>
 #include <vector>
@@ -84,6 +87,15 @@ static int rotor_main()
         t.wca = &wsa;
         t.patches = &pots;
         b.run(nsteps, t, nz.data());
+    } else if (g_job.at("mode") == "output") {
+        // Condensate::run(runtime, every, strbase) with its CSV output, from dir
+        StepTerms t;
+        t.wca = &wsa;
+        t.patches = &pots;
+        if (chdir(g_dir.c_str()) != 0)
+            throw mdx_error(MDX_ERR_ARG, "cannot enter " + g_dir);
+        b.run(nsteps, ji("every"), "_test", t);
+        g_dir = ".";
     } else {
         // Condensate::run, call by call
         int tb;
@@ -184,6 +196,18 @@ int main(int argc, char **argv)
     while (f >> k >> v)
         g_job[k] = v;
     try {
+        if (g_job.at("integ") == "csv") {
+            // outfunc of a matrix read from x.bin (N x M), and importcsv of it (no GPU)
+            const int N = ji("N"), M = ji("M");
+            matrix<double> x = to_mat(rd<double>("x.bin", (size_t)N * M), N, M);
+            outfunc(x, g_dir + "/out");
+            bool err = false;
+            matrix<double> y = importcsv(g_dir + "/out.csv", 0.0, err);
+            if (err || y.getNsafe() != N || y.getncols() != M)
+                throw mdx_error(MDX_ERR_ARG, "importcsv failed");
+            wr("out_import.bin", y.getdata(), (size_t)N * M);
+            return 0;
+        }
         if (g_job.at("integ") == "stencil") {
             // cube::generate_boxes_relationships only (no GPU)
             vector1<bool> pb(3, true);

@@ -310,6 +310,19 @@ def fixture_calcg1():
     np.savez_compressed(os.path.join(OUT, "calcg1.npz"), **out)
 
 
+def fixture_csv():
+    """the reference's CSV text of a matrix (operator<<=, matrix2.cpp:898-912)"""
+    rng = np.random.default_rng(17)
+    x = np.concatenate([rng.uniform(-300, 300, (40, 3)),
+                        np.array([[0.0, 1.0, -0.5], [1e-7, 123456789.0, 2.0 / 3.0],
+                                  [1e20, -1e-300, 5e-324]])])
+    d = job_dir("csv")
+    po.run_ref("csv", d, {"N": len(x), "M": 3}, {"x": x})
+    with open(os.path.join(d, "ref.csv")) as f:
+        text = f.read()
+    np.savez_compressed(os.path.join(OUT, "csv.npz"), x=x, text=np.array(text))
+
+
 def fixture_traj_rotor(N, nsteps=1000, snap=100, seed=77):
     s = wl.patchy_tetra(N)
     L = float(s["box"][0])
@@ -364,6 +377,7 @@ def main():
     fixture_traj_rotor(512)
     fixture_traj_bead()
     fixture_calcg1()
+    fixture_csv()
     print("fixtures written to", OUT)
 
 

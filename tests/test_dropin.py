@@ -59,6 +59,45 @@ def test_stencil_matches_reference_fixture(tmp_path, golden, n):
         assert boxes.sum() == g[f"boxes_n{n}_sum"]
 
 
+def test_csv_output_matches_reference_bytes(tmp_path, golden):
+    """outfunc writes the reference's CSV text byte for byte; importcsv reads
+    it back as the reference's reader does (atof)"""
+    g = golden("csv.npz")
+    x = g["x"]
+    run_driver(str(tmp_path), {"integ": "csv", "N": x.shape[0], "M": x.shape[1]}, {"x": x})
+    with open(os.path.join(str(tmp_path), "out.csv")) as f:
+        assert f.read() == str(g["text"])
+    back = np.fromfile(os.path.join(str(tmp_path), "out_import.bin")).reshape(x.shape)
+    ref = np.array([[float(v) for v in line.split(",")] for line in str(g["text"]).splitlines()])
+    assert np.array_equal(back, ref)
+
+
+@pytest.mark.gpu
+def test_condensate_run_writes_reference_files(tmp_path):
+    """Condensate::run(runtime, every): the output files' names (steps 0, 10,
+    20 of 25; index zero-padded to the digits of ceil(25 / 10)) and shapes,
+    positions inside the box and orthonormal orientations"""
+    s = wl.patchy_tetra(512)
+    N, nsteps, every = s["N"], 25, 10
+    e, d, th = s["patches"]["params"][0]
+    job = {"integ": "rotor", "mode": "output", "N": N, "L": repr(float(s["box"][0])),
+           "nsteps": nsteps, "every": every, "dt": s["dt"], "gamma": s["gamma"],
+           "gammar": repr(float(s["gammar"])), "kT": s["kT"], "precision": 0,
+           "eps_p": repr(float(e)), "d": repr(float(d)), "thetam": repr(float(th))}
+    nz = po.noise(1, 0, (nsteps + 1) * N * 6)
+    run_driver(str(tmp_path), job, {"x": s["x"], "Q": s["Q"], "noise": nz})
+    names = sorted(f for f in os.listdir(str(tmp_path)) if f.endswith(".csv"))
+    assert names == ["orientation_test_i=0.csv", "orientation_test_i=1.csv",
+                     "orientation_test_i=2.csv", "pos_test_i=0.csv", "pos_test_i=1.csv",
+                     "pos_test_i=2.csv"]
+    pos = np.loadtxt(os.path.join(str(tmp_path), "pos_test_i=2.csv"), delimiter=",")
+    ori = np.loadtxt(os.path.join(str(tmp_path), "orientation_test_i=2.csv"), delimiter=",")
+    assert pos.shape == (N, 3) and ori.shape == (N, 9)
+    assert np.all((pos >= 0) & (pos <= s["box"][0]))
+    assert np.allclose(np.einsum("nij,nkj->nik", ori.reshape(N, 3, 3), ori.reshape(N, 3, 3)),
+                       np.eye(3), atol=1e-4)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["stages", "run"])
 def test_rotor_condensate_loop_vs_oracle(tmp_path, mode):
