@@ -129,10 +129,18 @@ struct mdx_engine {
     int use_compact = 1;     // compacted patch kernel + integrate kernel (MDX_COMPACT=0: one finish kernel)
     int use_walkc = 1;       // compacted list walk (MDX_WALKC=0: one lane per position)
     int use_graph = 1;       // list steps replayed from captured graphs (MDX_GRAPH=0: eager)
+    // list steps integrate in the walk / patch kernels (MDX_FUSE=1), in a
+    // kernel of their own (0), or the former when at most a quarter of the
+    // particles carry patches (2, the default): integrating in the walk costs
+    // the walk occupancy, which pays only where few particles go on to the
+    // patch kernel
+    int use_fuse = 2;
+    int n_patchy = 0; // particles whose type carries patches
     bool list_eager = false; // an eager list step ran since the last list build
     GraphSlot graphs[2];     // per current state buffer
     Buf nbr, nn, bcell;  // neighbour list
-    Buf bmax;            // per-block drift maxima
+    Buf bmax, bmax2;     // per-block drift maxima (walk / patch kernel grids)
+    int drift_nb2 = 0;   // patch-kernel blocks that wrote bmax2 in the last gather
     hipEvent_t prof_mid = nullptr;  // recorded between walk and finish while profiling
     hipEvent_t prof_mid2 = nullptr; // recorded between patch and integrate while profiling
     bool last_split = false;       // the last gather launch used the split form
@@ -370,6 +378,7 @@ template <class R> static KArgs<R> make_args(mdx_engine *e)
     a.rl2 = rl * rl;
     a.skin = e->skin;
     a.bmax = (float *)e->bmax.p;
+    a.bmax2 = (float *)e->bmax2.p;
     return a;
 }
 
@@ -444,8 +453,36 @@ static int launch_gather(mdx_engine *e, const double *noise_dev, int lmode = 0)
         a.ncand = (int *)e->ncand.p;
         a.cand = (int *)e->cand.p;
         a.Tp = (R *)e->Tp.p;
+        // positions per patch block: as many as keep about 512 blocks
+        int pblk = 256;
+        while (pblk < PBLK && (long long)e->N / (2 * pblk) >= 512)
+            pblk *= 2;
+        // fused form (list steps): the walk integrates the particles without
+        // patch candidates, the patch kernel the others
+        const bool fuse_ok = e->use_fuse == 1 ||
+                             (e->use_fuse == 2 && (!a.has_patches || 4LL * e->n_patchy <= e->N));
+        const bool fuse = lmode && e->use_walkc && fuse_ok && e->use_compact &&
+                          (MODE == M_FUSED || MODE == M_STEP || MODE == M_BEAD);
+        e->drift_nb2 = 0;
+        if (fuse) {
+            hipLaunchKernelGGL((k_walkc<R, MODE, true>), dim3(nblk(e->N)), dim3(256), 0, e->stream,
+                               a);
+            if (e->prof_mid)
+                HIPCHK(hipEventRecord(e->prof_mid, e->stream));
+            e->last_split = true;
+            if (!bead_list) {
+                hipLaunchKernelGGL((k_patch<R, MODE, true>), dim3(nblk(e->N, pblk)), dim3(256), 0,
+                                   e->stream, a, pblk);
+                e->drift_nb2 = (int)nblk(e->N, pblk);
+            }
+            if (e->prof_mid2)
+                HIPCHK(hipEventRecord(e->prof_mid2, e->stream));
+            HIPCHK(hipGetLastError());
+            return MDX_OK;
+        }
         if (lmode && e->use_walkc)
-            hipLaunchKernelGGL((k_walkc<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
+            hipLaunchKernelGGL((k_walkc<R, MODE, false>), dim3(nblk(e->N)), dim3(256), 0, e->stream,
+                               a);
         else
             hipLaunchKernelGGL((k_walk<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
         if (e->prof_mid)
@@ -457,12 +494,8 @@ static int launch_gather(mdx_engine *e, const double *noise_dev, int lmode = 0)
             hipLaunchKernelGGL((k_integrate<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream,
                                a);
         } else if (e->use_compact) {
-            // positions per block: as many as keep about 512 blocks
-            int pblk = 256;
-            while (pblk < PBLK && (long long)e->N / (2 * pblk) >= 512)
-                pblk *= 2;
-            hipLaunchKernelGGL((k_patch<R, MODE>), dim3(nblk(e->N, pblk)), dim3(256), 0, e->stream,
-                               a, pblk);
+            hipLaunchKernelGGL((k_patch<R, MODE, false>), dim3(nblk(e->N, pblk)), dim3(256), 0,
+                               e->stream, a, pblk);
             if (e->prof_mid2)
                 HIPCHK(hipEventRecord(e->prof_mid2, e->stream));
             hipLaunchKernelGGL((k_integrate<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream,
@@ -554,7 +587,8 @@ static int nbr_alloc(mdx_engine *e)
     int rc;
     if ((rc = dalloc(e, e->nbr, 4 * (size_t)e->kmax * e->N)) ||
         (rc = dalloc(e, e->nn, 4 * (size_t)e->N)) || (rc = dalloc(e, e->bcell, 4 * (size_t)e->N)) ||
-        (rc = dalloc(e, e->bmax, 4 * (size_t)nblk(e->N))))
+        (rc = dalloc(e, e->bmax, 4 * (size_t)nblk(e->N))) ||
+        (rc = dalloc(e, e->bmax2, 4 * (size_t)nblk(e->N))))
         return rc;
     return MDX_OK;
 }
@@ -564,7 +598,8 @@ static int nbr_alloc(mdx_engine *e)
 static int decide(mdx_engine *e, int epoch_inc)
 {
     hipLaunchKernelGGL(k_decide, dim3(1), dim3(256), 0, e->stream, (StepCtl *)e->ctl.p,
-                       (const float *)e->bmax.p, (int)nblk(e->N), e->skin, epoch_inc);
+                       (const float *)e->bmax.p, (int)nblk(e->N), (const float *)e->bmax2.p,
+                       e->drift_nb2, e->skin, epoch_inc);
     HIPCHK(hipGetLastError());
     return MDX_OK;
 }
@@ -756,6 +791,7 @@ template <class R> static int enqueue_rotor(mdx_engine *e, int nsteps, const dou
             HIPCHK(hipEventRecord(kev[0], e->stream));
         hipLaunchKernelGGL(k_kick_drift<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
         HIPCHK(hipGetLastError());
+        e->drift_nb2 = 0; // only the kick-drift grid wrote drift maxima
         if (a.lmode && (rc = decide(e, 0)))
             return rc;
         if (kev)
@@ -926,6 +962,8 @@ int mdx_create(const mdx_desc *d, mdx_engine **out)
         e->use_tiles = std::atoi(t) != 0;
     if (const char *t = std::getenv("MDX_SPLIT"))
         e->use_split = std::atoi(t) != 0;
+    if (const char *t = std::getenv("MDX_FUSE"))
+        e->use_fuse = std::atoi(t);
     if (const char *t = std::getenv("MDX_GRAPH"))
         e->use_graph = std::atoi(t) != 0;
     if (const char *t = std::getenv("MDX_WALKC"))
@@ -994,7 +1032,7 @@ void mdx_destroy(mdx_engine *e)
                    &e->act_next, &e->noise,  &e->ctl,    &e->partial, &e->tmpd,   &e->tmpi,
                    &e->tmpi2,   &e->tmpi3,   &e->tmpll,  &e->tmpx,    &e->Fw,
                    &e->ncand,   &e->cand,    &e->nbr,    &e->nn,     &e->bcell,
-                   &e->bmax,    &e->Tp};
+                   &e->bmax,    &e->Tp,      &e->bmax2};
     for (Buf *b : bufs)
         dfree(*b);
     for (GraphSlot &g : e->graphs)
@@ -1232,6 +1270,9 @@ int mdx_set_patches(mdx_engine *e, int ntypes, const int *type_of_particle,
     for (int r = 0; r < e->nrep; r++)
         e->prow_raw.insert(e->prow_raw.end(), params, params + 3 * hj);
     e->maxd2 = maxd2;
+    e->n_patchy = 0;
+    for (int i = 0; i < e->N; i++)
+        e->n_patchy += npatch_per_type[type_of_particle[i]] > 0;
     for (int r = 0; r < e->nrep; r++)
         for (int k = 0; k < e->nper[r]; k++) {
             const int i = e->first[r] + k;

@@ -81,7 +81,8 @@ template <class R> struct KArgs {
     int *nbrw, *nn, *bcell;
     int kmax;
     double rl2, skin;
-    float *bmax; // per block: max drift of the block's particles
+    float *bmax;  // per block: max drift of the block's particles
+    float *bmax2; // the same for the blocks of the patch kernel (fused form)
     // WCA (HSPotential)
     int wca;
     double wca_eps, wca_s6, wca_s12, wca_pl, wca_pl2, wca_pl2lo;
@@ -284,7 +285,7 @@ __device__ __forceinline__ void min_image(const KArgs<R> &a, const DevRep &rp, d
 #endif
 constexpr int PMAX = 8;   // deferred patch candidates per particle held in LDS
 constexpr int PMAXG = 16; // patch candidates per particle handed from walk to finish
-enum { PH_ALL = 0, PH_WALK = 1, PH_FINISH = 2, PH_INTEGRATE = 3 };
+enum { PH_ALL = 0, PH_WALK = 1, PH_FINISH = 2, PH_INTEGRATE = 3, PH_INTEGRATE_F = 4 };
 
 // Workgroups go to the 8 XCDs round-robin; this gives each XCD one contiguous
 // range of blocks (of cell-ordered slots), so the neighbours a block gathers
@@ -666,9 +667,11 @@ __device__ __forceinline__ bool own_screen(const KArgs<R> &a, uint32_t oi, int t
 // only (WCA force and patch candidates to global memory); PH_FINISH: the
 // patches from those candidates, topology and the integrator.  The split
 // keeps the register-heavy patch code out of the neighbour walk.
+// PH_INTEGRATE_F: as PH_INTEGRATE with the force given in fin and no patch
+// torque (a particle without patch candidates, integrated by the walk)
 template <class R, int MODE, int PH, class Trav>
 __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *plist, Trav &&trav,
-                                                 float &drift)
+                                                 float &drift, const R *fin = nullptr)
 {
     constexpr bool FP64 = sizeof(R) == 8;
     const int i = a.lmode ? s : a.list[s];
@@ -685,7 +688,8 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
 
     R qi[9];
     // the bead integrator carries no orientations: no patch code in its kernels
-    const int npi = (MODE != M_BEAD && PH != PH_INTEGRATE && a.has_patches) ? a.np[ti] : 0;
+    constexpr bool INTG = PH == PH_INTEGRATE || PH == PH_INTEGRATE_F;
+    const int npi = (MODE != M_BEAD && !INTG && a.has_patches) ? a.np[ti] : 0;
     if (MODE != M_BEAD && a.rotor) {
 #pragma unroll
         for (int k = 0; k < 9; k++)
@@ -745,7 +749,13 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
             acc.T[k] = pt ? a.Tp[3 * (size_t)s + k] : (R)0;
         }
     }
-    if (PH != PH_FINISH && PH != PH_INTEGRATE && (a.wca || npi > 0)) {
+    if (PH == PH_INTEGRATE_F) {
+        for (int k = 0; k < 3; k++) {
+            acc.F[k] = fin[k];
+            acc.T[k] = (R)0;
+        }
+    }
+    if (PH != PH_FINISH && !INTG && (a.wca || npi > 0)) {
         int npc = 0;
         trav(i, rp, [&](const PosRec &xj, int j) {
             double d[3];
@@ -1066,7 +1076,9 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_walk(KArgs
 // patch candidates in list order -- the same sums in the same order as the
 // one-lane walk.  Positions with more than kmax entries walk the build's
 // cells in their own lane.
-template <class R, int MODE> __global__ void __launch_bounds__(256) k_walkc(KArgs<R> a)
+// INT: positions without patch candidates are integrated here (their force
+// is complete), the others are left to the patch kernel
+template <class R, int MODE, bool INT> __global__ void __launch_bounds__(256) k_walkc(KArgs<R> a)
 {
     __shared__ int off[257];
     __shared__ PosRec sx[256];
@@ -1177,25 +1189,57 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_walkc(KArg
         }
         __syncthreads();
     }
-    if (!live)
-        return;
-    if (fb) {
-        const DevRep rp = rep_rec(a, xi.meta);
-        Acc<R> pa;
-        pa.F[0] = pa.F[1] = pa.F[2] = 0;
-        for_each_in_cells(a, rp, a.bcell[s], s, [&](const PosRec &xj, int j) {
-            if (visit(xi, sq + 9 * tid, sexc[2 * tid], sexc[2 * tid + 1], xj, pa)) {
-                if (npc < PMAXG)
-                    a.cand[(size_t)npc * a.N + s] = j;
-                npc++;
-            }
-        });
-        for (int k = 0; k < 3; k++)
-            F[k] = pa.F[k];
+    float drift = 0.f;
+    if (live) {
+        if (fb) {
+            const DevRep rp = rep_rec(a, xi.meta);
+            Acc<R> pa;
+            pa.F[0] = pa.F[1] = pa.F[2] = 0;
+            for_each_in_cells(a, rp, a.bcell[s], s, [&](const PosRec &xj, int j) {
+                if (visit(xi, sq + 9 * tid, sexc[2 * tid], sexc[2 * tid + 1], xj, pa)) {
+                    if (npc < PMAXG)
+                        a.cand[(size_t)npc * a.N + s] = j;
+                    npc++;
+                }
+            });
+            for (int k = 0; k < 3; k++)
+                F[k] = pa.F[k];
+        }
+        a.ncand[s] = npc;
+        if (!(INT && npc == 0))
+            for (int k = 0; k < 3; k++)
+                a.Fw[3 * (size_t)s + k] = F[k];
     }
-    for (int k = 0; k < 3; k++)
-        a.Fw[3 * (size_t)s + k] = F[k];
-    a.ncand[s] = npc;
+    if (INT) {
+        // the positions without candidates, compacted to the first lanes
+        // (slot in pj, force in pf), are integrated one per lane
+        const bool here = live && npc == 0;
+        const unsigned long long m = __ballot(here);
+        const int lane = tid & 63;
+        const int rank = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0)
+            wsum[tid >> 6] = __popcll(m);
+        __syncthreads();
+        int at = rank;
+        for (int w = 0; w < (tid >> 6); w++)
+            at += wsum[w];
+        const int nint = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        if (here) {
+            pj[at] = s;
+            for (int k = 0; k < 3; k++)
+                pf[3 * at + k] = F[k];
+        }
+        __syncthreads();
+        if (tid < nint) {
+            const R fin[3] = {pf[3 * tid], pf[3 * tid + 1], pf[3 * tid + 2]};
+            process_particle<R, MODE, PH_INTEGRATE_F>(
+                a, pj[tid], nullptr,
+                [&](int i, const DevRep &rp, auto &&fn) { for_each_near(a, rp, i, fn); }, drift,
+                fin);
+        }
+        if (MODE == M_FUSED || MODE == M_BEAD)
+            drift_bound(a, drift);
+    }
 }
 
 // ... then the patch pairs, topology and the integrator
@@ -1222,7 +1266,10 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_finish(KAr
 // these patch sets).  Positions without candidates are not touched (the
 // integrator reads ncand).
 constexpr int PBLK = 2048; // largest block of positions (pblk: 256 .. PBLK, a multiple of 256)
-template <class R, int MODE> __global__ void __launch_bounds__(256) k_patch(KArgs<R> a, int pblk)
+// INT: the positions with candidates are then integrated here (the walk
+// integrated the others); block drift maxima go to bmax2
+template <class R, int MODE, bool INT>
+__global__ void __launch_bounds__(256) k_patch(KArgs<R> a, int pblk)
 {
     const int PER = pblk / 256;
     __shared__ int off[PBLK + 1];
@@ -1357,6 +1404,43 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_patch(KArg
             a.Tp[3 * (size_t)so + k2] = acc.T[k2];
         }
     }
+    if (INT) {
+        // the positions with candidates, compacted into off[], integrated one
+        // per lane
+        __threadfence_block();
+        __syncthreads();
+        int cnt = 0;
+        for (int k = 0; k < PER; k++) {
+            const int so = base + k * 256 + tid;
+            const bool here = so < a.N && a.ncand[so] > 0;
+            const unsigned long long m = __ballot(here);
+            const int lane = tid & 63;
+            const int rank = __popcll(m & ((1ull << lane) - 1ull));
+            if (lane == 0)
+                wsum[tid >> 6] = __popcll(m);
+            __syncthreads();
+            int wb = 0;
+            for (int w = 0; w < (tid >> 6); w++)
+                wb += wsum[w];
+            if (here)
+                off[cnt + wb + rank] = so;
+            cnt += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+            __syncthreads();
+        }
+        float drift = 0.f;
+        for (int t = tid; t < cnt; t += 256) {
+            float d = 0.f;
+            process_particle<R, MODE, PH_INTEGRATE>(
+                a, off[t], nullptr,
+                [&](int i, const DevRep &rp, auto &&fn) { for_each_near(a, rp, i, fn); }, d);
+            drift = fmaxf(drift, d);
+        }
+        if (MODE == M_FUSED || MODE == M_BEAD) {
+            KArgs<R> b = a;
+            b.bmax = a.bmax2;
+            drift_bound(b, drift);
+        }
+    }
 }
 
 // ... and the topology terms and the integrator
@@ -1424,6 +1508,7 @@ template <class R> __global__ void __launch_bounds__(256) k_nbuild(KArgs<R> a, c
     });
     a.nn[s] = n | (kept_excluded ? LIST_EXC : 0);
 }
+
 
 // ------------------------------------------------------- LDS-staged tiles
 // A block owns 256 consecutive cell-list positions.  When they lie in one
@@ -1754,13 +1839,16 @@ __global__ void k_list_reset(StepCtl *ctl)
 // the grid max of the block drifts of the kernel before is added to the
 // displacement bound D; a list rebuild is requested once 2 D reaches the skin
 // (a pair within reach now was within reach + skin at the build)
-__global__ void __launch_bounds__(256) k_decide(StepCtl *ctl, const float *bmax, int nb, double skin,
+__global__ void __launch_bounds__(256) k_decide(StepCtl *ctl, const float *bmax, int nb,
+                                                const float *bmax2, int nb2, double skin,
                                                 int epoch_inc)
 {
     __shared__ float wmax[4];
     float m = 0.f;
     for (int b = threadIdx.x; b < nb; b += 256)
         m = fmaxf(m, bmax[b]);
+    for (int b = threadIdx.x; b < nb2; b += 256)
+        m = fmaxf(m, bmax2[b]);
     for (int o = 32; o > 0; o >>= 1)
         m = fmaxf(m, __shfl_xor(m, o));
     if ((threadIdx.x & 63) == 0)

@@ -413,6 +413,37 @@ def test_graph_replay_bitwise_equal_eager(monkeypatch, builder):
         assert np.array_equal(outs[0][k], outs[1][k]), k
 
 
+@pytest.mark.parametrize("precision", [mdx.PREC_MIXED, mdx.PREC_FP64])
+@pytest.mark.parametrize("builder", [
+    lambda: wl.condensate_ab(1 << 16, phi=0.1),
+    lambda: wl.mt_bundle(2, 13, 60, spacing=6.0, box=(12.0, 12.0, 60.0)),
+])
+def test_fused_integrator_bitwise_equal_separate(monkeypatch, builder, precision):
+    """List steps that integrate inside the walk (particles without patch
+    candidates) and the patch kernel (the others) give the trajectory of the
+    separate integrator kernel bit for bit, with the same list rebuilds (the
+    drift bound is reduced over both kernels' blocks)."""
+    s = builder()
+    say("fused vs separate", s["name"], s["N"], precision)
+    outs = []
+    for f in ("1", "0"):
+        monkeypatch.setenv("MDX_FUSE", f)
+        with engine_for(s, precision, seed=5) as e:
+            e.set_neighbor_list(0.3, 64, 40)
+            if s["integrator"] == "rotor":
+                e.prime()
+            for _ in range(4):
+                e.step(60)
+            out = e.download()
+            out["builds"] = e.info()["list_builds"]
+            outs.append(out)
+    say("builds", outs[0]["builds"], outs[1]["builds"])
+    assert outs[0]["builds"] == outs[1]["builds"] and outs[0]["builds"] > 4
+    keys = ("x", "p", "Q", "L") if s["integrator"] == "rotor" else ("x", "p")
+    for k in keys:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
 # ---------------------------------------------------------------- analysis
 @pytest.mark.parametrize("case", ["c4096", "c512", "c512n1"])
 def test_pair_histogram_bit_exact(golden, case):
