@@ -167,6 +167,16 @@ int mdx_pair_histogram(mdx_engine *e, int replica, double max, double dx, long l
  * the force loop's own FP64 expressions.  No reference counterpart (the
  * reference's analysis counts bonds from its binding store). */
 int mdx_patch_bonds(mdx_engine *e, int *bonds);
+/* distance_graph (Analysis/AnalyzeGrowth.cpp:136-177) for replica r: the
+ * pairs of MD::calculatepairs_parallel with n[3] cells per side within cut
+ * (cube::distance_less_than), kept when both local indices lie outside
+ * [ex_lo, ex_hi] (the excludea predicate, AnalyzeGrowth.cpp:123-134;
+ * ex_lo > ex_hi keeps every pair, index_test), and labels[i] (N_r ints) = the
+ * label ConnectedComponentsParallel_Old (Bindings/GraphAlgorithms.cpp:672-810)
+ * converges to: the smallest local index of i's connected component.
+ * *nclusters (nullable) = number of components.  Bit-exact. */
+int mdx_clusters(mdx_engine *e, int replica, const int n[3], double cut, int ex_lo, int ex_hi,
+                 int *labels, int *nclusters);
 
 /* Time stepping. */
 /* Rotor integrator: computes the carried F, T at the current state

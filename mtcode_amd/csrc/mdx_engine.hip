@@ -1622,14 +1622,11 @@ int mdx_patch_bonds(mdx_engine *e, int *bonds)
                                     : patch_bonds_t<float>(e, bonds);
 }
 
-int mdx_neighbor_pairs(mdx_engine *e, int replica, const int n[3], double cut, int *ij,
-                       long long cap, long long *npairs)
+// the pairs of MD::calculatepairs as a set (k_pairs: count pass, offsets,
+// write pass) into the device buffer out, at most cap of them; *npairs = all
+static int pair_list(mdx_engine *e, int replica, const int n[3], double cut, Buf &out,
+                     long long cap, long long *npairs)
 {
-    if (!e || replica < 0 || replica >= e->nrep || !n || !npairs || cap < 0 || (cap > 0 && !ij))
-        return set_err(MDX_ERR_ARG, "bad arguments");
-    if (n[0] < 1 || n[1] < 1 || n[2] < 1 || !(cut > 0))
-        return set_err(MDX_ERR_ARG, "n >= 1 and cut > 0 required");
-    HIPCHK(hipSetDevice(e->device));
     const int cnt = e->nper[replica];
     const double *b = &e->box[3 * replica];
     int rc;
@@ -1649,29 +1646,108 @@ int mdx_neighbor_pairs(mdx_engine *e, int replica, const int n[3], double cut, i
     for (int k = 0; k < cnt; k++)
         off[k + 1] = off[k] + pc[k];
     *npairs = off[cnt];
-    if (cap > 0 && off[cnt] > 0) {
-        const long long w = std::min(cap, off[cnt]);
-        if ((rc = upload(e, e->tmpll, off.data(), off.size())))
-            return rc;
-        Buf out;
-        if ((rc = dalloc(e, out, 8 * (size_t)w)))
+    const long long w = std::min(cap, off[cnt]);
+    if (w > 0) {
+        if ((rc = upload(e, e->tmpll, off.data(), off.size())) ||
+            (rc = dalloc(e, out, 8 * (size_t)w)))
             return rc;
         hipLaunchKernelGGL(k_pairs, dim3(nblk(cnt)), dim3(256), 0, e->stream, X, cnt, b[0],
                            b[1], b[2], e->periodic, n[0], n[1], n[2], cut, (const int *)e->tmpi.p,
                            (const int *)e->tmpi3.p, (const int *)e->tmpi2.p, (int *)nullptr,
                            (const long long *)e->tmpll.p, (int *)out.p, w, 1);
-        hipError_t le = hipGetLastError();
-        if (le == hipSuccess)
-            le = hipMemcpyAsync(ij, out.p, 8 * (size_t)w, hipMemcpyDeviceToHost, e->stream);
+        HIPCHK(hipGetLastError());
+    }
+    return MDX_OK;
+}
+
+static void release(mdx_engine *e, Buf &b)
+{
+    e->bytes_held -= b.bytes;
+    dfree(b);
+}
+
+int mdx_neighbor_pairs(mdx_engine *e, int replica, const int n[3], double cut, int *ij,
+                       long long cap, long long *npairs)
+{
+    if (!e || replica < 0 || replica >= e->nrep || !n || !npairs || cap < 0 || (cap > 0 && !ij))
+        return set_err(MDX_ERR_ARG, "bad arguments");
+    if (n[0] < 1 || n[1] < 1 || n[2] < 1 || !(cut > 0))
+        return set_err(MDX_ERR_ARG, "n >= 1 and cut > 0 required");
+    HIPCHK(hipSetDevice(e->device));
+    Buf out;
+    int rc = pair_list(e, replica, n, cut, out, cap, npairs);
+    const long long w = std::min(cap, *npairs);
+    if (rc == MDX_OK && w > 0) {
+        hipError_t le = hipMemcpyAsync(ij, out.p, 8 * (size_t)w, hipMemcpyDeviceToHost, e->stream);
         if (le == hipSuccess)
             le = hipStreamSynchronize(e->stream);
-        e->bytes_held -= out.bytes;
-        dfree(out);
         if (le != hipSuccess)
-            return set_err(MDX_ERR_HIP, "pair listing failed: %s", hipGetErrorString(le));
+            rc = set_err(MDX_ERR_HIP, "pair listing failed: %s", hipGetErrorString(le));
     }
-    if (off[cnt] > cap)
-        return set_err(MDX_ERR_CAPACITY, "%lld pairs, capacity %lld", off[cnt], cap);
+    release(e, out);
+    if (rc)
+        return rc;
+    if (*npairs > cap)
+        return set_err(MDX_ERR_CAPACITY, "%lld pairs, capacity %lld", *npairs, cap);
+    return MDX_OK;
+}
+
+int mdx_clusters(mdx_engine *e, int replica, const int n[3], double cut, int ex_lo, int ex_hi,
+                 int *labels, int *nclusters)
+{
+    if (!e || replica < 0 || replica >= e->nrep || !n || !labels)
+        return set_err(MDX_ERR_ARG, "bad arguments");
+    if (n[0] < 1 || n[1] < 1 || n[2] < 1 || !(cut > 0))
+        return set_err(MDX_ERR_ARG, "n >= 1 and cut > 0 required");
+    HIPCHK(hipSetDevice(e->device));
+    const int cnt = e->nper[replica];
+    Buf pairs, lab, flag;
+    long long np = 0;
+    int rc = pair_list(e, replica, n, cut, pairs, (long long)1 << 40, &np);
+    if (!rc && !(rc = dalloc(e, lab, 4 * (size_t)cnt)) && !(rc = dalloc(e, flag, 4))) {
+        hipLaunchKernelGGL(k_cc_init, dim3(nblk(cnt)), dim3(256), 0, e->stream, (int *)lab.p, cnt);
+        // every round changes at least one label and labels only decrease:
+        // the rounds end; the cap only guards against a broken invariant
+        const int max_rounds = 100000;
+        int round = 0;
+        for (; round < max_rounds; round++) {
+            hipError_t le = hipMemsetAsync(flag.p, 0, 4, e->stream);
+            if (le != hipSuccess) {
+                rc = set_err(MDX_ERR_HIP, "%s", hipGetErrorString(le));
+                break;
+            }
+            if (np > 0)
+                hipLaunchKernelGGL(k_cc_hook, dim3(nblk(np)), dim3(256), 0, e->stream,
+                                   (const int *)pairs.p, np, ex_lo, ex_hi, (int *)lab.p,
+                                   (int *)flag.p);
+            hipLaunchKernelGGL(k_cc_jump, dim3(nblk(cnt)), dim3(256), 0, e->stream,
+                               (int *)lab.p, cnt, (int *)flag.p);
+            int changed = 0;
+            if ((le = hipGetLastError()) != hipSuccess ||
+                (le = hipMemcpyAsync(&changed, flag.p, 4, hipMemcpyDeviceToHost, e->stream)) !=
+                    hipSuccess ||
+                (le = hipStreamSynchronize(e->stream)) != hipSuccess) {
+                rc = set_err(MDX_ERR_HIP, "connected components: %s", hipGetErrorString(le));
+                break;
+            }
+            if (!changed)
+                break;
+        }
+        if (!rc && round == max_rounds)
+            rc = set_err(MDX_ERR_STATE, "connected components did not settle");
+        if (!rc)
+            rc = download(e, lab, labels, (size_t)cnt);
+    }
+    release(e, pairs);
+    release(e, lab);
+    release(e, flag);
+    if (rc)
+        return rc;
+    int nc = 0;
+    for (int i = 0; i < cnt; i++)
+        nc += labels[i] == i;
+    if (nclusters)
+        *nclusters = nc;
     return MDX_OK;
 }
 

@@ -1979,6 +1979,60 @@ __global__ void k_pairs(const double *X, int count, double l0, double l1, double
     }
 }
 
+// distance_graph (Analysis/AnalyzeGrowth.cpp:136-177): connected components
+// of the pair graph, each particle labelled with the smallest index of its
+// component -- the fixed point of the reference's ConnectedComponentsParallel_Old
+// (Bindings/GraphAlgorithms.cpp:672-810), which starts from ascending labels
+// and only moves a root to a smaller adjacent label.  Here a label always
+// names a particle of the same component and never increases, so the order of
+// the concurrent updates does not change the fixed point; the host repeats
+// hook + jump until neither changes a label.
+__global__ void k_cc_init(int *lab, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        lab[i] = i;
+}
+
+// each kept pair (both indices outside [lo, hi] when lo <= hi): the larger
+// of the two labels is moved to the smaller
+__global__ void k_cc_hook(const int *ij, long long np, int lo, int hi, int *lab, int *changed)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= np)
+        return;
+    const int u = ij[2 * k], v = ij[2 * k + 1];
+    if (lo <= hi && !((u < lo || u > hi) && (v < lo || v > hi)))
+        return;
+    const int lu = lab[u], lv = lab[v];
+    if (lu == lv)
+        return;
+    const int m = min(lu, lv), M = max(lu, lv);
+    if (atomicMin(&lab[M], m) > m)
+        atomicOr(changed, 1);
+}
+
+// each label to the root of its chain (labels decrease along a chain, so the
+// walk ends within n reads)
+__global__ void k_cc_jump(int *lab, int n, int *changed)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const int l0 = lab[i];
+    int x = l0;
+    for (int t = 0; t < n; t++) {
+        const int y = lab[x];
+        if (y >= x)
+            break;
+        x = y;
+    }
+    if (x != l0) {
+        lab[i] = x;
+        atomicOr(changed, 1);
+    }
+}
+
 // calcg1 (Analysis/PairCorrelation.cpp:9-155): every unordered pair of
 // stencil-adjacent cells once (j > k), the distance of cube::distance
 // (geometry.h:107-123), bin floor(dis / dx) counted when below gs.  Counts are

@@ -537,6 +537,25 @@ class LangevinNVT {
         return P;
     }
 
+    // ---- analysis: distance_graph (Analysis/AnalyzeGrowth.cpp:136-177) on
+    // the resident positions: the pairs within binding_distance on the
+    // stencil table's cells, without those touching indices ex_lo..ex_hi when
+    // ex_lo <= ex_hi; label = smallest index of the connected component
+    vector1<int> distance_graph(matrix<int> &boxes, double binding_distance, int ex_lo = 0,
+                                int ex_hi = -1)
+    {
+        int n = (int)std::lround(std::cbrt((double)boxes.getNsafe()));
+        if (n < 1 || n * n * n != boxes.getNsafe())
+            throw mdx_error(MDX_ERR_ARG, "stencil table must have n^3 rows");
+        const int nn[3] = {n, n, n};
+        std::vector<int> lab((size_t)N());
+        check(mdx_clusters(engine(), 0, nn, binding_distance, ex_lo, ex_hi, lab.data(), nullptr));
+        vector1<int> out(N());
+        for (int i = 0; i < N(); i++)
+            out[i] = lab[(size_t)i];
+        return out;
+    }
+
     // ---- forces (MD::calculateforces, MD.cpp:1684-1764, and friends); the
     // pair/bond lists are the terms' support (see the header comment)
     matrix<double> calculateforces(matrix<int> &, HSPotential &p)
@@ -779,6 +798,41 @@ class LangevinNVTR : public LangevinNVT {
         }
     }
 };
+
+// index_test / excludea (Analysis/AnalyzeGrowth.cpp:115-134): which pairs
+// the distance graph keeps -- all, or those with both indices outside
+// [N1, N2]
+struct index_test {
+    virtual ~index_test() {}
+    virtual void excluded_range(int &lo, int &hi) const
+    {
+        lo = 0;
+        hi = -1;
+    }
+};
+struct excludea : public index_test {
+    int N1, N2;
+    excludea(int N11, int N22) : N1(N11), N2(N22) {}
+    void excluded_range(int &lo, int &hi) const override
+    {
+        lo = N1;
+        hi = N2;
+    }
+};
+
+// distance_graph(pos, boxes, geo, binding_distance, i1)
+// (Analysis/AnalyzeGrowth.cpp:136-177): connected components of the pairs
+// within binding_distance, computed on the GPU
+inline vector1<int> distance_graph(matrix<double> &pos, matrix<int> &boxes, cube &geo,
+                                   double binding_distance, index_test *i1)
+{
+    LangevinNVT A(geo);
+    A.setdat(pos);
+    int lo = 0, hi = -1;
+    if (i1)
+        i1->excluded_range(lo, hi);
+    return A.distance_graph(boxes, binding_distance, lo, hi);
+}
 
 } // namespace mtamd
 

@@ -78,6 +78,8 @@ def lib():
     L.mdx_pair_histogram.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double,
                                      C.POINTER(C.c_longlong), C.c_int, C.POINTER(C.c_int)]
     L.mdx_patch_bonds.argtypes = [C.c_void_p, _ip]
+    L.mdx_clusters.argtypes = [C.c_void_p, C.c_int, _ip, C.c_double, C.c_int, C.c_int, _ip,
+                               C.POINTER(C.c_int)]
     L.mdx_autotune.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.POINTER(C.c_int)]
     L.mdx_stage_advance_pos.argtypes = [C.c_void_p]
     L.mdx_stage_rotate.argtypes = [C.c_void_p]
@@ -302,6 +304,18 @@ class Engine:
                                          out.ctypes.data_as(C.POINTER(C.c_longlong)), cap,
                                          C.byref(nb)))
         return out[:nb.value]
+
+    def clusters(self, n, cut, exclude=None, replica=0):
+        """distance_graph labels (smallest index of each particle's connected
+        component among the pairs within cut, n cells per side; exclude=(lo,
+        hi) drops the pairs touching indices lo..hi) and the component count"""
+        n3 = np.ascontiguousarray(np.broadcast_to(np.asarray(n), (3,)), dtype=np.int32)
+        lo, hi = exclude if exclude is not None else (0, -1)
+        out = np.zeros(int(self.n_per_replica[replica]), dtype=np.int32)
+        nc = C.c_int(0)
+        _check(lib().mdx_clusters(self.h, int(replica), n3.ctypes.data_as(_ip), float(cut),
+                                  int(lo), int(hi), out.ctypes.data_as(_ip), C.byref(nc)))
+        return out, nc.value
 
     def patch_bonds(self):
         out = np.zeros(self.N, dtype=np.int32)

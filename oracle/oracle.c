@@ -962,3 +962,40 @@ int orc_run_beads(const orc_params *P, const orc_box *b, int N, double *x, doubl
     free(F);
     return 0;
 }
+
+/* -------------------------------------------------------- distance_graph */
+
+static int uf_root(int *lab, int x)
+{
+    while (lab[x] != x) {
+        lab[x] = lab[lab[x]];
+        x = lab[x];
+    }
+    return x;
+}
+
+long long orc_dgraph_labels(const orc_box *b, int N, const double *x, const int n[3], double cut,
+                            int lo, int hi, int *labels)
+{
+    long long np = orc_pairs_reference(b, N, x, n, cut, NULL, 0);
+    int *ij = (int *)malloc(sizeof(int) * 2 * (size_t)(np > 0 ? np : 1));
+    if (!ij)
+        return -1;
+    orc_pairs_reference(b, N, x, n, cut, ij, np);
+    for (int i = 0; i < N; i++)
+        labels[i] = i;
+    for (long long k = 0; k < np; k++) {
+        const int p1 = ij[2 * k], p2 = ij[2 * k + 1];
+        if (lo <= hi && !((p1 < lo || p1 > hi) && (p2 < lo || p2 > hi)))
+            continue;
+        const int r1 = uf_root(labels, p1), r2 = uf_root(labels, p2);
+        if (r1 < r2)
+            labels[r2] = r1;
+        else if (r2 < r1)
+            labels[r1] = r2;
+    }
+    for (int i = 0; i < N; i++)
+        labels[i] = uf_root(labels, i);
+    free(ij);
+    return np;
+}

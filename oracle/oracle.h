@@ -32,6 +32,9 @@
  *   orc_philox4x32_10            : the Random123 known-answer vectors
  *   orc_noise_uniform            : moments / support (its own definition)
  *   orc_calcg1                   : golden fixture from the compiled reference
+ *   orc_dgraph_labels            : golden fixture from the compiled reference
+ *                                  (distance_graph); scipy's connected
+ *                                  components in the tests
  */
 #ifndef MTCODE_ORACLE_H
 #define MTCODE_ORACLE_H
@@ -229,6 +232,19 @@ int orc_run_beads(const orc_params *P, const orc_box *b, int N, double *x, doubl
  * dis >= max and fall outside the histogram).  Returns the bin count. */
 int orc_calcg1(const orc_box *b, int N, const double *x, double max, double dx,
                long long *counts, int cap);
+
+/* distance_graph (Analysis/AnalyzeGrowth.cpp:136-177): the pairs of
+ * calculatepairs with n cells per side within cut (orc_pairs_reference),
+ * kept when both indices lie outside [lo, hi] (lo > hi keeps all; the
+ * excludea predicate, AnalyzeGrowth.cpp:123-134), and the label each particle
+ * ends with in ConnectedComponentsParallel_Old
+ * (Bindings/GraphAlgorithms.cpp:672-810).  That iteration starts from
+ * ascending labels and only ever moves a root to a smaller label of an
+ * adjacent vertex, so its fixed point is the smallest index of each connected
+ * component; computed here by union-find with the smaller index as root.
+ * Returns the number of pairs, or -1 when out of memory. */
+long long orc_dgraph_labels(const orc_box *b, int N, const double *x, const int n[3], double cut,
+                            int lo, int hi, int *labels);
 
 /* measured_temperature (LangevinR.cpp:170-193): out[0..2] <p^2/2m>, out[3..5] <L^2/2I0> */
 void orc_measured_temperature(const orc_params *P, int N, const double *p, const double *L,

@@ -67,6 +67,9 @@ def lib():
         L.orc_calcg1.argtypes = [C.POINTER(Box), C.c_int, _dp, C.c_double, C.c_double,
                                  C.POINTER(C.c_longlong), C.c_int]
         L.orc_assign_all.argtypes = [C.POINTER(Box), C.c_int, _dp, _ip, _ip]
+        L.orc_dgraph_labels.restype = C.c_longlong
+        L.orc_dgraph_labels.argtypes = [C.POINTER(Box), C.c_int, _dp, _ip, C.c_double, C.c_int,
+                                        C.c_int, _ip]
         L.orc_boxes_relationships.restype = C.c_int
         L.orc_boxes_relationships.argtypes = [_ip, _ip]
         L.orc_pairs_reference.restype = C.c_longlong
@@ -305,6 +308,21 @@ def calcg1(box, x, max_, dx, periodic=True):
     gs = lib().orc_calcg1(C.byref(b), len(x), x.ctypes.data_as(_dp), C.c_double(max_),
                           C.c_double(dx), out.ctypes.data_as(C.POINTER(C.c_longlong)), cap)
     return out[:gs]
+
+
+def dgraph_labels(box, x, n, cut, lo=0, hi=-1, periodic=True):
+    """distance_graph labels: the smallest index of each particle's connected
+    component among the pairs within cut (orc_dgraph_labels)"""
+    b = make_box(box, periodic)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n3 = np.ascontiguousarray(np.broadcast_to(np.asarray(n), (3,)), dtype=np.int32)
+    out = np.empty(len(x), dtype=np.int32)
+    np_ = lib().orc_dgraph_labels(C.byref(b), len(x), x.ctypes.data_as(_dp),
+                                  n3.ctypes.data_as(_ip), C.c_double(cut), int(lo), int(hi),
+                                  out.ctypes.data_as(_ip))
+    if np_ < 0:
+        raise MemoryError("orc_dgraph_labels")
+    return out
 
 
 def noise(seed, first, count):

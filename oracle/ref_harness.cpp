@@ -607,6 +607,43 @@ static int cmd_calcg1()
     return 0;
 }
 
+static int cmd_dgraph()
+{
+    /* distance_graph (Analysis/AnalyzeGrowth.cpp:136-177): the pairs of
+       calculatepairs_parallel within the binding distance, kept when both
+       indices lie outside [ex_lo, ex_hi] (the excludea predicate,
+       AnalyzeGrowth.cpp:123-134; ex_lo > ex_hi keeps all, index_test), then
+       ConnectedComponentsParallel_Old (Bindings/GraphAlgorithms.cpp:672-810)
+       from ascending labels */
+    int N = (int)ji("N");
+    cube geo = make_cube();
+    LangevinNVT b(geo);
+    matrix<double> x = to_mat(rd<double>("x", (size_t)N * 3), N, 3);
+    b.setdat(x);
+    int tb;
+    matrix<int> boxes = geo.generate_boxes_relationships((int)ji("n"), tb);
+    matrix<int> *pairs = b.calculatepairs_parallel(boxes, jd("cut"));
+    const int lo = (int)ji("ex_lo"), hi = (int)ji("ex_hi");
+    vector<mdpair> edgelist;
+    for (int i = 0; i < pairs->getNsafe(); i++) {
+        int p1 = (*pairs)(i, 0), p2 = (*pairs)(i, 1);
+        bool keep = true;
+        if (lo <= hi)
+            keep = (p1 < lo || p1 > hi) && (p2 < lo || p2 > hi);
+        if (keep)
+            edgelist.push_back(mdpair(p1, p2));
+    }
+    string sg = "a";
+    vector1<int> idx(N, sg);
+    ConnectedComponentsParallel_Old(edgelist, idx);
+    vector<int> out(N);
+    for (int i = 0; i < N; i++)
+        out[i] = idx[i];
+    wr("labels", out.data(), out.size());
+    delete pairs;
+    return 0;
+}
+
 static int cmd_csv()
 {
     /* the reference's matrix CSV output (operator<<=, matrix2.cpp:898-912) */
@@ -644,6 +681,8 @@ int main(int argc, char **argv)
         return cmd_calcg1();
     if (cmd == "csv")
         return cmd_csv();
+    if (cmd == "dgraph")
+        return cmd_dgraph();
     die("unknown command " + cmd);
     return 2;
 }

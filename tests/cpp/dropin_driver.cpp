@@ -18,6 +18,8 @@
 //   integ "stencil" with key n: writes the stencil table out_stencil.bin (no GPU)
 //   integ "csv" with N, M: outfunc / importcsv of x.bin (no GPU)
 //   rotor mode "output": Condensate::run(nsteps, every) with its CSV files
+//   integ "dgraph" with N, L, n, cut, ex_lo, ex_hi: distance_graph labels of
+//          x.bin to out_labels.bin
 #include "mtcode_amd.hpp"
 
 #include <cstdio>
@@ -206,6 +208,24 @@ int main(int argc, char **argv)
             if (err || y.getNsafe() != N || y.getncols() != M)
                 throw mdx_error(MDX_ERR_ARG, "importcsv failed");
             wr("out_import.bin", y.getdata(), (size_t)N * M);
+            return 0;
+        }
+        if (g_job.at("integ") == "dgraph") {
+            // distance_graph(pos, boxes, geo, cut, i1) with index_test or excludea
+            const int N = ji("N");
+            vector1<bool> pb(3, true);
+            cube geo(jd("L"), pb, 3);
+            matrix<double> x = to_mat(rd<double>("x.bin", (size_t)N * 3), N, 3);
+            int tb;
+            matrix<int> boxes = geo.generate_boxes_relationships(ji("n"), tb);
+            index_test all;
+            excludea ex(ji("ex_lo"), ji("ex_hi"));
+            index_test *i1 = ji("ex_lo") <= ji("ex_hi") ? &ex : &all;
+            vector1<int> lab = distance_graph(x, boxes, geo, jd("cut"), i1);
+            std::vector<int> out(N);
+            for (int i = 0; i < N; i++)
+                out[i] = lab[i];
+            wr("out_labels.bin", out.data(), (size_t)N);
             return 0;
         }
         if (g_job.at("integ") == "stencil") {

@@ -310,6 +310,34 @@ def fixture_calcg1():
     np.savez_compressed(os.path.join(OUT, "calcg1.npz"), **out)
 
 
+def fixture_dgraph():
+    """distance_graph labels of the reference (Analysis/AnalyzeGrowth.cpp:
+    136-177: calculatepairs_parallel + ConnectedComponentsParallel_Old), at the
+    1.25 binding distance of Condensate::run_singlebond, with and without the
+    excludea index range, on 2 cells per side, and on bead chains"""
+    out = {}
+    u = wl.patchy_tetra(4096, phi=0.15, seed_pos=21)
+    p = wl.active_polymer(16, 48, phi=0.15, seed=9)
+    cases = [("u4096", u["x"], float(u["box"][0]), 1.25, 0, -1),
+             ("u4096ex", u["x"], float(u["box"][0]), 1.25, 1000, 2999),
+             ("u4096c16", u["x"], float(u["box"][0]), 1.6, 0, -1),
+             ("n2", wl.patchy_tetra(150, phi=0.15, seed_pos=22)["x"],
+              float(wl.box_for_packing(150, 0.15)), 1.25, 0, -1),
+             ("chains", p["x"], float(p["box"][0]), 1.05, 0, -1)]
+    for name, x, L, cut, lo, hi in cases:
+        n = max(int(np.floor(L / 3.5)), 1) if name != "n2" else 2
+        d = job_dir(f"dgraph_{name}")
+        po.run_ref("dgraph", d, {"N": len(x), "L": repr(L), "periodic": 1, "n": n,
+                                 "cut": repr(cut), "ex_lo": lo, "ex_hi": hi}, {"x": x})
+        out[f"{name}_x"] = x
+        out[f"{name}_L"] = L
+        out[f"{name}_n"] = n
+        out[f"{name}_cut"] = cut
+        out[f"{name}_ex"] = np.array([lo, hi])
+        out[f"{name}_labels"] = po.read_ref(d, "labels", dtype=np.int32)
+    np.savez_compressed(os.path.join(OUT, "dgraph.npz"), **out)
+
+
 def fixture_csv():
     """the reference's CSV text of a matrix (operator<<=, matrix2.cpp:898-912)"""
     rng = np.random.default_rng(17)
@@ -378,6 +406,7 @@ def main():
     fixture_traj_bead()
     fixture_calcg1()
     fixture_csv()
+    fixture_dgraph()
     print("fixtures written to", OUT)
 
 

@@ -142,3 +142,19 @@ def test_bead_loop_vs_oracle(tmp_path, mode):
         err = relmax(rd(str(tmp_path), f"out_{k}.bin", (N, 3)), ref[k])
         say(k, err)
         assert err < 1e-10, k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["u4096", "u4096ex", "n2", "chains"])
+def test_distance_graph_matches_reference(tmp_path, golden, case):
+    """distance_graph(pos, boxes, geo, d, i1) of the C++ classes (index_test or
+    excludea) against the reference's labels"""
+    g = golden("dgraph.npz")
+    x = g[f"{case}_x"]
+    lo, hi = (int(v) for v in g[f"{case}_ex"])
+    say("distance_graph", case, len(x))
+    run_driver(str(tmp_path), {"integ": "dgraph", "N": len(x), "L": repr(float(g[f"{case}_L"])),
+                               "n": int(g[f"{case}_n"]), "cut": repr(float(g[f"{case}_cut"])),
+                               "ex_lo": lo, "ex_hi": hi}, {"x": x})
+    lab = np.fromfile(os.path.join(str(tmp_path), "out_labels.bin"), dtype=np.int32)
+    assert np.array_equal(lab, g[f"{case}_labels"])

@@ -458,6 +458,47 @@ def test_pair_histogram_bit_exact(golden, case):
     assert np.array_equal(c, g[f"{case}_counts"])
 
 
+@pytest.mark.parametrize("case", ["u4096", "u4096ex", "u4096c16", "n2", "chains"])
+def test_clusters_bit_exact(golden, case):
+    """mdx_clusters against the labels of the reference's distance_graph
+    (calculatepairs_parallel + ConnectedComponentsParallel_Old)"""
+    g = golden("dgraph.npz")
+    x = g[f"{case}_x"]
+    lo, hi = (int(v) for v in g[f"{case}_ex"])
+    say("clusters", case, len(x))
+    with mdx.Engine(len(x), float(g[f"{case}_L"]), 1.0, 0.005) as e:
+        e.upload(x)
+        lab, nc = e.clusters(int(g[f"{case}_n"]), float(g[f"{case}_cut"]), (lo, hi))
+    assert np.array_equal(lab, g[f"{case}_labels"])
+    assert nc == len(np.unique(g[f"{case}_labels"]))
+
+
+def test_clusters_large_and_replica_vs_oracle():
+    """262,144 condensate particles after 200 steps (binding distances 1.25
+    and 1.6, the latter percolating), and one replica of a batch, against the
+    oracle's union-find labels"""
+    s = wl.condensate_ab(1 << 18, phi=0.1)
+    L = float(s["box"][0])
+    n = int(L // 3.5)
+    with mdx.engine_from_system(s, precision=mdx.PREC_MIXED, seed=3) as e:
+        e.prime()
+        e.step(200)
+        x = e.download()["x"]
+        for cut in (1.25, 1.6):
+            say("clusters large", s["N"], cut)
+            lab, nc = e.clusters(n, cut)
+            ref = po.dgraph_labels([L] * 3, x, n, cut)
+            assert np.array_equal(lab, ref)
+            say("components", nc, "largest", int(np.bincount(lab).max()))
+    reps = wl.replicas(5, seed=8, nmin=300, nmax=2000)
+    with mdx.engine_from_system(None, replicas=reps) as e:
+        for r in (0, 3):
+            Lr = float(reps[r]["box"][0])
+            nr = max(int(Lr // 3.5), 1)
+            lab, _ = e.clusters(nr, 1.3, (10, 40), replica=r)
+            assert np.array_equal(lab, po.dgraph_labels([Lr] * 3, reps[r]["x"], nr, 1.3, 10, 40))
+
+
 def test_autotune_tries_every_candidate_and_keeps_the_fastest():
     s = wl.condensate_ab(1 << 16, phi=0.1)
     with engine_for(s, mdx.PREC_MIXED, seed=2) as e:

@@ -124,3 +124,40 @@ def test_calcg1_histogram_bit_exact(golden, case):
     L = float(g[f"{case}_L"])
     c = po.calcg1(np.array([L] * 3), g[f"{case}_x"], float(g[f"{case}_max"]), float(g[f"{case}_dx"]))
     assert np.array_equal(c, g[f"{case}_counts"])
+
+
+DGRAPH_CASES = ["u4096", "u4096ex", "u4096c16", "n2", "chains"]
+
+
+@pytest.mark.parametrize("case", DGRAPH_CASES)
+def test_distance_graph_labels_bit_exact(golden, case):
+    """distance_graph (Analysis/AnalyzeGrowth.cpp:136-177): the oracle's
+    union-find labels equal the labels the reference's
+    ConnectedComponentsParallel_Old converges to"""
+    g = golden("dgraph.npz")
+    L = float(g[f"{case}_L"])
+    lo, hi = (int(v) for v in g[f"{case}_ex"])
+    lab = po.dgraph_labels([L] * 3, g[f"{case}_x"], int(g[f"{case}_n"]), float(g[f"{case}_cut"]),
+                           lo, hi)
+    assert np.array_equal(lab, g[f"{case}_labels"])
+
+
+@pytest.mark.parametrize("case", DGRAPH_CASES)
+def test_distance_graph_labels_are_component_minima(golden, case):
+    """the same labels from scipy's connected components over the oracle's
+    pair set: each particle carries the smallest index of its component"""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    g = golden("dgraph.npz")
+    x, L = g[f"{case}_x"], float(g[f"{case}_L"])
+    lo, hi = (int(v) for v in g[f"{case}_ex"])
+    ij = po.pairs_reference(np.array([L] * 3), x, int(g[f"{case}_n"]), float(g[f"{case}_cut"]))
+    if lo <= hi:
+        out = lambda v: (v < lo) | (v > hi)  # noqa: E731
+        ij = ij[out(ij[:, 0]) & out(ij[:, 1])]
+    N = len(x)
+    A = coo_matrix((np.ones(len(ij)), (ij[:, 0], ij[:, 1])), shape=(N, N))
+    _, comp = connected_components(A, directed=False)
+    first = np.full(comp.max() + 1, N)
+    np.minimum.at(first, comp, np.arange(N))
+    assert np.array_equal(first[comp], g[f"{case}_labels"])
