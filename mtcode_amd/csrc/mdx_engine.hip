@@ -397,8 +397,8 @@ static int finish_bins(mdx_engine *e, const StepCtl *gate = nullptr, const int *
                        (int *)e->start.p, (const int *)e->bsum.p, n, gate);
     hipLaunchKernelGGL(k_fill, dim3(nblk(e->N)), dim3(256), 0, e->stream, e->N, cells,
                        (const int *)e->rank.p, (const int *)e->start.p, (int *)e->list.p, gate);
-    hipLaunchKernelGGL(k_cellsort, dim3(nblk(n)), dim3(256), 0, e->stream, n,
-                       (const int *)e->start.p, (int *)e->list.p, gate);
+    hipLaunchKernelGGL(k_cellsort, dim3(nblk(e->N)), dim3(256), 0, e->stream, e->N, cells,
+                       (const int *)e->rank.p, (const int *)e->start.p, (int *)e->list.p, gate);
     HIPCHK(hipGetLastError());
     return MDX_OK;
 }
@@ -597,7 +597,7 @@ static int nbr_alloc(mdx_engine *e)
 // bmax); epoch_inc: also count the step's force evaluation
 static int decide(mdx_engine *e, int epoch_inc)
 {
-    hipLaunchKernelGGL(k_decide, dim3(1), dim3(256), 0, e->stream, (StepCtl *)e->ctl.p,
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(1024), 0, e->stream, (StepCtl *)e->ctl.p,
                        (const float *)e->bmax.p, (int)nblk(e->N), (const float *)e->bmax2.p,
                        e->drift_nb2, e->skin, epoch_inc);
     HIPCHK(hipGetLastError());

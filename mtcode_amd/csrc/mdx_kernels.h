@@ -1800,16 +1800,21 @@ __global__ void __launch_bounds__(256) k_fill(int N, const int *cell, const int 
 }
 
 // ascending slot order inside each cell: makes the gather order, and so the
-// floating-point sums, independent of atomic arrival order
-__global__ void __launch_bounds__(256) k_cellsort(int ncells, const int *start, int *list,
+// floating-point sums, independent of atomic arrival order.  One thread per
+// particle; the particle that arrived first in its cell (rank 0, unique per
+// non-empty cell and not changed by the sort) sorts the cell, so empty cells
+// cost nothing.
+__global__ void __launch_bounds__(256) k_cellsort(int N, const int *cell, const int *rank,
+                                                  const int *start, int *list,
                                                   const StepCtl *gate)
 {
     if (gate && !gate->rebuild)
         return;
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N || rank[i] != 0)
         return;
-    int s0 = start[c], s1 = start[c + 1];
+    const int c = cell[i];
+    const int s0 = start[c], s1 = start[c + 1];
     for (int s = s0 + 1; s < s1; s++) {
         int v = list[s];
         int t = s - 1;
@@ -1839,15 +1844,27 @@ __global__ void k_list_reset(StepCtl *ctl)
 // the grid max of the block drifts of the kernel before is added to the
 // displacement bound D; a list rebuild is requested once 2 D reaches the skin
 // (a pair within reach now was within reach + skin at the build)
-__global__ void __launch_bounds__(256) k_decide(StepCtl *ctl, const float *bmax, int nb,
-                                                const float *bmax2, int nb2, double skin,
-                                                int epoch_inc)
+__global__ void __launch_bounds__(1024) k_decide(StepCtl *ctl, const float *bmax, int nb,
+                                                 const float *bmax2, int nb2, double skin,
+                                                 int epoch_inc)
 {
-    __shared__ float wmax[4];
+    __shared__ float wmax[16];
+    // the first 4 x 1024 entries of each array as independent loads (their
+    // latencies overlap), then the rest
+    float v[8];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int b = threadIdx.x + 1024 * k;
+        v[k] = b < nb ? bmax[b] : 0.f;
+        v[4 + k] = b < nb2 ? bmax2[b] : 0.f;
+    }
     float m = 0.f;
-    for (int b = threadIdx.x; b < nb; b += 256)
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        m = fmaxf(m, v[k]);
+    for (int b = threadIdx.x + 4096; b < nb; b += 1024)
         m = fmaxf(m, bmax[b]);
-    for (int b = threadIdx.x; b < nb2; b += 256)
+    for (int b = threadIdx.x + 4096; b < nb2; b += 1024)
         m = fmaxf(m, bmax2[b]);
     for (int o = 32; o > 0; o >>= 1)
         m = fmaxf(m, __shfl_xor(m, o));
@@ -1855,7 +1872,9 @@ __global__ void __launch_bounds__(256) k_decide(StepCtl *ctl, const float *bmax,
         wmax[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
-        m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+        m = 0.f;
+        for (int w = 0; w < 16; w++)
+            m = fmaxf(m, wmax[w]);
         // margin for the rounding of the drift and of the FP64 position update
         const double D = ctl->D + (double)m * (1 + 1e-6) + 1e-9;
         ctl->D = D;
