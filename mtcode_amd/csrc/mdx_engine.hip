@@ -138,7 +138,7 @@ struct mdx_engine {
     int n_patchy = 0; // particles whose type carries patches
     bool list_eager = false; // an eager list step ran since the last list build
     GraphSlot graphs[2];     // per current state buffer
-    Buf nbr, nn, bcell;  // neighbour list
+    Buf nbr, nbrc, nn, bcell; // neighbour list (column-major, per-block compact copy)
     Buf bmax, bmax2;     // per-block drift maxima (walk / patch kernel grids)
     int drift_nb2 = 0;   // patch-kernel blocks that wrote bmax2 in the last gather
     hipEvent_t prof_mid = nullptr;  // recorded between walk and finish while profiling
@@ -371,6 +371,8 @@ template <class R> static KArgs<R> make_args(mdx_engine *e)
     a.lmode = 0;
     a.nbr = (const int *)e->nbr.p;
     a.nbrw = (int *)e->nbr.p;
+    a.nbrc = (const int *)e->nbrc.p;
+    a.nbrcw = (int *)e->nbrc.p;
     a.nn = (int *)e->nn.p;
     a.bcell = (int *)e->bcell.p;
     a.kmax = e->kmax;
@@ -586,6 +588,7 @@ static int nbr_alloc(mdx_engine *e)
 {
     int rc;
     if ((rc = dalloc(e, e->nbr, 4 * (size_t)e->kmax * e->N)) ||
+        (rc = dalloc(e, e->nbrc, 4 * (size_t)e->kmax * nblk(e->N) * 256)) ||
         (rc = dalloc(e, e->nn, 4 * (size_t)e->N)) || (rc = dalloc(e, e->bcell, 4 * (size_t)e->N)) ||
         (rc = dalloc(e, e->bmax, 4 * (size_t)nblk(e->N))) ||
         (rc = dalloc(e, e->bmax2, 4 * (size_t)nblk(e->N))))
@@ -1031,7 +1034,7 @@ void mdx_destroy(mdx_engine *e)
                    &e->ang_ijk, &e->ang_k,   &e->ang_t0, &e->exc_off, &e->exc_j,  &e->act_prev,
                    &e->act_next, &e->noise,  &e->ctl,    &e->partial, &e->tmpd,   &e->tmpi,
                    &e->tmpi2,   &e->tmpi3,   &e->tmpll,  &e->tmpx,    &e->Fw,
-                   &e->ncand,   &e->cand,    &e->nbr,    &e->nn,     &e->bcell,
+                   &e->ncand,   &e->cand,    &e->nbr,    &e->nbrc,   &e->nn,     &e->bcell,
                    &e->bmax,    &e->Tp,      &e->bmax2};
     for (Buf *b : bufs)
         dfree(*b);

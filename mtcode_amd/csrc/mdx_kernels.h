@@ -75,10 +75,13 @@ template <class R> struct KArgs {
     R *Tp;             // patch torque per position (compacted patch kernel)
     int nrep; // replicas (1: the replica record is loaded uniformly)
     // neighbour list (lmode): slot s walks nbr[k * N + s], k < nn[s] (<= kmax),
-    // else the 27 cells of its cell at the last build, bcell[s]
+    // else the 27 cells of its cell at the last build, bcell[s].  nbrc: the
+    // same entries per block of 256 slots, back to back in slot order from
+    // nbrc[first slot * kmax] (slots over kmax contribute none), for the
+    // compacted walk
     int lmode;
-    const int *nbr;
-    int *nbrw, *nn, *bcell;
+    const int *nbr, *nbrc;
+    int *nbrw, *nbrcw, *nn, *bcell;
     int kmax;
     double rl2, skin;
     float *bmax;  // per block: max drift of the block's particles
@@ -1171,7 +1174,7 @@ __global__ void __launch_bounds__(256, (sizeof(R) == 4 && !INT) ? 6 : 1) k_walkc
             }
             const int o = lo;
             const int so = base + o;
-            const int j = a.nbr[(size_t)(e - off[o]) * a.N + so];
+            const int j = a.nbrc[(size_t)base * a.kmax + e];
             const PosRec xo = sx[o];
             Acc<R> pa;
             pa.F[0] = pa.F[1] = pa.F[2] = 0;
@@ -1474,9 +1477,12 @@ __global__ void __launch_bounds__(256, 8) k_nbuild(KArgs<R> a, const int *cells,
         return;
     if (blockIdx.x == 0 && threadIdx.x == 0)
         atomicAdd(&a.ctl->nbuilds, 1u);
-    const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (s >= a.N)
-        return;
+    __shared__ int wsum[4];
+    const int tid = threadIdx.x;
+    const int base = xcd_block(blockIdx.x, gridDim.x) * blockDim.x;
+    const int s = base + tid;
+    int cnt = 0;
+    if (s < a.N) {
     const PosRec xi = a.X[s];
     const DevRep rp = rep_rec(a, xi.meta);
     const int c = cells[s];
@@ -1516,6 +1522,24 @@ __global__ void __launch_bounds__(256, 8) k_nbuild(KArgs<R> a, const int *cells,
         n++;
     });
     a.nn[s] = n | (kept_excluded ? LIST_EXC : 0);
+    cnt = n <= a.kmax ? n : 0;
+    }
+    // the block's lists back to back (exclusive scan of the counts)
+    int v = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(v, o);
+        if ((tid & 63) >= o)
+            v += t;
+    }
+    if ((tid & 63) == 63)
+        wsum[tid >> 6] = v;
+    __syncthreads();
+    int at = v - cnt;
+    for (int w = 0; w < (tid >> 6); w++)
+        at += wsum[w];
+    int *seg = a.nbrcw + (size_t)base * a.kmax + at;
+    for (int k = 0; k < cnt; k++)
+        seg[k] = a.nbrw[(size_t)k * a.N + s];
 }
 
 
