@@ -1161,24 +1161,40 @@ __global__ void __launch_bounds__(256, (sizeof(R) == 4 && !INT) ? 6 : 1) k_walkc
 
     R F[3] = {0, 0, 0};
     int npc = 0;
+    auto owner_of = [&](int e) -> int {
+        int lo = 0, hi = 255;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (off[mid] <= e)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        return lo;
+    };
+    const int *lst = a.nbrc + (size_t)base * a.kmax;
+    // software pipeline: round r evaluates the record loaded in round r - 1
+    // and loads the record of round r + 1 and the list entry of round r + 2
+    int j1 = tid < total ? lst[tid] : 0;
+    int j2 = tid + 256 < total ? lst[tid + 256] : 0;
+    PosRec x1;
+    if (tid < total)
+        x1 = a.X[j1];
     for (int r0 = 0; r0 < total; r0 += 256) {
         const int e = r0 + tid;
+        const int j = j1;
+        const PosRec xj = x1;
+        j1 = j2;
+        if (e + 256 < total)
+            x1 = a.X[j1];
+        if (e + 512 < total)
+            j2 = lst[e + 512];
         if (e < total) {
-            int lo = 0, hi = 255;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (off[mid] <= e)
-                    lo = mid;
-                else
-                    hi = mid - 1;
-            }
-            const int o = lo;
-            const int so = base + o;
-            const int j = a.nbrc[(size_t)base * a.kmax + e];
+            const int o = owner_of(e);
             const PosRec xo = sx[o];
             Acc<R> pa;
             pa.F[0] = pa.F[1] = pa.F[2] = 0;
-            const bool c = visit(xo, sq + 9 * o, sexc[2 * o], sexc[2 * o + 1], a.X[j], pa);
+            const bool c = visit(xo, sq + 9 * o, sexc[2 * o], sexc[2 * o + 1], xj, pa);
             for (int k = 0; k < 3; k++)
                 pf[3 * tid + k] = pa.F[k];
             pj[tid] = c ? j : -1;
@@ -1261,6 +1277,51 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_finish(KAr
             drift);
     if ((MODE == M_FUSED || MODE == M_BEAD) && a.lmode)
         drift_bound(a, drift);
+}
+
+// the patch candidates of slot so beyond the PMAXG handed over by the walk:
+// the neighbours are walked again in walk order and the terms after the
+// first PMAXG candidates added to Fw / Tp (k_patch has added the others)
+template <class R, int MODE>
+__device__ __forceinline__ void patch_overflow(const KArgs<R> &a, int so)
+{
+    const int i = a.lmode ? so : a.list[so];
+    const PosRec xi = a.X[i];
+    const int ti = xi.meta & 255;
+    const DevRep rp = rep_rec(a, xi.meta);
+    R qi[9];
+    for (int k2 = 0; k2 < 9; k2++)
+        qi[k2] = a.Q[9 * (size_t)i + k2];
+    Acc<R> acc;
+    for (int k2 = 0; k2 < 3; k2++) {
+        acc.F[k2] = a.Fw[3 * (size_t)so + k2];
+        acc.T[k2] = a.Tp[3 * (size_t)so + k2];
+    }
+    const int npi = a.np[ti];
+    int seen = 0;
+    for_each_near(a, rp, i, [&](const PosRec &xj, int j) {
+        double d[3];
+        min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
+        const double r2s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (!(r2s < a.rmax2) || !(npi > 0 && r2s < a.maxd2))
+            return;
+        const int tj = xj.meta & 255;
+        if (a.np[tj] == 0)
+            return;
+        int ta, tc;
+        patch_roles(xi.id, ti, xj.id, tj, ta, tc);
+        if (!((a.active_mask >> (ta * MAXT + tc)) & 1ull))
+            return;
+        if (MDX_WALK_SCREEN && !own_screen(a, xi.id, ti, qi, xj, d, r2s))
+            return;
+        const int js = j >= 0 ? j : a.slot[xj.id];
+        if (seen++ >= PMAXG)
+            patch_pair<R, MODE>(a, rp, xi, ti, qi, js, acc);
+    });
+    for (int k2 = 0; k2 < 3; k2++) {
+        a.Fw[3 * (size_t)so + k2] = acc.F[k2];
+        a.Tp[3 * (size_t)so + k2] = acc.T[k2];
+    }
 }
 
 // split form, compacted: the patch candidates of a block of pblk positions
@@ -1369,49 +1430,14 @@ __global__ void __launch_bounds__(256) k_patch(KArgs<R> a, int pblk)
         __threadfence_block();
         __syncthreads();
     }
-    // candidates beyond the handed-over ones: walk again, in walk order
-    for (int k = 0; k < PER; k++) {
-        const int so = base + k * 256 + tid;
-        if (so >= a.N || a.ncand[so] <= PMAXG)
-            continue;
-        const int i = a.lmode ? so : a.list[so];
-        const PosRec xi = a.X[i];
-        const int ti = xi.meta & 255;
-        const DevRep rp = rep_rec(a, xi.meta);
-        R qi[9];
-        for (int k2 = 0; k2 < 9; k2++)
-            qi[k2] = a.Q[9 * (size_t)i + k2];
-        Acc<R> acc;
-        for (int k2 = 0; k2 < 3; k2++) {
-            acc.F[k2] = a.Fw[3 * (size_t)so + k2];
-            acc.T[k2] = a.Tp[3 * (size_t)so + k2];
+    // candidates beyond the handed-over ones (rare): inline in the fused form,
+    // else k_patch_ovf after this kernel (keeps this kernel at 3 waves per SIMD)
+    if (INT)
+        for (int k = 0; k < PER; k++) {
+            const int so = base + k * 256 + tid;
+            if (so < a.N && a.ncand[so] > PMAXG)
+                patch_overflow<R, MODE>(a, so);
         }
-        const int npi = a.np[ti];
-        int seen = 0;
-        for_each_near(a, rp, i, [&](const PosRec &xj, int j) {
-            double d[3];
-            min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
-            const double r2s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-            if (!(r2s < a.rmax2) || !(npi > 0 && r2s < a.maxd2))
-                return;
-            const int tj = xj.meta & 255;
-            if (a.np[tj] == 0)
-                return;
-            int ta, tc;
-            patch_roles(xi.id, ti, xj.id, tj, ta, tc);
-            if (!((a.active_mask >> (ta * MAXT + tc)) & 1ull))
-                return;
-            if (MDX_WALK_SCREEN && !own_screen(a, xi.id, ti, qi, xj, d, r2s))
-                return;
-            const int js = j >= 0 ? j : a.slot[xj.id];
-            if (seen++ >= PMAXG)
-                patch_pair<R, MODE>(a, rp, xi, ti, qi, js, acc);
-        });
-        for (int k2 = 0; k2 < 3; k2++) {
-            a.Fw[3 * (size_t)so + k2] = acc.F[k2];
-            a.Tp[3 * (size_t)so + k2] = acc.T[k2];
-        }
-    }
     if (INT) {
         // the positions with candidates, compacted into off[], integrated one
         // per lane
@@ -1449,6 +1475,14 @@ __global__ void __launch_bounds__(256) k_patch(KArgs<R> a, int pblk)
             drift_bound(b, drift);
         }
     }
+}
+
+// the candidates beyond PMAXG of the separate form's positions (after k_patch)
+template <class R, int MODE> __global__ void __launch_bounds__(256) k_patch_ovf(KArgs<R> a)
+{
+    const int so = blockIdx.x * blockDim.x + threadIdx.x;
+    if (so < a.N && a.ncand[so] > PMAXG)
+        patch_overflow<R, MODE>(a, so);
 }
 
 // ... and the topology terms and the integrator
