@@ -378,6 +378,14 @@ template <class R> static KArgs<R> make_args(mdx_engine *e)
     a.kmax = e->kmax;
     const double rl = reach(e) + e->skin;
     a.rl2 = rl * rl;
+    // (the decisions on the WCA cut-off carry a 1e-12 relative margin); one
+    // radius when every type pair is patch-active (no per-pair test needed)
+    bool all_active = a.has_patches;
+    for (int ta = 0; ta < e->ntypes && all_active; ta++)
+        for (int tc = ta; tc < e->ntypes; tc++)
+            all_active &= (e->active_mask >> (ta * MAXT + tc)) & 1ull;
+    const double rw = e->wca ? e->wca_pl * (1 + 1e-12) + e->skin : 0.0;
+    a.rl2w = all_active ? a.rl2 : (e->wca ? std::min(rw * rw, a.rl2) : 0.0);
     a.skin = e->skin;
     a.bmax = (float *)e->bmax.p;
     a.bmax2 = (float *)e->bmax2.p;

@@ -83,7 +83,9 @@ template <class R> struct KArgs {
     const int *nbr, *nbrc;
     int *nbrw, *nbrcw, *nn, *bcell;
     int kmax;
-    double rl2, skin;
+    // list radii squared: interaction reach + skin; WCA reach + skin for the
+    // pairs that cannot interact through patches
+    double rl2, rl2w, skin;
     float *bmax;  // per block: max drift of the block's particles
     float *bmax2; // the same for the blocks of the patch kernel (fused form)
     // WCA (HSPotential)
@@ -1517,46 +1519,52 @@ __global__ void __launch_bounds__(256, 8) k_nbuild(KArgs<R> a, const int *cells,
     const int s = base + tid;
     int cnt = 0;
     if (s < a.N) {
-    const PosRec xi = a.X[s];
-    const DevRep rp = rep_rec(a, xi.meta);
-    const int c = cells[s];
-    a.bcell[s] = c;
-    int exc0 = 0, exc1 = 0;
-    if (a.exc_off) {
-        exc0 = a.exc_off[xi.id];
-        exc1 = a.exc_off[xi.id + 1];
-    }
-    const int ti = xi.meta & 255;
-    int n = 0;
-    bool kept_excluded = false;
-    for_each_in_cells(a, rp, c, s, [&](const PosRec &xj, int j) {
-        double d[3];
-        min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
-        if (!(d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < a.rl2))
-            return;
-        // a pair without WCA (exclusion) is listed only if it can still
-        // interact through patches
-        bool ex = false;
-        for (int e = exc0; e < exc1; e++)
-            ex |= (uint32_t)a.exc_j[e] == xj.id;
-        if (ex) {
-            const int tj = xj.meta & 255;
-            bool patchy = a.has_patches && a.np[ti] > 0 && a.np[tj] > 0;
-            if (patchy) {
-                int ta, tc;
-                patch_roles(xi.id, ti, xj.id, tj, ta, tc);
-                patchy = (a.active_mask >> (ta * MAXT + tc)) & 1ull;
-            }
-            if (!patchy)
-                return;
-            kept_excluded = true;
+        const PosRec xi = a.X[s];
+        const DevRep rp = rep_rec(a, xi.meta);
+        const int c = cells[s];
+        a.bcell[s] = c;
+        int exc0 = 0, exc1 = 0;
+        if (a.exc_off) {
+            exc0 = a.exc_off[xi.id];
+            exc1 = a.exc_off[xi.id + 1];
         }
-        if (n < a.kmax)
-            a.nbrw[(size_t)n * a.N + s] = j;
-        n++;
-    });
-    a.nn[s] = n | (kept_excluded ? LIST_EXC : 0);
-    cnt = n <= a.kmax ? n : 0;
+        const int ti = xi.meta & 255;
+        // the pair's types carry an active patch row
+        auto patch_active = [&](const PosRec &xj) -> bool {
+            const int tj = xj.meta & 255;
+            if (!(a.has_patches && a.np[ti] > 0 && a.np[tj] > 0))
+                return false;
+            int ta, tc;
+            patch_roles(xi.id, ti, xj.id, tj, ta, tc);
+            return (a.active_mask >> (ta * MAXT + tc)) & 1ull;
+        };
+        int n = 0;
+        bool kept_excluded = false;
+        for_each_in_cells(a, rp, c, s, [&](const PosRec &xj, int j) {
+            double d[3];
+            min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
+            const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (!(r2 < a.rl2))
+                return;
+            // beyond the WCA reach: listed only for a type pair with patch terms
+            if (!(r2 < a.rl2w) && !patch_active(xj))
+                return;
+            // a pair without WCA (exclusion) is listed only if it can still
+            // interact through patches
+            bool ex = false;
+            for (int e = exc0; e < exc1; e++)
+                ex |= (uint32_t)a.exc_j[e] == xj.id;
+            if (ex) {
+                if (!patch_active(xj))
+                    return;
+                kept_excluded = true;
+            }
+            if (n < a.kmax)
+                a.nbrw[(size_t)n * a.N + s] = j;
+            n++;
+        });
+        a.nn[s] = n | (kept_excluded ? LIST_EXC : 0);
+        cnt = n <= a.kmax ? n : 0;
     }
     // the block's lists back to back (exclusive scan of the counts)
     int v = cnt;
