@@ -127,7 +127,9 @@ int mdx_set_neighbor_list(mdx_engine *e, double skin, int kmax, int resort_every
 #define MDX_TERM_ACTIVE 32
 #define MDX_TERM_ALL 63
 int mdx_clear_interactions(mdx_engine *e, int terms);
-/* Autotuner (no reference counterpart): runs 2 x trial_steps steps with each
+/* Autotuner (no reference counterpart): after 4 x trial_steps warm-up steps
+ * at the current settings (a system started from rest heats up for about
+ * 1 / gamma), runs 2 x trial_steps steps with each
  * candidate neighbour-list skin (0.25, 0.4, 0.6, 0.9) and walk form
  * (compacted lanes, one lane per particle), timing the second half with HIP
  * events, and keeps the fastest.  The steps are real steps (the state

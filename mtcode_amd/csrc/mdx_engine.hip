@@ -672,10 +672,13 @@ template <class R, int MODE> static int list_step_graph(mdx_engine *e)
 {
     KArgs<R> a = make_args<R>(e);
     a.lmode = 1;
-    std::vector<char> sig(sizeof a + 2 * sizeof(int));
+    // the arguments, the grid and the launch-form switches (which kernels
+    // the step launches)
+    const int form[7] = {e->ncells,      e->N,         e->use_split, e->use_compact,
+                         e->use_walkc,   e->use_fuse,  e->use_tiles};
+    std::vector<char> sig(sizeof a + sizeof form);
     memcpy(sig.data(), &a, sizeof a);
-    memcpy(sig.data() + sizeof a, &e->ncells, sizeof(int));
-    memcpy(sig.data() + sizeof a + sizeof(int), &e->N, sizeof(int));
+    memcpy(sig.data() + sizeof a, form, sizeof form);
     GraphSlot &g = e->graphs[e->cur];
     if (!g.exec || g.sig != sig) {
         if (g.exec) {
@@ -2048,6 +2051,13 @@ template <class R> static int autotune_t(mdx_engine *e, int trial, int cap, doub
     int k = 0, best = -1, rc = MDX_OK;
     double best_ms = 1e300;
     const bool rotor = e->integ == MDX_INTEG_ROTOR;
+    // warm-up at the current settings first: a system started from rest is
+    // still heating towards kT for about 1 / gamma, and its slow particles
+    // would make the first candidates' list rebuilds look rare
+    if (rotor && !e->primed && (rc = do_prime<R>(e, nullptr)))
+        return rc;
+    if ((rc = rotor ? enqueue_rotor<R>(e, 4 * trial, nullptr) : enqueue_bead<R>(e, 4 * trial, nullptr)))
+        return rc;
     for (double sk : skins)
         for (int wv : walks) {
             if (sk != e->skin)
