@@ -506,8 +506,8 @@ static int launch_gather(mdx_engine *e, const double *noise_dev, int lmode = 0)
         } else if (e->use_compact) {
             hipLaunchKernelGGL((k_patch<R, MODE, false>), dim3(nblk(e->N, pblk)), dim3(256), 0,
                                e->stream, a, pblk);
-            hipLaunchKernelGGL((k_patch_ovf<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream,
-                               a);
+            hipLaunchKernelGGL((k_patch_ovf<R, MODE>), dim3(std::min(nblk(e->N), 512u)), dim3(256),
+                               0, e->stream, a);
             if (e->prof_mid2)
                 HIPCHK(hipEventRecord(e->prof_mid2, e->stream));
             hipLaunchKernelGGL((k_integrate<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream,

@@ -1479,12 +1479,13 @@ __global__ void __launch_bounds__(256) k_patch(KArgs<R> a, int pblk)
     }
 }
 
-// the candidates beyond PMAXG of the separate form's positions (after k_patch)
+// the candidates beyond PMAXG of the separate form's positions (after k_patch;
+// grid-stride over a small grid: almost every step has none)
 template <class R, int MODE> __global__ void __launch_bounds__(256) k_patch_ovf(KArgs<R> a)
 {
-    const int so = blockIdx.x * blockDim.x + threadIdx.x;
-    if (so < a.N && a.ncand[so] > PMAXG)
-        patch_overflow<R, MODE>(a, so);
+    for (int so = blockIdx.x * blockDim.x + threadIdx.x; so < a.N; so += gridDim.x * blockDim.x)
+        if (a.ncand[so] > PMAXG)
+            patch_overflow<R, MODE>(a, so);
 }
 
 // ... and the topology terms and the integrator
