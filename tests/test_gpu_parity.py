@@ -334,12 +334,15 @@ def _run(s, precision, nsteps, skin, kmax=64, resort=200, seed=7):
     return out
 
 
+@pytest.mark.parametrize("split", ["1", "0"])
 @pytest.mark.parametrize("kmax", [64, 4])
-def test_neighbor_list_rotor_matches_cell_walk(kmax):
+def test_neighbor_list_rotor_matches_cell_walk(monkeypatch, kmax, split):
     """Steps that walk the neighbour list (rebuilt on the device from the
     displacement bound, re-sorted every 25 steps; kmax=4 sends most particles
     to the cell fallback) evaluate the same pairs as the exhaustive cell walk;
-    only the summation order differs, so FP64 trajectories agree to rounding."""
+    only the summation order differs, so FP64 trajectories agree to rounding.
+    Both the walk / patch / integrator kernels and the one-kernel form."""
+    monkeypatch.setenv("MDX_SPLIT", split)
     s = wl.condensate_ab(1 << 15, phi=0.2)
     say("list vs cell walk (rotor)", s["N"], "kmax", kmax)
     ref = _run(s, mdx.PREC_FP64, 80, 0.0)
@@ -352,13 +355,15 @@ def test_neighbor_list_rotor_matches_cell_walk(kmax):
         assert err < 1e-9, k
 
 
+@pytest.mark.parametrize("split", ["1", "0"])
 @pytest.mark.parametrize("builder", [
     lambda: wl.mt_bundle(2, 13, 60, spacing=6.0, box=(12.0, 12.0, 60.0)),
     lambda: wl.active_polymer(nchains=64, nbeads=64, phi=0.15, seed=3),
 ])
-def test_neighbor_list_topology_matches_cell_walk(builder):
+def test_neighbor_list_topology_matches_cell_walk(monkeypatch, builder, split):
     """Bead integrator (microtubule bundle: bonds, bending, exclusions) and a
     rotor system with topology (active polymer with patchy end beads)."""
+    monkeypatch.setenv("MDX_SPLIT", split)
     s = builder()
     say("list vs cell walk", s["name"], s["integrator"], s["N"])
     ref = _run(s, mdx.PREC_FP64, 80, 0.0)
@@ -382,15 +387,17 @@ def test_neighbor_list_mixed_close_to_cell_walk():
 
 
 # ------------------------------------------------------------------ graphs
+@pytest.mark.parametrize("split", ["1", "0"])
 @pytest.mark.parametrize("builder", [
     lambda: wl.condensate_ab(1 << 16, phi=0.1),
     lambda: wl.mt_bundle(2, 13, 60, spacing=6.0, box=(12.0, 12.0, 60.0)),
 ])
-def test_graph_replay_bitwise_equal_eager(monkeypatch, builder):
+def test_graph_replay_bitwise_equal_eager(monkeypatch, builder, split):
     """List steps replayed from captured graphs (two graphs, alternating
     state buffers, interleaved with eager re-sort steps, list rebuilds and
     host synchronisations between calls) give the same trajectory, bit for
     bit, as eager launches."""
+    monkeypatch.setenv("MDX_SPLIT", split)
     s = builder()
     say("graph vs eager", s["name"], s["N"])
     outs = []
@@ -423,6 +430,7 @@ def test_fused_integrator_bitwise_equal_separate(monkeypatch, builder, precision
     candidates) and the patch kernel (the others) give the trajectory of the
     separate integrator kernel bit for bit, with the same list rebuilds (the
     drift bound is reduced over both kernels' blocks)."""
+    monkeypatch.setenv("MDX_SPLIT", "1")
     s = builder()
     say("fused vs separate", s["name"], s["N"], precision)
     outs = []
@@ -499,7 +507,8 @@ def test_clusters_large_and_replica_vs_oracle():
             assert np.array_equal(lab, po.dgraph_labels([Lr] * 3, reps[r]["x"], nr, 1.3, 10, 40))
 
 
-def test_autotune_tries_every_candidate_and_keeps_the_fastest():
+def test_autotune_tries_every_candidate_and_keeps_the_fastest(monkeypatch):
+    monkeypatch.setenv("MDX_SPLIT", "1")
     s = wl.condensate_ab(1 << 16, phi=0.1)
     with engine_for(s, mdx.PREC_MIXED, seed=2) as e:
         e.prime()
