@@ -49,3 +49,57 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(mdx, "_lib", None)
     with pytest.raises(RuntimeError, match="no CPU fallback"):
         mdx.lib()
+
+
+def _desc(n, box=10.0, dt=0.005, precision=None, integrator=None, kT=1.0, nrep=None):
+    nn = (ctypes.c_int * len(n))(*n)
+    bx = (ctypes.c_double * (3 * len(n)))(*([box] * 3 * len(n)))
+    kt = (ctypes.c_double * len(n))(*([kT] * len(n)))
+    d = mdx.Desc()
+    d.nreplicas = len(n) if nrep is None else nrep
+    d.n_per_replica = ctypes.cast(nn, ctypes.POINTER(ctypes.c_int))
+    d.box = ctypes.cast(bx, ctypes.POINTER(ctypes.c_double))
+    d.periodic = 1
+    d.kT = ctypes.cast(kt, ctypes.POINTER(ctypes.c_double))
+    d.dt, d.m, d.gamma, d.gammar = dt, 1.0, 1.0, 1.0
+    for k in range(3):
+        d.inertia[k] = 0.1
+    d.precision = mdx.PREC_FP64 if precision is None else precision
+    d.integrator = mdx.INTEG_ROTOR if integrator is None else integrator
+    return d, (nn, bx, kt)
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(n=[0]), b"no particles"),
+    (dict(n=[10, 0]), b"replica 1 has no particles"),
+    (dict(n=[10], nrep=0), b"nreplicas"),
+    (dict(n=[10], box=0.0), b"box edge"),
+    (dict(n=[10], box=-2.0), b"box edge"),
+    (dict(n=[10], dt=0.0), b"dt"),
+    (dict(n=[10], kT=-1.0), b"kT"),
+    (dict(n=[10], precision=7), b"precision"),
+    (dict(n=[10], integrator=9), b"integrator"),
+])
+def test_descriptor_errors_before_any_device_call(kw, msg):
+    """Empty systems and replicas, non-positive boxes and steps, unknown
+    modes: rejected with MDX_ERR_ARG and a message, validated before the
+    device is touched (so this runs without a GPU)"""
+    if not os.path.exists(mdx.LIB_PATH):
+        pytest.skip("libmdx.so not built")
+    L = mdx.lib()
+    d, keep = _desc(**kw)
+    h = ctypes.c_void_p()
+    rc = L.mdx_create(ctypes.byref(d), ctypes.byref(h))
+    assert rc == mdx.MDX_ERR_ARG
+    assert msg in L.mdx_last_error()
+    assert not h.value
+
+
+def test_null_engine_is_an_argument_error():
+    if not os.path.exists(mdx.LIB_PATH):
+        pytest.skip("libmdx.so not built")
+    L = mdx.lib()
+    assert L.mdx_step(None, 1, None) == mdx.MDX_ERR_ARG
+    assert L.mdx_prime(None, None) == mdx.MDX_ERR_ARG
+    assert L.mdx_download_state(None, None, None, None, None) == mdx.MDX_ERR_ARG
+    assert L.mdx_upload_state(None, None, None, None, None) == mdx.MDX_ERR_ARG

@@ -183,6 +183,70 @@ def test_bead_trajectory_fp64_vs_reference(golden):
     assert relmax(st["p"], g["p_out"]) < 1e-9
 
 
+def _first(s, k):
+    """the first k particles of a rotor system (same box and parameters)"""
+    t = dict(s)
+    t["N"] = k
+    for key in ("x", "p", "Q", "L"):
+        t[key] = np.array(s[key][:k])
+    t["patches"] = dict(s["patches"])
+    t["patches"]["types"] = np.array(s["patches"]["types"][:k])
+    return t
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_one_and_two_particle_systems_vs_oracle(k):
+    """Edge sizes: a lone particle (no pairs; friction, noise and rotation
+    only) and an interacting pair, 30 FP64 steps with injected noise; a call
+    of zero steps changes nothing"""
+    s = _first(wl.patchy_tetra(64, phi=0.05), k)
+    if k == 2:
+        # in WCA range, with particle 1 on particle 0's first patch axis
+        n0 = s["Q"][0].reshape(3, 3).T @ wl.TETRA_DIRS[0]
+        s["x"][1] = (s["x"][0] + 1.05 * n0) % s["box"]
+    nsteps, seed = 30, 11
+    say("edge size", k)
+    noise = po.noise(seed, 0, (nsteps + 1) * k * 6).reshape(nsteps + 1, k, 6)
+    with engine_for(s) as e:
+        e.prime(noise[0])
+        before = e.download()
+        e.step(0)
+        same = e.download()
+        e.step(nsteps, noise[1:])
+        st = e.download()
+    for key in ("x", "p", "Q", "L"):
+        assert np.array_equal(before[key], same[key]), key
+    ref = po.run_rotors(s, nsteps, seed, 0, list_cut=3.5, rebuild_every=20)
+    for key in ("x", "p", "Q", "L"):
+        err = relmax(st[key], ref[key])
+        say(key, err)
+        assert err < 1e-10, key
+
+
+def test_ragged_batch_with_tiny_replicas_matches_independent_runs():
+    """A batch whose replicas hold 1, 2, 700 and 3 particles (ragged, with
+    replicas smaller than a cell stencil) equals the replicas run alone"""
+    base = [wl.patchy_tetra(64, phi=0.05, seed_pos=40 + r) for r in range(4)]
+    reps = [_first(base[0], 1), _first(base[1], 2), wl.patchy_tetra(700), _first(base[3], 3)]
+    nsteps = 15
+    Ntot = sum(r["N"] for r in reps)
+    noise = po.noise(6, 0, (nsteps + 1) * Ntot * 6).reshape(nsteps + 1, Ntot, 6)
+    offs = np.concatenate([[0], np.cumsum([r["N"] for r in reps])])
+    say("ragged batch", [r["N"] for r in reps])
+    with mdx.engine_from_system(None, replicas=reps) as e:
+        e.prime(noise[0])
+        e.step(nsteps, noise[1:])
+        batch = e.download()
+    for r, s in enumerate(reps):
+        sl = slice(offs[r], offs[r + 1])
+        with engine_for(s) as e:
+            e.prime(noise[0, sl])
+            e.step(nsteps, noise[1:, sl])
+            one = e.download()
+        for key in ("x", "p", "Q", "L"):
+            assert relmax(batch[key][sl], one[key]) < 1e-12, (r, key)
+
+
 # ------------------------------------------------------------ mixed precision
 def f32_rounded(s):
     s = dict(s)
