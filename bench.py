@@ -7,8 +7,10 @@ Workload (default): the 1,048,576-particle two-species patchy condensate
 (packing fraction 0.05, A: 3 patches, B: 2, A-B eps_p=8 + WCA, kT=1,
 dt=0.005) in mixed precision.  One step = one Langevin step of every particle
 (binning, 27-cell pair loop with WCA + patches + torques, rotor update), state
-resident in HBM.  Prints one JSON line.  Also reported: the plain FP64 GPU
-form on the same system, the mixed-vs-FP64 force error on the same frozen
+resident in HBM.  Prints one JSON line.  Also reported: the same engine in
+FP64 (fp64_gpu_value), the plain FP64 GPU form (plain_fp64_gpu_value: one
+kernel per step walking the 27 cells of every particle, no neighbour list),
+the mixed-vs-FP64 force error on the same frozen
 configuration, the step's HBM roofline fraction (kernel times from HIP events
 on the engine's stream; measured HBM traffic per step from the rocprofv3
 counter summary in profiles/ when present), and the reference's own CPU path
@@ -135,6 +137,29 @@ def time_engine(s, precision, steps, warmup, seed=1, reps=None, autotune=False):
         prof = e.profile(min(steps, 20))
         info = e.info()
     return (t1 - t0), prof, info
+
+
+def time_plain_fp64(s, steps, warmup, seed=1, reps=None):
+    """the plain FP64 GPU form of the step: every step bins the particles and
+    walks the 27 cells in one kernel (walk, patches and integrator together),
+    no neighbour list, no walk / patch / integrator split, FP64 throughout"""
+    old = os.environ.get("MDX_SPLIT")
+    os.environ["MDX_SPLIT"] = "0"
+    try:
+        with mdx.engine_from_system(s, precision=mdx.PREC_FP64, seed=seed, replicas=reps) as e:
+            e.set_neighbor_list(0.0)
+            if s["integrator"] == "rotor":
+                e.prime()
+            e.step(warmup)
+            t0 = time.perf_counter()
+            e.step(steps)
+            t1 = time.perf_counter()
+    finally:
+        if old is None:
+            del os.environ["MDX_SPLIT"]
+        else:
+            os.environ["MDX_SPLIT"] = old
+    return t1 - t0
 
 
 def cpu_baseline(s, nsteps, threads):
@@ -279,6 +304,10 @@ def main():
         out["fp64_gpu_value"] = v64
         out["mixed_over_fp64"] = value / v64
         out["fp64_kernel_ms"] = prof64
+        tp = time_plain_fp64(s, fsteps, 2, reps=reps)
+        log("plain fp64 ms/step", 1e3 * tp / fsteps)
+        out["plain_fp64_gpu_value"] = N * fsteps / tp
+        out["mixed_over_plain_fp64"] = value * tp / (N * fsteps)
         out.update(force_error(s, reps))
         log("force error", {k: out[k] for k in out if "err" in k})
     if not args.no_cpu and args.workload in ("condensate_ab", "patchy_tetra"):
