@@ -62,3 +62,42 @@ def test_long_run_statistics_match_reference(golden, precision):
     z = zscore(a, b)[sel]
     say("g(r) bins", int(sel.sum()), "max z", float(z.max()), "frac z<3", float((z < 3).mean()))
     assert (z < 3).mean() > 0.95 and z.max() < 5.5
+
+
+def test_condensate_mixed_statistics_match_fp64():
+    """The two-species condensate (262,144 particles, packing fraction 0.05,
+    A-B attraction eps_p = 8) from the same start with the same Philox noise
+    in mixed precision and in FP64 for 10000 steps: the trajectories separate
+    (chaos), the ensemble does not -- g(r) (calcg1 counts) and patch-bond
+    counts over steps 5000-10000 agree within their frame-to-frame spread"""
+    s = wl.condensate_ab(1 << 18, phi=0.05)
+    runs = {}
+    for prec in (mdx.PREC_FP64, mdx.PREC_MIXED):
+        say("condensate statistics", prec, s["N"])
+        hist, bonds = [], []
+        with mdx.engine_from_system(s, precision=prec, seed=77) as e:
+            e.prime()
+            e.step(5000)
+            for f in range(5):
+                e.step(1000)
+                hist.append(e.pair_histogram(3.0, 0.1).astype(np.float64))
+                bonds.append(e.patch_bonds().astype(np.float64))
+        runs[prec] = (np.array(hist), np.array(bonds))
+    (h64, b64), (h32, b32) = runs[mdx.PREC_FP64], runs[mdx.PREC_MIXED]
+    # patch bonds per particle: mean and the 0..3 histogram
+    m64, m32 = b64.mean(1), b32.mean(1)
+    spread = np.sqrt(m64.var(ddof=1) + m32.var(ddof=1) + 1e-12)
+    say("bonds per particle", m64.mean(), m32.mean(), "spread", spread)
+    assert abs(m64.mean() - m32.mean()) < max(4 * spread, 0.01 * m64.mean())
+    for k in range(4):
+        f64 = (b64 == k).mean(1).mean()
+        f32 = (b32 == k).mean(1).mean()
+        say("bond fraction", k, f64, f32)
+        assert abs(f64 - f32) < 0.01 + 0.03 * f64
+    # g(r) counts in the bins with data, relative to the frame-averaged FP64 counts
+    g64, g32 = h64.mean(0), h32.mean(0)
+    sel = g64 > 1000
+    sd = np.sqrt(h64[:, sel].var(0, ddof=1) + h32[:, sel].var(0, ddof=1)) / np.sqrt(5)
+    z = np.abs(g64[sel] - g32[sel]) / np.maximum(sd, np.sqrt(g64[sel]))
+    say("g(r) bins", int(sel.sum()), "max z", float(z.max()), "median z", float(np.median(z)))
+    assert np.median(z) < 2 and z.max() < 6
