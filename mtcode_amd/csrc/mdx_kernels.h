@@ -1821,7 +1821,7 @@ __global__ void __launch_bounds__(256) k_scan1(const int *count, int *start, int
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int off = 1; off < 256; off <<= 1) {
-        int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+        int t = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
         __syncthreads();
         sh[threadIdx.x] += t;
         __syncthreads();
@@ -1850,7 +1850,7 @@ __global__ void __launch_bounds__(256) k_scan3(int *count, int *start, const int
     sh[threadIdx.x] = part;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o)
+        if ((int)threadIdx.x < o)
             sh[threadIdx.x] += sh[threadIdx.x + o];
         __syncthreads();
     }
@@ -2403,7 +2403,7 @@ __global__ void __launch_bounds__(256) k_obs(int first, int count, const R *p, c
         sh[threadIdx.x] = acc[k];
         __syncthreads();
         for (int off = 128; off > 0; off >>= 1) {
-            if (threadIdx.x < off)
+            if ((int)threadIdx.x < off)
                 sh[threadIdx.x] += sh[threadIdx.x + off];
             __syncthreads();
         }
