@@ -318,8 +318,8 @@ template <class R> static KArgs<R> make_args(mdx_engine *e)
         a.rstart[k] = k < (int)e->rstart.size() ? e->rstart[k] : 0;
     a.active_mask = e->active_mask;
     a.pdirs = (const double *)e->pdirs_d.p;
-    // the walk's own-patch screen reads the directions in R
-    a.pdirs_r = sizeof(R) == 8 ? (const R *)e->pdirs_d.p : (const R *)e->pdirs_f.p;
+    // the own-patch screen reads the directions in FP32 (both precisions)
+    a.pdirs_f = (const float *)e->pdirs_f.p;
     a.wca_pl2lo = e->wca_pl * e->wca_pl * (1 - 1e-12);
     a.prows = (const PatchRow *)e->prows.p;
     a.maxd2 = e->maxd2;
@@ -327,6 +327,7 @@ template <class R> static KArgs<R> make_args(mdx_engine *e)
                        a.has_patches ? e->maxd2 : 0.0);
     // the screen of the cone test runs in R precision: FP32 needs a margin
     a.costm_screen = e->costm_min - (e->prec == MDX_PREC_FP64 ? 1e-12 : 1e-5);
+    a.costm_screen_f = e->costm_min - 1e-5;
     if (e->nbonds) {
         a.bond_off = (const int *)e->bond_off.p;
         a.bond_j = (const int *)e->bond_j.p;

@@ -98,9 +98,10 @@ template <class R> struct KArgs {
     int rstart[MAXT * MAXT];
     unsigned long long active_mask; // bit ta * MAXT + tc: type pair with a nonzero row
     const double *pdirs;
-    const R *pdirs_r; // the same directions in R
+    const float *pdirs_f; // the same directions in FP32 (the own-patch screen)
     const PatchRow *prows;
     double maxd2, rmax2, costm_screen;
+    double costm_screen_f; // the own-patch screen's threshold (FP32 margin)
     // topology (indexed by original index)
     const int *bond_off, *bond_j;
     const double *bond_k, *bond_r0;
@@ -625,36 +626,33 @@ __device__ __forceinline__ void wca_term(const KArgs<R> &a, const double d[3], d
 }
 
 // The cone screen of patch_pair on the particle's own patches alone (no
-// load of the partner): false when none of its patches passes, in which case
-// patch_pair adds exactly nothing for the pair.  Evaluated in R with a
-// reciprocal square root: its error (~1e-7 in FP32, ~1e-16 in FP64) is far
-// inside the screen's margin below cos(theta_m) (1e-5 FP32, 1e-12 FP64), so a
-// pair it rejects fails patch_pair's exact FP64 cone test as well.  The walk
-// uses it to hand over only candidates that can bind.
+// load of the partner), in FP32 for both precisions: false when none of its
+// patches passes, in which case patch_pair adds exactly nothing for the pair.
+// Its error (~1e-7 with the orientation rounded to FP32 and a reciprocal
+// square root) is far inside the screen's margin below cos(theta_m) (1e-5),
+// so a pair it rejects fails patch_pair's exact FP64 cone test as well.  The
+// walk uses it to hand over only candidates that can bind; every walk form
+// and the overflow path use this same screen, so they agree on the candidates.
 template <class R>
-__device__ __forceinline__ bool own_screen(const KArgs<R> &a, uint32_t oi, int ti, const R *qi,
+__device__ __forceinline__ bool own_screen(const KArgs<R> &a, uint32_t oi, int ti, const float *qi,
                                            const PosRec &xj, const double d[3], double r2s)
 {
     const int tj = xj.meta & 255;
     int ta, tc;
     const bool i_is_a = patch_roles(oi, ti, xj.id, tj, ta, tc);
     // u = (d / r), sign folded into the test below
-    R ri;
-    if constexpr (sizeof(R) == 8)
-        ri = (R)1 / sqrt((R)r2s);
-    else
-        ri = rsqrtf((float)r2s);
-    const R ur[3] = {(R)d[0] * ri, (R)d[1] * ri, (R)d[2] * ri};
-    const R v[3] = {qi[0] * ur[0] + qi[1] * ur[1] + qi[2] * ur[2],
-                    qi[3] * ur[0] + qi[4] * ur[1] + qi[5] * ur[2],
-                    qi[6] * ur[0] + qi[7] * ur[1] + qi[8] * ur[2]};
+    const float ri = rsqrtf((float)r2s);
+    const float ur[3] = {(float)d[0] * ri, (float)d[1] * ri, (float)d[2] * ri};
+    const float v[3] = {qi[0] * ur[0] + qi[1] * ur[1] + qi[2] * ur[2],
+                        qi[3] * ur[0] + qi[4] * ur[1] + qi[5] * ur[2],
+                        qi[6] * ur[0] + qi[7] * ur[1] + qi[8] * ur[2]};
     const int t = i_is_a ? ta : tc;
     const int np = a.np[t];
-    const R *dd = a.pdirs_r + 3 * a.pstart[t];
-    const R scr = (R)a.costm_screen;
+    const float *dd = a.pdirs_f + 3 * a.pstart[t];
+    const float scr = (float)a.costm_screen_f;
     bool pass = false;
     for (int k = 0; k < np; k++) {
-        const R sd = dd[3 * k] * v[0] + dd[3 * k + 1] * v[1] + dd[3 * k + 2] * v[2];
+        const float sd = dd[3 * k] * v[0] + dd[3 * k + 1] * v[1] + dd[3 * k + 2] * v[2];
         // u = +d/r for role a, whose test is -(n . u); u = -d/r for role c,
         // whose test is n . u: both are -(n . d/r)
         pass |= -sd > scr;
@@ -692,13 +690,16 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
         acc.vir[k] = 0;
 
     R qi[9];
+    float qs[9]; // qi in FP32, for the own-patch screen
     // the bead integrator carries no orientations: no patch code in its kernels
     constexpr bool INTG = PH == PH_INTEGRATE || PH == PH_INTEGRATE_F;
     const int npi = (MODE != M_BEAD && !INTG && a.has_patches) ? a.np[ti] : 0;
     if (MODE != M_BEAD && a.rotor) {
 #pragma unroll
-        for (int k = 0; k < 9; k++)
+        for (int k = 0; k < 9; k++) {
             qi[k] = a.Q[9 * (size_t)i + k];
+            qs[k] = (float)qi[k];
+        }
     }
     int exc0 = 0, exc1 = 0;
     if (needs_exclusions(a, i)) {
@@ -721,7 +722,7 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
     // the candidates the split form hands over: those that pass the screen
     auto is_listed_candidate = [&](const PosRec &xj, int js, const double d[3], double r2s) -> bool {
         (void)js;
-        return is_patch_candidate(xj, r2s) && own_screen(a, oi, ti, qi, xj, d, r2s);
+        return is_patch_candidate(xj, r2s) && own_screen(a, oi, ti, qs, xj, d, r2s);
     };
     if (PH == PH_FINISH) {
         for (int k = 0; k < 3; k++)
@@ -1092,7 +1093,7 @@ __global__ void __launch_bounds__(256, (sizeof(R) == 4 && !INT) ? 6 : 1) k_walkc
 {
     __shared__ int off[257];
     __shared__ PosRec sx[256];
-    __shared__ R sq[256 * 9];
+    __shared__ float sq[256 * 9]; // orientations in FP32 for the own-patch screen
     __shared__ int sexc[256 * 2];
     __shared__ R pf[256 * 3];
     __shared__ int pj[256];
@@ -1112,7 +1113,7 @@ __global__ void __launch_bounds__(256, (sizeof(R) == 4 && !INT) ? 6 : 1) k_walkc
         if (a.has_patches && a.np[xi.meta & 255] > 0) {
 #pragma unroll
             for (int k = 0; k < 9; k++)
-                sq[9 * tid + k] = a.Q[9 * (size_t)s + k];
+                sq[9 * tid + k] = (float)a.Q[9 * (size_t)s + k];
         }
         const bool ne = needs_exclusions(a, s);
         sexc[2 * tid] = ne ? a.exc_off[xi.id] : 0;
@@ -1139,7 +1140,7 @@ __global__ void __launch_bounds__(256, (sizeof(R) == 4 && !INT) ? 6 : 1) k_walkc
 
     // WCA of a pair into acc; true for a patch candidate that passes the
     // own-patch screen (the one-lane walk's rule)
-    auto visit = [&](const PosRec &xo, const R *qo, int e0, int e1, const PosRec &xj,
+    auto visit = [&](const PosRec &xo, const float *qo, int e0, int e1, const PosRec &xj,
                      Acc<R> &acc) -> bool {
         const DevRep rp = rep_rec(a, xo.meta);
         double d[3];
@@ -1292,8 +1293,11 @@ __device__ __forceinline__ void patch_overflow(const KArgs<R> &a, int so)
     const int ti = xi.meta & 255;
     const DevRep rp = rep_rec(a, xi.meta);
     R qi[9];
-    for (int k2 = 0; k2 < 9; k2++)
+    float qs[9]; // qi in FP32, for the own-patch screen
+    for (int k2 = 0; k2 < 9; k2++) {
         qi[k2] = a.Q[9 * (size_t)i + k2];
+        qs[k2] = (float)qi[k2];
+    }
     Acc<R> acc;
     for (int k2 = 0; k2 < 3; k2++) {
         acc.F[k2] = a.Fw[3 * (size_t)so + k2];
@@ -1314,7 +1318,7 @@ __device__ __forceinline__ void patch_overflow(const KArgs<R> &a, int so)
         patch_roles(xi.id, ti, xj.id, tj, ta, tc);
         if (!((a.active_mask >> (ta * MAXT + tc)) & 1ull))
             return;
-        if (MDX_WALK_SCREEN && !own_screen(a, xi.id, ti, qi, xj, d, r2s))
+        if (MDX_WALK_SCREEN && !own_screen(a, xi.id, ti, qs, xj, d, r2s))
             return;
         const int js = j >= 0 ? j : a.slot[xj.id];
         if (seen++ >= PMAXG)
