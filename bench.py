@@ -85,19 +85,24 @@ WORKLOADS = {
 
 
 def build_workload(name):
-    """(system dict or None, list of replica dicts or None)"""
-    if name == "condensate_ab":
-        return wl.condensate_ab(), None
-    if name == "patchy_tetra":
-        return wl.patchy_tetra(), None
+    """(system dict or None, list of replica dicts or None), with momenta
+    drawn at each system's temperature, so that the timed steps run at the
+    steady temperature (and list-rebuild rate) from the start"""
     if name == "replicas":
-        reps = wl.replicas(progress=log)
+        reps = [wl.thermal_momenta(r, seed=99 + k) for k, r in
+                enumerate(wl.replicas(progress=log))]
         return reps[0], reps
-    if name == "mt_bundle":
-        return wl.mt_bundle(), None
-    if name == "active_polymer":
-        return wl.active_polymer(progress=log), None
-    raise SystemExit(f"unknown workload {name}")
+    if name == "condensate_ab":
+        s = wl.condensate_ab()
+    elif name == "patchy_tetra":
+        s = wl.patchy_tetra()
+    elif name == "mt_bundle":
+        s = wl.mt_bundle()
+    elif name == "active_polymer":
+        s = wl.active_polymer(progress=log)
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return wl.thermal_momenta(s), None
 
 
 def force_error(s, reps=None):
@@ -270,7 +275,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 (f64 positions and decisions)",
-        "data": "synthetic (seeded uniform positions with overlap rejection, Arvo random rotations)",
+        "data": ("synthetic (seeded uniform positions with overlap rejection, Arvo random rotations, "
+                 "Maxwell-Boltzmann momenta at kT)"),
         "config": {
             "workload": f"{args.workload}: N={N}, " + WORKLOADS[args.workload],
             "precision": "mixed",

@@ -171,6 +171,21 @@ def _base(name: str, n: int, box, dt: float, kT: float, gamma: float, gammar: fl
     }
 
 
+def thermal_momenta(s: dict, seed: int = 99) -> dict:
+    """A copy of system s with momenta drawn from the Maxwell-Boltzmann
+    distribution at its kT: p ~ N(0, m kT) per component, and for rotors the
+    body-frame angular momenta L ~ N(0, I_k kT).  The generators start from
+    rest; a run from these momenta is at its steady temperature from the first
+    step (the friction would otherwise take about 1 / gamma to heat it)."""
+    t = dict(s)
+    rng = np.random.default_rng(seed)
+    n = s["N"]
+    t["p"] = rng.normal(0.0, np.sqrt(s["m"] * s["kT"]), (n, 3))
+    if s["integrator"] == "rotor":
+        t["L"] = rng.normal(0.0, 1.0, (n, 3)) * np.sqrt(np.asarray(s["I"]) * s["kT"])
+    return t
+
+
 def patchy_tetra(n: int = 4096, phi: float = 0.1, seed_pos: int = 1, seed_q: int = 2,
                  eps_p: float = 6.0, kT: float = 1.0) -> dict:
     """N patchy particles, 4 tetrahedral patches, cubic periodic box at packing
