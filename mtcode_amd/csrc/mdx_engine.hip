@@ -142,8 +142,7 @@ struct mdx_engine {
     bool list_eager = false; // an eager list step ran since the last list build
     GraphSlot graphs[2];     // per current state buffer
     Buf nbr, nbrc, nn, bcell; // neighbour list (column-major, per-block compact copy)
-    Buf bmax, bmax2;     // per-block drift maxima (walk / patch kernel grids)
-    int drift_nb2 = 0;   // patch-kernel blocks that wrote bmax2 in the last gather
+    Buf dslot;           // drift maxima slots (NDSLOT), cleared by k_decide
     hipEvent_t prof_mid = nullptr;  // recorded between walk and finish while profiling
     hipEvent_t prof_mid2 = nullptr; // recorded between patch and integrate while profiling
     bool last_split = false;       // the last gather launch used the split form
@@ -391,8 +390,7 @@ template <class R> static KArgs<R> make_args(mdx_engine *e)
     const double rw = e->wca ? e->wca_pl * (1 + 1e-12) + e->skin : 0.0;
     a.rl2w = all_active ? a.rl2 : (e->wca ? std::min(rw * rw, a.rl2) : 0.0);
     a.skin = e->skin;
-    a.bmax = (float *)e->bmax.p;
-    a.bmax2 = (float *)e->bmax2.p;
+    a.dslot = (unsigned int *)e->dslot.p;
     return a;
 }
 
@@ -477,7 +475,6 @@ static int launch_gather(mdx_engine *e, const double *noise_dev, int lmode = 0)
                              (e->use_fuse == 2 && (!a.has_patches || 4LL * e->n_patchy <= e->N));
         const bool fuse = lmode && e->use_walkc && fuse_ok && e->use_compact &&
                           (MODE == M_FUSED || MODE == M_STEP || MODE == M_BEAD);
-        e->drift_nb2 = 0;
         if (fuse) {
             hipLaunchKernelGGL((k_walkc<R, MODE, true>), dim3(nblk(e->N)), dim3(256), 0, e->stream,
                                a);
@@ -487,7 +484,6 @@ static int launch_gather(mdx_engine *e, const double *noise_dev, int lmode = 0)
             if (!bead_list) {
                 hipLaunchKernelGGL((k_patch<R, MODE, true>), dim3(nblk(e->N, pblk)), dim3(256), 0,
                                    e->stream, a, pblk);
-                e->drift_nb2 = (int)nblk(e->N, pblk);
             }
             if (e->prof_mid2)
                 HIPCHK(hipEventRecord(e->prof_mid2, e->stream));
@@ -604,19 +600,17 @@ static int nbr_alloc(mdx_engine *e)
     if ((rc = dalloc(e, e->nbr, 4 * (size_t)e->kmax * e->N)) ||
         (rc = dalloc(e, e->nbrc, 4 * (size_t)e->kmax * nblk(e->N) * 256)) ||
         (rc = dalloc(e, e->nn, 4 * (size_t)e->N)) || (rc = dalloc(e, e->bcell, 4 * (size_t)e->N)) ||
-        (rc = dalloc(e, e->bmax, 4 * (size_t)nblk(e->N))) ||
-        (rc = dalloc(e, e->bmax2, 4 * (size_t)nblk(e->N))))
+        (rc = dalloc(e, e->dslot, 4 * (size_t)NDSLOT)))
         return rc;
     return MDX_OK;
 }
 
 // after a kernel that moved the particles in list mode (its blocks wrote
-// bmax); epoch_inc: also count the step's force evaluation
+// drift slots); epoch_inc: also count the step's force evaluation
 static int decide(mdx_engine *e, int epoch_inc)
 {
-    hipLaunchKernelGGL(k_decide, dim3(1), dim3(1024), 0, e->stream, (StepCtl *)e->ctl.p,
-                       (const float *)e->bmax.p, (int)nblk(e->N), (const float *)e->bmax2.p,
-                       e->drift_nb2, e->skin, epoch_inc);
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(NDSLOT), 0, e->stream, (StepCtl *)e->ctl.p,
+                       (unsigned int *)e->dslot.p, e->skin, epoch_inc);
     HIPCHK(hipGetLastError());
     return MDX_OK;
 }
@@ -811,7 +805,6 @@ template <class R> static int enqueue_rotor(mdx_engine *e, int nsteps, const dou
             HIPCHK(hipEventRecord(kev[0], e->stream));
         hipLaunchKernelGGL(k_kick_drift<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
         HIPCHK(hipGetLastError());
-        e->drift_nb2 = 0; // only the kick-drift grid wrote drift maxima
         if (a.lmode && (rc = decide(e, 0)))
             return rc;
         if (kev)
@@ -1056,7 +1049,7 @@ void mdx_destroy(mdx_engine *e)
                    &e->act_next, &e->noise,  &e->ctl,    &e->partial, &e->tmpd,   &e->tmpi,
                    &e->tmpi2,   &e->tmpi3,   &e->tmpll,  &e->tmpx,    &e->Fw,
                    &e->ncand,   &e->cand,    &e->nbr,    &e->nbrc,   &e->nn,     &e->bcell,
-                   &e->bmax,    &e->Tp,      &e->bmax2};
+                   &e->dslot,   &e->Tp};
     for (Buf *b : bufs)
         dfree(*b);
     for (GraphSlot &g : e->graphs)

@@ -86,8 +86,9 @@ template <class R> struct KArgs {
     // list radii squared: interaction reach + skin; WCA reach + skin for the
     // pairs that cannot interact through patches
     double rl2, rl2w, skin;
-    float *bmax;  // per block: max drift of the block's particles
-    float *bmax2; // the same for the blocks of the patch kernel (fused form)
+    // drift maxima of the moving kernels' blocks (float bits, atomic max into
+    // slot block % NDSLOT); k_decide reduces and clears them
+    unsigned int *dslot;
     // WCA (HSPotential)
     int wca;
     double wca_eps, wca_s6, wca_s12, wca_pl, wca_pl2, wca_pl2lo;
@@ -430,7 +431,10 @@ __device__ __forceinline__ void for_each_near(const KArgs<R> &a, const DevRep &r
         for_each_neighbor(a, rp, i, fn);
 }
 
-// block max of the drift into bmax[block]; k_decide reduces them
+// block max of the drift into one of NDSLOT slots (atomic max on the float
+// bits, which order like the values for drifts >= 0; a non-finite drift
+// counts as infinite, so it forces a rebuild); k_decide reduces the slots
+constexpr int NDSLOT = 64;
 template <class R> __device__ __forceinline__ void drift_bound(const KArgs<R> &a, float d)
 {
     __shared__ float wmax[4];
@@ -443,7 +447,9 @@ template <class R> __device__ __forceinline__ void drift_bound(const KArgs<R> &a
         float m = 0.f;
         for (int w = 0; w < (int)(blockDim.x >> 6); w++)
             m = fmaxf(m, wmax[w]);
-        a.bmax[blockIdx.x] = m;
+        if (!(m <= 3.0e38f))
+            m = __int_as_float(0x7f800000);
+        atomicMax(&a.dslot[blockIdx.x & (NDSLOT - 1)], __float_as_uint(m));
     }
 }
 
@@ -1342,7 +1348,7 @@ __device__ __forceinline__ void patch_overflow(const KArgs<R> &a, int so)
 // integrator reads ncand).
 constexpr int PBLK = 2048; // largest block of positions (pblk: 256 .. PBLK, a multiple of 256)
 // INT: the positions with candidates are then integrated here (the walk
-// integrated the others); block drift maxima go to bmax2
+// integrated the others)
 template <class R, int MODE, bool INT>
 __global__ void __launch_bounds__(256) k_patch(KArgs<R> a, int pblk)
 {
@@ -1475,11 +1481,8 @@ __global__ void __launch_bounds__(256) k_patch(KArgs<R> a, int pblk)
                 [&](int i, const DevRep &rp, auto &&fn) { for_each_near(a, rp, i, fn); }, d);
             drift = fmaxf(drift, d);
         }
-        if (MODE == M_FUSED || MODE == M_BEAD) {
-            KArgs<R> b = a;
-            b.bmax = a.bmax2;
-            drift_bound(b, drift);
-        }
+        if (MODE == M_FUSED || MODE == M_BEAD)
+            drift_bound(a, drift);
     }
 }
 
@@ -1924,37 +1927,15 @@ __global__ void k_list_reset(StepCtl *ctl)
 // the grid max of the block drifts of the kernel before is added to the
 // displacement bound D; a list rebuild is requested once 2 D reaches the skin
 // (a pair within reach now was within reach + skin at the build)
-__global__ void __launch_bounds__(1024) k_decide(StepCtl *ctl, const float *bmax, int nb,
-                                                 const float *bmax2, int nb2, double skin,
-                                                 int epoch_inc)
+__global__ void __launch_bounds__(64) k_decide(StepCtl *ctl, unsigned int *dslot, double skin,
+                                               int epoch_inc)
 {
-    __shared__ float wmax[16];
-    // the first 4 x 1024 entries of each array as independent loads (their
-    // latencies overlap), then the rest
-    float v[8];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const int b = threadIdx.x + 1024 * k;
-        v[k] = b < nb ? bmax[b] : 0.f;
-        v[4 + k] = b < nb2 ? bmax2[b] : 0.f;
-    }
-    float m = 0.f;
-#pragma unroll
-    for (int k = 0; k < 8; k++)
-        m = fmaxf(m, v[k]);
-    for (int b = threadIdx.x + 4096; b < nb; b += 1024)
-        m = fmaxf(m, bmax[b]);
-    for (int b = threadIdx.x + 4096; b < nb2; b += 1024)
-        m = fmaxf(m, bmax2[b]);
+    // one wave: each lane takes and clears one slot
+    float m = __uint_as_float(dslot[threadIdx.x]);
+    dslot[threadIdx.x] = 0u;
     for (int o = 32; o > 0; o >>= 1)
         m = fmaxf(m, __shfl_xor(m, o));
-    if ((threadIdx.x & 63) == 0)
-        wmax[threadIdx.x >> 6] = m;
-    __syncthreads();
     if (threadIdx.x == 0) {
-        m = 0.f;
-        for (int w = 0; w < 16; w++)
-            m = fmaxf(m, wmax[w]);
         // margin for the rounding of the drift and of the FP64 position update
         const double D = ctl->D + (double)m * (1 + 1e-6) + 1e-9;
         ctl->D = D;
