@@ -44,6 +44,9 @@ struct StepCtl {
     unsigned int rebuild; // 1: the next step rebuilds the list before its walk
     unsigned int nbuilds; // list builds so far
     double D;             // sum over steps of the max drift since the last build
+    // a walk queued more than PMAXG patch candidates for some position (the
+    // overflow kernel's gate; cleared by the integrator kernel after it)
+    unsigned int ovf;
 };
 
 enum { M_CONS = 0, M_PRIME = 1, M_STEP = 2, M_FUSED = 3, M_BEAD = 4, M_ENERGY = 5 };
@@ -793,6 +796,8 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
             for (int k = 0; k < 3; k++)
                 a.Fw[3 * (size_t)s + k] = acc.F[k];
             a.ncand[s] = npc;
+            if (npc > PMAXG)
+                atomicOr(&a.ctl->ovf, 1u);
             return;
         }
         // -------- patches (LangevinR.cpp:911-1047), in neighbour-loop order
@@ -1239,6 +1244,8 @@ __global__ void __launch_bounds__(256, (sizeof(R) == 4 && !INT) ? 6 : 1) k_walkc
                 F[k] = pa.F[k];
         }
         a.ncand[s] = npc;
+        if (npc > PMAXG)
+            atomicOr(&a.ctl->ovf, 1u);
         if (!(INT && npc == 0))
             for (int k = 0; k < 3; k++)
                 a.Fw[3 * (size_t)s + k] = F[k];
@@ -1487,9 +1494,12 @@ __global__ void __launch_bounds__(256) k_patch(KArgs<R> a, int pblk)
 }
 
 // the candidates beyond PMAXG of the separate form's positions (after k_patch;
-// grid-stride over a small grid: almost every step has none)
+// grid-stride over a small grid; almost every step has none, and then the
+// walk left the gate closed)
 template <class R, int MODE> __global__ void __launch_bounds__(256) k_patch_ovf(KArgs<R> a)
 {
+    if (!a.ctl->ovf)
+        return;
     for (int so = blockIdx.x * blockDim.x + threadIdx.x; so < a.N; so += gridDim.x * blockDim.x)
         if (a.ncand[so] > PMAXG)
             patch_overflow<R, MODE>(a, so);
@@ -1499,6 +1509,8 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_patch_ovf(
 template <class R, int MODE> __global__ void __launch_bounds__(256) k_integrate(KArgs<R> a)
 {
     const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        a.ctl->ovf = 0u; // the overflow kernel (before this one) has read it
     float drift = 0.f;
     if (s < a.N)
         process_particle<R, MODE, PH_INTEGRATE>(
