@@ -386,6 +386,29 @@ def test_split_candidate_overflow_matches_oracle(monkeypatch):
     assert relmax(T, To) < 1e-12
 
 
+@pytest.mark.parametrize("precision", [mdx.PREC_FP64, mdx.PREC_MIXED])
+def test_overflow_steps_bitwise_equal_one_kernel(monkeypatch, precision):
+    """The same simple-cubic cluster stepped 20 times: in the split form the
+    walk raises the overflow gate, k_patch_ovf adds the candidates beyond 16
+    and the integrator closes the gate each step; the one-kernel form handles
+    them inline.  Same trajectory bit for bit."""
+    s = wl.patchy_tetra(512, phi=0.1)
+    g = np.arange(8) * 0.95
+    lat = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    s["x"] = lat + (s["box"] - 7 * 0.95) / 2
+    say("overflow steps", precision)
+    outs = []
+    for split in ("1", "0"):
+        monkeypatch.setenv("MDX_SPLIT", split)
+        with engine_for(s, precision, seed=12) as e:
+            e.prime()
+            for _ in range(4):
+                e.step(5)
+            outs.append(e.download())
+    for k in ("x", "p", "Q", "L"):
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
 # ------------------------------------------------------------- neighbour list
 def _run(s, precision, nsteps, skin, kmax=64, resort=200, seed=7):
     with engine_for(s, precision, seed=seed) as e:
