@@ -230,13 +230,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--cpu-steps-1core", type=int, default=8,
+                    help="steps of the one-thread CPU sample (0 skips it)")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--autotune", action="store_true",
                     help="let the engine's autotuner pick skin and walk form before timing")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the engine has no multi-GPU path in this build "
-              "(see DESIGN.md for how replicas and large systems would shard)", file=sys.stderr)
+              "(replicas would shard whole per GPU, a large system by z-slabs)", file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:
         copy_probe()
@@ -321,6 +323,11 @@ def main():
         log("cpu baseline (reference harness),", threads, "threads")
         cb = cpu_baseline(s, args.cpu_steps, threads)
         log("cpu baseline", cb)
+        if cb is not None and args.cpu_steps_1core > 0 and threads != 1:
+            log("cpu baseline (reference harness), 1 thread")
+            c1 = cpu_baseline(s, args.cpu_steps_1core, 1)
+            log("cpu baseline 1 thread", c1)
+            cb["one_core"] = {k: c1[k] for k in ("value", "cores", "sample")}
         out["cpu_baseline"] = cb
     else:
         out["cpu_baseline"] = None
