@@ -237,7 +237,7 @@ def main():
                     help="let the engine's autotuner pick skin and walk form before timing")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: the engine has no multi-GPU path in this build "
+        print("bench.py runs on one GPU only: the engine has no multi-GPU path "
               "(replicas would shard whole per GPU, a large system by z-slabs)", file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:

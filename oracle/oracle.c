@@ -1,7 +1,7 @@
 /*
  * oracle.c -- FP64 CPU restatement of mtcode's per-timestep Langevin step.
  *
- * TEST INFRASTRUCTURE ONLY (the checker).  Nothing in the product links or
+ * TEST INFRASTRUCTURE ONLY (a result check).  Nothing in the product links or
  * calls this file.  Reference citations are relative to the reference's Code/
  * directory.  See oracle.h for what pins each function.
  */

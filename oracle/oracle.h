@@ -4,7 +4,7 @@
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (the mtcode_amd package,
  * libmdx.so, the drop-in host classes) may include, link or call this code.
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
- * and only as the checker.
+ * and only to check results.
  *
  * Every function restates one reference function; the citation is given next
  * to its declaration (paths relative to the reference's Code/ directory).

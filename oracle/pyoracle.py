@@ -1,7 +1,7 @@
 """ctypes front end of liboracle.so (the FP64 C restatement in oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, as the checker.  The product never imports it.
+bench.py's cpu_baseline leg, to check results.  The product never imports it.
 """
 from __future__ import annotations
 
