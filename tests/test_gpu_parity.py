@@ -606,3 +606,37 @@ def test_autotune_tries_every_candidate_and_keeps_the_fastest(monkeypatch):
         assert e.info()["skin"] == best["skin"]
         e.step(20)
         assert np.isfinite(e.download()["x"]).all()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fuse", ["2", "1"])
+def test_full_size_momentum_conserved_without_thermostat(monkeypatch, fuse):
+    """The 1M two-species condensate (N = 1,048,576, L = 222.27) stepped 200
+    times through the production list path (split kernels, compacted walk,
+    gated list rebuilds; fuse "1" also integrates inside the walk) in FP64
+    with gamma = gammar = 0: the noise amplitude sqrt(2 gamma kT / dt) is then
+    zero and there is no friction, so each half kick adds dt/2 * F and the
+    total momentum moves only by the sum of the forces, which pairwise forces
+    make zero (Newton's third law; MD.cpp:1684-1764 and LangevinR.cpp:911-1047
+    add +F to i and -F to j).  A pair seen from one side only, or twice from
+    one side, moves it by about dt * |F_ij|, orders above the FP64 roundoff
+    allowed here.  The particles move well past the list skin, so the run
+    crosses several rebuilds."""
+    monkeypatch.setenv("MDX_FUSE", fuse)
+    s = wl.thermal_momenta(wl.condensate_ab())
+    s["gamma"] = 0.0
+    s["gammar"] = 0.0
+    say("momentum conservation, condensate", s["N"], "fuse", fuse)
+    with mdx.engine_from_system(s, precision=mdx.PREC_FP64, seed=5) as e:
+        st0 = e.download()
+        e.prime()
+        e.step(200)
+        st = e.download()
+    P0, P = st0["p"].sum(0), st["p"].sum(0)
+    L = float(s["box"][0])
+    dx = st["x"] - st0["x"]
+    dx -= L * np.round(dx / L)
+    say("P0", P0, "P", P, "max displacement", np.abs(dx).max())
+    assert np.isfinite(st["p"]).all() and np.isfinite(st["x"]).all()
+    assert np.abs(dx).max() > 0.5
+    assert np.abs(P - P0).max() < 1e-8
