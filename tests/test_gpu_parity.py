@@ -609,34 +609,44 @@ def test_autotune_tries_every_candidate_and_keeps_the_fastest(monkeypatch):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("fuse", ["2", "1"])
-def test_full_size_momentum_conserved_without_thermostat(monkeypatch, fuse):
-    """The 1M two-species condensate (N = 1,048,576, L = 222.27) stepped 200
-    times through the production list path (split kernels, compacted walk,
-    gated list rebuilds; fuse "1" also integrates inside the walk) in FP64
-    with gamma = gammar = 0: the noise amplitude sqrt(2 gamma kT / dt) is then
-    zero and there is no friction, so each half kick adds dt/2 * F and the
-    total momentum moves only by the sum of the forces, which pairwise forces
-    make zero (Newton's third law; MD.cpp:1684-1764 and LangevinR.cpp:911-1047
-    add +F to i and -F to j).  A pair seen from one side only, or twice from
-    one side, moves it by about dt * |F_ij|, orders above the FP64 roundoff
-    allowed here.  The particles move well past the list skin, so the run
-    crosses several rebuilds."""
+@pytest.mark.parametrize("name,builder,fuse,steps", [
+    ("condensate", lambda: wl.condensate_ab(), "2", 200),
+    ("condensate", lambda: wl.condensate_ab(), "1", 200),
+    ("mt_bundle", lambda: wl.mt_bundle(), "2", 500),
+    ("polymer_fa0", lambda: wl.active_polymer(fa=0.0), "2", 200),
+])
+def test_full_size_momentum_conserved_without_thermostat(monkeypatch, name, builder, fuse, steps):
+    """Full-size systems stepped through the production list path (split
+    kernels, compacted walk, gated list rebuilds; fuse "1" also integrates
+    inside the walk) in FP64 with gamma = gammar = 0: the 1M two-species
+    condensate (WCA + patches), the 416,000-bead microtubule bundle (WCA +
+    harmonic bonds + harmonic angles, bead integrator) and the 2.1M-bead
+    polymer melt with its propulsion switched off (fa = 0: WCA + bonds +
+    bending + patchy chain-end rotors).  The noise amplitude
+    sqrt(2 gamma kT / dt) is then zero and there is no friction, so each half
+    kick adds dt/2 * F and the total momentum moves only by the sum of the
+    forces, which pair and three-body forces make zero (Newton's third law;
+    MD.cpp:1684-1764, MD.cpp:1929-1983 and LangevinR.cpp:911-1047 add +F to i
+    and -F to j, and the three angle terms sum to zero).  A pair seen from one
+    side only, or twice from one side, moves it by about dt * |F_ij|, orders
+    above the FP64 roundoff allowed here.  A particle moving more than half
+    the list skin (0.4) forces a rebuild, so the run crosses at least one."""
     monkeypatch.setenv("MDX_FUSE", fuse)
-    s = wl.thermal_momenta(wl.condensate_ab())
+    s = wl.thermal_momenta(builder())
     s["gamma"] = 0.0
     s["gammar"] = 0.0
-    say("momentum conservation, condensate", s["N"], "fuse", fuse)
+    say("momentum conservation", name, s["N"], "fuse", fuse, "steps", steps)
     with mdx.engine_from_system(s, precision=mdx.PREC_FP64, seed=5) as e:
         st0 = e.download()
-        e.prime()
-        e.step(200)
+        if s["integrator"] == "rotor":
+            e.prime()
+        e.step(steps)
         st = e.download()
     P0, P = st0["p"].sum(0), st["p"].sum(0)
-    L = float(s["box"][0])
+    box = np.asarray(s["box"], dtype=np.float64)
     dx = st["x"] - st0["x"]
-    dx -= L * np.round(dx / L)
-    say("P0", P0, "P", P, "max displacement", np.abs(dx).max())
+    dx -= box * np.round(dx / box)
+    say("P0", P0, "P", P, "|dP|", np.abs(P - P0).max(), "max displacement", np.abs(dx).max())
     assert np.isfinite(st["p"]).all() and np.isfinite(st["x"]).all()
-    assert np.abs(dx).max() > 0.5
+    assert np.abs(dx).max() > 0.2
     assert np.abs(P - P0).max() < 1e-8
