@@ -76,6 +76,22 @@ int mdx_download_state(mdx_engine *e, double *x, double *p, double *Q, double *L
 /* total force (lab) and torque (body) carried between steps: the F and T of
  * Condensate::run after create_forces_and_torques_sphere (Condensate.cpp:370) */
 int mdx_download_forces(mdx_engine *e, double *F, double *T);
+/* Resume.  The reference continues a run from its newest CSV files with fresh
+ * rand() noise (Condensate::run_singlebond_continue, Condensate.cpp:620-660;
+ * Basic/Scripts/continue_run.sh:9-20).  Here the noise of force evaluation k
+ * (k counts mdx_prime and every step since mdx_create) is a pure function of
+ * the seed, k and the particle's original index, so a resumed run sets the
+ * counter and continues the stream instead of repeating it.
+ * mdx_get_noise_counter gives the k of the next force evaluation. */
+int mdx_get_noise_counter(mdx_engine *e, unsigned long long *counter);
+int mdx_set_noise_counter(mdx_engine *e, unsigned long long counter);
+/* carried F (lab) and T (body) in the layout of mdx_download_forces, in place
+ * of mdx_prime: after mdx_upload_state + mdx_set_noise_counter, a run
+ * continued from downloaded state, forces and counter takes the same steps
+ * as the uninterrupted run (bit for bit when the uninterrupted run re-sorts
+ * its state at the step where the continuation starts, as the continuation
+ * does at its first step) */
+int mdx_upload_forces(mdx_engine *e, const double *F, const double *T);
 
 /* Interactions. */
 /* HSPotential(eps, sigma) as used by MD::calculateforces (potential.h:79-125);
@@ -150,6 +166,15 @@ int mdx_cell_assign(mdx_engine *e, int replica, const int n[3], int *cell_of);
  * (geometry.h:145-164), sorted by (i, j).  n[3] cells per side, n >= 1. */
 int mdx_neighbor_pairs(mdx_engine *e, int replica, const int n[3], double cut, int *ij,
                        long long cap, long long *npairs);
+/* The Verlet list of the step (the reference's MD::calculatepairs(boxes, 3.5)
+ * rebuilt every 20 steps, Condensate.cpp:311, 345) as a set: the list the
+ * step's own kernels build at the current positions (pairs within reach +
+ * skin; type pairs without patch rows within the WCA reach + skin), as
+ * unordered pairs of original indices (i < j, ascending), at most cap;
+ * r2cut (nullable, 2 doubles) receives the two squared radii used.  Checks
+ * that the full list holds each pair from both sides.  The next mdx_step
+ * re-sorts and rebuilds its list. */
+int mdx_list_pairs(mdx_engine *e, int *ij, long long cap, long long *npairs, double *r2cut);
 /* conservative force F (lab) and patch torque T (lab) at the current state:
  * calculateforces(pairs, HSPotential) + bonds + angles + active +
  * calculate_forces_and_torques3D(pairs, ComboPatch, F, T) (LangevinR.cpp:911-1047) */
@@ -193,6 +218,14 @@ int mdx_step(mdx_engine *e, int nsteps, const double *noise);
  * out[3..5] <L_k^2/2I_0> (LangevinR.cpp:170-193), out[6] potential energy,
  * out[7..15] virial sum_pairs r_ij (x) f_ij (calculatestress, MD.cpp:2381-2424) */
 int mdx_observables(mdx_engine *e, int replica, double *out);
+
+/* MD::calculatestress (MD.cpp:2381-2424) of replica r (or -1: all) with the
+ * pair interaction of mdx_set_pair_wca as the MD's interaction
+ * (HSPotential::force2mdx, potential.h:126-139): out[3k+j] = sum over pairs
+ * within the cut-off of f(r^2) sqrt(r^2) d_k d_j, d the minimum-imaged
+ * separation (the reference's own expression: r times the pair's virial
+ * term; the virial itself is mdx_observables out[7..15]) */
+int mdx_stress(mdx_engine *e, int replica, double *out);
 
 /* Single stages of the step, one reference call each, on the engine's state
  * (host arrays by original index, the reference's layouts).  For drivers that

@@ -77,6 +77,46 @@ __device__ __forceinline__ int axis_cell(double x, int n, double l)
     return m;
 }
 
+// cube::correct_position_and_momentum (geometry.h:403-445) for one
+// coordinate: the reference's loop of single box-length shifts (periodic) or
+// one reflection (walls), so a coordinate one step outside the box rounds as
+// the reference's does.  Bounded: after WRAP_MAX shifts (a coordinate that
+// far out cannot come from one step with a finite force) the remainder is
+// taken by floor, and a result outside [0, l] (a reflection from more than a
+// box length out) is clamped.  Returns false, leaving x and p untouched, for
+// a non-finite x or p; the caller then keeps the particle's previous state.
+constexpr int WRAP_MAX = 64;
+template <class R>
+__host__ __device__ __forceinline__ bool wrap1(double &x, R &p, double l, int periodic)
+{
+    if (!(__builtin_isfinite(x) && __builtin_isfinite((double)p)))
+        return false;
+    if (periodic) {
+        int it = 0;
+        if (x < 0) {
+            while (x < 0 && it++ < WRAP_MAX)
+                x += l;
+        } else if (x > l) {
+            while (x > l && it++ < WRAP_MAX)
+                x -= l;
+        }
+        if (x < 0 || x > l) {
+            x -= l * __builtin_floor(x / l);
+            x = x < 0 ? 0.0 : (x > l ? l : x);
+        }
+    } else {
+        if (x < 0) {
+            x = -x;
+            p = -p;
+        } else if (x > l) {
+            x = l - (x - l);
+            p = -p;
+        }
+        x = x < 0 ? 0.0 : (x > l ? l : x);
+    }
+    return true;
+}
+
 // ------------------------------------------------------------------- pairs
 // HSPotential::force (potential.h:112-125)
 __device__ __forceinline__ double hs_force(double x, double epsilon, double sigma6,

@@ -410,7 +410,8 @@ static int finish_bins(mdx_engine *e, const StepCtl *gate = nullptr, const int *
     hipLaunchKernelGGL(k_fill, dim3(nblk(e->N)), dim3(256), 0, e->stream, e->N, cells,
                        (const int *)e->rank.p, (const int *)e->start.p, (int *)e->list.p, gate);
     hipLaunchKernelGGL(k_cellsort, dim3(nblk(e->N)), dim3(256), 0, e->stream, e->N, cells,
-                       (const int *)e->rank.p, (const int *)e->start.p, (int *)e->list.p, gate);
+                       (const int *)e->rank.p, (const int *)e->start.p, (int *)e->list.p,
+                       (const PosRec *)e->X[e->cur].p, gate);
     HIPCHK(hipGetLastError());
     return MDX_OK;
 }
@@ -529,27 +530,43 @@ static int launch_gather(mdx_engine *e, const double *noise_dev, int lmode = 0)
     return MDX_OK;
 }
 
-static int read_status(mdx_engine *e, unsigned int *st)
+// status of the calls since the last read (and clear it): st = status bits,
+// *step = the force evaluation that first raised one, counted from the start
+// of the current call (0: the carried-force evaluation of mdx_prime, or the
+// first step's kernels; k: step k)
+static int read_status(mdx_engine *e, unsigned int *st, long long *step = nullptr)
 {
     StepCtl c;
     HIPCHK(hipMemcpyAsync(&c, e->ctl.p, sizeof c, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     *st = c.status;
+    if (step)
+        *step = c.status ? (long long)(c.fail_epoch - c.call_base) : -1;
     if (c.status) {
         unsigned int z = 0;
+        unsigned long long none = ~0ull;
         HIPCHK(hipMemcpyAsync((char *)e->ctl.p + offsetof(StepCtl, status), &z, sizeof z,
                               hipMemcpyHostToDevice, e->stream));
+        HIPCHK(hipMemcpyAsync((char *)e->ctl.p + offsetof(StepCtl, fail_epoch), &none,
+                              sizeof none, hipMemcpyHostToDevice, e->stream));
         HIPCHK(hipStreamSynchronize(e->stream));
     }
     return MDX_OK;
 }
 
-static int status_to_rc(unsigned int st)
+// step >= 1: the step of the call whose force evaluation raised the status
+// (the steps after it moved no particle); 0: mdx_prime's evaluation
+static int status_to_rc(unsigned int st, long long step = -1)
 {
+    char where[96] = "";
+    if (step == 0)
+        snprintf(where, sizeof where, " in the carried-force evaluation");
+    else if (step > 0)
+        snprintf(where, sizeof where, " at step %lld of the call (no particle moved after it)", step);
     if (st & 2u)
-        return set_err(MDX_ERR_NAN, "non-finite force or torque");
+        return set_err(MDX_ERR_NAN, "non-finite force, torque or updated state%s", where);
     if (st & 1u)
-        return set_err(MDX_ERR_FORCE, "pair force above the guard (force too big)");
+        return set_err(MDX_ERR_FORCE, "pair force above the guard (force too big)%s", where);
     return MDX_OK;
 }
 
@@ -588,10 +605,11 @@ template <class R> static int do_prime(mdx_engine *e, const double *noise0)
     e->counted = false;
     e->list_valid = false;
     unsigned int st;
-    if ((rc = read_status(e, &st)))
+    long long at;
+    if ((rc = read_status(e, &st, &at)))
         return rc;
     e->primed = true;
-    return status_to_rc(st);
+    return status_to_rc(st, at);
 }
 
 static int nbr_alloc(mdx_engine *e)
@@ -835,9 +853,11 @@ template <class R> static int do_step_rotor(mdx_engine *e, int nsteps, const dou
     if ((rc = enqueue_rotor<R>(e, nsteps, nd)))
         return rc;
     unsigned int st;
-    if ((rc = read_status(e, &st)))
+    long long at;
+    if ((rc = read_status(e, &st, &at)))
         return rc;
-    return status_to_rc(st);
+    // step k of the call evaluates its forces at epoch call_base + k - 1
+    return status_to_rc(st, at >= 0 ? at + 1 : at);
 }
 
 template <class R> static int enqueue_bead(mdx_engine *e, int nsteps, const double *nd,
@@ -863,9 +883,10 @@ template <class R> static int do_step_bead(mdx_engine *e, int nsteps, const doub
     if ((rc = enqueue_bead<R>(e, nsteps, nd)))
         return rc;
     unsigned int st;
-    if ((rc = read_status(e, &st)))
+    long long at;
+    if ((rc = read_status(e, &st, &at)))
         return rc;
-    return status_to_rc(st);
+    return status_to_rc(st, at >= 0 ? at + 1 : at);
 }
 
 // ids of the current slots (host copy)
@@ -1025,6 +1046,12 @@ int mdx_create(const mdx_desc *d, mdx_engine **out)
     rc |= dalloc(e, e->rank, 4 * N);
     rc |= dalloc(e, e->list, 4 * N);
     rc |= dalloc(e, e->ctl, sizeof(StepCtl));
+    if (!rc) {
+        StepCtl c0;
+        memset(&c0, 0, sizeof c0);
+        c0.fail_epoch = ~0ull;
+        rc |= upload(e, e->ctl, &c0, 1);
+    }
     if (rc) {
         std::string msg = g_err;
         mdx_destroy(e);
@@ -1116,8 +1143,12 @@ int mdx_upload_state(mdx_engine *e, const double *x, const double *p, const doub
         return set_err(MDX_ERR_ARG, "engine and positions required");
     HIPCHK(hipSetDevice(e->device));
     for (long long k = 0; k < 3ll * e->N; k++)
-        if (!std::isfinite(x[k]))
-            return set_err(MDX_ERR_ARG, "non-finite position at particle %lld", k / 3);
+        if (!std::isfinite(x[k]) || (p && !std::isfinite(p[k])) || (L && !std::isfinite(L[k])))
+            return set_err(MDX_ERR_ARG, "non-finite position or momentum at particle %lld", k / 3);
+    if (Q)
+        for (long long k = 0; k < 9ll * e->N; k++)
+            if (!std::isfinite(Q[k]))
+                return set_err(MDX_ERR_ARG, "non-finite orientation at particle %lld", k / 9);
     int rc = e->prec == MDX_PREC_FP64 ? upload_state_t<double>(e, x, p, Q, L)
                                       : upload_state_t<float>(e, x, p, Q, L);
     e->binned = false;
@@ -1850,7 +1881,51 @@ template <class R> static int observables_t(mdx_engine *e, int replica, double *
     }
     for (int k = 0; k < 6; k++)
         out[k] /= (double)cnt;
+    unsigned int st;
+    if ((rc = read_status(e, &st)))
+        return rc;
+    return status_to_rc(st & 2u);
+}
+
+template <class R> static int stress_t(mdx_engine *e, int replica, double *out)
+{
+    int rc;
+    if (!e->binned && (rc = bin_current<R>(e)))
+        return rc;
+    if ((rc = dalloc(e, e->obs, 80 * (size_t)e->N)))
+        return rc;
+    KArgs<R> a = make_args<R>(e);
+    hipLaunchKernelGGL(k_stress<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, a,
+                       (double *)e->obs.p);
+    const int first = replica < 0 ? 0 : e->first[replica];
+    const int cnt = replica < 0 ? e->N : e->nper[replica];
+    const int nb = (int)std::min(1024u, nblk(cnt));
+    if ((rc = dalloc(e, e->partial, 9 * 8 * (size_t)nb)))
+        return rc;
+    hipLaunchKernelGGL(k_sum_rows<9>, dim3(nb), dim3(256), 0, e->stream, first, cnt,
+                       (const double *)e->obs.p, (double *)e->partial.p);
+    HIPCHK(hipGetLastError());
+    std::vector<double> part(9 * (size_t)nb);
+    if ((rc = download(e, e->partial, part.data(), part.size())))
+        return rc;
+    for (int k = 0; k < 9; k++) {
+        double v = 0;
+        for (int b = 0; b < nb; b++)
+            v += part[9 * (size_t)b + k];
+        out[k] = v;
+    }
     return MDX_OK;
+}
+
+int mdx_stress(mdx_engine *e, int replica, double *out)
+{
+    if (!e || !out || replica < -1 || replica >= e->nrep)
+        return set_err(MDX_ERR_ARG, "bad arguments");
+    if (!e->wca)
+        return set_err(MDX_ERR_STATE, "calculatestress needs the pair interaction (mdx_set_pair_wca)");
+    HIPCHK(hipSetDevice(e->device));
+    return e->prec == MDX_PREC_FP64 ? stress_t<double>(e, replica, out)
+                                    : stress_t<float>(e, replica, out);
 }
 
 int mdx_observables(mdx_engine *e, int replica, double *out)
@@ -1860,6 +1935,132 @@ int mdx_observables(mdx_engine *e, int replica, double *out)
     HIPCHK(hipSetDevice(e->device));
     return e->prec == MDX_PREC_FP64 ? observables_t<double>(e, replica, out)
                                     : observables_t<float>(e, replica, out);
+}
+
+// ------------------------------------------------------ noise counter, resume
+int mdx_get_noise_counter(mdx_engine *e, unsigned long long *counter)
+{
+    if (!e || !counter)
+        return set_err(MDX_ERR_ARG, "bad arguments");
+    HIPCHK(hipSetDevice(e->device));
+    StepCtl c;
+    HIPCHK(hipMemcpyAsync(&c, e->ctl.p, sizeof c, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    *counter = c.epoch;
+    return MDX_OK;
+}
+
+int mdx_set_noise_counter(mdx_engine *e, unsigned long long counter)
+{
+    if (!e)
+        return set_err(MDX_ERR_ARG, "null engine");
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipMemcpyAsync((char *)e->ctl.p + offsetof(StepCtl, epoch), &counter, sizeof counter,
+                          hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return MDX_OK;
+}
+
+template <class R> static int upload_forces_t(mdx_engine *e, const double *F, const double *T)
+{
+    const size_t n3 = 3 * (size_t)e->N;
+    int rc;
+    if ((rc = upload(e, e->tmpd, F, n3)) || (rc = upload(e, e->tmpx, T, n3)))
+        return rc;
+    const int c = e->cur;
+    hipLaunchKernelGGL((k_by_id_to_slots<R>), dim3(nblk(e->N)), dim3(256), 0, e->stream,
+                       (const PosRec *)e->X[c].p, e->N, (const double *)e->tmpd.p, (R *)e->Fc[c].p,
+                       3);
+    hipLaunchKernelGGL((k_by_id_to_slots<R>), dim3(nblk(e->N)), dim3(256), 0, e->stream,
+                       (const PosRec *)e->X[c].p, e->N, (const double *)e->tmpx.p, (R *)e->Tc[c].p,
+                       3);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->primed = true;
+    return MDX_OK;
+}
+
+int mdx_upload_forces(mdx_engine *e, const double *F, const double *T)
+{
+    if (!e || !F || !T)
+        return set_err(MDX_ERR_ARG, "engine, F and T required");
+    if (e->integ != MDX_INTEG_ROTOR)
+        return set_err(MDX_ERR_STATE, "carried forces belong to the rotor integrator");
+    for (long long k = 0; k < 3ll * e->N; k++)
+        if (!std::isfinite(F[k]) || !std::isfinite(T[k]))
+            return set_err(MDX_ERR_ARG, "non-finite carried force or torque at particle %lld",
+                           k / 3);
+    HIPCHK(hipSetDevice(e->device));
+    return e->prec == MDX_PREC_FP64 ? upload_forces_t<double>(e, F, T)
+                                    : upload_forces_t<float>(e, F, T);
+}
+
+// -------------------------------------------------- the step's neighbour list
+// The list mdx_step walks, built at the current positions by the step's own
+// kernels (binning, scan, fill, per-cell order, k_nbuild), returned as the
+// set of unordered pairs (original indices, i < j, ascending).  The list is
+// full (each pair from both sides): both sides are checked to agree.
+template <class R>
+static int list_pairs_t(mdx_engine *e, int *ij, long long cap, long long *npairs, double *r2)
+{
+    if (!(e->skin > 0))
+        return set_err(MDX_ERR_STATE, "no neighbour list (skin 0)");
+    int rc;
+    if ((rc = build_grid(e)) || (rc = nbr_alloc(e)))
+        return rc;
+    if (!e->counted && (rc = count_current<R>(e, (int *)e->cell[e->cur].p, nullptr)))
+        return rc;
+    if ((rc = finish_bins(e)) || (rc = build_list<R>(e, (const int *)e->cell[e->cur].p, 0)))
+        return rc;
+    e->counted = false;
+    e->binned = true;
+    // the step rebuilds its list (and its displacement bound) at its next step
+    e->list_valid = false;
+    const int N = e->N;
+    std::vector<int> nn(N), nb((size_t)e->kmax * N);
+    std::vector<PosRec> rec;
+    if ((rc = download(e, e->nn, nn.data(), nn.size())) ||
+        (rc = download(e, e->nbr, nb.data(), nb.size())) || (rc = slot_ids(e, rec)))
+        return rc;
+    std::vector<long long> fwd, bwd;
+    for (int s = 0; s < N; s++) {
+        const int n = nn[s] & (LIST_EXC - 1);
+        if (n > e->kmax)
+            return set_err(MDX_ERR_STATE,
+                           "particle %u has %d list entries, more than kmax = %d (it walks its "
+                           "cells instead)", rec[s].id, n, e->kmax);
+        for (int k = 0; k < n; k++) {
+            const long long a = rec[s].id, b = rec[nb[(size_t)k * N + s]].id;
+            (a < b ? fwd : bwd).push_back(a < b ? (a << 32) | b : (b << 32) | a);
+        }
+    }
+    std::sort(fwd.begin(), fwd.end());
+    std::sort(bwd.begin(), bwd.end());
+    if (fwd != bwd)
+        return set_err(MDX_ERR_STATE, "neighbour list not symmetric (%zu vs %zu entries)",
+                       fwd.size(), bwd.size());
+    *npairs = (long long)fwd.size();
+    for (long long k = 0; k < std::min(cap, *npairs); k++) {
+        ij[2 * k] = (int)(fwd[k] >> 32);
+        ij[2 * k + 1] = (int)(fwd[k] & 0xffffffffll);
+    }
+    if (r2) {
+        KArgs<R> a = make_args<R>(e);
+        r2[0] = a.rl2;
+        r2[1] = a.rl2w;
+    }
+    if (*npairs > cap)
+        return set_err(MDX_ERR_CAPACITY, "%lld pairs, capacity %lld", *npairs, cap);
+    return MDX_OK;
+}
+
+int mdx_list_pairs(mdx_engine *e, int *ij, long long cap, long long *npairs, double *r2cut)
+{
+    if (!e || !npairs || cap < 0 || (cap > 0 && !ij))
+        return set_err(MDX_ERR_ARG, "bad arguments");
+    HIPCHK(hipSetDevice(e->device));
+    return e->prec == MDX_PREC_FP64 ? list_pairs_t<double>(e, ij, cap, npairs, r2cut)
+                                    : list_pairs_t<float>(e, ij, cap, npairs, r2cut);
 }
 
 int mdx_info(mdx_engine *e, double *out)
@@ -1942,11 +2143,13 @@ template <class R> static int stage_advance_pos_t(mdx_engine *e)
         return rc;
     hipLaunchKernelGGL(k_stage_advance_pos<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, e->N,
                        (PosRec *)e->X[e->cur].p, (R *)e->P[e->cur].p, (const DevRep *)e->dreps.p,
-                       e->dt / e->m, e->periodic);
+                       e->dt / e->m, e->periodic, (StepCtl *)e->ctl.p);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(e->stream));
     stage_done(e, true);
-    return MDX_OK;
+    unsigned int st;
+    if ((rc = read_status(e, &st)))
+        return rc;
+    return status_to_rc(st);
 }
 
 int mdx_stage_advance_pos(mdx_engine *e)

@@ -47,6 +47,9 @@ struct StepCtl {
     // a walk queued more than PMAXG patch candidates for some position (the
     // overflow kernel's gate; cleared by the integrator kernel after it)
     unsigned int ovf;
+    // the first force evaluation (epoch) that raised a status bit; ~0 while
+    // the status is clear.  From then on no moving kernel advances a particle.
+    unsigned long long fail_epoch;
 };
 
 enum { M_CONS = 0, M_PRIME = 1, M_STEP = 2, M_FUSED = 3, M_BEAD = 4, M_ENERGY = 5 };
@@ -207,33 +210,30 @@ template <class R> __device__ __forceinline__ void reortho(R q[9])
         }
 }
 
-// cube::correct_position_and_momentum (geometry.h:403-445) for one coordinate
-template <class R>
-__device__ __forceinline__ void wrap1(double &x, R &p, double l, int periodic)
+// status bits: 1 a pair or bond term above the force guard, 2 a non-finite
+// force, torque or updated state; the first epoch that raised one is kept
+template <class R> __device__ __forceinline__ void raise_status(const KArgs<R> &a, unsigned bit)
 {
-    if (periodic) {
-        if (x < 0) {
-            while (x < 0)
-                x += l;
-        } else if (x > l) {
-            while (x > l)
-                x -= l;
-        }
-    } else {
-        if (x < 0) {
-            x = -x;
-            p = -p;
-        } else if (x > l) {
-            x = l - (x - l);
-            p = -p;
-        }
-    }
+    atomicOr(&a.ctl->status, bit);
+    atomicMin(&a.ctl->fail_epoch, a.ctl->epoch);
 }
 
-template <class R> __device__ __forceinline__ void flag_force(const KArgs<R> &a, double f)
+// |f| of one term above the guard (the reference prints "force too big" and
+// pauses: MD.cpp:1711-1749)
+template <class R> __device__ __forceinline__ bool flag_force(const KArgs<R> &a, double f)
 {
-    if (a.fguard > 0 && fabs(f) > a.fguard)
-        atomicOr(&a.ctl->status, 1u);
+    if (a.fguard > 0 && fabs(f) > a.fguard) {
+        raise_status(a, 1u);
+        return true;
+    }
+    return false;
+}
+
+// a status bit is raised (by this kernel so far or an earlier one): the run
+// stops, and moving kernels keep every particle's state from here on
+template <class R> __device__ __forceinline__ bool stopped(const KArgs<R> &a)
+{
+    return __hip_atomic_load(&a.ctl->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
 }
 
 // noise of particle (original index) o for the current force evaluation
@@ -459,6 +459,7 @@ template <class R> __device__ __forceinline__ void drift_bound(const KArgs<R> &a
 template <class R> struct Acc {
     R F[3], T[3];
     double epot, vir[9];
+    bool bad; // a term of this particle was above the force guard
 };
 
 // roles of a patch pair: "a" = lower original index, exchanged when its type
@@ -596,7 +597,7 @@ __device__ __forceinline__ void wca_term(const KArgs<R> &a, const double d[3], d
             if (!ex) {
                 if constexpr (FP64) {
                     const double f1 = hs_force(rij, a.wca_eps, a.wca_s6, a.wca_s12);
-                    flag_force(a, f1);
+                    acc.bad |= flag_force(a, f1);
                     for (int q = 0; q < 3; q++) {
                         const double fac = f1 * d[q] / rij;
                         acc.F[q] += fac;
@@ -615,7 +616,7 @@ __device__ __forceinline__ void wca_term(const KArgs<R> &a, const double d[3], d
                     const float ir6 = ir2 * ir2 * ir2;
                     const float f1 = 4.f * (float)a.wca_eps * ir * ir6 *
                                      (12.f * (float)a.wca_s12 * ir6 - 6.f * (float)a.wca_s6);
-                    flag_force(a, f1);
+                    acc.bad |= flag_force(a, f1);
                     const float fr = f1 * ir;
                     for (int q = 0; q < 3; q++) {
                         const float fac = fr * (float)d[q];
@@ -695,6 +696,7 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
     acc.F[0] = acc.F[1] = acc.F[2] = 0;
     acc.T[0] = acc.T[1] = acc.T[2] = 0;
     acc.epot = 0;
+    acc.bad = false;
     for (int k = 0; k < 9; k++)
         acc.vir[k] = 0;
 
@@ -842,7 +844,7 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
                 f1 = -k * r / (1 - (r * r) / (r0 * r0)); // FENEPotential::force (potential.h:356)
             else
                 f1 = -k * (r - r0);
-            flag_force(a, f1);
+            acc.bad |= flag_force(a, f1);
             for (int c = 0; c < 3; c++) {
                 const double fac = f1 * d[c] / r;
                 Fi[c] += (R)fac;
@@ -878,14 +880,29 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
             for (int k = 0; k < 3; k++)
                 Fi[k] += out[k];
             if (MODE == M_ENERGY && role == 1) {
-                double c = (ab[0] * bc[0] + ab[1] * bc[1] + ab[2] * bc[2]) /
-                           sqrt((ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2]) *
-                                (bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2]));
-                c = fmin(1.0, fmax(-1.0, c));
-                if (a.ang_form == MDX_ANGLE_HARMONIC)
+                // the triplet's energy (BendingPotential::energy,
+                // potential3.h:32-41, or k (1 - cos theta)) and virial
+                // sum_i F_i x_i = -F1 ab + F3 bc, once, by the middle member
+                const double ac = ab[0] * bc[0] + ab[1] * bc[1] + ab[2] * bc[2];
+                const double absq = ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2];
+                const double bcsq = bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2];
+                if (a.ang_form == MDX_ANGLE_HARMONIC) {
+                    const double c = fmin(1.0, fmax(-1.0, ac / (sqrt(absq) * sqrt(bcsq))));
                     epot += 0.5 * a.ang_k[t] * (acos(c) - a.ang_t0[t]) * (acos(c) - a.ang_t0[t]);
-                else
-                    epot += a.ang_k[t] * (1 - c);
+                } else {
+                    epot += a.ang_k[t] * (1 - ac / sqrt(absq * bcsq));
+                }
+                R f1[3], f3[3];
+                if (a.ang_form == MDX_ANGLE_HARMONIC) {
+                    bend_harmonic(abr, bcr, (R)a.ang_k[t], (R)a.ang_t0[t], 0, f1);
+                    bend_harmonic(abr, bcr, (R)a.ang_k[t], (R)a.ang_t0[t], 2, f3);
+                } else {
+                    bend_cosine(abr, bcr, (R)a.ang_k[t], 0, f1);
+                    bend_cosine(abr, bcr, (R)a.ang_k[t], 2, f3);
+                }
+                for (int q2 = 0; q2 < 3; q2++)
+                    for (int l = 0; l < 3; l++)
+                        vir[3 * q2 + l] += -(double)f1[q2] * ab[l] + (double)f3[q2] * bc[l];
             }
         }
     }
@@ -903,9 +920,11 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
         }
     }
 
-    if (!(isfinite((double)Fi[0]) && isfinite((double)Fi[1]) && isfinite((double)Fi[2]) &&
-          isfinite((double)Ti[0]) && isfinite((double)Ti[1]) && isfinite((double)Ti[2])))
-        atomicOr(&a.ctl->status, 2u);
+    const bool finite_ft = isfinite((double)Fi[0]) && isfinite((double)Fi[1]) &&
+                           isfinite((double)Fi[2]) && isfinite((double)Ti[0]) &&
+                           isfinite((double)Ti[1]) && isfinite((double)Ti[2]);
+    if (!finite_ft)
+        raise_status(a, 2u);
 
     // ------------------------------------------------------------ epilogues
     if (MODE == M_CONS) {
@@ -922,6 +941,12 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
             o[1 + k] = vir[k];
         return;
     }
+    // The moving epilogues keep a particle's state (and carry no force) when
+    // its force or torque is non-finite, one of its terms is above the guard,
+    // its updated state is non-finite, or the run has stopped (a status bit
+    // raised by any particle so far); the inputs are still intact in the
+    // current buffers, so the kept state is re-read from there.
+    const bool hold = !finite_ft || acc.bad || stopped(a);
     // list mode keeps the slots: both slot maps already hold them
     if (!a.lmode)
         a.slotn[oi] = s;
@@ -936,6 +961,7 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
         double xn[3] = {xi.x, xi.y, xi.z};
         R pk[3];
         double dd = 0;
+        bool ok = !hold;
         for (int k = 0; k < 3; k++) {
             pk[k] = a.p[3 * (size_t)i + k];
             pk[k] = (R)a.b_c5c2 * pk[k] + (R)a.b_c5c3q * Fi[k] + (c5 * c4 + rr) * nz[k];
@@ -946,9 +972,20 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
                 step = (double)((R)a.c1 * pk[k]);
             xn[k] = xn[k] + step;
             dd += step * step;
-            wrap1(xn[k], pk[k], rp.l[k], a.periodic);
-            a.pn[3 * (size_t)s + k] = pk[k];
+            ok = ok && wrap1(xn[k], pk[k], rp.l[k], a.periodic);
         }
+        if (!ok) {
+            if (!hold)
+                raise_status(a, 2u);
+            xn[0] = xi.x;
+            xn[1] = xi.y;
+            xn[2] = xi.z;
+            for (int k = 0; k < 3; k++)
+                pk[k] = a.p[3 * (size_t)i + k];
+            dd = 0;
+        }
+        for (int k = 0; k < 3; k++)
+            a.pn[3 * (size_t)s + k] = pk[k];
         PosRec o = xi;
         o.x = xn[0];
         o.y = xn[1];
@@ -999,6 +1036,19 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
             ll[k] += hdt * Ti[k];
         }
     if (MODE == M_PRIME || MODE == M_STEP) {
+        bool ok = !hold;
+        for (int k = 0; k < 3; k++)
+            ok = ok && isfinite((double)pl[k]) && isfinite((double)ll[k]) &&
+                 isfinite((double)Fi[k]) && isfinite((double)Ti[k]);
+        if (!ok) {
+            if (!hold)
+                raise_status(a, 2u);
+            for (int k = 0; k < 3; k++) {
+                pl[k] = a.p[3 * (size_t)i + k];
+                ll[k] = a.L[3 * (size_t)i + k];
+                Fi[k] = Ti[k] = (R)0;
+            }
+        }
         a.Xn[s] = xi;
         for (int k = 0; k < 9; k++)
             a.Qn[9 * (size_t)s + k] = qi[k];
@@ -1017,6 +1067,7 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
     }
     double xn[3] = {xi.x, xi.y, xi.z};
     double dd = 0;
+    bool ok = !hold;
     for (int k = 0; k < 3; k++) {
         double step;
         if constexpr (FP64)
@@ -1025,12 +1076,30 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
             step = (double)((R)a.c1 * pl[k]);
         xn[k] = xn[k] + step;
         dd += step * step;
-        wrap1(xn[k], pl[k], rp.l[k], a.periodic);
+        ok = ok && wrap1(xn[k], pl[k], rp.l[k], a.periodic);
     }
     const R Ir[3] = {(R)a.I[0], (R)a.I[1], (R)a.I[2]};
     rotate((R)a.dt, Ir, ll, qi);
     if (!FP64 && a.reortho_every > 0 && ((a.ctl->epoch + 1) % a.reortho_every) == 0)
         reortho(qi);
+    for (int k = 0; k < 3; k++)
+        ok = ok && isfinite((double)ll[k]);
+    for (int k = 0; k < 9; k++)
+        ok = ok && isfinite((double)qi[k]);
+    if (!ok) {
+        if (!hold)
+            raise_status(a, 2u);
+        xn[0] = xi.x;
+        xn[1] = xi.y;
+        xn[2] = xi.z;
+        for (int k = 0; k < 3; k++) {
+            pl[k] = a.p[3 * (size_t)i + k];
+            ll[k] = a.L[3 * (size_t)i + k];
+        }
+        for (int k = 0; k < 9; k++)
+            qi[k] = a.Q[9 * (size_t)i + k];
+        dd = 0;
+    }
     for (int k = 0; k < 3; k++) {
         a.pn[3 * (size_t)s + k] = pl[k];
         a.Ln[3 * (size_t)s + k] = ll[k];
@@ -1208,6 +1277,7 @@ __global__ void __launch_bounds__(256, (sizeof(R) == 4 && !INT) ? 6 : 1) k_walkc
             const PosRec xo = sx[o];
             Acc<R> pa;
             pa.F[0] = pa.F[1] = pa.F[2] = 0;
+            pa.bad = false;
             const bool c = visit(xo, sq + 9 * o, sexc[2 * o], sexc[2 * o + 1], xj, pa);
             for (int k = 0; k < 3; k++)
                 pf[3 * tid + k] = pa.F[k];
@@ -1233,6 +1303,7 @@ __global__ void __launch_bounds__(256, (sizeof(R) == 4 && !INT) ? 6 : 1) k_walkc
             const DevRep rp = rep_rec(a, xi.meta);
             Acc<R> pa;
             pa.F[0] = pa.F[1] = pa.F[2] = 0;
+            pa.bad = false;
             for_each_in_cells(a, rp, a.bcell[s], s, [&](const PosRec &xj, int j) {
                 if (visit(xi, sq + 9 * tid, sexc[2 * tid], sexc[2 * tid + 1], xj, pa)) {
                     if (npc < PMAXG)
@@ -1312,6 +1383,7 @@ __device__ __forceinline__ void patch_overflow(const KArgs<R> &a, int so)
         qs[k2] = (float)qi[k2];
     }
     Acc<R> acc;
+    acc.bad = false;
     for (int k2 = 0; k2 < 3; k2++) {
         acc.F[k2] = a.Fw[3 * (size_t)so + k2];
         acc.T[k2] = a.Tp[3 * (size_t)so + k2];
@@ -1418,6 +1490,7 @@ __global__ void __launch_bounds__(256) k_patch(KArgs<R> a, int pblk)
             Acc<R> ea;
             for (int k = 0; k < 3; k++)
                 ea.F[k] = ea.T[k] = 0;
+            ea.bad = false;
             patch_pair<R, MODE>(a, rp, xo, xo.meta & 255, qo, a.cand[(size_t)q * a.N + so], ea);
             for (int k = 0; k < 3; k++) {
                 ent[6 * tid + k] = ea.F[k];
@@ -1766,6 +1839,10 @@ template <class R> __device__ __forceinline__ float kick_drift_one(const KArgs<R
     for (int k = 0; k < 9; k++)
         q[k] = a.Q[9 * (size_t)i + k];
     double dd = 0;
+    // a stopped run (status raised) keeps every particle's state, and a
+    // particle whose updated state is non-finite keeps its own
+    const bool hold = stopped(a);
+    bool ok = !hold;
     for (int k = 0; k < 3; k++) {
         double step;
         if constexpr (FP64)
@@ -1774,12 +1851,30 @@ template <class R> __device__ __forceinline__ float kick_drift_one(const KArgs<R
             step = (double)((R)a.c1 * pl[k]);
         xn[k] = xn[k] + step;
         dd += step * step;
-        wrap1(xn[k], pl[k], rp.l[k], a.periodic);
+        ok = ok && wrap1(xn[k], pl[k], rp.l[k], a.periodic);
     }
     const R Ir[3] = {(R)a.I[0], (R)a.I[1], (R)a.I[2]};
     rotate((R)a.dt, Ir, ll, q);
     if (!FP64 && a.reortho_every > 0 && ((a.ctl->epoch + 1) % a.reortho_every) == 0)
         reortho(q);
+    for (int k = 0; k < 3; k++)
+        ok = ok && isfinite((double)ll[k]);
+    for (int k = 0; k < 9; k++)
+        ok = ok && isfinite((double)q[k]);
+    if (!ok) {
+        if (!hold)
+            raise_status(a, 2u);
+        xn[0] = xr.x;
+        xn[1] = xr.y;
+        xn[2] = xr.z;
+        for (int k = 0; k < 3; k++) {
+            pl[k] = a.p[3 * (size_t)i + k];
+            ll[k] = a.L[3 * (size_t)i + k];
+        }
+        for (int k = 0; k < 9; k++)
+            q[k] = a.Q[9 * (size_t)i + k];
+        dd = 0;
+    }
     for (int k = 0; k < 3; k++) {
         a.pn[3 * (size_t)i + k] = pl[k];
         a.Ln[3 * (size_t)i + k] = ll[k];
@@ -1894,13 +1989,15 @@ __global__ void __launch_bounds__(256) k_fill(int N, const int *cell, const int 
         list[start[cell[i]] + rank[i]] = i;
 }
 
-// ascending slot order inside each cell: makes the gather order, and so the
-// floating-point sums, independent of atomic arrival order.  One thread per
-// particle; the particle that arrived first in its cell (rank 0, unique per
-// non-empty cell and not changed by the sort) sorts the cell, so empty cells
-// cost nothing.
+// ascending original index inside each cell: makes the gather order, and so
+// the floating-point sums, independent of atomic arrival order and of the
+// slot numbering (a state uploaded in original order and the same state held
+// in cell order walk their cells, and build their lists, in the same order).
+// One thread per particle; the particle that arrived first in its cell (rank
+// 0, unique per non-empty cell and not changed by the sort) sorts the cell,
+// so empty cells cost nothing.
 __global__ void __launch_bounds__(256) k_cellsort(int N, const int *cell, const int *rank,
-                                                  const int *start, int *list,
+                                                  const int *start, int *list, const PosRec *X,
                                                   const StepCtl *gate)
 {
     if (gate && !gate->rebuild)
@@ -1911,9 +2008,10 @@ __global__ void __launch_bounds__(256) k_cellsort(int N, const int *cell, const 
     const int c = cell[i];
     const int s0 = start[c], s1 = start[c + 1];
     for (int s = s0 + 1; s < s1; s++) {
-        int v = list[s];
+        const int v = list[s];
+        const uint32_t idv = X[v].id;
         int t = s - 1;
-        while (t >= s0 && list[t] > v) {
+        while (t >= s0 && X[list[t]].id > idv) {
             list[t + 1] = list[t];
             t--;
         }
@@ -2273,7 +2371,7 @@ __global__ void __launch_bounds__(256) k_stage_halfkick(int N, const PosRec *X, 
 // (geometry.h:403-445)
 template <class R>
 __global__ void __launch_bounds__(256) k_stage_advance_pos(int N, PosRec *X, R *p, const DevRep *reps,
-                                                           double c1, int periodic)
+                                                           double c1, int periodic, StepCtl *ctl)
 {
     constexpr bool FP64 = sizeof(R) == 8;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2282,15 +2380,23 @@ __global__ void __launch_bounds__(256) k_stage_advance_pos(int N, PosRec *X, R *
     PosRec xr = X[i];
     const DevRep rp = reps[xr.meta >> 8];
     double x[3] = {xr.x, xr.y, xr.z};
+    R pk[3];
+    bool ok = true;
     for (int k = 0; k < 3; k++) {
-        R pk = p[3 * (size_t)i + k];
+        pk[k] = p[3 * (size_t)i + k];
         if constexpr (FP64)
-            x[k] = x[k] + c1 * pk;
+            x[k] = x[k] + c1 * pk[k];
         else
-            x[k] = x[k] + (double)((R)c1 * pk);
-        wrap1(x[k], pk, rp.l[k], periodic);
-        p[3 * (size_t)i + k] = pk;
+            x[k] = x[k] + (double)((R)c1 * pk[k]);
+        ok = ok && wrap1(x[k], pk[k], rp.l[k], periodic);
     }
+    if (!ok) {
+        // non-finite: the particle keeps its position and momentum
+        atomicOr(&ctl->status, 2u);
+        return;
+    }
+    for (int k = 0; k < 3; k++)
+        p[3 * (size_t)i + k] = pk[k];
     xr.x = x[0];
     xr.y = x[1];
     xr.z = x[2];
@@ -2410,12 +2516,87 @@ __global__ void __launch_bounds__(256) k_obs(int first, int count, const R *p, c
     }
 }
 
+// MD::calculatestress (MD.cpp:2381-2424) with the engine's WCA as the MD's
+// interaction (HSPotential::force2mdx, potential.h:126-139): per particle the
+// half of its pair terms f(r^2) sqrt(r^2) d_k d_j, d = x_i - x_j
+// minimum-imaged as distance3v (geometry.h:250-276); every pair within the
+// cut-off is visited from both sides (the 27 cells of the current cell list)
+template <class R>
+__global__ void __launch_bounds__(256) k_stress(KArgs<R> a, double *out)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.N)
+        return;
+    const int i = a.list[s];
+    const PosRec xi = a.X[i];
+    const DevRep rp = rep_rec(a, xi.meta);
+    const double pl2 = a.wca_pl * a.wca_pl;
+    double st[9];
+    for (int k = 0; k < 9; k++)
+        st[k] = 0;
+    if (a.wca)
+        for_each_neighbor(a, rp, i, [&](const PosRec &xj, int j) {
+            (void)j;
+            double d[3];
+            min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
+            const double dis = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (!(dis < pl2))
+                return;
+            const double x3 = dis * dis * dis;
+            const double f1 = 4 * a.wca_eps * (12 * a.wca_s12 / (dis * (x3 * x3)) - 6 * a.wca_s6 / (dis * x3));
+            for (int k = 0; k < 3; k++)
+                for (int l = 0; l < 3; l++)
+                    st[3 * k + l] += 0.5 * (f1 * sqrt(dis) * d[k] * d[l]);
+        });
+    for (int k = 0; k < 9; k++)
+        out[9 * (size_t)xi.id + k] = st[k];
+}
+
+// per-block sums (fixed order) of width-value rows first..first+count of in
+template <int W>
+__global__ void __launch_bounds__(256) k_sum_rows(int first, int count, const double *in,
+                                                  double *partial)
+{
+    __shared__ double sh[256];
+    double acc[W];
+    for (int k = 0; k < W; k++)
+        acc[k] = 0;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < count; s += gridDim.x * blockDim.x)
+        for (int k = 0; k < W; k++)
+            acc[k] += in[(size_t)W * (first + s) + k];
+    for (int k = 0; k < W; k++) {
+        sh[threadIdx.x] = acc[k];
+        __syncthreads();
+        for (int off = 128; off > 0; off >>= 1) {
+            if ((int)threadIdx.x < off)
+                sh[threadIdx.x] += sh[threadIdx.x + off];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            partial[W * blockIdx.x + k] = sh[0];
+        __syncthreads();
+    }
+}
+
 // precision conversion (device <-> host FP64 layout)
 template <class A, class B> __global__ void k_convert(const A *in, B *out, size_t n)
 {
     size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n)
         out[k] = (B)in[k];
+}
+
+// a host array by original index (width values per particle, FP64) into the
+// slot order of the current buffer
+template <class R>
+__global__ void k_by_id_to_slots(const PosRec *X, int N, const double *in, R *out, int width)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N)
+        return;
+    const size_t o = X[s].id;
+    for (int k = 0; k < width; k++)
+        out[(size_t)width * s + k] = (R)in[width * o + k];
 }
 
 // new (replica << 8 | type) words for the current slots from an array by

@@ -95,6 +95,11 @@ def lib():
     L.mdx_profile_step.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_char_p), _dp,
                                    _ip]
     L.mdx_info.argtypes = [C.c_void_p, _dp]
+    L.mdx_get_noise_counter.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong)]
+    L.mdx_set_noise_counter.argtypes = [C.c_void_p, C.c_ulonglong]
+    L.mdx_upload_forces.argtypes = [C.c_void_p, _dp, _dp]
+    L.mdx_list_pairs.argtypes = [C.c_void_p, _ip, C.c_longlong, C.POINTER(C.c_longlong), _dp]
+    L.mdx_stress.argtypes = [C.c_void_p, C.c_int, _dp]
     _lib = L
     return L
 
@@ -188,6 +193,31 @@ class Engine:
                                         Q.ctypes.data_as(_dp), L.ctypes.data_as(_dp)))
         return {"x": x, "p": p, "Q": Q, "L": L}
 
+    def noise_counter(self):
+        c = C.c_ulonglong(0)
+        _check(lib().mdx_get_noise_counter(self.h, C.byref(c)))
+        return int(c.value)
+
+    def set_noise_counter(self, counter):
+        _check(lib().mdx_set_noise_counter(self.h, C.c_ulonglong(int(counter))))
+
+    def upload_forces(self, F, T):
+        F, fp = _d(F)
+        T, tp = _d(T)
+        _check(lib().mdx_upload_forces(self.h, fp, tp))
+
+    def list_pairs(self):
+        """the step's neighbour list as (pairs (P, 2) int32, (rl2, rl2w))"""
+        cnt = C.c_longlong(0)
+        r2 = np.zeros(2)
+        rc = lib().mdx_list_pairs(self.h, None, 0, C.byref(cnt), r2.ctypes.data_as(_dp))
+        if rc not in (MDX_OK, MDX_ERR_CAPACITY):
+            _check(rc)
+        out = np.empty((max(cnt.value, 1), 2), dtype=np.int32)
+        _check(lib().mdx_list_pairs(self.h, out.ctypes.data_as(_ip), cnt.value, C.byref(cnt),
+                                    r2.ctypes.data_as(_dp)))
+        return out[: cnt.value], (float(r2[0]), float(r2[1]))
+
     def carried_forces(self):
         F, T = np.empty((self.N, 3)), np.empty((self.N, 3))
         _check(lib().mdx_download_forces(self.h, F.ctypes.data_as(_dp), T.ctypes.data_as(_dp)))
@@ -271,6 +301,11 @@ class Engine:
         out = np.zeros(16)
         _check(lib().mdx_observables(self.h, int(replica), out.ctypes.data_as(_dp)))
         return {"ke_t": out[0:3], "ke_r": out[3:6], "epot": out[6], "virial": out[7:16].reshape(3, 3)}
+
+    def stress(self, replica=-1):
+        out = np.zeros(9)
+        _check(lib().mdx_stress(self.h, int(replica), out.ctypes.data_as(_dp)))
+        return out.reshape(3, 3)
 
     def profile(self, nsteps):
         names = (C.c_char_p * 16)()

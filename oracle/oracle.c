@@ -461,7 +461,8 @@ static void patch_layout(const orc_patchset *ps, int pstart[ORC_MAXTYPES],
 
 static void patch_pair(const orc_box *b, const double *x, const double *Q, const orc_patchset *ps,
                        int pstart[ORC_MAXTYPES], int rstart[ORC_MAXTYPES][ORC_MAXTYPES],
-                       double maxd2, int p1, int p2, double *F, double *T, double *energy)
+                       double maxd2, int p1, int p2, double *F, double *T, double *energy,
+                       double *W)
 {
     /* LangevinR.cpp:923-1044 for one listed pair */
     if (p2 < p1) {
@@ -496,11 +497,19 @@ static void patch_pair(const orc_box *b, const double *x, const double *Q, const
             if (energy) {
                 *energy += orc_kf2_energy(un, dis, Q + 9 * a, Q + 9 * c, nb1, nb2, prm[0], prm[1],
                                           prm[2]);
-                continue;
+                if (!W)
+                    continue;
             }
             double o[9];
             orc_kf2_force_torque(un, dis, Q + 9 * a, Q + 9 * c, nb1, nb2, prm[0], prm[1], prm[2],
                                  o);
+            if (W) {
+                /* force on a times x_a - x_c = un * dis */
+                for (int k = 0; k < 3; k++)
+                    for (int l = 0; l < 3; l++)
+                        W[3 * k + l] += o[k] * (un[l] * dis);
+                continue;
+            }
             F[3 * a + 0] += o[0];
             F[3 * a + 1] += o[1];
             F[3 * a + 2] += o[2];
@@ -523,7 +532,7 @@ void orc_patch_forces(const orc_box *b, int N, const double *x, const double *Q,
     int pstart[ORC_MAXTYPES], rstart[ORC_MAXTYPES][ORC_MAXTYPES];
     patch_layout(ps, pstart, rstart);
     for (long long i = 0; i < np; ++i)
-        patch_pair(b, x, Q, ps, pstart, rstart, maxd2, ij[2 * i], ij[2 * i + 1], F, T, NULL);
+        patch_pair(b, x, Q, ps, pstart, rstart, maxd2, ij[2 * i], ij[2 * i + 1], F, T, NULL, NULL);
 }
 
 double orc_patch_energy(const orc_box *b, int N, const double *x, const double *Q, const int *ij,
@@ -534,7 +543,8 @@ double orc_patch_energy(const orc_box *b, int N, const double *x, const double *
     patch_layout(ps, pstart, rstart);
     double e = 0;
     for (long long i = 0; i < np; ++i)
-        patch_pair(b, x, Q, ps, pstart, rstart, maxd2, ij[2 * i], ij[2 * i + 1], NULL, NULL, &e);
+        patch_pair(b, x, Q, ps, pstart, rstart, maxd2, ij[2 * i], ij[2 * i + 1], NULL, NULL, &e,
+                   NULL);
     return e;
 }
 
@@ -890,6 +900,121 @@ int orc_compute_forces(const orc_box *b, int N, const double *x, const double *Q
     if (!pb.ij || !pb.wca_ij || build_pairs(b, N, x, it, &pb))
         return -1;
     conservative(b, N, x, Q, it, &pb, F, T);
+    free(pb.ij);
+    free(pb.wca_ij);
+    return 0;
+}
+
+void orc_stress(const orc_box *b, int N, const double *x, const int *ij, long long np,
+                double eps, double sigma, double rcut, double out[9])
+{
+    (void)N;
+    /* MD::calculatestress (MD.cpp:2381-2424) with HSPotential::force2mdx
+     * (potential.h:126-139) as the MD's interaction: d from distance3v
+     * (geometry.h:250-276, x_p1 - x_p2 minimum-imaged, not normalised),
+     * stress(k, j) += force2mdx(r^2) * sqrt(r^2) * d_k * d_j */
+    double paramlimit = rcut > 0 ? rcut : 1.12246 * sigma;
+    double s3 = sigma * sigma * sigma;
+    double sigma6 = SQR(s3), sigma12 = SQR(sigma6);
+    for (int k = 0; k < 9; k++)
+        out[k] = 0;
+    for (long long i = 0; i < np; i++) {
+        const double *xa = x + 3 * ij[2 * i], *xc = x + 3 * ij[2 * i + 1];
+        double d[3] = {xa[0] - xc[0], xa[1] - xc[1], xa[2] - xc[2]};
+        if (b->periodic)
+            for (int k = 0; k < 3; k++)
+                if (SQR(d[k]) > SQR(b->l[k] / 2.))
+                    d[k] = d[k] - (d[k] >= 0 ? fabs(b->l[k]) : -fabs(b->l[k]));
+        double dis = SQR(d[0]) + SQR(d[1]) + SQR(d[2]);
+        double f1 = 0;
+        if (dis < SQR(paramlimit)) {
+            double a = dis * dis * dis;
+            f1 = 4 * eps * (12 * sigma12 / (dis * SQR(a)) - 6 * sigma6 / (dis * a));
+        }
+        for (int k = 0; k < 3; k++)
+            for (int j = 0; j < 3; j++)
+                out[3 * k + j] += f1 * sqrt(dis) * d[k] * d[j];
+    }
+}
+
+/* pair term force on a (f1 along d = x_a - x_c) into W and U */
+static void ev_pair(double f1, double e, const double d[3], double r, double *U, double W[9])
+{
+    *U += e;
+    for (int k = 0; k < 3; k++) {
+        double fk = f1 * d[k] / r;
+        for (int l = 0; l < 3; l++)
+            W[3 * k + l] += fk * d[l];
+    }
+}
+
+int orc_energy_virial(const orc_box *b, int N, const double *x, const double *Q,
+                      const orc_interactions *it, double *U, double W[9])
+{
+    pairbuf pb = {NULL, 0, 0, NULL, 0};
+    pb.cap = (long long)N * 64 + 64;
+    pb.ij = (int *)malloc(sizeof(int) * 2 * pb.cap);
+    pb.wca_ij = (int *)malloc(sizeof(int) * 2 * pb.cap);
+    if (!pb.ij || !pb.wca_ij || build_pairs(b, N, x, it, &pb))
+        return -1;
+    *U = 0;
+    for (int k = 0; k < 9; k++)
+        W[k] = 0;
+    if (it->wca_on) {
+        /* HSPotential::force and WCAPotential::energy (potential.h:112-125,
+         * 179-185) on the listed pairs minus exclusions */
+        const int *wij = it->nexcl > 0 ? pb.wca_ij : pb.ij;
+        long long nw = it->nexcl > 0 ? pb.nwca : pb.np;
+        for (long long i = 0; i < nw; i++) {
+            double d[3], dis;
+            orc_distance_vector(b, x + 3 * wij[2 * i], x + 3 * wij[2 * i + 1], d, &dis);
+            double r = sqrt(dis);
+            ev_pair(orc_hs_force(r, it->wca_eps, it->wca_sigma, it->wca_rcut),
+                    orc_wca_energy(r, it->wca_eps, it->wca_sigma, it->wca_rcut), d, r, U, W);
+        }
+    }
+    for (long long i = 0; i < it->nbonds; i++) {
+        /* HarmonicPotential / FENEPotential energy and force (potential.h:265-297, 345-376) */
+        double d[3], dis;
+        orc_distance_vector(b, x + 3 * it->bonds[2 * i], x + 3 * it->bonds[2 * i + 1], d, &dis);
+        double r = sqrt(dis), k = it->bond_k[i], r0 = it->bond_r0[i], f1, e;
+        if (it->bond_kind == 2) {
+            f1 = -k * r / (1 - SQR(r) / SQR(r0));
+            e = r < r0 ? -(k / 2.) * SQR(r0) * log(1.0 - SQR(r) / SQR(r0)) : 1.E10 * SQR(r);
+        } else {
+            f1 = -k * (r - r0);
+            e = 0.5 * k * SQR(r - r0);
+        }
+        ev_pair(f1, e, d, r, U, W);
+    }
+    for (long long t = 0; t < it->nangles; t++) {
+        /* BendingPotential::energy (potential3.h:32-41) or k (1 - cos theta);
+         * the triplet's virial sum_i F_i x_i = -F1 ab + F3 bc */
+        int p1 = it->angles[3 * t], p2 = it->angles[3 * t + 1], p3 = it->angles[3 * t + 2];
+        double ab[3], bc[3], d1, d2, F[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        orc_distance_vector(b, x + 3 * p2, x + 3 * p1, ab, &d1);
+        orc_distance_vector(b, x + 3 * p3, x + 3 * p2, bc, &d2);
+        double xs[9] = {0, 0, 0, ab[0], ab[1], ab[2], ab[0] + bc[0], ab[1] + bc[1], ab[2] + bc[2]};
+        int tri[3] = {0, 1, 2};
+        double kk = it->angle_k[t], th0 = it->angle_theta0[t];
+        orc_box open = {{1e300, 1e300, 1e300}, 0};
+        orc_angle_forces(&open, 3, xs, tri, 1, it->angle_form, &kk, &th0, F);
+        double a = dot3(ab, bc), mab = sqrt(dot3(ab, ab)), mbc = sqrt(dot3(bc, bc));
+        if (it->angle_form == 0)
+            *U += 0.5 * kk * SQR(acos(a / (mab * mbc)) - th0);
+        else
+            *U += kk * (1 - a / sqrt(dot3(ab, ab) * dot3(bc, bc)));
+        for (int k = 0; k < 3; k++)
+            for (int l = 0; l < 3; l++)
+                W[3 * k + l] += -F[k] * ab[l] + F[6 + k] * bc[l];
+    }
+    if (it->patches) {
+        int pstart[ORC_MAXTYPES], rstart[ORC_MAXTYPES][ORC_MAXTYPES];
+        patch_layout(it->patches, pstart, rstart);
+        for (long long i = 0; i < pb.np; ++i)
+            patch_pair(b, x, Q, it->patches, pstart, rstart, it->maxd2, pb.ij[2 * i],
+                       pb.ij[2 * i + 1], NULL, NULL, U, W);
+    }
     free(pb.ij);
     free(pb.wca_ij);
     return 0;

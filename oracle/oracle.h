@@ -246,6 +246,25 @@ int orc_calcg1(const orc_box *b, int N, const double *x, double max, double dx,
 long long orc_dgraph_labels(const orc_box *b, int N, const double *x, const int n[3], double cut,
                             int lo, int hi, int *labels);
 
+/* MD::calculatestress (MD.cpp:2381-2424) over the pairs ij with the MD's
+ * interaction an HSPotential(eps, sigma) (force2mdx, potential.h:126-139;
+ * rcut <= 0: 1.12246 sigma): out[3k+j] = sum f(r^2) sqrt(r^2) d_k d_j with
+ * d = x_i - x_j minimum-imaged (distance3v, geometry.h:250-276) */
+void orc_stress(const orc_box *b, int N, const double *x, const int *ij, long long np,
+                double eps, double sigma, double rcut, double out[9]);
+
+/* Energy and virial of the conservative terms on a frozen configuration
+ * (pairs built as orc_compute_forces builds them): U = WCA energy
+ * (WCAPotential::energy, attrepsilon 0) on the listed pairs minus exclusions
+ * + KernFrenkelOnePatch2::energy on the listed pairs within maxd2 + bond
+ * energies (HarmonicPotential / FENEPotential::energy) + angle energies
+ * (BendingPotential::energy, potential3.h:32-41, or k (1 - cos theta));
+ * W[3k+l] = sum_i F_i,k x_i,l over the terms: pairs F_a,k (x_a - x_c)_l
+ * minimum-imaged, triplets -F1,k ab_l + F3,k bc_l.  The active force is not
+ * conservative and is left out.  Returns 0, or -1 when out of memory. */
+int orc_energy_virial(const orc_box *b, int N, const double *x, const double *Q,
+                      const orc_interactions *it, double *U, double W[9]);
+
 /* measured_temperature (LangevinR.cpp:170-193): out[0..2] <p^2/2m>, out[3..5] <L^2/2I0> */
 void orc_measured_temperature(const orc_params *P, int N, const double *p, const double *L,
                               double out[6]);

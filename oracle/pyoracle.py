@@ -115,6 +115,11 @@ def lib():
         L.orc_run_beads.argtypes = [C.POINTER(Params), C.POINTER(Box), C.c_int, _dp, _dp,
                                     C.POINTER(Interactions), C.c_int, _dp, C.c_int, _dp]
         L.orc_measured_temperature.argtypes = [C.POINTER(Params), C.c_int, _dp, _dp, _dp]
+        L.orc_stress.argtypes = [C.POINTER(Box), C.c_int, _dp, _ip, C.c_longlong, C.c_double,
+                                 C.c_double, C.c_double, _dp]
+        L.orc_energy_virial.restype = C.c_int
+        L.orc_energy_virial.argtypes = [C.POINTER(Box), C.c_int, _dp, _dp,
+                                        C.POINTER(Interactions), _dp, _dp]
         _lib = L
     return _lib
 
@@ -271,6 +276,35 @@ def compute_forces(s, list_cut=None, n_side=None, x=None, Q=None):
     if rc:
         raise RuntimeError("orc_compute_forces failed")
     return F, T
+
+
+def energy_virial(s, list_cut=None, n_side=None, x=None):
+    """total conservative energy U and virial W (3 x 3, W[k, l] = sum_i
+    F_i,k x_i,l over the terms) on a frozen configuration"""
+    keep = _Keep()
+    it = make_interactions(s, keep, list_cut, 20, n_side)
+    b = make_box(s["box"], s.get("periodic", True))
+    x = s["x"] if x is None else x
+    N = len(x)
+    Q = s.get("Q")
+    U = np.zeros(1)
+    W = np.zeros(9)
+    rc = lib().orc_energy_virial(C.byref(b), N, keep.d(x),
+                                 keep.d(Q if Q is not None else np.tile(np.eye(3).ravel(), (N, 1))),
+                                 C.byref(it), U.ctypes.data_as(_dp), W.ctypes.data_as(_dp))
+    if rc:
+        raise RuntimeError("orc_energy_virial failed")
+    return float(U[0]), W.reshape(3, 3)
+
+
+def stress(box, x, ij, eps=1.0, sigma=1.0, rcut=0.0, periodic=True):
+    """MD::calculatestress over the pairs ij with an HSPotential interaction"""
+    b = make_box(box, periodic)
+    keep = _Keep()
+    out = np.zeros(9)
+    lib().orc_stress(C.byref(b), len(x), keep.d(x), keep.i(ij), len(ij), float(eps),
+                     float(sigma), float(rcut), out.ctypes.data_as(_dp))
+    return out.reshape(3, 3)
 
 
 def patch_energy(s, list_cut=3.5, n_side=None):

@@ -25,6 +25,9 @@
  *   long_rotor x,Q -> snapshots of a long Condensate::run loop (rand() noise)
  *   calcg1    x    -> calcg1 pair-distance histogram (Analysis/PairCorrelation.cpp)
  *   csv       x    -> the matrix written by the reference's CSV operator
+ *   temperature p, L -> LangevinNVTR::measured_temperature
+ *   pairpot   r    -> WCAPotential::energy, HSPotential::force and force2mdx
+ *   stress    x    -> MD::calculatestress with an HSPotential interaction
  */
 #include <stdlib.h>
 #include <iostream>
@@ -654,6 +657,71 @@ static int cmd_csv()
     return 0;
 }
 
+static int cmd_temperature()
+{
+    /* LangevinNVTR::measured_temperature(vector1<double>&) (LangevinR.cpp:170-193)
+       on given momenta and angular momenta */
+    int N = (int)ji("N");
+    cube geo = make_cube();
+    LangevinNVTR b(geo);
+    matrix<double> x(N, 3);
+    matrix<double> p = to_mat(rd<double>("p", (size_t)N * 3), N, 3);
+    matrix<double> L = to_mat(rd<double>("L", (size_t)N * 3), N, 3);
+    matrix<double> Q(N, 9);
+    for (int i = 0; i < N; i++)
+        Q(i, 0) = Q(i, 4) = Q(i, 8) = 1.0;
+    b.initialize(x, p, Q, L);
+    if (g_job.count("m"))
+        b.setm(jd("m"));
+    vector1<double> out(6);
+    b.measured_temperature(out);
+    vector<double> v(6);
+    for (int k = 0; k < 6; k++)
+        v[k] = out[k];
+    wr("temperature", v.data(), v.size());
+    return 0;
+}
+
+static int cmd_pairpot()
+{
+    /* WCAPotential::energy and HSPotential::force / force2mdx at given r
+       (potential.h:79-200) */
+    int n = (int)ji("n");
+    vector<double> r = rd<double>("r", (size_t)n);
+    WCAPotential w(jd("eps"), jd("sigma"), jd("attr", 0.0));
+    HSPotential h(jd("eps"), jd("sigma"));
+    vector<double> e(n), f(n), f2(n);
+    for (int i = 0; i < n; i++) {
+        e[i] = w.energy(r[i]);
+        f[i] = h.force(r[i]);
+        f2[i] = h.force2mdx(r[i] * r[i]);
+    }
+    wr("energy", e.data(), e.size());
+    wr("force", f.data(), f.size());
+    wr("force2mdx", f2.data(), f2.size());
+    return 0;
+}
+
+static int cmd_stress()
+{
+    /* MD::calculatestress (MD.cpp:2381-2424) with an HSPotential as the MD's
+       interaction (MD::setinteractions), over the pairs of calculatepairs */
+    int N = (int)ji("N");
+    cube geo = make_cube();
+    LangevinNVT b(geo);
+    matrix<double> x = to_mat(rd<double>("x", (size_t)N * 3), N, 3);
+    b.setdat(x);
+    HSPotential h(jd("wca_eps"), jd("wca_sigma"));
+    b.setinteractions(h);
+    matrix<int> *pairs = make_pairs(b, geo, (int)ji("n"), jd("cut"));
+    matrix<double> st = b.calculatestress(*pairs);
+    vector<double> v = from_mat(st);
+    wr("stress", v.data(), v.size());
+    write_pairs(pairs);
+    delete pairs;
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc != 3)
@@ -683,6 +751,12 @@ int main(int argc, char **argv)
         return cmd_csv();
     if (cmd == "dgraph")
         return cmd_dgraph();
+    if (cmd == "temperature")
+        return cmd_temperature();
+    if (cmd == "pairpot")
+        return cmd_pairpot();
+    if (cmd == "stress")
+        return cmd_stress();
     die("unknown command " + cmd);
     return 2;
 }

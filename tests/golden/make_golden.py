@@ -351,12 +351,12 @@ def fixture_csv():
     np.savez_compressed(os.path.join(OUT, "csv.npz"), x=x, text=np.array(text))
 
 
-def fixture_traj_rotor(N, nsteps=1000, snap=100, seed=77):
+def fixture_traj_rotor(N, nsteps=1000, snap=100, seed=77, periodic=1):
     s = wl.patchy_tetra(N)
     L = float(s["box"][0])
     n = max(int(L // 4), 1)
-    d = job_dir(f"traj_rotor_{N}")
-    job = {"N": N, "L": repr(L), "periodic": 1, "n": n, "cut": 3.5, "rebuild": 20,
+    d = job_dir(f"traj_rotor_{N}_{periodic}")
+    job = {"N": N, "L": repr(L), "periodic": periodic, "n": n, "cut": 3.5, "rebuild": 20,
            "wca_eps": 1.0, "wca_sigma": 1.0, "dt": s["dt"], "gamma": s["gamma"],
            "gammar": repr(float(s["gammar"])), "kT": s["kT"], "nsteps": nsteps, "snap_every": snap,
            "noise_seed": seed}
@@ -369,7 +369,58 @@ def fixture_traj_rotor(N, nsteps=1000, snap=100, seed=77):
            "Q_out": po.read_ref(d, "Q_out").reshape(N, 9),
            "L_out": po.read_ref(d, "L_out").reshape(N, 3),
            "snaps": po.read_ref(d, "snaps").reshape(-1, N, 3)}
-    np.savez_compressed(os.path.join(OUT, f"traj_rotor{N}.npz"), **out)
+    name = f"traj_rotor{N}.npz" if periodic else f"traj_rotor_walls{N}.npz"
+    np.savez_compressed(os.path.join(OUT, name), **out)
+
+
+def fixture_traj_walls():
+    """1000 steps of the reference's Condensate::run loop with reflecting
+    walls (cube with pb = false, the reference's default:
+    Condensate.cpp:17-18; geometry.h:433-444)"""
+    fixture_traj_rotor(256, seed=79, periodic=0)
+
+
+def fixture_observables():
+    """LangevinNVTR::measured_temperature (LangevinR.cpp:170-193),
+    WCAPotential::energy / HSPotential::force / force2mdx
+    (potential.h:79-200) at given distances, and MD::calculatestress
+    (MD.cpp:2381-2424) with an HSPotential interaction"""
+    out = {}
+    rng = np.random.default_rng(23)
+    for tag, m in (("m1", 1.0), ("m2", 2.0)):
+        N = 777
+        p = rng.normal(0, 1.3, (N, 3))
+        L = rng.normal(0, 0.4, (N, 3))
+        d = job_dir(f"temperature_{tag}")
+        po.run_ref("temperature", d, {"N": N, "L": 10.0, "periodic": 1, "m": repr(m)},
+                   {"p": p, "L": L})
+        out[f"temp_{tag}_p"] = p
+        out[f"temp_{tag}_L"] = L
+        out[f"temp_{tag}_m"] = m
+        out[f"temp_{tag}_out"] = po.read_ref(d, "temperature")
+    for tag, eps, sig in (("e1s1", 1.0, 1.0), ("e25s13", 2.5, 1.3)):
+        pl = 1.12246 * sig
+        r = np.concatenate([np.linspace(0.75 * sig, 1.3 * sig, 200),
+                            [pl, np.nextafter(pl, 0), np.nextafter(pl, 10), sig]])
+        d = job_dir(f"pairpot_{tag}")
+        po.run_ref("pairpot", d, {"n": len(r), "eps": repr(eps), "sigma": repr(sig)}, {"r": r})
+        out[f"pot_{tag}_r"] = r
+        out[f"pot_{tag}_eps"] = eps
+        out[f"pot_{tag}_sigma"] = sig
+        for k in ("energy", "force", "force2mdx"):
+            out[f"pot_{tag}_{k}"] = po.read_ref(d, k)
+    for tag, s in (("c1", wl.patchy_tetra()), ("c3", wl.condensate_ab(4096))):
+        L = float(s["box"][0])
+        n = int(L // 4)
+        d = job_dir(f"stress_{tag}")
+        po.run_ref("stress", d, {"N": s["N"], "L": repr(L), "periodic": 1, "n": n, "cut": 3.5,
+                                 "wca_eps": 1.0, "wca_sigma": 1.0}, {"x": s["x"]})
+        out[f"stress_{tag}_x"] = s["x"]
+        out[f"stress_{tag}_L"] = L
+        out[f"stress_{tag}_n"] = n
+        out[f"stress_{tag}_out"] = po.read_ref(d, "stress").reshape(3, 3)
+        out[f"stress_{tag}_npairs"] = len(po.read_ref(d, "pairs", np.int32)) // 2
+    np.savez_compressed(os.path.join(OUT, "observables.npz"), **out)
 
 
 def fixture_traj_bead(nsteps=1000, snap=100, seed=88):
@@ -407,6 +458,8 @@ def main():
     fixture_calcg1()
     fixture_csv()
     fixture_dgraph()
+    fixture_traj_walls()
+    fixture_observables()
     print("fixtures written to", OUT)
 
 

@@ -161,3 +161,56 @@ def test_distance_graph_labels_are_component_minima(golden, case):
     first = np.full(comp.max() + 1, N)
     np.minimum.at(first, comp, np.arange(N))
     assert np.array_equal(first[comp], g[f"{case}_labels"])
+
+
+def test_rotor_trajectory_reflecting_walls_1000_steps(golden):
+    """The Condensate::run loop in a box with reflecting walls (cube with
+    pb = false, the reference's default: Condensate.cpp:17-18; reflection
+    in cube::correct_position_and_momentum, geometry.h:433-444; distances
+    without minimum image)"""
+    g = golden("traj_rotor_walls256.npz")
+    N = len(g["x"])
+    s = wl.patchy_tetra(N)
+    s["periodic"] = False
+    assert np.array_equal(s["x"], g["x"])
+    r = po.run_rotors(s, int(g["nsteps"]), int(g["noise_seed"]), int(g["snap_every"]), 3.5, 20,
+                      int(g["n"]))
+    assert np.max(np.abs(r["snaps"][0] - g["snaps"][0])) < 1e-13
+    assert np.max(np.abs(r["x"] - g["x_out"])) < 1e-7
+    assert relmax(r["p"], g["p_out"]) < 1e-6
+    # the walls were hit: some particle ends within 0.1 of a face
+    L = float(g["box"])
+    assert np.min(np.minimum(g["x_out"], L - g["x_out"])) < 0.1
+
+
+@pytest.mark.parametrize("tag", ["m1", "m2"])
+def test_measured_temperature_vs_reference(golden, tag):
+    """LangevinNVTR::measured_temperature (LangevinR.cpp:170-193)"""
+    g = golden("observables.npz")
+    s = wl.patchy_tetra(64)
+    s["m"] = float(g[f"temp_{tag}_m"])
+    out = po.measured_temperature(s, g[f"temp_{tag}_p"], g[f"temp_{tag}_L"])
+    assert relmax(out, g[f"temp_{tag}_out"]) < 1e-15
+
+
+@pytest.mark.parametrize("tag", ["e1s1", "e25s13"])
+def test_pair_potential_values_vs_reference(golden, tag):
+    """WCAPotential::energy (with its +epsilon shift) and HSPotential::force at
+    given distances, including the cut-off 1.12246 sigma and its neighbours"""
+    g = golden("observables.npz")
+    eps, sig = float(g[f"pot_{tag}_eps"]), float(g[f"pot_{tag}_sigma"])
+    r = g[f"pot_{tag}_r"]
+    e = np.array([po.lib().orc_wca_energy(float(v), eps, sig, 0) for v in r])
+    f = np.array([po.lib().orc_hs_force(float(v), eps, sig, 0) for v in r])
+    assert np.array_equal(e, g[f"pot_{tag}_energy"])
+    assert np.array_equal(f, g[f"pot_{tag}_force"])
+
+
+@pytest.mark.parametrize("tag", ["c1", "c3"])
+def test_stress_vs_reference(golden, tag):
+    """MD::calculatestress (MD.cpp:2381-2424) over the 3.5 Verlet list"""
+    g = golden("observables.npz")
+    L, n, x = float(g[f"stress_{tag}_L"]), int(g[f"stress_{tag}_n"]), g[f"stress_{tag}_x"]
+    pr = po.pairs_reference(L, x, n, 3.5)
+    assert len(pr) == int(g[f"stress_{tag}_npairs"])
+    assert relmax(po.stress(L, x, pr), g[f"stress_{tag}_out"]) < 1e-14

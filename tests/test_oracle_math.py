@@ -306,3 +306,140 @@ def test_injected_noise_moments():
     assert abs(z.var() - 1.0) < 0.01
     assert z.min() >= -1.732050808 and z.max() <= 1.732050808 + 1e-9
     assert np.array_equal(po.noise(123, 1000, 10), z[1000:1010])
+
+
+def relmax(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def brute_pairs_ortho(x, box, cut, periodic=True):
+    box = np.asarray(box, dtype=np.float64)
+    d = x[:, None, :] - x[None, :, :]
+    if periodic:
+        d = np.where(np.abs(d) > box * 0.5, d - np.sign(d) * box, d)
+    r2 = np.sum(d * d, axis=2)
+    i, j = np.nonzero(np.triu(r2 < cut * cut, 1))
+    return set(zip(i.tolist(), j.tolist()))
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_orthorhombic_pair_set_brute_force(periodic):
+    """The oracle's box is an orthorhombic generalisation of the reference's
+    cube (per-axis edges and cell counts); its pair set equals an O(N^2)
+    search with the per-axis minimum image, on a 1:1.5:3 box"""
+    rng = np.random.default_rng(12)
+    box = np.array([9.0, 13.5, 27.0])
+    x = rng.uniform(0, 1, (900, 3)) * box
+    for cut in (1.12246, 1.949, 3.5):
+        n = np.maximum(np.floor(box / cut), 1).astype(int)
+        pr = po.pairs_reference(box, x, n, cut, periodic)
+        got = {(min(a, b), max(a, b)) for a, b in pr.tolist()}
+        assert len(got) == len(pr)
+        assert got == brute_pairs_ortho(x, box, cut, periodic)
+
+
+def test_orthorhombic_wca_forces_brute_force():
+    """WCA forces on an orthorhombic periodic box against a vectorised
+    all-pairs evaluation of 24 eps (2 sigma^12 / r^14 - sigma^6 / r^8) d with
+    the per-axis minimum image (the closed form of HSPotential::force)"""
+    rng = np.random.default_rng(13)
+    box = np.array([8.0, 12.0, 24.0])
+    x = rng.uniform(0, 1, (700, 3)) * box
+    s = {"N": len(x), "box": box, "periodic": True, "x": x, "Q": None,
+         "wca": (1.0, 1.0, 1.12246), "patches": None}
+    F, _ = po.compute_forces(s, list_cut=1.5)
+    d = x[:, None, :] - x[None, :, :]
+    d = np.where(np.abs(d) > box * 0.5, d - np.sign(d) * box, d)
+    r2 = np.sum(d * d, axis=2)
+    np.fill_diagonal(r2, np.inf)
+    m = r2 < 1.12246 ** 2
+    f = np.where(m, 24 * (2 / r2 ** 7 - 1 / r2 ** 4), 0.0)
+    Fb = np.sum(f[:, :, None] * d, axis=1)
+    assert np.count_nonzero(m) > 20
+    assert relmax(F, Fb) < 1e-12
+
+
+def _open_system(builder):
+    """a system moved into a box with reflecting walls and a margin, so an
+    affine deformation of the positions deforms every pair vector exactly"""
+    s = builder()
+    s = dict(s)
+    s["periodic"] = False
+    x = np.array(s["x"], dtype=np.float64)
+    box = np.asarray(s["box"], dtype=np.float64)
+    # unwrap chains first: bead chains of the generator may cross the faces
+    if s.get("bonds") is not None:
+        for a, b in np.asarray(s["bonds"]):
+            dd = x[b] - x[a]
+            x[b] = x[a] + dd - box * np.round(dd / box)
+    x = x - x.min(0) + 5.0
+    s["box"] = np.full(3, float(x.max() + 5.0))
+    s["x"] = x
+    return s
+
+
+@pytest.mark.parametrize("builder", [
+    lambda: wl.patchy_tetra(400, phi=0.2),
+    lambda: wl.condensate_ab(600, phi=0.2),
+    lambda: wl.active_polymer(nchains=6, nbeads=30, phi=0.1, seed=3, fa=0.0),
+    lambda: wl.mt_bundle(1, 13, 12, spacing=6.0, box=(8.0, 8.0, 14.0)),
+])
+def test_energy_virial_is_minus_strain_derivative(builder):
+    """orc_energy_virial's W[k, l] = sum_i F_i,k x_i,l equals
+    -dU/d eps of the energy under the affine deformation x -> (1 + eps e_k e_l^T) x
+    (central differences), for every k, l: pins the virial of the pair, patch
+    (non-central forces: W is not symmetric), bond and three-body terms on
+    the energy, which the reference fixtures pin"""
+    s = _open_system(builder)
+    U, W = po.energy_virial(s)
+    assert W.any()
+    h = 1e-6
+    for k in range(3):
+        for l in range(3):
+            E = np.zeros((3, 3))
+            E[k, l] = 1.0
+            xp = s["x"] + h * s["x"] @ E.T
+            xm = s["x"] - h * s["x"] @ E.T
+            Up, _ = po.energy_virial(s, x=xp)
+            Um, _ = po.energy_virial(s, x=xm)
+            fd = -(Up - Um) / (2 * h)
+            assert abs(fd - W[k, l]) < 2e-5 * max(1.0, np.abs(W).max()), (k, l, fd, W[k, l])
+
+
+def test_energy_parts_pinned():
+    """U of orc_energy_virial = the WCA energies (fixture-pinned
+    orc_wca_energy) + the patch energy (fixture-pinned orc_patch_energy) on
+    the same list; for bonds and angles, -dU/dx equals the fixture-pinned
+    forces and U vanishes at the rest length / rest angle"""
+    s = wl.patchy_tetra(1000, phi=0.2)
+    U, _ = po.energy_virial(s, list_cut=3.5)
+    L = float(s["box"][0])
+    pr = po.pairs_reference(L, s["x"], int(L // 4), 3.5)
+    d = s["x"][pr[:, 0]] - s["x"][pr[:, 1]]
+    d = np.where(np.abs(d) > L / 2, d - np.sign(d) * L, d)
+    r = np.sqrt(np.sum(d * d, 1))
+    ew = sum(po.lib().orc_wca_energy(float(v), 1.0, 1.0, 0) for v in r)
+    assert abs(U - (ew + po.patch_energy(s))) < 1e-12 * max(1.0, abs(U))
+    # bonds and angles: energy derivative = force
+    c = _open_system(lambda: wl.active_polymer(nchains=4, nbeads=12, phi=0.1, seed=5, fa=0.0))
+    c["wca"] = None
+    c["patches"] = None
+    for form in (0, 1):
+        c["angle_form"] = form
+        c["angle_theta0"] = np.full(len(c["angles"]), 0.4 if form == 0 else 0.0)
+        U0, _ = po.energy_virial(c)
+        F, _ = po.compute_forces(c)
+        i = 7
+        for k in range(3):
+            xp, xm = c["x"].copy(), c["x"].copy()
+            xp[i, k] += 1e-6
+            xm[i, k] -= 1e-6
+            fd = -(po.energy_virial(c, x=xp)[0] - po.energy_virial(c, x=xm)[0]) / 2e-6
+            assert abs(fd - F[i, k]) < 1e-5 * max(1.0, np.abs(F).max())
+    rest = {"N": 3, "box": np.full(3, 50.0), "periodic": False,
+            "x": np.array([[10.0, 10, 10], [11.0, 10, 10], [11.0 + np.cos(0.4), 10 + np.sin(0.4), 10]]),
+            "Q": None, "wca": None, "patches": None, "bonds": np.array([[0, 1], [1, 2]]),
+            "bond_k": np.array([100.0, 100.0]), "bond_r0": np.array([1.0, 1.0]), "bond_kind": 1,
+            "angles": np.array([[0, 1, 2]]), "angle_k": np.array([50.0]),
+            "angle_theta0": np.array([0.4]), "angle_form": 0}
+    assert abs(po.energy_virial(rest)[0]) < 1e-20
