@@ -171,10 +171,11 @@ int mdx_neighbor_pairs(mdx_engine *e, int replica, const int n[3], double cut, i
  * step's own kernels build at the current positions (pairs within reach +
  * skin; type pairs without patch rows within the WCA reach + skin), as
  * unordered pairs of original indices (i < j, ascending), at most cap;
- * r2cut (nullable, 2 doubles) receives the two squared radii used.  Checks
- * that the full list holds each pair from both sides.  The next mdx_step
- * re-sorts and rebuilds its list. */
-int mdx_list_pairs(mdx_engine *e, int *ij, long long cap, long long *npairs, double *r2cut);
+ * radii (nullable, 2 doubles) receives the list radius and the radius of the
+ * type pairs without patch rows (a pair is listed when r^2 is below the
+ * square of its radius).  Checks that the full list holds each pair from
+ * both sides.  The next mdx_step re-sorts and rebuilds its list. */
+int mdx_list_pairs(mdx_engine *e, int *ij, long long cap, long long *npairs, double *radii);
 /* conservative force F (lab) and patch torque T (lab) at the current state:
  * calculateforces(pairs, HSPotential) + bonds + angles + active +
  * calculate_forces_and_torques3D(pairs, ComboPatch, F, T) (LangevinR.cpp:911-1047) */

@@ -813,6 +813,9 @@ template <class R> static int enqueue_rotor(mdx_engine *e, int nsteps, const dou
                                             hipEvent_t *ev = nullptr, hipEvent_t *kev = nullptr)
 {
     int rc;
+    // the first kernel reads the replica table and bins into the cell grid
+    if ((rc = build_grid(e)))
+        return rc;
     launch_small(e, k_set_call_base);
     {
         KArgs<R> a = make_args<R>(e);
@@ -844,9 +847,14 @@ template <class R> static int enqueue_rotor(mdx_engine *e, int nsteps, const dou
 
 template <class R> static int do_step_rotor(mdx_engine *e, int nsteps, const double *noise)
 {
-    if (!e->primed)
-        return set_err(MDX_ERR_STATE, "mdx_step before mdx_prime (the carried F, T are unset)");
     int rc;
+    // a change of the interactions or of the list skin since mdx_prime
+    // rebuilds the grid and drops the carried forces computed with the old ones
+    if ((rc = build_grid(e)))
+        return rc;
+    if (!e->primed)
+        return set_err(MDX_ERR_STATE, "mdx_step before mdx_prime or mdx_upload_forces (the carried "
+                                      "F, T are unset, or the interactions changed since)");
     const double *nd;
     if ((rc = noise_upload(e, noise, (size_t)nsteps * e->N * 6, &nd)))
         return rc;
@@ -1965,7 +1973,9 @@ template <class R> static int upload_forces_t(mdx_engine *e, const double *F, co
 {
     const size_t n3 = 3 * (size_t)e->N;
     int rc;
-    if ((rc = upload(e, e->tmpd, F, n3)) || (rc = upload(e, e->tmpx, T, n3)))
+    // the step that follows bins the particles: the cell grid (replica table,
+    // cell counts) must exist, as mdx_prime would have built it
+    if ((rc = build_grid(e)) || (rc = upload(e, e->tmpd, F, n3)) || (rc = upload(e, e->tmpx, T, n3)))
         return rc;
     const int c = e->cur;
     hipLaunchKernelGGL((k_by_id_to_slots<R>), dim3(nblk(e->N)), dim3(256), 0, e->stream,
@@ -2001,7 +2011,7 @@ int mdx_upload_forces(mdx_engine *e, const double *F, const double *T)
 // set of unordered pairs (original indices, i < j, ascending).  The list is
 // full (each pair from both sides): both sides are checked to agree.
 template <class R>
-static int list_pairs_t(mdx_engine *e, int *ij, long long cap, long long *npairs, double *r2)
+static int list_pairs_t(mdx_engine *e, int *ij, long long cap, long long *npairs, double *radii)
 {
     if (!(e->skin > 0))
         return set_err(MDX_ERR_STATE, "no neighbour list (skin 0)");
@@ -2044,23 +2054,26 @@ static int list_pairs_t(mdx_engine *e, int *ij, long long cap, long long *npairs
         ij[2 * k] = (int)(fwd[k] >> 32);
         ij[2 * k + 1] = (int)(fwd[k] & 0xffffffffll);
     }
-    if (r2) {
+    if (radii) {
+        // the build compares r^2 with the squares of these (make_args)
         KArgs<R> a = make_args<R>(e);
-        r2[0] = a.rl2;
-        r2[1] = a.rl2w;
+        const double rl = reach(e) + e->skin;
+        const double rw = e->wca ? e->wca_pl * (1 + 1e-12) + e->skin : 0.0;
+        radii[0] = rl;
+        radii[1] = a.rl2w == a.rl2 ? rl : rw;
     }
     if (*npairs > cap)
         return set_err(MDX_ERR_CAPACITY, "%lld pairs, capacity %lld", *npairs, cap);
     return MDX_OK;
 }
 
-int mdx_list_pairs(mdx_engine *e, int *ij, long long cap, long long *npairs, double *r2cut)
+int mdx_list_pairs(mdx_engine *e, int *ij, long long cap, long long *npairs, double *radii)
 {
     if (!e || !npairs || cap < 0 || (cap > 0 && !ij))
         return set_err(MDX_ERR_ARG, "bad arguments");
     HIPCHK(hipSetDevice(e->device));
-    return e->prec == MDX_PREC_FP64 ? list_pairs_t<double>(e, ij, cap, npairs, r2cut)
-                                    : list_pairs_t<float>(e, ij, cap, npairs, r2cut);
+    return e->prec == MDX_PREC_FP64 ? list_pairs_t<double>(e, ij, cap, npairs, radii)
+                                    : list_pairs_t<float>(e, ij, cap, npairs, radii);
 }
 
 int mdx_info(mdx_engine *e, double *out)

@@ -207,7 +207,8 @@ class Engine:
         _check(lib().mdx_upload_forces(self.h, fp, tp))
 
     def list_pairs(self):
-        """the step's neighbour list as (pairs (P, 2) int32, (rl2, rl2w))"""
+        """the step's neighbour list as (pairs (P, 2) int32, (list radius,
+        radius of the type pairs without patch rows))"""
         cnt = C.c_longlong(0)
         r2 = np.zeros(2)
         rc = lib().mdx_list_pairs(self.h, None, 0, C.byref(cnt), r2.ctypes.data_as(_dp))
