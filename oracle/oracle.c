@@ -815,14 +815,38 @@ void orc_noise_fill(uint64_t seed, uint64_t first, long long count, double *out)
 
 /* ------------------------------------------------------- step compositions */
 
-static int is_excluded(const orc_interactions *it, int a, int c)
+static int cmp_ll(const void *a, const void *b)
 {
-    for (long long k = 0; k < it->nexcl; k++) {
-        int e0 = it->excl[2 * k], e1 = it->excl[2 * k + 1];
-        if ((e0 == a && e1 == c) || (e0 == c && e1 == a))
-            return 1;
+    long long x = *(const long long *)a, y = *(const long long *)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* the exclusion pairs as sorted keys min * 2^32 + max, for a binary search */
+static long long *exclusion_keys(const orc_interactions *it)
+{
+    long long *k = (long long *)malloc(sizeof(long long) * (size_t)(it->nexcl > 0 ? it->nexcl : 1));
+    if (!k)
+        return NULL;
+    for (long long q = 0; q < it->nexcl; q++) {
+        long long e0 = it->excl[2 * q], e1 = it->excl[2 * q + 1];
+        k[q] = e0 < e1 ? (e0 << 32) | e1 : (e1 << 32) | e0;
     }
-    return 0;
+    qsort(k, (size_t)it->nexcl, sizeof(long long), cmp_ll);
+    return k;
+}
+
+static int is_excluded(const long long *keys, long long n, int a, int c)
+{
+    long long key = a < c ? ((long long)a << 32) | c : ((long long)c << 32) | a;
+    long long lo = 0, hi = n;
+    while (lo < hi) {
+        long long mid = (lo + hi) / 2;
+        if (keys[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo < n && keys[lo] == key;
 }
 
 typedef struct {
@@ -849,14 +873,18 @@ static int build_pairs(const orc_box *b, int N, const double *x, const orc_inter
     pb->np = np;
     pb->nwca = 0;
     if (it->nexcl > 0) {
+        long long *keys = exclusion_keys(it);
+        if (!keys)
+            return -1;
         for (long long k = 0; k < np; k++) {
             int a = pb->ij[2 * k], c = pb->ij[2 * k + 1];
-            if (is_excluded(it, a, c))
+            if (is_excluded(keys, it->nexcl, a, c))
                 continue;
             pb->wca_ij[2 * pb->nwca] = a;
             pb->wca_ij[2 * pb->nwca + 1] = c;
             pb->nwca++;
         }
+        free(keys);
     }
     return 0;
 }
