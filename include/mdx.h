@@ -228,6 +228,17 @@ int mdx_observables(mdx_engine *e, int replica, double *out);
  * term; the virial itself is mdx_observables out[7..15]) */
 int mdx_stress(mdx_engine *e, int replica, double *out);
 
+/* mdx_stress without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same result as
+ * mdx_stress (FP64 sums in another order) for any N: its kernels reduce into
+ * a fixed static device scratch of 9 x 2049 doubles (147 KB) per GPU, held
+ * until the process ends.  Needs the engine's cell grid built (after
+ * mdx_prime, mdx_step, mdx_stress or mdx_observables), else MDX_ERR_STATE.
+ * Calls on one GPU share that scratch: calls that overlap (from several
+ * threads or engines on one GPU) race. */
+int mdx_stress_noalloc(mdx_engine *e, int replica, double *out);
+
 /* Single stages of the step, one reference call each, on the engine's state
  * (host arrays by original index, the reference's layouts).  For drivers that
  * compose the step themselves; afterwards mdx_step needs mdx_prime again. */

@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -196,6 +197,8 @@ template <class T> static int download(mdx_engine *e, const Buf &b, T *h, size_t
 }
 
 static inline unsigned nblk(long long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+static int scratch_setup(int dev); // device address of the no-allocation scratch
 
 // interaction reach: WCA cut-off and the patch reach min(sqrt(maxd2), 1.4 d)
 static double reach(const mdx_engine *e)
@@ -986,6 +989,8 @@ int mdx_create(const mdx_desc *d, mdx_engine **out)
     if (d->device < 0 || d->device >= ndev)
         return set_err(MDX_ERR_ARG, "device %d not present (%d devices)", d->device, ndev);
     HIPCHK(hipSetDevice(d->device));
+    if (int src = scratch_setup(d->device))
+        return src;
     mdx_engine *e = new mdx_engine();
     e->nrep = d->nreplicas;
     e->periodic = d->periodic ? 1 : 0;
@@ -1923,6 +1928,77 @@ template <class R> static int stress_t(mdx_engine *e, int replica, double *out)
         out[k] = v;
     }
     return MDX_OK;
+}
+
+// ------------------------------------------------ no-allocation stress
+// Scratch of mdx_stress_noalloc: per-block partial sums of the stress (9
+// doubles per block of a grid of at most NOALLOC_BLOCKS blocks) and the final
+// 9 sums after them.  One copy per GPU, held until the process ends; host
+// code uses the address hipGetSymbolAddress gives for the current device
+// (g_scratch_ptr), never the array's name.
+constexpr int NOALLOC_BLOCKS = 2048;
+static __device__ __align__(256) double scratch[9 * (NOALLOC_BLOCKS + 1)];
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[MAX_DEVICES];
+
+// the device address of scratch on device dev (current); once per device
+static int scratch_setup(int dev)
+{
+    if (dev < 0 || dev >= MAX_DEVICES)
+        return set_err(MDX_ERR_ARG, "device %d beyond the %d the library tracks", dev, MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev])
+        return MDX_OK;
+    void *p = nullptr;
+    hipError_t he = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (he != hipSuccess || !p)
+        return set_err(MDX_ERR_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                       hipGetErrorString(he));
+    g_scratch_ptr[dev] = (double *)p;
+    return MDX_OK;
+}
+
+template <class R> static int stress_noalloc_t(mdx_engine *e, int replica, double *out)
+{
+    // the cell grid and its buffers must exist already: building them
+    // allocates
+    if (e->grid_dirty || !e->count.p || !e->start.p || !e->bsum.p)
+        return set_err(MDX_ERR_STATE, "mdx_stress_noalloc needs the cell grid built (call "
+                                      "mdx_prime, mdx_step, mdx_stress or mdx_observables first)");
+    double *scr = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        scr = g_scratch_ptr[e->device];
+    }
+    if (!scr)
+        return set_err(MDX_ERR_STATE, "scratch not set up on device %d", e->device);
+    int rc;
+    if (!e->binned && (rc = bin_current<R>(e)))
+        return rc;
+    KArgs<R> a = make_args<R>(e);
+    const int first = replica < 0 ? 0 : e->first[replica];
+    const int cnt = replica < 0 ? e->N : e->nper[replica];
+    const int nb = (int)std::min<unsigned>(NOALLOC_BLOCKS, nblk(e->N));
+    hipLaunchKernelGGL(k_stress_partial<R>, dim3(nb), dim3(256), 0, e->stream, a, first, cnt, scr);
+    hipLaunchKernelGGL(k_sum_partials9, dim3(1), dim3(64), 0, e->stream, (const double *)scr, nb,
+                       scr + 9 * (size_t)NOALLOC_BLOCKS);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, scr + 9 * (size_t)NOALLOC_BLOCKS, 9 * sizeof(double),
+                          hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return MDX_OK;
+}
+
+int mdx_stress_noalloc(mdx_engine *e, int replica, double *out)
+{
+    if (!e || !out || replica < -1 || replica >= e->nrep)
+        return set_err(MDX_ERR_ARG, "bad arguments");
+    if (!e->wca)
+        return set_err(MDX_ERR_STATE, "calculatestress needs the pair interaction (mdx_set_pair_wca)");
+    HIPCHK(hipSetDevice(e->device));
+    return e->prec == MDX_PREC_FP64 ? stress_noalloc_t<double>(e, replica, out)
+                                    : stress_noalloc_t<float>(e, replica, out);
 }
 
 int mdx_stress(mdx_engine *e, int replica, double *out)

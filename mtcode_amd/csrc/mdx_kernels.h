@@ -2522,6 +2522,26 @@ __global__ void __launch_bounds__(256) k_obs(int first, int count, const R *p, c
 // minimum-imaged as distance3v (geometry.h:250-276); every pair within the
 // cut-off is visited from both sides (the 27 cells of the current cell list)
 template <class R>
+__device__ __forceinline__ void stress_terms(const KArgs<R> &a, int i, const PosRec &xi, double st[9])
+{
+    const DevRep rp = rep_rec(a, xi.meta);
+    const double pl2 = a.wca_pl * a.wca_pl;
+    for_each_neighbor(a, rp, i, [&](const PosRec &xj, int j) {
+        (void)j;
+        double d[3];
+        min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
+        const double dis = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (!(dis < pl2))
+            return;
+        const double x3 = dis * dis * dis;
+        const double f1 = 4 * a.wca_eps * (12 * a.wca_s12 / (dis * (x3 * x3)) - 6 * a.wca_s6 / (dis * x3));
+        for (int k = 0; k < 3; k++)
+            for (int l = 0; l < 3; l++)
+                st[3 * k + l] += 0.5 * (f1 * sqrt(dis) * d[k] * d[l]);
+    });
+}
+
+template <class R>
 __global__ void __launch_bounds__(256) k_stress(KArgs<R> a, double *out)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2529,27 +2549,56 @@ __global__ void __launch_bounds__(256) k_stress(KArgs<R> a, double *out)
         return;
     const int i = a.list[s];
     const PosRec xi = a.X[i];
-    const DevRep rp = rep_rec(a, xi.meta);
-    const double pl2 = a.wca_pl * a.wca_pl;
     double st[9];
     for (int k = 0; k < 9; k++)
         st[k] = 0;
     if (a.wca)
-        for_each_neighbor(a, rp, i, [&](const PosRec &xj, int j) {
-            (void)j;
-            double d[3];
-            min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
-            const double dis = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-            if (!(dis < pl2))
-                return;
-            const double x3 = dis * dis * dis;
-            const double f1 = 4 * a.wca_eps * (12 * a.wca_s12 / (dis * (x3 * x3)) - 6 * a.wca_s6 / (dis * x3));
-            for (int k = 0; k < 3; k++)
-                for (int l = 0; l < 3; l++)
-                    st[3 * k + l] += 0.5 * (f1 * sqrt(dis) * d[k] * d[l]);
-        });
+        stress_terms(a, i, xi, st);
     for (int k = 0; k < 9; k++)
         out[9 * (size_t)xi.id + k] = st[k];
+}
+
+// the same stress terms summed per block (grid-stride over the cell list,
+// particles with original index in [first, first + count)), block sums in
+// a fixed order into partial[9 * block + k]
+template <class R>
+__global__ void __launch_bounds__(256) k_stress_partial(KArgs<R> a, int first, int count,
+                                                        double *partial)
+{
+    __shared__ double sh[256];
+    double st[9];
+    for (int k = 0; k < 9; k++)
+        st[k] = 0;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < a.N; s += gridDim.x * blockDim.x) {
+        const int i = a.list[s];
+        const PosRec xi = a.X[i];
+        if (a.wca && (int)xi.id >= first && (int)xi.id < first + count)
+            stress_terms(a, i, xi, st);
+    }
+    for (int k = 0; k < 9; k++) {
+        sh[threadIdx.x] = st[k];
+        __syncthreads();
+        for (int off = 128; off > 0; off >>= 1) {
+            if ((int)threadIdx.x < off)
+                sh[threadIdx.x] += sh[threadIdx.x + off];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            partial[9 * (size_t)blockIdx.x + k] = sh[0];
+        __syncthreads();
+    }
+}
+
+// out[k] = sum over b < nb of partial[9 b + k], in block order (one wave)
+__global__ void __launch_bounds__(64) k_sum_partials9(const double *partial, int nb, double *out)
+{
+    const int k = threadIdx.x;
+    if (k >= 9)
+        return;
+    double v = 0;
+    for (int b = 0; b < nb; b++)
+        v += partial[9 * (size_t)b + k];
+    out[k] = v;
 }
 
 // per-block sums (fixed order) of width-value rows first..first+count of in

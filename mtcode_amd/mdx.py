@@ -100,6 +100,7 @@ def lib():
     L.mdx_upload_forces.argtypes = [C.c_void_p, _dp, _dp]
     L.mdx_list_pairs.argtypes = [C.c_void_p, _ip, C.c_longlong, C.POINTER(C.c_longlong), _dp]
     L.mdx_stress.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.mdx_stress_noalloc.argtypes = [C.c_void_p, C.c_int, _dp]
     _lib = L
     return L
 
@@ -306,6 +307,11 @@ class Engine:
     def stress(self, replica=-1):
         out = np.zeros(9)
         _check(lib().mdx_stress(self.h, int(replica), out.ctypes.data_as(_dp)))
+        return out.reshape(3, 3)
+
+    def stress_noalloc(self, replica=-1):
+        out = np.zeros(9)
+        _check(lib().mdx_stress_noalloc(self.h, int(replica), out.ctypes.data_as(_dp)))
         return out.reshape(3, 3)
 
     def profile(self, nsteps):
