@@ -107,15 +107,16 @@ def build_workload(name):
 
 def force_error(s, reps=None):
     """mixed vs FP64 conservative forces/torques on the same frozen
-    configuration (orientations rounded to FP32, as the mixed engine holds them)"""
+    configuration: the FP64 engine gets the orientations the mixed engine's
+    kernels use (its FP32 unit quaternions as matrices)"""
     def rounded(x):
         x = dict(x)
         if x.get("Q") is not None:
-            x["Q"] = x["Q"].astype(np.float32).astype(np.float64)
+            x["Q"] = wl.mixed_orientations(x["Q"])
         return x
-    s = rounded(s)
-    reps = [rounded(r) for r in reps] if reps else None
-    with mdx.engine_from_system(s, precision=mdx.PREC_FP64, replicas=reps) as e:
+    s64 = rounded(s)
+    reps64 = [rounded(r) for r in reps] if reps else None
+    with mdx.engine_from_system(s64, precision=mdx.PREC_FP64, replicas=reps64) as e:
         F64, T64 = e.compute_forces()
     with mdx.engine_from_system(s, precision=mdx.PREC_MIXED, replicas=reps) as e:
         F32, T32 = e.compute_forces()

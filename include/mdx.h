@@ -88,9 +88,11 @@ int mdx_set_noise_counter(mdx_engine *e, unsigned long long counter);
 /* carried F (lab) and T (body) in the layout of mdx_download_forces, in place
  * of mdx_prime: after mdx_upload_state + mdx_set_noise_counter, a run
  * continued from downloaded state, forces and counter takes the same steps
- * as the uninterrupted run (bit for bit when the uninterrupted run re-sorts
- * its state at the step where the continuation starts, as the continuation
- * does at its first step) */
+ * as the uninterrupted run (FP64: bit for bit when the uninterrupted run
+ * re-sorts its state at the step where the continuation starts, as the
+ * continuation does at its first step; mixed precision: to FP32 rounding,
+ * since its orientations are unit quaternions whose FP32 norm the N x 9
+ * matrices do not carry) */
 int mdx_upload_forces(mdx_engine *e, const double *F, const double *T);
 
 /* Interactions. */

@@ -117,6 +117,72 @@ __host__ __device__ __forceinline__ bool wrap1(double &x, R &p, double l, int pe
     return true;
 }
 
+// ------------------------------------------------------------- orientations
+// The mixed-precision engine stores each orientation as a unit quaternion
+// (w, x, y, z) in 16 B instead of the 36 B of an FP32 3x3 matrix, and
+// rebuilds the lab -> body matrix on load:
+//   m = [[1-2(y^2+z^2), 2(xy-wz),     2(xz+wy)    ],
+//        [2(xy+wz),     1-2(x^2+z^2), 2(yz-wx)    ],
+//        [2(xz-wy),     2(yz+wx),     1-2(x^2+y^2)]]
+// A rotated matrix goes back to a quaternion by Shepperd's method (the
+// branch on the largest of the trace and the diagonal keeps the square root
+// well conditioned) and is normalised, which also keeps the orientation
+// orthonormal.  R(q) = R(-q) bit for bit, so the sign the conversion picks
+// does not matter to the dynamics.
+template <class R> __host__ __device__ __forceinline__ void quat_to_mat(R w, R x, R y, R z, R m[9])
+{
+    const R xx = x * x, yy = y * y, zz = z * z;
+    const R xy = x * y, xz = x * z, yz = y * z, wx = w * x, wy = w * y, wz = w * z;
+    m[0] = (R)1 - (R)2 * (yy + zz);
+    m[1] = (R)2 * (xy - wz);
+    m[2] = (R)2 * (xz + wy);
+    m[3] = (R)2 * (xy + wz);
+    m[4] = (R)1 - (R)2 * (xx + zz);
+    m[5] = (R)2 * (yz - wx);
+    m[6] = (R)2 * (xz - wy);
+    m[7] = (R)2 * (yz + wx);
+    m[8] = (R)1 - (R)2 * (xx + yy);
+}
+
+__host__ __device__ __forceinline__ float sqrt_(float v) { return __builtin_sqrtf(v); }
+__host__ __device__ __forceinline__ double sqrt_(double v) { return __builtin_sqrt(v); }
+
+template <class R> __host__ __device__ __forceinline__ void mat_to_quat(const R m[9], R q[4])
+{
+    const R tr = m[0] + m[4] + m[8];
+    R w, x, y, z;
+    if (tr > (R)0) {
+        const R S = sqrt_(tr + (R)1) * (R)2;
+        w = (R)0.25 * S;
+        x = (m[7] - m[5]) / S;
+        y = (m[2] - m[6]) / S;
+        z = (m[3] - m[1]) / S;
+    } else if (m[0] > m[4] && m[0] > m[8]) {
+        const R S = sqrt_((R)1 + m[0] - m[4] - m[8]) * (R)2;
+        w = (m[7] - m[5]) / S;
+        x = (R)0.25 * S;
+        y = (m[1] + m[3]) / S;
+        z = (m[2] + m[6]) / S;
+    } else if (m[4] > m[8]) {
+        const R S = sqrt_((R)1 + m[4] - m[0] - m[8]) * (R)2;
+        w = (m[2] - m[6]) / S;
+        x = (m[1] + m[3]) / S;
+        y = (R)0.25 * S;
+        z = (m[5] + m[7]) / S;
+    } else {
+        const R S = sqrt_((R)1 + m[8] - m[0] - m[4]) * (R)2;
+        w = (m[3] - m[1]) / S;
+        x = (m[2] + m[6]) / S;
+        y = (m[5] + m[7]) / S;
+        z = (R)0.25 * S;
+    }
+    const R inv = (R)1 / sqrt_(w * w + x * x + y * y + z * z);
+    q[0] = w * inv;
+    q[1] = x * inv;
+    q[2] = y * inv;
+    q[3] = z * inv;
+}
+
 // ------------------------------------------------------------------- pairs
 // HSPotential::force (potential.h:112-125)
 __device__ __forceinline__ double hs_force(double x, double epsilon, double sigma6,

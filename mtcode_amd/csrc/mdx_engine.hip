@@ -95,7 +95,6 @@ struct mdx_engine {
     double fa = 0;
     int has_active = 0;
     double fguard = 1e4;
-    int reortho_every = 100;
     int use_tiles = 0; // LDS-staged neighbour tiles (MDX_TILES=1 enables)
     // walk / patch / integrator kernels (1) or one kernel per step (0); by
     // default the one-kernel form up to SPLIT_MIN_N particles, where launches
@@ -368,7 +367,6 @@ template <class R> static KArgs<R> make_args(mdx_engine *e)
     a.b_c2 = c2;
     a.b_c5c2 = c5 * c2;
     a.b_c5c3q = (c5 * (c3) + q);
-    a.reortho_every = e->reortho_every;
     a.key0 = (uint32_t)e->seed;
     a.key1 = (uint32_t)(e->seed >> 32);
     a.noise = nullptr;
@@ -1050,7 +1048,7 @@ int mdx_create(const mdx_desc *d, mdx_engine **out)
         rc |= dalloc(e, e->slot[b], 4 * N);
         rc |= dalloc(e, e->cell[b], 4 * N);
         if (e->integ == MDX_INTEG_ROTOR) {
-            rc |= dalloc(e, e->Q[b], 9 * rs * N);
+            rc |= dalloc(e, e->Q[b], (size_t)(rs == 8 ? 72 : 16) * N);
             rc |= dalloc(e, e->Lm[b], 3 * rs * N);
             rc |= dalloc(e, e->Fc[b], 3 * rs * N);
             rc |= dalloc(e, e->Tc[b], 3 * rs * N);
@@ -1143,8 +1141,19 @@ static int upload_state_t(mdx_engine *e, const double *x, const double *p, const
             for (size_t i = 0; i < N; i++)
                 eye[9 * i] = eye[9 * i + 4] = eye[9 * i + 8] = 1.0;
         }
-        if ((rc = upload_real<R>(e, e->Q[0], Q ? Q : eye.data(), 9 * N)))
-            return rc;
+        const double *Qh = Q ? Q : eye.data();
+        if (sizeof(R) == 8) {
+            if ((rc = upload(e, e->Q[0], Qh, 9 * N)))
+                return rc;
+        } else {
+            // the mixed engine stores unit quaternions
+            if ((rc = upload(e, e->tmpd, Qh, 9 * N)))
+                return rc;
+            hipLaunchKernelGGL(k_mats_to_quats, dim3(nblk((long long)N)), dim3(256), 0, e->stream,
+                               (const double *)e->tmpd.p, (int)N, (float *)e->Q[0].p);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(e->stream));
+        }
     }
     return MDX_OK;
 }
@@ -1195,9 +1204,19 @@ int mdx_download_state(mdx_engine *e, double *x, double *p, double *Q, double *L
         return rc;
     if (Q) {
         if (rot) {
-            if ((rc = f64 ? download_by_id<double>(e, e->Q[c], rec, 9, Q)
-                          : download_by_id<float>(e, e->Q[c], rec, 9, Q)))
-                return rc;
+            if (f64) {
+                if ((rc = download_by_id<double>(e, e->Q[c], rec, 9, Q)))
+                    return rc;
+            } else {
+                // quaternions as FP64 matrices, then by original index
+                if ((rc = dalloc(e, e->tmpx, 72 * (size_t)e->N)))
+                    return rc;
+                hipLaunchKernelGGL(k_quats_to_mats, dim3(nblk(e->N)), dim3(256), 0, e->stream,
+                                   (const float *)e->Q[c].p, e->N, (double *)e->tmpx.p);
+                HIPCHK(hipGetLastError());
+                if ((rc = download_by_id<double>(e, e->tmpx, rec, 9, Q)))
+                    return rc;
+            }
         } else {
             for (int i = 0; i < e->N; i++)
                 for (int k = 0; k < 9; k++)

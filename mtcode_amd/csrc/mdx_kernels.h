@@ -123,13 +123,61 @@ template <class R> struct KArgs {
     // integrator
     double dt, hdt, c1, m, gamma, gammar, I[3];
     double b_c5c2, b_c5c3q, b_c2;
-    int reortho_every;
     // noise
     uint32_t key0, key1;
     const double *noise; // injected rows (ncomp per particle, original order) or NULL
     StepCtl *ctl;
     double fguard;
 };
+
+// ------------------------------------------------------- orientation store
+// FP64: the 3 x 3 lab -> body matrix (9 doubles per particle, the reference's
+// layout, LangevinR.h:67-68).  Mixed: a unit quaternion, 4 floats (16 B)
+// per particle, the matrix rebuilt on load (quat_to_mat).
+template <class R> struct Orient {
+    static constexpr int W = 9;
+};
+template <> struct Orient<float> {
+    static constexpr int W = 4;
+};
+
+template <class R> __device__ __forceinline__ void load_orient(const R *Q, size_t i, R m[9])
+{
+    if constexpr (sizeof(R) == 8) {
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+            m[k] = Q[9 * i + k];
+    } else {
+        const float4 q = reinterpret_cast<const float4 *>(Q)[i];
+        quat_to_mat<float>(q.x, q.y, q.z, q.w, m);
+    }
+}
+
+// store the rotated matrix m (mixed: as its normalised quaternion)
+template <class R> __device__ __forceinline__ void store_orient(R *Q, size_t i, const R m[9])
+{
+    if constexpr (sizeof(R) == 8) {
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+            Q[9 * i + k] = m[k];
+    } else {
+        float q[4];
+        mat_to_quat<float>(m, q);
+        reinterpret_cast<float4 *>(Q)[i] = make_float4(q[0], q[1], q[2], q[3]);
+    }
+}
+
+// an orientation that did not rotate, copied as stored (no round trip)
+template <class R> __device__ __forceinline__ void copy_orient(R *Qn, size_t s, const R *Q, size_t i)
+{
+    if constexpr (sizeof(R) == 8) {
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+            Qn[9 * s + k] = Q[9 * i + k];
+    } else {
+        reinterpret_cast<float4 *>(Qn)[s] = reinterpret_cast<const float4 *>(Q)[i];
+    }
+}
 
 // ------------------------------------------------------------ small helpers
 // n = Q^T n_body in FP64 with the reference's operation order
@@ -191,23 +239,6 @@ template <class R> __device__ __forceinline__ void rotate(R dt, const R I[3], R 
     q[6] = o[0] * rr[2] + o[3] * rr[5] + o[6] * rr[8];
     q[7] = o[1] * rr[2] + o[4] * rr[5] + o[7] * rr[8];
     q[8] = o[2] * rr[2] + o[5] * rr[5] + o[8] * rr[8];
-}
-
-// symmetric re-orthonormalisation in FP64: Q <- 1.5 Q - 0.5 Q Q^T Q (one
-// Newton-Schulz step towards Q (Q^T Q)^(-1/2); FP32 drift is ~1e-7 per step)
-template <class R> __device__ __forceinline__ void reortho(R q[9])
-{
-    double a[9], m[9];
-    for (int k = 0; k < 9; k++)
-        a[k] = (double)q[k];
-    for (int r = 0; r < 3; r++)
-        for (int c = 0; c < 3; c++)
-            m[3 * r + c] = a[3 * r] * a[3 * c] + a[3 * r + 1] * a[3 * c + 1] + a[3 * r + 2] * a[3 * c + 2];
-    for (int r = 0; r < 3; r++)
-        for (int c = 0; c < 3; c++) {
-            double s = m[3 * r] * a[c] + m[3 * r + 1] * a[3 + c] + m[3 * r + 2] * a[6 + c];
-            q[3 * r + c] = (R)(1.5 * a[3 * r + c] - 0.5 * s);
-        }
 }
 
 // status bits: 1 a pair or bond term above the force guard, 2 a non-finite
@@ -497,9 +528,7 @@ __device__ __forceinline__ void patch_pair(const KArgs<R> &a, const DevRep &rp, 
     const double sg = i_is_a ? 1.0 : -1.0;
     const double u[3] = {sg * d[0] / rij, sg * d[1] / rij, sg * d[2] / rij};
     R qj[9];
-#pragma unroll
-    for (int q = 0; q < 9; q++)
-        qj[q] = a.Q[9 * (size_t)j + q];
+    load_orient(a.Q, (size_t)j, qj);
     const R *qa = i_is_a ? qi : qj;
     const R *qc = i_is_a ? qj : qi;
     // body-frame components of u screen the cone tests; the decisions are
@@ -706,11 +735,10 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
     constexpr bool INTG = PH == PH_INTEGRATE || PH == PH_INTEGRATE_F;
     const int npi = (MODE != M_BEAD && !INTG && a.has_patches) ? a.np[ti] : 0;
     if (MODE != M_BEAD && a.rotor) {
+        load_orient(a.Q, (size_t)i, qi);
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
-            qi[k] = a.Q[9 * (size_t)i + k];
+        for (int k = 0; k < 9; k++)
             qs[k] = (float)qi[k];
-        }
     }
     int exc0 = 0, exc1 = 0;
     if (needs_exclusions(a, i)) {
@@ -1050,8 +1078,7 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
             }
         }
         a.Xn[s] = xi;
-        for (int k = 0; k < 9; k++)
-            a.Qn[9 * (size_t)s + k] = qi[k];
+        copy_orient(a.Qn, (size_t)s, a.Q, (size_t)i);
         for (int k = 0; k < 3; k++) {
             a.pn[3 * (size_t)s + k] = pl[k];
             a.Ln[3 * (size_t)s + k] = ll[k];
@@ -1080,8 +1107,6 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
     }
     const R Ir[3] = {(R)a.I[0], (R)a.I[1], (R)a.I[2]};
     rotate((R)a.dt, Ir, ll, qi);
-    if (!FP64 && a.reortho_every > 0 && ((a.ctl->epoch + 1) % a.reortho_every) == 0)
-        reortho(qi);
     for (int k = 0; k < 3; k++)
         ok = ok && isfinite((double)ll[k]);
     for (int k = 0; k < 9; k++)
@@ -1096,16 +1121,16 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
             pl[k] = a.p[3 * (size_t)i + k];
             ll[k] = a.L[3 * (size_t)i + k];
         }
-        for (int k = 0; k < 9; k++)
-            qi[k] = a.Q[9 * (size_t)i + k];
         dd = 0;
     }
     for (int k = 0; k < 3; k++) {
         a.pn[3 * (size_t)s + k] = pl[k];
         a.Ln[3 * (size_t)s + k] = ll[k];
     }
-    for (int k = 0; k < 9; k++)
-        a.Qn[9 * (size_t)s + k] = qi[k];
+    if (ok)
+        store_orient(a.Qn, (size_t)s, qi);
+    else
+        copy_orient(a.Qn, (size_t)s, a.Q, (size_t)i);
     PosRec o = xi;
     o.x = xn[0];
     o.y = xn[1];
@@ -1191,9 +1216,11 @@ __global__ void __launch_bounds__(256, (sizeof(R) == 4 && !INT) ? 6 : 1) k_walkc
         xi = a.X[s];
         sx[tid] = xi;
         if (a.has_patches && a.np[xi.meta & 255] > 0) {
+            R qm[9];
+            load_orient(a.Q, (size_t)s, qm);
 #pragma unroll
             for (int k = 0; k < 9; k++)
-                sq[9 * tid + k] = (float)a.Q[9 * (size_t)s + k];
+                sq[9 * tid + k] = (float)qm[k];
         }
         const bool ne = needs_exclusions(a, s);
         sexc[2 * tid] = ne ? a.exc_off[xi.id] : 0;
@@ -1378,10 +1405,9 @@ __device__ __forceinline__ void patch_overflow(const KArgs<R> &a, int so)
     const DevRep rp = rep_rec(a, xi.meta);
     R qi[9];
     float qs[9]; // qi in FP32, for the own-patch screen
-    for (int k2 = 0; k2 < 9; k2++) {
-        qi[k2] = a.Q[9 * (size_t)i + k2];
+    load_orient(a.Q, (size_t)i, qi);
+    for (int k2 = 0; k2 < 9; k2++)
         qs[k2] = (float)qi[k2];
-    }
     Acc<R> acc;
     acc.bad = false;
     for (int k2 = 0; k2 < 3; k2++) {
@@ -1484,9 +1510,7 @@ __global__ void __launch_bounds__(256) k_patch(KArgs<R> a, int pblk)
             const PosRec xo = a.X[i];
             const DevRep rp = rep_rec(a, xo.meta);
             R qo[9];
-#pragma unroll
-            for (int k = 0; k < 9; k++)
-                qo[k] = a.Q[9 * (size_t)i + k];
+            load_orient(a.Q, (size_t)i, qo);
             Acc<R> ea;
             for (int k = 0; k < 3; k++)
                 ea.F[k] = ea.T[k] = 0;
@@ -1836,8 +1860,7 @@ template <class R> __device__ __forceinline__ float kick_drift_one(const KArgs<R
         pl[k] = a.p[3 * (size_t)i + k] + hdt * a.F[3 * (size_t)i + k];
         ll[k] = a.L[3 * (size_t)i + k] + hdt * a.T[3 * (size_t)i + k];
     }
-    for (int k = 0; k < 9; k++)
-        q[k] = a.Q[9 * (size_t)i + k];
+    load_orient(a.Q, (size_t)i, q);
     double dd = 0;
     // a stopped run (status raised) keeps every particle's state, and a
     // particle whose updated state is non-finite keeps its own
@@ -1855,8 +1878,6 @@ template <class R> __device__ __forceinline__ float kick_drift_one(const KArgs<R
     }
     const R Ir[3] = {(R)a.I[0], (R)a.I[1], (R)a.I[2]};
     rotate((R)a.dt, Ir, ll, q);
-    if (!FP64 && a.reortho_every > 0 && ((a.ctl->epoch + 1) % a.reortho_every) == 0)
-        reortho(q);
     for (int k = 0; k < 3; k++)
         ok = ok && isfinite((double)ll[k]);
     for (int k = 0; k < 9; k++)
@@ -1871,16 +1892,16 @@ template <class R> __device__ __forceinline__ float kick_drift_one(const KArgs<R
             pl[k] = a.p[3 * (size_t)i + k];
             ll[k] = a.L[3 * (size_t)i + k];
         }
-        for (int k = 0; k < 9; k++)
-            q[k] = a.Q[9 * (size_t)i + k];
         dd = 0;
     }
     for (int k = 0; k < 3; k++) {
         a.pn[3 * (size_t)i + k] = pl[k];
         a.Ln[3 * (size_t)i + k] = ll[k];
     }
-    for (int k = 0; k < 9; k++)
-        a.Qn[9 * (size_t)i + k] = q[k];
+    if (ok)
+        store_orient(a.Qn, (size_t)i, q);
+    else
+        copy_orient(a.Qn, (size_t)i, a.Q, (size_t)i);
     xr.x = xn[0];
     xr.y = xn[1];
     xr.z = xn[2];
@@ -2297,8 +2318,7 @@ __global__ void __launch_bounds__(256) k_patch_bonds(KArgs<R> a, int *bonds)
     int nb = 0;
     if (a.has_patches && a.np[ti] > 0) {
         R qi[9];
-        for (int k = 0; k < 9; k++)
-            qi[k] = a.Q[9 * (size_t)i + k];
+        load_orient(a.Q, (size_t)i, qi);
         for_each_neighbor(a, rp, i, [&](const PosRec &xj, int j) {
             double d[3];
             min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
@@ -2316,8 +2336,7 @@ __global__ void __launch_bounds__(256) k_patch_bonds(KArgs<R> a, int *bonds)
             const double sg = i_is_a ? 1.0 : -1.0;
             const double u[3] = {sg * d[0] / rij, sg * d[1] / rij, sg * d[2] / rij};
             R qj[9];
-            for (int k = 0; k < 9; k++)
-                qj[k] = a.Q[9 * (size_t)j + k];
+            load_orient(a.Q, (size_t)j, qj);
             const R *qa = i_is_a ? qi : qj;
             const R *qc = i_is_a ? qj : qi;
             const int npa = a.np[ta], npc = a.np[tc];
@@ -2414,14 +2433,12 @@ __global__ void __launch_bounds__(256) k_stage_rotate(int N, R *L, R *Q, double 
     R l[3], q[9];
     for (int k = 0; k < 3; k++)
         l[k] = L[3 * (size_t)i + k];
-    for (int k = 0; k < 9; k++)
-        q[k] = Q[9 * (size_t)i + k];
+    load_orient(Q, (size_t)i, q);
     const R Ir[3] = {(R)I0, (R)I1, (R)I2};
     rotate((R)dt, Ir, l, q);
     for (int k = 0; k < 3; k++)
         L[3 * (size_t)i + k] = l[k];
-    for (int k = 0; k < 9; k++)
-        Q[9 * (size_t)i + k] = q[k];
+    store_orient(Q, (size_t)i, q);
 }
 
 // LangevinNVTR::create_forces_and_torques_sphere (LangevinR.cpp:475-578):
@@ -2438,7 +2455,8 @@ __global__ void __launch_bounds__(256) k_stage_sphere(int N, const PosRec *X, co
     const PosRec xr = X[i];
     const size_t o = xr.id;
     const DevRep rp = reps[xr.meta >> 8];
-    const R *q = Q + 9 * (size_t)i;
+    R q[9];
+    load_orient(Q, (size_t)i, q);
     const R pl[3] = {p[3 * (size_t)i], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2]};
     const R ll[3] = {L[3 * (size_t)i], L[3 * (size_t)i + 1], L[3 * (size_t)i + 2]};
     const R pbx = q[0] * pl[0] + q[1] * pl[1] + q[2] * pl[2];
@@ -2646,6 +2664,37 @@ __global__ void k_by_id_to_slots(const PosRec *X, int N, const double *in, R *ou
     const size_t o = X[s].id;
     for (int k = 0; k < width; k++)
         out[(size_t)width * s + k] = (R)in[width * o + k];
+}
+
+// host matrices (FP64, 9 per particle, slot order) into the mixed engine's
+// quaternions; Shepperd's method in FP64, rounded once
+__global__ void k_mats_to_quats(const double *in, int N, float *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    double m[9], q[4];
+    for (int k = 0; k < 9; k++)
+        m[k] = in[9 * (size_t)i + k];
+    mat_to_quat<double>(m, q);
+    reinterpret_cast<float4 *>(out)[i] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
+}
+
+// the mixed engine's quaternions as FP64 rotation matrices (9 per particle,
+// slot order), normalised in FP64 first (the stored FP32 norm is 1 only to
+// FP32 rounding)
+__global__ void k_quats_to_mats(const float *in, int N, double *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    const float4 q = reinterpret_cast<const float4 *>(in)[i];
+    const double w = q.x, x = q.y, y = q.z, z = q.w;
+    const double inv = 1.0 / sqrt(w * w + x * x + y * y + z * z);
+    double m[9];
+    quat_to_mat<double>(w * inv, x * inv, y * inv, z * inv, m);
+    for (int k = 0; k < 9; k++)
+        out[9 * (size_t)i + k] = m[k];
 }
 
 // new (replica << 8 | type) words for the current slots from an array by

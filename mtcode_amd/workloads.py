@@ -171,6 +171,50 @@ def _base(name: str, n: int, box, dt: float, kT: float, gamma: float, gammar: fl
     }
 
 
+def mixed_quaternions(Q: np.ndarray) -> np.ndarray:
+    """The unit quaternions (N x 4 float32, w x y z) the mixed-precision
+    engine stores for the orientations Q (N x 9 lab -> body matrices): its
+    upload's Shepperd's method in FP64, normalised, rounded once to FP32,
+    with the same operations."""
+    m = np.asarray(Q, dtype=np.float64).reshape(-1, 9).T
+    tr = (m[0] + m[4]) + m[8]
+    c0 = tr > 0
+    c1 = ~c0 & (m[0] > m[4]) & (m[0] > m[8])
+    c2 = ~c0 & ~c1 & (m[4] > m[8])
+    q = np.zeros((4, m.shape[1]))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        S0 = np.sqrt(tr + 1) * 2
+        S1 = np.sqrt(((1 + m[0]) - m[4]) - m[8]) * 2
+        S2 = np.sqrt(((1 + m[4]) - m[0]) - m[8]) * 2
+        S3 = np.sqrt(((1 + m[8]) - m[0]) - m[4]) * 2
+        br = [(c0, [0.25 * S0, (m[7] - m[5]) / S0, (m[2] - m[6]) / S0, (m[3] - m[1]) / S0]),
+              (c1, [(m[7] - m[5]) / S1, 0.25 * S1, (m[1] + m[3]) / S1, (m[2] + m[6]) / S1]),
+              (c2, [(m[2] - m[6]) / S2, (m[1] + m[3]) / S2, 0.25 * S2, (m[5] + m[7]) / S2]),
+              (~(c0 | c1 | c2), [(m[3] - m[1]) / S3, (m[2] + m[6]) / S3, (m[5] + m[7]) / S3,
+                                 0.25 * S3])]
+    for c, comp in br:
+        for k in range(4):
+            q[k][c] = comp[k][c]
+    inv = 1 / np.sqrt(((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]) + q[3] * q[3])
+    return (q * inv).astype(np.float32).T
+
+
+def mixed_orientations(Q: np.ndarray) -> np.ndarray:
+    """The matrices (N x 9, FP64) that the mixed-precision engine's kernels
+    use for the orientations Q it is given: the quaternions of
+    mixed_quaternions, turned back into matrices in FP32 arithmetic with the
+    kernels' operations.  An FP64 engine handed them sees exactly the
+    orientations a mixed engine handed Q works with."""
+    w, x, y, z = mixed_quaternions(Q).T
+    one, two = np.float32(1), np.float32(2)
+    xx, yy, zz = x * x, y * y, z * z
+    xy, xz, yz, wx, wy, wz = x * y, x * z, y * z, w * x, w * y, w * z
+    out = np.stack([one - two * (yy + zz), two * (xy - wz), two * (xz + wy),
+                    two * (xy + wz), one - two * (xx + zz), two * (yz - wx),
+                    two * (xz - wy), two * (yz + wx), one - two * (xx + yy)], 1)
+    return out.astype(np.float64)
+
+
 def thermal_momenta(s: dict, seed: int = 99) -> dict:
     """A copy of system s with momenta drawn from the Maxwell-Boltzmann
     distribution at its kT: p ~ N(0, m kT) per component, and for rotors the

@@ -178,10 +178,10 @@ def test_replica_batch_forces_fp64_and_mixed(c4):
         worst = max(worst, ef, et)
         assert ef < 1e-12 and et < 1e-12, (r, ef, et)
     say("replicas checked", len(pick), "N range", Ns[pick].min(), Ns[pick].max(), "worst", worst)
-    rounded = [dict(r, Q=r["Q"].astype(np.float32).astype(np.float64)) for r in c4]
+    rounded = [dict(r, Q=wl.mixed_orientations(r["Q"])) for r in c4]
     with mdx.engine_from_system(None, precision=mdx.PREC_FP64, replicas=rounded) as e:
         F64, T64 = e.compute_forces()
-    with mdx.engine_from_system(None, precision=mdx.PREC_MIXED, replicas=rounded) as e:
+    with mdx.engine_from_system(None, precision=mdx.PREC_MIXED, replicas=c4) as e:
         F32, T32 = e.compute_forces()
     ef, et = relmax(F32, F64), relmax(T32, T64)
     rms = float(np.linalg.norm(F32 - F64) / np.linalg.norm(F64))

@@ -1,4 +1,4 @@
-"""Host build of the integrator's position wrap (mdx::wrap1 in
+"""Host builds of device helpers.  The integrator's position wrap (mdx::wrap1 in
 mtcode_amd/csrc/mdx_device.h, cube::correct_position_and_momentum of the
 reference's MDBase/geometry.h:403-445), fed NaN, +-inf and huge values: every
 call returns, non-finite inputs are reported (the kernels then keep the
@@ -14,16 +14,32 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 
 
-@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
-def test_wrap_bounded_on_host(tmp_path):
-    exe = str(tmp_path / "wrap_check")
+def host_build_and_run(tmp_path, name):
+    exe = str(tmp_path / name)
     subprocess.run([HIPCC, "-x", "hip", "--cuda-host-only", "-O2", "-std=c++17",
                     "-ffp-contract=off", f"-I{os.path.join(ROOT, 'include')}",
                     f"-I{os.path.join(ROOT, 'mtcode_amd', 'csrc')}",
-                    os.path.join(ROOT, "tests", "cpp", "wrap_check.cpp"), "-o", exe],
+                    os.path.join(ROOT, "tests", "cpp", name + ".cpp"), "-o", exe],
                    check=True, capture_output=True)
-    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    return subprocess.run([exe], capture_output=True, text=True, timeout=120)
+
+
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
+def test_wrap_bounded_on_host(tmp_path):
+    r = host_build_and_run(tmp_path, "wrap_check")
     print(r.stdout)
     assert r.returncode == 0, r.stdout
     assert r.stdout.strip().endswith("ok")
     assert "FAIL" not in r.stdout
+
+
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
+def test_orientation_store_on_host(tmp_path):
+    """The mixed engine's quaternion orientation store (quat_to_mat /
+    mat_to_quat of mdx_device.h): matrix -> quaternion -> matrix on 200,000
+    random rotations (all branches of Shepperd's method), unit norm, and
+    the download / upload round trip within FP32 rounding"""
+    r = host_build_and_run(tmp_path, "orient_check")
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout
+    assert r.stdout.strip().endswith("ok")
