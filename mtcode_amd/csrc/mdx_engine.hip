@@ -453,9 +453,10 @@ static int launch_gather(mdx_engine *e, const double *noise_dev, int lmode = 0)
     // split form: patch systems, and list steps of the bead integrator (walk,
     // then the integrator kernel; no patch kernel)
     const bool bead_list = MODE == M_BEAD && lmode && e->use_walkc;
-    if (e->use_split && ((a.has_patches && (MODE == M_PRIME || MODE == M_STEP ||
-                                           MODE == M_FUSED || MODE == M_CONS)) ||
-                         bead_list)) {
+    const bool split = e->use_split && ((a.has_patches && (MODE == M_PRIME || MODE == M_STEP ||
+                                                           MODE == M_FUSED || MODE == M_CONS)) ||
+                                        bead_list);
+    if (split) {
         int rc;
         if ((rc = dalloc(e, e->Fw, 3 * e->rsz() * (size_t)e->N)) ||
             (rc = dalloc(e, e->ncand, 4 * (size_t)e->N)) ||
@@ -517,16 +518,19 @@ static int launch_gather(mdx_engine *e, const double *noise_dev, int lmode = 0)
         } else {
             if (e->prof_mid2)
                 HIPCHK(hipEventRecord(e->prof_mid2, e->stream));
-            hipLaunchKernelGGL((k_finish<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
+            hipLaunchKernelGGL((k_finish<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream,
+                               a);
         }
         HIPCHK(hipGetLastError());
         return MDX_OK;
     }
     e->last_split = false;
+    // (k_tile: no list mode, nothing to close)
     if (e->use_tiles && !lmode)
         hipLaunchKernelGGL((k_tile<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
     else
-        hipLaunchKernelGGL((k_gather<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
+        hipLaunchKernelGGL((k_gather<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream,
+                           a);
     HIPCHK(hipGetLastError());
     return MDX_OK;
 }
@@ -625,11 +629,12 @@ static int nbr_alloc(mdx_engine *e)
 }
 
 // after a kernel that moved the particles in list mode (its blocks wrote
-// drift slots); epoch_inc: also count the step's force evaluation
-static int decide(mdx_engine *e, int epoch_inc)
+// drift slots); epoch_inc: also count the step's force evaluation;
+// step_end: the list step's kernels are done (clears fresh)
+static int decide(mdx_engine *e, int epoch_inc, int step_end)
 {
     hipLaunchKernelGGL(k_decide, dim3(1), dim3(NDSLOT), 0, e->stream, (StepCtl *)e->ctl.p,
-                       (unsigned int *)e->dslot.p, e->skin, epoch_inc);
+                       (unsigned int *)e->dslot.p, e->skin, epoch_inc, step_end);
     HIPCHK(hipGetLastError());
     return MDX_OK;
 }
@@ -674,10 +679,10 @@ static int launch_list_step(mdx_engine *e, const double *nd, hipEvent_t *ev, int
     if (rc)
         return rc;
     if (MODE != M_STEP) {
-        if ((rc = decide(e, 1)))
+        if ((rc = decide(e, 1, 1)))
             return rc;
     } else {
-        launch_small(e, k_epoch_inc);
+        launch_small(e, k_step_end);
     }
     return MDX_OK;
 }
@@ -827,7 +832,7 @@ template <class R> static int enqueue_rotor(mdx_engine *e, int nsteps, const dou
             HIPCHK(hipEventRecord(kev[0], e->stream));
         hipLaunchKernelGGL(k_kick_drift<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
         HIPCHK(hipGetLastError());
-        if (a.lmode && (rc = decide(e, 0)))
+        if (a.lmode && (rc = decide(e, 0, 0)))
             return rc;
         if (kev)
             HIPCHK(hipEventRecord(kev[1], e->stream));

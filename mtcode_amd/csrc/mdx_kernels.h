@@ -50,6 +50,10 @@ struct StepCtl {
     // the first force evaluation (epoch) that raised a status bit; ~0 while
     // the status is clear.  From then on no moving kernel advances a particle.
     unsigned long long fail_epoch;
+    // 1: a list build has written nbr since the compacted walk last laid its
+    // entries out per block (the next compacted walk does so); cleared by the
+    // kernel that closes the list step (k_decide, k_step_end)
+    unsigned int fresh;
 };
 
 enum { M_CONS = 0, M_PRIME = 1, M_STEP = 2, M_FUSED = 3, M_BEAD = 4, M_ENERGY = 5 };
@@ -83,8 +87,8 @@ template <class R> struct KArgs {
     // neighbour list (lmode): slot s walks nbr[k * N + s], k < nn[s] (<= kmax),
     // else the 27 cells of its cell at the last build, bcell[s].  nbrc: the
     // same entries per block of 256 slots, back to back in slot order from
-    // nbrc[first slot * kmax] (slots over kmax contribute none), for the
-    // compacted walk
+    // nbrc[first slot * kmax] (slots over kmax contribute none), laid out by
+    // the first compacted walk after each build
     int lmode;
     const int *nbr, *nbrc;
     int *nbrw, *nbrcw, *nn, *bcell;
@@ -1142,7 +1146,6 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
         bin_particle(a, s, rp, xn, a.celln);
 }
 
-// one thread per cell-list position, neighbours read from global memory
 // the first kernel after a list build clears the request and the bound
 template <class R> __device__ __forceinline__ void list_built(const KArgs<R> &a)
 {
@@ -1283,6 +1286,17 @@ __global__ void __launch_bounds__(256, (sizeof(R) == 4 && !INT) ? 6 : 1) k_walkc
         return lo;
     };
     const int *lst = a.nbrc + (size_t)base * a.kmax;
+    // a list built since the last compacted walk: this block's entries are
+    // first laid out back to back (and read back below by this block)
+    if (a.ctl->fresh) {
+        int *seg = a.nbrcw + (size_t)base * a.kmax;
+        for (int e = tid; e < total; e += 256) {
+            const int o = owner_of(e);
+            seg[e] = a.nbr[(size_t)(e - off[o]) * a.N + base + o];
+        }
+        __threadfence_block();
+        __syncthreads();
+    }
     // software pipeline: round r evaluates the record loaded in round r - 1
     // and loads the record of round r + 1 and the list entry of round r + 2
     int j1 = tid < total ? lst[tid] : 0;
@@ -1617,90 +1631,293 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_integrate(
         drift_bound(a, drift);
 }
 
-// neighbour list of every slot from the cell list of the current positions:
-// the slots within sqrt(rl2) in the order of the 27-cell walk.  gated: only
-// when a rebuild was requested.
-// 64 VGPRs (8 waves per SIMD, no spill): the build is a divergent,
-// latency-bound walk over the 27 cells, and more resident waves hide more of
-// it (rebuilds of the 2M-bead polymer 1.19 -> 1.06 ms)
+// neighbour list of every particle from the cell list of the current
+// positions: the slots within sqrt(rl2) in the order of the 27-cell walk
+// (z rows -1, 0, +1 outer, y rows -1, 0, +1 inner, then x along the row, as
+// for_each_in_cells), written to nbr[k * N + slot] with the count in nn;
+// the compacted walk lays the entries out per block afterwards (fresh).
+// gated: only when a rebuild was requested.
+// One thread per cell-list position: a block's particles then lie in a few
+// consecutive x-rows of cells (the list is in cell order).  For each z offset
+// in turn the block stages in LDS the distinct neighbour rows of its rows --
+// their cell offsets, and each row's particles (position record and slot),
+// a contiguous run of the list read with coalesced loads -- and walks them
+// there, with no global load per candidate.  A block whose particles span
+// more than NB_W rows or two replicas, a grid with fewer than 3 cells on a
+// side, and a z offset whose rows hold more than NB_CAP particles or
+// NB_ROWCAP cell offsets walk global memory instead.
+constexpr int NB_W = 16;        // own x-rows of a block walked from LDS
+constexpr int NB_CAP = 896;     // staged particles per z offset
+constexpr int NB_ROWCAP = 1280; // staged cell offsets per z offset
+
+// the list ranges [r0, r1) and [r2, r3) of the x neighbours of x-cell cx in
+// a row whose cell offsets are off[0..n0] (one range, or two at the periodic
+// seam; the whole row when it has fewer than 3 cells)
+template <class T>
+__device__ __forceinline__ void row_ranges(const T *off, int n0, int cx, int &r0, int &r1, int &r2,
+                                           int &r3)
+{
+    r2 = r3 = 0;
+    if (n0 >= 3) {
+        if (cx == 0) {
+            r0 = off[0];
+            r1 = off[2];
+            r2 = off[n0 - 1];
+            r3 = off[n0];
+        } else if (cx == n0 - 1) {
+            r0 = off[n0 - 2];
+            r1 = off[n0];
+            r2 = off[0];
+            r3 = off[1];
+        } else {
+            r0 = off[cx - 1];
+            r1 = off[cx + 2];
+        }
+    } else {
+        r0 = off[0];
+        r1 = off[n0];
+    }
+}
+
 template <class R>
-__global__ void __launch_bounds__(256, 8) k_nbuild(KArgs<R> a, const int *cells, int gated)
+__global__ void __launch_bounds__(256) k_nbuild(KArgs<R> a, const int *cells, int gated)
 {
     if (gated && !a.ctl->rebuild)
         return;
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        atomicAdd(&a.ctl->nbuilds, 1u);
-    __shared__ int wsum[4];
+    __shared__ PosRec lpos[NB_CAP];
+    __shared__ int lslot[NB_CAP];
+    __shared__ int rowoff[NB_ROWCAP];  // n0 + 1 cell offsets per staged row
+    __shared__ int crow[3 * NB_W];     // first cell of each candidate row of a z offset
+    __shared__ int hrow[3 * NB_W];     // its staged row
+    __shared__ int hcell[3 * NB_W];    // first cell of each staged row
+    __shared__ int hoff[3 * NB_W + 1]; // LDS index of each staged row's first particle
+    __shared__ int ends[4];            // replica and cell of the first and last position
+    __shared__ int binfo[5];           // replica (-1: global walk), first own row, own rows,
+                                       // staged rows, staged particles
     const int tid = threadIdx.x;
-    const int base = xcd_block(blockIdx.x, gridDim.x) * blockDim.x;
-    const int s = base + tid;
-    int cnt = 0;
-    if (s < a.N) {
-        const PosRec xi = a.X[s];
-        const DevRep rp = rep_rec(a, xi.meta);
-        const int c = cells[s];
+    if (blockIdx.x == 0 && tid == 0) {
+        atomicAdd(&a.ctl->nbuilds, 1u);
+        a.ctl->fresh = 1u;
+    }
+    const int p0 = xcd_block(blockIdx.x, gridDim.x) * blockDim.x;
+    const int pos = p0 + tid;
+    const int plast = min(p0 + (int)blockDim.x, a.N) - 1;
+    const bool live = pos <= plast;
+    int s = 0, c = 0;
+    PosRec xi;
+    if (live) {
+        s = a.list[pos];
+        xi = a.X[s];
+        c = cells[s];
         a.bcell[s] = c;
-        int exc0 = 0, exc1 = 0;
+        if (pos == p0) {
+            ends[0] = (int)(xi.meta >> 8);
+            ends[1] = c;
+        }
+        if (pos == plast) {
+            ends[2] = (int)(xi.meta >> 8);
+            ends[3] = c;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        // the first and last positions bound the block's cells (cell order)
+        int use = -1, rf = 0, w = 0;
+        if (ends[0] == ends[2]) {
+            const DevRep &rp = a.reps[ends[0]];
+            const int n0 = rp.nc[0];
+            rf = (ends[1] - rp.coff) / n0;
+            w = (ends[3] - rp.coff) / n0 - rf + 1;
+            if (n0 >= 3 && rp.nc[1] >= 3 && rp.nc[2] >= 3 && w <= NB_W)
+                use = ends[0];
+        }
+        binfo[0] = use;
+        binfo[1] = rf;
+        binfo[2] = w;
+    }
+    __syncthreads();
+
+    int n = 0;
+    bool kept_excluded = false;
+    int exc0 = 0, exc1 = 0, ti = 0;
+    DevRep rp;
+    if (live) {
+        rp = rep_rec(a, xi.meta);
         if (a.exc_off) {
             exc0 = a.exc_off[xi.id];
             exc1 = a.exc_off[xi.id + 1];
         }
-        const int ti = xi.meta & 255;
-        // the pair's types carry an active patch row
-        auto patch_active = [&](const PosRec &xj) -> bool {
-            const int tj = xj.meta & 255;
-            if (!(a.has_patches && a.np[ti] > 0 && a.np[tj] > 0))
-                return false;
-            int ta, tc;
-            patch_roles(xi.id, ti, xj.id, tj, ta, tc);
-            return (a.active_mask >> (ta * MAXT + tc)) & 1ull;
-        };
-        int n = 0;
-        bool kept_excluded = false;
-        for_each_in_cells(a, rp, c, s, [&](const PosRec &xj, int j) {
-            double d[3];
-            min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
-            const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-            if (!(r2 < a.rl2))
-                return;
-            // beyond the WCA reach: listed only for a type pair with patch terms
-            if (!(r2 < a.rl2w) && !patch_active(xj))
-                return;
-            // a pair without WCA (exclusion) is listed only if it can still
-            // interact through patches
-            bool ex = false;
-            for (int e = exc0; e < exc1; e++)
-                ex |= (uint32_t)a.exc_j[e] == xj.id;
-            if (ex) {
-                if (!patch_active(xj))
-                    return;
-                kept_excluded = true;
-            }
-            if (n < a.kmax)
-                a.nbrw[(size_t)n * a.N + s] = j;
-            n++;
-        });
-        a.nn[s] = n | (kept_excluded ? LIST_EXC : 0);
-        cnt = n <= a.kmax ? n : 0;
+        ti = xi.meta & 255;
     }
-    // the block's lists back to back (exclusive scan of the counts)
-    int v = cnt;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int t = __shfl_up(v, o);
-        if ((tid & 63) >= o)
-            v += t;
-    }
-    if ((tid & 63) == 63)
-        wsum[tid >> 6] = v;
-    __syncthreads();
-    int at = v - cnt;
-    for (int w = 0; w < (tid >> 6); w++)
-        at += wsum[w];
-    int *seg = a.nbrcw + (size_t)base * a.kmax + at;
-    for (int k = 0; k < cnt; k++)
-        seg[k] = a.nbrw[(size_t)k * a.N + s];
-}
+    // the pair's types carry an active patch row
+    auto patch_active = [&](const PosRec &xj) -> bool {
+        const int tj = xj.meta & 255;
+        if (!(a.has_patches && a.np[ti] > 0 && a.np[tj] > 0))
+            return false;
+        int ta, tc;
+        patch_roles(xi.id, ti, xj.id, tj, ta, tc);
+        return (a.active_mask >> (ta * MAXT + tc)) & 1ull;
+    };
+    auto consider = [&](const PosRec &xj, int j) {
+        double d[3];
+        min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
+        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (!(r2 < a.rl2))
+            return;
+        // beyond the WCA reach: listed only for a type pair with patch terms
+        if (!(r2 < a.rl2w) && !patch_active(xj))
+            return;
+        // a pair without WCA (exclusion) is listed only if it can still
+        // interact through patches
+        bool ex = false;
+        for (int e = exc0; e < exc1; e++)
+            ex |= (uint32_t)a.exc_j[e] == xj.id;
+        if (ex) {
+            if (!patch_active(xj))
+                return;
+            kept_excluded = true;
+        }
+        if (n < a.kmax)
+            a.nbrw[(size_t)n * a.N + s] = j;
+        n++;
+    };
+    auto global_range = [&](int r0, int r1) {
+        for (int k = r0; k < r1; k++) {
+            const int j = a.list[k];
+            if (j != s)
+                consider(a.X[j], j);
+        }
+    };
 
+    if (binfo[0] < 0) {
+        if (live)
+            for_each_in_cells(a, rp, c, s, [&](const PosRec &xj, int j) { consider(xj, j); });
+    } else {
+        const DevRep &rb = a.reps[binfo[0]];
+        const int n0 = rb.nc[0], n1 = rb.nc[1], n2 = rb.nc[2];
+        const int rf = binfo[1], w = binfo[2];
+        const int cc = c - rb.coff;
+        const int cx = cc % n0, ro = cc / n0 - rf;
+        for (int dz = -1; dz <= 1; dz++) {
+            // the candidate rows of this z offset: own row rf + m / 3 moved
+            // by m % 3 - 1 in y and by dz in z; the distinct ones are staged
+            // in order of first appearance
+            if (tid < 64) {
+                const bool act = tid < 3 * w;
+                int rid = -1;
+                if (act) {
+                    const int r = rf + tid / 3;
+                    int yy = r % n1 + tid % 3 - 1, zz = r / n1 + dz;
+                    yy = yy < 0 ? yy + n1 : (yy >= n1 ? yy - n1 : yy);
+                    zz = zz < 0 ? zz + n2 : (zz >= n2 ? zz - n2 : zz);
+                    rid = rb.coff + n0 * (yy + n1 * zz);
+                    crow[tid] = rid;
+                }
+                // the first lane holding the same row
+                int first = tid;
+                for (int m = 3 * w - 1; m >= 0; m--) {
+                    const int o = __shfl(rid, m);
+                    if (m < tid && o == rid)
+                        first = m;
+                }
+                const bool uniq = act && first == tid;
+                const unsigned long long bm = __ballot(uniq);
+                const int h = __popcll(bm & ((1ull << tid) - 1ull));
+                if (uniq)
+                    hcell[h] = rid;
+                const int hf = __shfl(h, first);
+                if (act)
+                    hrow[tid] = hf;
+                if (tid == 0)
+                    binfo[3] = __popcll(bm);
+            }
+            __syncthreads();
+            const int nh = binfo[3];
+            const bool rows_fit = nh * (n0 + 1) <= NB_ROWCAP;
+            if (rows_fit)
+                for (int t = tid; t < nh * (n0 + 1); t += blockDim.x) {
+                    const int h = t / (n0 + 1);
+                    rowoff[t] = a.start[hcell[h] + t - h * (n0 + 1)];
+                }
+            __syncthreads();
+            if (tid < 64) {
+                // LDS index of each staged row's first particle (exclusive
+                // scan of the rows' particle counts)
+                const int cnt = rows_fit && tid < nh
+                                    ? rowoff[tid * (n0 + 1) + n0] - rowoff[tid * (n0 + 1)]
+                                    : 0;
+                int v = cnt;
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int t = __shfl_up(v, o);
+                    if (tid >= o)
+                        v += t;
+                }
+                if (tid <= nh)
+                    hoff[tid] = v - cnt;
+                if (tid == 63)
+                    binfo[4] = rows_fit ? v : NB_CAP + 1;
+            }
+            __syncthreads();
+            const int total = binfo[4];
+            const bool staged = total <= NB_CAP;
+            if (staged)
+                for (int f = tid; f < total; f += blockDim.x) {
+                    int lo = 0, hi = nh - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (hoff[mid] <= f)
+                            lo = mid;
+                        else
+                            hi = mid - 1;
+                    }
+                    const int j = a.list[rowoff[lo * (n0 + 1)] + f - hoff[lo]];
+                    lslot[f] = j;
+                    lpos[f] = a.X[j];
+                }
+            __syncthreads();
+            if (live && staged) {
+                // the thread's candidates of this z offset, in walk order: the
+                // LDS ranges of its 3 rows (two at the seam) walked as one loop
+                // (the wave's lanes then differ only in their total count)
+                int qs[6], cum[6], tot = 0;
+                for (int dy = 0; dy < 3; dy++) {
+                    // LDS index of list position k of row h: qb + k
+                    const int h = hrow[3 * ro + dy];
+                    const int *off = rowoff + h * (n0 + 1);
+                    const int qb = hoff[h] - off[0];
+                    int r0, r1, r2, r3;
+                    row_ranges(off, n0, cx, r0, r1, r2, r3);
+                    qs[2 * dy] = qb + r0;
+                    tot += r1 - r0;
+                    cum[2 * dy] = tot;
+                    qs[2 * dy + 1] = qb + r2;
+                    tot += r3 - r2;
+                    cum[2 * dy + 1] = tot;
+                }
+                for (int t = 0; t < tot; t++) {
+                    int q = qs[0] + t;
+#pragma unroll
+                    for (int u = 0; u < 5; u++)
+                        if (t >= cum[u])
+                            q = qs[u + 1] + t - cum[u];
+                    const int j = lslot[q];
+                    if (j != s)
+                        consider(lpos[q], j);
+                }
+            } else if (live) {
+                for (int dy = 0; dy < 3; dy++) {
+                    int r0, r1, r2, r3;
+                    row_ranges(a.start + crow[3 * ro + dy], n0, cx, r0, r1, r2, r3);
+                    global_range(r0, r1);
+                    global_range(r2, r3);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (live)
+        a.nn[s] = n | (kept_excluded ? LIST_EXC : 0);
+}
 
 // ------------------------------------------------------- LDS-staged tiles
 // A block owns 256 consecutive cell-list positions.  When they lie in one
@@ -1939,15 +2156,11 @@ template <class R> __global__ void __launch_bounds__(256) k_bin(KArgs<R> a, cons
 
 // --------------------------------------------------------------- cell lists
 // exclusive scan of count[0..n) into start[0..n], counts zeroed after use.
-// phase 1: per block (1024 elements) local scan and block total
-// gate: when non-null, each kernel runs only if a list rebuild was requested
-__global__ void __launch_bounds__(256) k_scan1(const int *count, int *start, int *bsum, int n,
-                                               const StepCtl *gate)
+// phase 1: per block b (1024 elements) local scan and block total
+__device__ __forceinline__ void scan1_block(const int *count, int *start, int *bsum, int n, int b)
 {
-    if (gate && !gate->rebuild)
-        return;
     __shared__ int sh[256];
-    const int base = blockIdx.x * 1024 + threadIdx.x * 4;
+    const int base = b * 1024 + threadIdx.x * 4;
     int v[4], s = 0;
     for (int k = 0; k < 4; k++) {
         v[k] = base + k < n ? count[base + k] : 0;
@@ -1968,20 +2181,18 @@ __global__ void __launch_bounds__(256) k_scan1(const int *count, int *start, int
         run += v[k];
     }
     if (threadIdx.x == 255)
-        bsum[blockIdx.x] = sh[255];
+        bsum[b] = sh[255];
 }
 
 // phase 3: add block offsets, zero the counts, close the array
-// (bsum holds the block totals of phase 1; each block sums those before it)
-__global__ void __launch_bounds__(256) k_scan3(int *count, int *start, const int *bsum, int n,
-                                               const StepCtl *gate)
+// (bsum holds the block totals of phase 1; block b of nb sums those before it)
+__device__ __forceinline__ void scan3_block(int *count, int *start, const int *bsum, int n, int b,
+                                            int nb)
 {
-    if (gate && !gate->rebuild)
-        return;
     __shared__ int sh[256];
     int part = 0;
-    for (int b = threadIdx.x; b < (int)blockIdx.x; b += 256)
-        part += bsum[b];
+    for (int q = threadIdx.x; q < b; q += 256)
+        part += bsum[q];
     sh[threadIdx.x] = part;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
@@ -1989,15 +2200,38 @@ __global__ void __launch_bounds__(256) k_scan3(int *count, int *start, const int
             sh[threadIdx.x] += sh[threadIdx.x + o];
         __syncthreads();
     }
-    const int base = blockIdx.x * 1024 + threadIdx.x * 4;
+    const int base = b * 1024 + threadIdx.x * 4;
     const int off = sh[0];
     for (int k = 0; k < 4; k++)
         if (base + k < n) {
             start[base + k] += off;
             count[base + k] = 0;
         }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
-        start[n] = off + bsum[blockIdx.x];
+    if (b == nb - 1 && threadIdx.x == 0)
+        start[n] = off + bsum[b];
+}
+
+// gate: when non-null, each kernel runs only if a list rebuild was requested
+__global__ void __launch_bounds__(256) k_scan1(const int *count, int *start, int *bsum, int n,
+                                               const StepCtl *gate)
+{
+    if (gate && !gate->rebuild)
+        return;
+    scan1_block(count, start, bsum, n, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_scan3(int *count, int *start, const int *bsum, int n,
+                                               const StepCtl *gate)
+{
+    if (gate && !gate->rebuild)
+        return;
+    scan3_block(count, start, bsum, n, blockIdx.x, gridDim.x);
+}
+
+__device__ __forceinline__ void fill_one(const int *cell, const int *rank, const int *start,
+                                         int *list, int i)
+{
+    list[start[cell[i]] + rank[i]] = i;
 }
 
 __global__ void __launch_bounds__(256) k_fill(int N, const int *cell, const int *rank,
@@ -2007,24 +2241,20 @@ __global__ void __launch_bounds__(256) k_fill(int N, const int *cell, const int 
         return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < N)
-        list[start[cell[i]] + rank[i]] = i;
+        fill_one(cell, rank, start, list, i);
 }
 
 // ascending original index inside each cell: makes the gather order, and so
 // the floating-point sums, independent of atomic arrival order and of the
 // slot numbering (a state uploaded in original order and the same state held
 // in cell order walk their cells, and build their lists, in the same order).
-// One thread per particle; the particle that arrived first in its cell (rank
-// 0, unique per non-empty cell and not changed by the sort) sorts the cell,
-// so empty cells cost nothing.
-__global__ void __launch_bounds__(256) k_cellsort(int N, const int *cell, const int *rank,
-                                                  const int *start, int *list, const PosRec *X,
-                                                  const StepCtl *gate)
+// The particle that arrived first in its cell (rank 0, unique per non-empty
+// cell and not changed by the sort) sorts the cell, so empty cells cost
+// nothing.
+__device__ __forceinline__ void cellsort_one(const int *cell, const int *rank, const int *start,
+                                             int *list, const PosRec *X, int i)
 {
-    if (gate && !gate->rebuild)
-        return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N || rank[i] != 0)
+    if (rank[i] != 0)
         return;
     const int c = cell[i];
     const int s0 = start[c], s1 = start[c + 1];
@@ -2040,10 +2270,31 @@ __global__ void __launch_bounds__(256) k_cellsort(int N, const int *cell, const 
     }
 }
 
+__global__ void __launch_bounds__(256) k_cellsort(int N, const int *cell, const int *rank,
+                                                  const int *start, int *list, const PosRec *X,
+                                                  const StepCtl *gate)
+{
+    if (gate && !gate->rebuild)
+        return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N)
+        cellsort_one(cell, rank, start, list, X, i);
+}
+
 __global__ void k_epoch_inc(StepCtl *ctl)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0)
         ctl->epoch += 1;
+}
+
+// the last kernel of a list step that moves nothing (M_STEP): the step's
+// force evaluation is counted, and the compacted walk has laid out any new list
+__global__ void k_step_end(StepCtl *ctl)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctl->epoch += 1;
+        ctl->fresh = 0u;
+    }
 }
 
 // a list was just built at the current positions: no rebuild pending, bound 0
@@ -2057,9 +2308,11 @@ __global__ void k_list_reset(StepCtl *ctl)
 
 // the grid max of the block drifts of the kernel before is added to the
 // displacement bound D; a list rebuild is requested once 2 D reaches the skin
-// (a pair within reach now was within reach + skin at the build)
+// (a pair within reach now was within reach + skin at the build).
+// epoch_inc: also count the step's force evaluation; step_end: the list
+// step's kernels are done (the compacted walk has laid out any new list)
 __global__ void __launch_bounds__(64) k_decide(StepCtl *ctl, unsigned int *dslot, double skin,
-                                               int epoch_inc)
+                                               int epoch_inc, int step_end)
 {
     // one wave: each lane takes and clears one slot
     float m = __uint_as_float(dslot[threadIdx.x]);
@@ -2073,9 +2326,12 @@ __global__ void __launch_bounds__(64) k_decide(StepCtl *ctl, unsigned int *dslot
         if (2 * D >= skin)
             ctl->rebuild = 1u;
         if (epoch_inc)
-            ctl->epoch += 1; // the step's force evaluation is done (k_epoch_inc)
+            ctl->epoch += 1;
+        if (step_end)
+            ctl->fresh = 0u;
     }
 }
+
 
 __global__ void k_set_call_base(StepCtl *ctl)
 {
