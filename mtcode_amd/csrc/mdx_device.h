@@ -147,33 +147,43 @@ template <class R> __host__ __device__ __forceinline__ void quat_to_mat(R w, R x
 __host__ __device__ __forceinline__ float sqrt_(float v) { return __builtin_sqrtf(v); }
 __host__ __device__ __forceinline__ double sqrt_(double v) { return __builtin_sqrt(v); }
 
+// a / S: FP64 divides (the conversion of uploaded FP64 matrices, bitwise
+// as the host's); FP32 multiplies by the one reciprocal iS = 1 / S
+template <class R> __host__ __device__ __forceinline__ R qdiv(R a, R S, R iS)
+{
+    if constexpr (sizeof(R) == 8)
+        return a / S;
+    else
+        return a * iS;
+}
+
 template <class R> __host__ __device__ __forceinline__ void mat_to_quat(const R m[9], R q[4])
 {
     const R tr = m[0] + m[4] + m[8];
     R w, x, y, z;
     if (tr > (R)0) {
-        const R S = sqrt_(tr + (R)1) * (R)2;
+        const R S = sqrt_(tr + (R)1) * (R)2, iS = (R)1 / S;
         w = (R)0.25 * S;
-        x = (m[7] - m[5]) / S;
-        y = (m[2] - m[6]) / S;
-        z = (m[3] - m[1]) / S;
+        x = qdiv(m[7] - m[5], S, iS);
+        y = qdiv(m[2] - m[6], S, iS);
+        z = qdiv(m[3] - m[1], S, iS);
     } else if (m[0] > m[4] && m[0] > m[8]) {
-        const R S = sqrt_((R)1 + m[0] - m[4] - m[8]) * (R)2;
-        w = (m[7] - m[5]) / S;
+        const R S = sqrt_((R)1 + m[0] - m[4] - m[8]) * (R)2, iS = (R)1 / S;
+        w = qdiv(m[7] - m[5], S, iS);
         x = (R)0.25 * S;
-        y = (m[1] + m[3]) / S;
-        z = (m[2] + m[6]) / S;
+        y = qdiv(m[1] + m[3], S, iS);
+        z = qdiv(m[2] + m[6], S, iS);
     } else if (m[4] > m[8]) {
-        const R S = sqrt_((R)1 + m[4] - m[0] - m[8]) * (R)2;
-        w = (m[2] - m[6]) / S;
-        x = (m[1] + m[3]) / S;
+        const R S = sqrt_((R)1 + m[4] - m[0] - m[8]) * (R)2, iS = (R)1 / S;
+        w = qdiv(m[2] - m[6], S, iS);
+        x = qdiv(m[1] + m[3], S, iS);
         y = (R)0.25 * S;
-        z = (m[5] + m[7]) / S;
+        z = qdiv(m[5] + m[7], S, iS);
     } else {
-        const R S = sqrt_((R)1 + m[8] - m[0] - m[4]) * (R)2;
-        w = (m[3] - m[1]) / S;
-        x = (m[2] + m[6]) / S;
-        y = (m[5] + m[7]) / S;
+        const R S = sqrt_((R)1 + m[8] - m[0] - m[4]) * (R)2, iS = (R)1 / S;
+        w = qdiv(m[3] - m[1], S, iS);
+        x = qdiv(m[2] + m[6], S, iS);
+        y = qdiv(m[5] + m[7], S, iS);
         z = (R)0.25 * S;
     }
     const R inv = (R)1 / sqrt_(w * w + x * x + y * y + z * z);

@@ -96,6 +96,9 @@ struct mdx_engine {
     int has_active = 0;
     double fguard = 1e4;
     int use_tiles = 0; // LDS-staged neighbour tiles (MDX_TILES=1 enables)
+    // positions per block of the compacted patch kernel (256 .. PBLK, a
+    // multiple of 256; 0: chosen from N) (MDX_PBLK)
+    int pblk = 0;
     // walk / patch / integrator kernels (1) or one kernel per step (0); by
     // default the one-kernel form up to SPLIT_MIN_N particles, where launches
     // dominate the step (MDX_SPLIT=0/1 forces either)
@@ -361,6 +364,11 @@ template <class R> static KArgs<R> make_args(mdx_engine *e)
     a.gammar = e->gammar;
     for (int k = 0; k < 3; k++)
         a.I[k] = e->I[k];
+    // the friction factors of the sphere step, divided here once (the same
+    // IEEE quotients the device would form)
+    a.gm_m = -e->gamma / e->m;
+    for (int k = 0; k < 3; k++)
+        a.gr_I[k] = -e->gammar / e->I[k];
     // LangevinNVT coefficients (Langevin.cpp:44-52); r depends on kT (per replica)
     const double d = (e->gamma * e->dt / 2.), q = (e->dt) / 2.;
     const double c2 = (1.0 / (1.0 + (d))), c3 = (1.0 / (1.0 + (d))) * q, c5 = (1 - (d));
@@ -472,6 +480,8 @@ static int launch_gather(mdx_engine *e, const double *noise_dev, int lmode = 0)
         int pblk = 256;
         while (pblk < PBLK && (long long)e->N / (2 * pblk) >= 512)
             pblk *= 2;
+        if (e->pblk > 0)
+            pblk = e->pblk;
         // fused form (list steps): the walk integrates the particles without
         // patch candidates, the patch kernel the others
         const bool fuse_ok = e->use_fuse == 1 ||
@@ -1014,6 +1024,8 @@ int mdx_create(const mdx_desc *d, mdx_engine **out)
     e->use_split = N > SPLIT_MIN_N;
     if (const char *t = std::getenv("MDX_TILES"))
         e->use_tiles = std::atoi(t) != 0;
+    if (const char *t = std::getenv("MDX_PBLK"))
+        e->pblk = std::min(PBLK, std::max(0, std::atoi(t) / 256 * 256));
     if (const char *t = std::getenv("MDX_SPLIT"))
         e->use_split = std::atoi(t) != 0;
     if (const char *t = std::getenv("MDX_FUSE"))

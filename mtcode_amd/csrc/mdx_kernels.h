@@ -126,6 +126,7 @@ template <class R> struct KArgs {
     double fa;
     // integrator
     double dt, hdt, c1, m, gamma, gammar, I[3];
+    double gm_m, gr_I[3]; // -gamma / m, -gammar / I[k]
     double b_c5c2, b_c5c3q, b_c2;
     // noise
     uint32_t key0, key1;
@@ -195,19 +196,38 @@ template <class R> __device__ __forceinline__ void lab_normal_exact(const R *q, 
 }
 
 // LangevinNVTR::genfullmat (LangevinR.cpp:392-461)
+// FP64 with the reference's divisions; FP32 with one hardware reciprocal
+// (v_rcp_f32, 1 ulp) per axis in place of its nine divisions
 template <class R>
 __device__ __forceinline__ void genfullmat(R dt, const R I[3], const R L[3], R up[9])
 {
-    R tempx = dt * L[0] / I[0];
-    R tempy = dt * L[1] / I[1];
-    R tempz = dt * L[2] / I[2];
-    R qx = tempx / (R)4., qy = tempy / (R)4., hz = tempz / (R)2.;
-    R fac1 = ((R)1 - qx * qx) / ((R)1 + qx * qx);
-    R fac2 = tempx / ((R)2. * ((R)1 + qx * qx));
-    R fac3 = ((R)1 - qy * qy) / ((R)1 + qy * qy);
-    R fac4 = tempy / ((R)2. * ((R)1 + qy * qy));
-    R fac5 = ((R)1 - hz * hz) / ((R)1 + hz * hz);
-    R fac6 = (tempz) / ((R)1 + hz * hz);
+    R fac1, fac2, fac3, fac4, fac5, fac6;
+    if constexpr (sizeof(R) == 8) {
+        R tempx = dt * L[0] / I[0];
+        R tempy = dt * L[1] / I[1];
+        R tempz = dt * L[2] / I[2];
+        R qx = tempx / (R)4., qy = tempy / (R)4., hz = tempz / (R)2.;
+        fac1 = ((R)1 - qx * qx) / ((R)1 + qx * qx);
+        fac2 = tempx / ((R)2. * ((R)1 + qx * qx));
+        fac3 = ((R)1 - qy * qy) / ((R)1 + qy * qy);
+        fac4 = tempy / ((R)2. * ((R)1 + qy * qy));
+        fac5 = ((R)1 - hz * hz) / ((R)1 + hz * hz);
+        fac6 = (tempz) / ((R)1 + hz * hz);
+    } else {
+        const float tempx = dt * L[0] * __builtin_amdgcn_rcpf(I[0]);
+        const float tempy = dt * L[1] * __builtin_amdgcn_rcpf(I[1]);
+        const float tempz = dt * L[2] * __builtin_amdgcn_rcpf(I[2]);
+        const float qx = tempx * 0.25f, qy = tempy * 0.25f, hz = tempz * 0.5f;
+        const float ix = __builtin_amdgcn_rcpf(1.f + qx * qx);
+        const float iy = __builtin_amdgcn_rcpf(1.f + qy * qy);
+        const float iz = __builtin_amdgcn_rcpf(1.f + hz * hz);
+        fac1 = (1.f - qx * qx) * ix;
+        fac2 = tempx * 0.5f * ix;
+        fac3 = (1.f - qy * qy) * iy;
+        fac4 = tempy * 0.5f * iy;
+        fac5 = (1.f - hz * hz) * iz;
+        fac6 = tempz * iz;
+    }
     up[0] = -fac4 * fac4 + fac3 * fac3 * fac5;
     up[1] = fac3 * (fac2 * fac4 * ((R)1 + fac5) - fac1 * fac6);
     up[2] = fac3 * (fac1 * fac4 * ((R)1 + fac5) + fac2 * fac6);
@@ -317,6 +337,26 @@ __device__ __forceinline__ void min_image(const KArgs<R> &a, const DevRep &rp, d
     d[0] = mimage(x0 - b.x, rp.l[0], rp.l2s[0], a.periodic);
     d[1] = mimage(x1 - b.y, rp.l[1], rp.l2s[1], a.periodic);
     d[2] = mimage(x2 - b.z, rp.l[2], rp.l2s[2], a.periodic);
+}
+
+// The raw difference x_i - x_j is the minimum image for every pair the list
+// (or the cells of its build) can hold when the box has walls, or when x_i
+// lies farther than two cells plus the skin from every face: such a pair was
+// in adjacent cells at the build (under two cells apart per axis) and each
+// side has moved less than half the skin since.  (min_image would return the
+// same value; this skips its tests.)
+template <class R>
+__device__ __forceinline__ bool face_clear(const KArgs<R> &a, const DevRep &rp, const PosRec &x)
+{
+    if (!a.periodic)
+        return true;
+    const double xs[3] = {x.x, x.y, x.z};
+    bool ok = true;
+    for (int k = 0; k < 3; k++) {
+        const double m = 2.0 * rp.l[k] / rp.nc[k] + a.skin;
+        ok = ok && xs[k] > m && xs[k] < rp.l[k] - m;
+    }
+    return ok;
 }
 
 // ------------------------------------------------------------- gather kernel
@@ -1040,13 +1080,13 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
         const R pbx = q[0] * pl[0] + q[1] * pl[1] + q[2] * pl[2];
         const R pby = q[3] * pl[0] + q[4] * pl[1] + q[5] * pl[2];
         const R pbz = q[6] * pl[0] + q[7] * pl[1] + q[8] * pl[2];
-        const R gm = (R)(-a.gamma / a.m);
+        const R gm = (R)a.gm_m;
         const R fx = gm * pbx + (R)rp.Rt * nz[0];
         const R fy = gm * pby + (R)rp.Rt * nz[1];
         const R fz = gm * pbz + (R)rp.Rt * nz[2];
-        const R tbx = (R)(-a.gammar / a.I[0]) * ll[0] + (R)rp.Rr * nz[3];
-        const R tby = (R)(-a.gammar / a.I[1]) * ll[1] + (R)rp.Rr * nz[4];
-        const R tbz = (R)(-a.gammar / a.I[2]) * ll[2] + (R)rp.Rr * nz[5];
+        const R tbx = (R)a.gr_I[0] * ll[0] + (R)rp.Rr * nz[3];
+        const R tby = (R)a.gr_I[1] * ll[1] + (R)rp.Rr * nz[4];
+        const R tbz = (R)a.gr_I[2] * ll[2] + (R)rp.Rr * nz[5];
         const R Fx = Fi[0] + fx * q[0] + fy * q[3] + fz * q[6];
         const R Fy = Fi[1] + fx * q[1] + fy * q[4] + fz * q[7];
         const R Fz = Fi[2] + fx * q[2] + fy * q[5] + fz * q[8];
@@ -1203,6 +1243,7 @@ __global__ void __launch_bounds__(256, (sizeof(R) == 4 && !INT) ? 6 : 1) k_walkc
     __shared__ PosRec sx[256];
     __shared__ float sq[256 * 9]; // orientations in FP32 for the own-patch screen
     __shared__ int sexc[256 * 2];
+    __shared__ unsigned char spl[256]; // face_clear of each position
     __shared__ R pf[256 * 3];
     __shared__ int pj[256];
     __shared__ int wsum[4];
@@ -1228,6 +1269,7 @@ __global__ void __launch_bounds__(256, (sizeof(R) == 4 && !INT) ? 6 : 1) k_walkc
         const bool ne = needs_exclusions(a, s);
         sexc[2 * tid] = ne ? a.exc_off[xi.id] : 0;
         sexc[2 * tid + 1] = ne ? a.exc_off[xi.id + 1] : 0;
+        spl[tid] = face_clear(a, rep_rec(a, xi.meta), xi);
     }
     // exclusive scan of the entry counts
     int v = cnt;
@@ -1250,11 +1292,17 @@ __global__ void __launch_bounds__(256, (sizeof(R) == 4 && !INT) ? 6 : 1) k_walkc
 
     // WCA of a pair into acc; true for a patch candidate that passes the
     // own-patch screen (the one-lane walk's rule)
-    auto visit = [&](const PosRec &xo, const float *qo, int e0, int e1, const PosRec &xj,
-                     Acc<R> &acc) -> bool {
+    auto visit = [&](const PosRec &xo, const float *qo, int e0, int e1, bool pl,
+                     const PosRec &xj, Acc<R> &acc) -> bool {
         const DevRep rp = rep_rec(a, xo.meta);
         double d[3];
-        min_image(a, rp, xo.x, xo.y, xo.z, xj, d);
+        if (pl) {
+            d[0] = xo.x - xj.x;
+            d[1] = xo.y - xj.y;
+            d[2] = xo.z - xj.z;
+        } else {
+            min_image(a, rp, xo.x, xo.y, xo.z, xj, d);
+        }
         const double r2s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
         if (!(r2s < a.rmax2))
             return false;
@@ -1319,7 +1367,7 @@ __global__ void __launch_bounds__(256, (sizeof(R) == 4 && !INT) ? 6 : 1) k_walkc
             Acc<R> pa;
             pa.F[0] = pa.F[1] = pa.F[2] = 0;
             pa.bad = false;
-            const bool c = visit(xo, sq + 9 * o, sexc[2 * o], sexc[2 * o + 1], xj, pa);
+            const bool c = visit(xo, sq + 9 * o, sexc[2 * o], sexc[2 * o + 1], spl[o], xj, pa);
             for (int k = 0; k < 3; k++)
                 pf[3 * tid + k] = pa.F[k];
             pj[tid] = c ? j : -1;
@@ -1346,7 +1394,7 @@ __global__ void __launch_bounds__(256, (sizeof(R) == 4 && !INT) ? 6 : 1) k_walkc
             pa.F[0] = pa.F[1] = pa.F[2] = 0;
             pa.bad = false;
             for_each_in_cells(a, rp, a.bcell[s], s, [&](const PosRec &xj, int j) {
-                if (visit(xi, sq + 9 * tid, sexc[2 * tid], sexc[2 * tid + 1], xj, pa)) {
+                if (visit(xi, sq + 9 * tid, sexc[2 * tid], sexc[2 * tid + 1], spl[tid], xj, pa)) {
                     if (npc < PMAXG)
                         a.cand[(size_t)npc * a.N + s] = j;
                     npc++;
@@ -1632,27 +1680,32 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_integrate(
 }
 
 // neighbour list of every particle from the cell list of the current
-// positions: the slots within sqrt(rl2) in the order of the 27-cell walk
-// (z rows -1, 0, +1 outer, y rows -1, 0, +1 inner, then x along the row, as
-// for_each_in_cells), written to nbr[k * N + slot] with the count in nn;
+// positions: the slots within sqrt(rl2) in the order of a 27-cell walk
+// (z rows -1, 0, +1 outer, y rows -1, 0, +1 inner, then the cells x - 1,
+// x, x + 1 of the row), written to nbr[k * N + slot] with the count in nn;
 // the compacted walk lays the entries out per block afterwards (fresh).
 // gated: only when a rebuild was requested.
 // One thread per cell-list position: a block's particles then lie in a few
-// consecutive x-rows of cells (the list is in cell order).  For each z offset
-// in turn the block stages in LDS the distinct neighbour rows of its rows --
-// their cell offsets, and each row's particles (position record and slot),
-// a contiguous run of the list read with coalesced loads -- and walks them
-// there, with no global load per candidate.  A block whose particles span
-// more than NB_W rows or two replicas, a grid with fewer than 3 cells on a
-// side, and a z offset whose rows hold more than NB_CAP particles or
-// NB_ROWCAP cell offsets walk global memory instead.
+// consecutive x-rows of cells (the list is in cell order).  The block first
+// finds, for each z offset (one wave each), the distinct neighbour rows of
+// its rows and loads their cell offsets into LDS; then, one z offset at a
+// time, it stages those rows' particles (position record and slot, read with
+// coalesced loads from the list) and walks them there.  Each staged row
+// holds its last cell, the row, then its first cell again, so that the three
+// cells about any x -- across the periodic seam too -- are one contiguous
+// run.  The walk first marks the candidates within the list radius in a bit
+// mask (a distance per candidate, no other work), then takes the marked ones
+// in order.  A block whose particles span more than NB_W rows or two
+// replicas, a grid with fewer than 3 cells on a side, and a z offset whose
+// rows hold more than NB_CAP particles or NB_ROWCAP cell offsets walk global
+// memory instead.
 constexpr int NB_W = 16;        // own x-rows of a block walked from LDS
-constexpr int NB_CAP = 896;     // staged particles per z offset
-constexpr int NB_ROWCAP = 1280; // staged cell offsets per z offset
+constexpr int NB_CAP = 800;     // staged particles per z offset
+constexpr int NB_ROWCAP = 1152; // staged cell offsets per z offset
 
-// the list ranges [r0, r1) and [r2, r3) of the x neighbours of x-cell cx in
-// a row whose cell offsets are off[0..n0] (one range, or two at the periodic
-// seam; the whole row when it has fewer than 3 cells)
+// the list ranges [r0, r1) and [r2, r3) of the cells x - 1, x, x + 1 about
+// x-cell cx, in that order, in a row whose cell offsets are off[0..n0]
+// (across the seam: two ranges; the whole row when it has fewer than 3 cells)
 template <class T>
 __device__ __forceinline__ void row_ranges(const T *off, int n0, int cx, int &r0, int &r1, int &r2,
                                            int &r3)
@@ -1660,10 +1713,10 @@ __device__ __forceinline__ void row_ranges(const T *off, int n0, int cx, int &r0
     r2 = r3 = 0;
     if (n0 >= 3) {
         if (cx == 0) {
-            r0 = off[0];
-            r1 = off[2];
-            r2 = off[n0 - 1];
-            r3 = off[n0];
+            r0 = off[n0 - 1];
+            r1 = off[n0];
+            r2 = off[0];
+            r3 = off[2];
         } else if (cx == n0 - 1) {
             r0 = off[n0 - 2];
             r1 = off[n0];
@@ -1686,14 +1739,17 @@ __global__ void __launch_bounds__(256) k_nbuild(KArgs<R> a, const int *cells, in
         return;
     __shared__ PosRec lpos[NB_CAP];
     __shared__ int lslot[NB_CAP];
-    __shared__ int rowoff[NB_ROWCAP];  // n0 + 1 cell offsets per staged row
-    __shared__ int crow[3 * NB_W];     // first cell of each candidate row of a z offset
-    __shared__ int hrow[3 * NB_W];     // its staged row
-    __shared__ int hcell[3 * NB_W];    // first cell of each staged row
-    __shared__ int hoff[3 * NB_W + 1]; // LDS index of each staged row's first particle
-    __shared__ int ends[4];            // replica and cell of the first and last position
-    __shared__ int binfo[5];           // replica (-1: global walk), first own row, own rows,
-                                       // staged rows, staged particles
+    // per z offset p: n0 + 1 cell offsets per staged row, relative to the
+    // row's first list position hk0
+    __shared__ unsigned short rowoff[3][NB_ROWCAP];
+    __shared__ int crow[3][3 * NB_W];     // first cell of each candidate row
+    __shared__ int hrow[3][3 * NB_W];     // its staged row
+    __shared__ int hcell[3][3 * NB_W];    // first cell of each staged row
+    __shared__ int hk0[3][3 * NB_W];      // list position of its first particle
+    __shared__ int hoff[3][3 * NB_W + 1]; // LDS index of its first staged particle
+    __shared__ int pinfo[3][3]; // staged rows; staged particles (> NB_CAP: global); a row over NB_CAP
+    __shared__ int ends[4];     // replica and cell of the first and last position
+    __shared__ int binfo[3];    // replica (-1: global walk), first own row, own rows
     const int tid = threadIdx.x;
     if (blockIdx.x == 0 && tid == 0) {
         atomicAdd(&a.ctl->nbuilds, 1u);
@@ -1758,17 +1814,26 @@ __global__ void __launch_bounds__(256) k_nbuild(KArgs<R> a, const int *cells, in
         patch_roles(xi.id, ti, xj.id, tj, ta, tc);
         return (a.active_mask >> (ta * MAXT + tc)) & 1ull;
     };
-    auto consider = [&](const PosRec &xj, int j) {
+    // plain: no candidate can be a periodic image (the cell is off the box
+    // faces, at least 5 cells per side, or walls), so the minimum image is the
+    // plain difference (the same value min_image returns)
+    bool plain = false;
+    auto dist2 = [&](const PosRec &xj) -> double {
         double d[3];
-        min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
-        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-        if (!(r2 < a.rl2))
-            return;
-        // beyond the WCA reach: listed only for a type pair with patch terms
+        if (plain) {
+            d[0] = xi.x - xj.x;
+            d[1] = xi.y - xj.y;
+            d[2] = xi.z - xj.z;
+        } else {
+            min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
+        }
+        return d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    };
+    // a candidate within the list radius (r2 < rl2): listed unless beyond
+    // the WCA reach without patch terms, or an exclusion without them
+    auto take = [&](const PosRec &xj, int j, double r2) {
         if (!(r2 < a.rl2w) && !patch_active(xj))
             return;
-        // a pair without WCA (exclusion) is listed only if it can still
-        // interact through patches
         bool ex = false;
         for (int e = exc0; e < exc1; e++)
             ex |= (uint32_t)a.exc_j[e] == xj.id;
@@ -1780,6 +1845,11 @@ __global__ void __launch_bounds__(256) k_nbuild(KArgs<R> a, const int *cells, in
         if (n < a.kmax)
             a.nbrw[(size_t)n * a.N + s] = j;
         n++;
+    };
+    auto consider = [&](const PosRec &xj, int j) {
+        const double r2 = dist2(xj);
+        if (r2 < a.rl2)
+            take(xj, j, r2);
     };
     auto global_range = [&](int r0, int r1) {
         for (int k = r0; k < r1; k++) {
@@ -1798,116 +1868,147 @@ __global__ void __launch_bounds__(256) k_nbuild(KArgs<R> a, const int *cells, in
         const int rf = binfo[1], w = binfo[2];
         const int cc = c - rb.coff;
         const int cx = cc % n0, ro = cc / n0 - rf;
-        for (int dz = -1; dz <= 1; dz++) {
-            // the candidate rows of this z offset: own row rf + m / 3 moved
-            // by m % 3 - 1 in y and by dz in z; the distinct ones are staged
-            // in order of first appearance
-            if (tid < 64) {
-                const bool act = tid < 3 * w;
-                int rid = -1;
-                if (act) {
-                    const int r = rf + tid / 3;
-                    int yy = r % n1 + tid % 3 - 1, zz = r / n1 + dz;
-                    yy = yy < 0 ? yy + n1 : (yy >= n1 ? yy - n1 : yy);
-                    zz = zz < 0 ? zz + n2 : (zz >= n2 ? zz - n2 : zz);
-                    rid = rb.coff + n0 * (yy + n1 * zz);
-                    crow[tid] = rid;
-                }
-                // the first lane holding the same row
-                int first = tid;
-                for (int m = 3 * w - 1; m >= 0; m--) {
-                    const int o = __shfl(rid, m);
-                    if (m < tid && o == rid)
-                        first = m;
-                }
-                const bool uniq = act && first == tid;
-                const unsigned long long bm = __ballot(uniq);
-                const int h = __popcll(bm & ((1ull << tid) - 1ull));
-                if (uniq)
-                    hcell[h] = rid;
-                const int hf = __shfl(h, first);
-                if (act)
-                    hrow[tid] = hf;
-                if (tid == 0)
-                    binfo[3] = __popcll(bm);
+        const int cy = (cc / n0) % n1, cz = cc / (n0 * n1);
+        plain = !a.periodic || (n0 >= 5 && n1 >= 5 && n2 >= 5 && cx >= 1 && cx <= n0 - 2 &&
+                                cy >= 1 && cy <= n1 - 2 && cz >= 1 && cz <= n2 - 2);
+        const int wv = tid >> 6, lane = tid & 63;
+        // wave p (z offset p - 1): the candidate rows, own row rf + m / 3
+        // moved by m % 3 - 1 in y and by p - 1 in z; the distinct ones are
+        // staged in order of first appearance
+        if (wv < 3) {
+            const bool act = lane < 3 * w;
+            int rid = -1;
+            if (act) {
+                const int r = rf + lane / 3;
+                int yy = r % n1 + lane % 3 - 1, zz = r / n1 + wv - 1;
+                yy = yy < 0 ? yy + n1 : (yy >= n1 ? yy - n1 : yy);
+                zz = zz < 0 ? zz + n2 : (zz >= n2 ? zz - n2 : zz);
+                rid = rb.coff + n0 * (yy + n1 * zz);
+                crow[wv][lane] = rid;
             }
-            __syncthreads();
-            const int nh = binfo[3];
-            const bool rows_fit = nh * (n0 + 1) <= NB_ROWCAP;
-            if (rows_fit)
+            // the first lane holding the same row
+            int first = lane;
+            for (int m = 3 * w - 1; m >= 0; m--) {
+                const int o = __shfl(rid, m);
+                if (m < lane && o == rid)
+                    first = m;
+            }
+            const bool uniq = act && first == lane;
+            const unsigned long long bm = __ballot(uniq);
+            const int h = __popcll(bm & ((1ull << lane) - 1ull));
+            if (uniq)
+                hcell[wv][h] = rid;
+            const int hf = __shfl(h, first);
+            if (act)
+                hrow[wv][lane] = hf;
+            if (lane == 0) {
+                pinfo[wv][0] = __popcll(bm);
+                pinfo[wv][2] = 0;
+            }
+        }
+        __syncthreads();
+        // the cell offsets of every staged row of the three z offsets
+        for (int p = 0; p < 3; p++) {
+            const int nh = pinfo[p][0];
+            if (nh * (n0 + 1) <= NB_ROWCAP)
                 for (int t = tid; t < nh * (n0 + 1); t += blockDim.x) {
                     const int h = t / (n0 + 1);
-                    rowoff[t] = a.start[hcell[h] + t - h * (n0 + 1)];
+                    const int x = t - h * (n0 + 1);
+                    const int k0 = a.start[hcell[p][h]];
+                    const int r = a.start[hcell[p][h] + x] - k0;
+                    rowoff[p][t] = (unsigned short)r;
+                    if (x == 0)
+                        hk0[p][h] = k0;
+                    if (r > NB_CAP)
+                        pinfo[p][2] = 1; // (then the 16-bit offsets may not hold it)
                 }
-            __syncthreads();
-            if (tid < 64) {
-                // LDS index of each staged row's first particle (exclusive
-                // scan of the rows' particle counts)
-                const int cnt = rows_fit && tid < nh
-                                    ? rowoff[tid * (n0 + 1) + n0] - rowoff[tid * (n0 + 1)]
-                                    : 0;
-                int v = cnt;
-                for (int o = 1; o < 64; o <<= 1) {
-                    const int t = __shfl_up(v, o);
-                    if (tid >= o)
-                        v += t;
-                }
-                if (tid <= nh)
-                    hoff[tid] = v - cnt;
-                if (tid == 63)
-                    binfo[4] = rows_fit ? v : NB_CAP + 1;
+        }
+        __syncthreads();
+        if (wv < 3) {
+            // LDS index of each staged row's first particle: exclusive scan
+            // of the staged rows' sizes (the row plus its last and first
+            // cells once more)
+            const int nh = pinfo[wv][0];
+            const bool fit = nh * (n0 + 1) <= NB_ROWCAP && !pinfo[wv][2];
+            int cnt = 0;
+            if (fit && lane < nh) {
+                const unsigned short *off = rowoff[wv] + lane * (n0 + 1);
+                cnt = off[n0] + (off[n0] - off[n0 - 1]) + off[1];
             }
-            __syncthreads();
-            const int total = binfo[4];
+            int v = cnt;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(v, o);
+                if (lane >= o)
+                    v += t;
+            }
+            if (lane <= nh)
+                hoff[wv][lane] = v - cnt;
+            if (lane == 63)
+                pinfo[wv][1] = fit ? v : NB_CAP + 1;
+        }
+        __syncthreads();
+        for (int p = 0; p < 3; p++) {
+            const int nh = pinfo[p][0];
+            const int total = pinfo[p][1];
             const bool staged = total <= NB_CAP;
             if (staged)
                 for (int f = tid; f < total; f += blockDim.x) {
                     int lo = 0, hi = nh - 1;
                     while (lo < hi) {
                         const int mid = (lo + hi + 1) >> 1;
-                        if (hoff[mid] <= f)
+                        if (hoff[p][mid] <= f)
                             lo = mid;
                         else
                             hi = mid - 1;
                     }
-                    const int j = a.list[rowoff[lo * (n0 + 1)] + f - hoff[lo]];
+                    // the staged row: last cell, the row, first cell
+                    const unsigned short *off = rowoff[p] + lo * (n0 + 1);
+                    const int cl = off[n0] - off[n0 - 1], cr = off[n0];
+                    int o = f - hoff[p][lo];
+                    o = o < cl ? off[n0 - 1] + o : (o < cl + cr ? o - cl : o - cl - cr);
+                    const int j = a.list[hk0[p][lo] + o];
                     lslot[f] = j;
                     lpos[f] = a.X[j];
                 }
             __syncthreads();
             if (live && staged) {
-                // the thread's candidates of this z offset, in walk order: the
-                // LDS ranges of its 3 rows (two at the seam) walked as one loop
-                // (the wave's lanes then differ only in their total count)
-                int qs[6], cum[6], tot = 0;
+                // the thread's candidates of this z offset, in walk order:
+                // three LDS runs, one per row (cells cx - 1 .. cx + 1 start at
+                // staged cell cx, the last cell standing first)
+                int qs[3], cum[3], tot = 0;
                 for (int dy = 0; dy < 3; dy++) {
-                    // LDS index of list position k of row h: qb + k
-                    const int h = hrow[3 * ro + dy];
-                    const int *off = rowoff + h * (n0 + 1);
-                    const int qb = hoff[h] - off[0];
-                    int r0, r1, r2, r3;
-                    row_ranges(off, n0, cx, r0, r1, r2, r3);
-                    qs[2 * dy] = qb + r0;
-                    tot += r1 - r0;
-                    cum[2 * dy] = tot;
-                    qs[2 * dy + 1] = qb + r2;
-                    tot += r3 - r2;
-                    cum[2 * dy + 1] = tot;
+                    const int h = hrow[p][3 * ro + dy];
+                    const unsigned short *off = rowoff[p] + h * (n0 + 1);
+                    const int cl = off[n0] - off[n0 - 1];
+                    const int b0 = cx == 0 ? 0 : cl + off[cx - 1];
+                    const int b1 = cx + 3 == n0 + 2 ? cl + off[n0] + off[1] : cl + off[cx + 2];
+                    qs[dy] = hoff[p][h] + b0;
+                    tot += b1 - b0;
+                    cum[dy] = tot;
                 }
-                for (int t = 0; t < tot; t++) {
-                    int q = qs[0] + t;
-#pragma unroll
-                    for (int u = 0; u < 5; u++)
-                        if (t >= cum[u])
-                            q = qs[u + 1] + t - cum[u];
-                    const int j = lslot[q];
-                    if (j != s)
-                        consider(lpos[q], j);
+                // marks, 64 candidates at a time, then the marked ones in order
+                for (int t0 = 0; t0 < tot; t0 += 64) {
+                    const int te = min(tot, t0 + 64);
+                    unsigned long long hit = 0ull;
+                    for (int t = t0; t < te; t++) {
+                        const int q = t < cum[0] ? qs[0] + t
+                                                 : (t < cum[1] ? qs[1] + t - cum[0] : qs[2] + t - cum[1]);
+                        const PosRec &xj = lpos[q];
+                        if (xj.id != xi.id && dist2(xj) < a.rl2)
+                            hit |= 1ull << (t - t0);
+                    }
+                    while (hit) {
+                        const int t = t0 + __builtin_ctzll(hit);
+                        hit &= hit - 1ull;
+                        const int q = t < cum[0] ? qs[0] + t
+                                                 : (t < cum[1] ? qs[1] + t - cum[0] : qs[2] + t - cum[1]);
+                        take(lpos[q], lslot[q], dist2(lpos[q]));
+                    }
                 }
             } else if (live) {
                 for (int dy = 0; dy < 3; dy++) {
                     int r0, r1, r2, r3;
-                    row_ranges(a.start + crow[3 * ro + dy], n0, cx, r0, r1, r2, r3);
+                    row_ranges(a.start + crow[p][3 * ro + dy], n0, cx, r0, r1, r2, r3);
                     global_range(r0, r1);
                     global_range(r2, r3);
                 }
