@@ -37,16 +37,25 @@ from mtcode_amd import workloads as wl  # noqa: E402
 
 METRIC = "particle-steps/s at 1M patchy particles (mixed vs FP64 GPU); force rel err vs FP64"
 HBM_PEAK_GBS = 8000.0
+# the engine's re-sort period (list steps between re-sorts; MDX_RESORT)
+RESORT_EVERY = int(os.environ.get("MDX_RESORT", "200"))
 
-# Algorithmic bytes per particle-step: one read and one write of every
-# per-particle state array the step updates, at the stored precision, plus one
-# read of the static per-particle type word.
-#   mixed: x FP64 24 + p FP32 12 + Q FP32 36 + L FP32 12 = 84 B state -> 2*84 + 4
-#   fp64 : x 24 + p 24 + Q 72 + L 24 = 144 B state -> 2*144 + 4
-ALGO_BYTES = {"mixed": 2 * 84 + 4, "fp64": 2 * 144 + 4}
-# beads: x 24 + p 12 (mixed) or 24 (fp64), plus the static type word and the
-# bead's bond (4 x 4 B) and angle (2 x 4 B) partners
-BEAD_BYTES = {"mixed": 2 * 36 + 28, "fp64": 2 * 48 + 28}
+# Algorithmic bytes per particle-step of the workload (a fixed figure of the
+# problem, not of this engine's storage): one read and one write of every
+# per-particle state array the step updates, at the precision the workload
+# specifies, plus one read of the static per-particle data.
+#   patchy, mixed: x split FP64+FP32 36 + p 12 + Q (3x3 FP32) 36 + L 12 = 96 B
+#                  state -> 2*96 + type 4 = 196
+#   patchy, fp64 : x 24 + p 24 + Q 72 + L 24 = 144 B state -> 2*144 + 4 = 292
+#   beads (bundle): x 36 (mixed split) + p 12 = 48 B state -> 2*48 + type 4 +
+#                  bond partners 16 + angle partners 8 = 124
+#   active polymer: bead state 48 B (2*48), its share of end-rotor state
+#                  (2/256 of 96 B), type 4 + chain neighbours 8 = 109
+# The engine stores less (x 24, p 12, Q as a 16 B quaternion, L 12: 64 B per
+# rotor), so its own traffic can sit below these figures.
+ALGO_BYTES = {"mixed": 196, "fp64": 292}
+BEAD_BYTES = {"mixed": 124, "fp64": 124}
+POLYMER_BYTES = {"mixed": 109, "fp64": 109}
 # kernels of one step and the phases of mdx_profile_step that time them
 STEP_PHASES = ("bins", "walk", "patch", "finish")
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_condensate_ab_1M_mixed.json")
@@ -54,12 +63,20 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_condensate_ab_1M_mixed.js
 
 def measured_traffic():
     """HBM bytes per step of each step kernel, from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (copy-probe scaled), or None"""
+    FETCH_SIZE / WRITE_SIZE summary (copy-probe scaled; see
+    mtcode_amd/profiling.py), or None -- also when it was measured on other
+    kernel sources than these (its source stamp differs)"""
     try:
         with open(TRAFFIC_FILE) as f:
-            return json.load(f)
+            tr = json.load(f)
     except (OSError, ValueError):
         return None
+    from mtcode_amd import profiling
+
+    if tr.get("source") != profiling.source_stamp():
+        log("recorded traffic is stale (measured on other kernel sources): not reported")
+        return None
+    return tr
 
 
 def log(*a):
@@ -142,7 +159,24 @@ def time_engine(s, precision, steps, warmup, seed=1, reps=None, autotune=False):
         t1 = time.perf_counter()
         prof = e.profile(min(steps, 20))
         info = e.info()
+        info["resort_ms"] = resort_cost(e, info) if info["skin"] > 0 else None
     return (t1 - t0), prof, info
+
+
+def resort_cost(e, info, reps=3):
+    """extra wall time of a re-sort step (state gathered into cell order,
+    list rebuilt; every RESORT_EVERY list steps) over a list step: two-step
+    calls with and without an invalidated list, best of reps"""
+    def two():
+        t = time.perf_counter()
+        e.step(2)
+        return time.perf_counter() - t
+    plain, resort = [], []
+    for _ in range(reps):
+        plain.append(two())
+        e.set_neighbor_list(info["skin"], info["kmax"], RESORT_EVERY)
+        resort.append(two())
+    return 1e3 * max(0.0, min(resort) - min(plain))
 
 
 def time_plain_fp64(s, steps, warmup, seed=1, reps=None):
@@ -208,18 +242,28 @@ def cpu_baseline(s, nsteps, threads):
     }
 
 
-def copy_probe(nbytes=1 << 28):
-    """a plain device copy of known size, for scaling rocprof byte counters"""
+def copy_probe(nbytes=1 << 28, reps=3):
+    """a plain device copy of known size (for scaling rocprof byte counters);
+    returns the copy's rate in GB/s (bytes read + written over the mean
+    duration, HIP events), or None"""
     try:
         import torch
 
         a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)
-        for _ in range(3):
+        b.copy_(a)
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(reps):
             b.copy_(a)
+        t1.record()
         torch.cuda.synchronize()
+        ms = t0.elapsed_time(t1) / reps
+        del a, b
+        return 2 * nbytes / (ms * 1e-3) / 1e9
     except Exception as exc:  # pragma: no cover - diagnostics only
         print(f"copy probe skipped: {exc}", file=sys.stderr)
+        return None
 
 
 def main():
@@ -241,14 +285,15 @@ def main():
         print("bench.py runs on one GPU only: the engine has no multi-GPU path "
               "(replicas would shard whole per GPU, a large system by z-slabs)", file=sys.stderr)
         sys.exit(2)
-    if args.copy_probe:
-        copy_probe()
+    # a plain device copy: the practical bandwidth beside the 8 TB/s peak
+    copy_gbs = copy_probe(reps=3 if args.copy_probe else 5)
 
     log("building workload", args.workload)
     s, reps = build_workload(args.workload)
     N = sum(r["N"] for r in reps) if reps else s["N"]
     rotor = s["integrator"] == "rotor"
-    algo = ALGO_BYTES if rotor else BEAD_BYTES
+    algo = (ALGO_BYTES if rotor else
+            POLYMER_BYTES if args.workload == "active_polymer" else BEAD_BYTES)
     log("timing mixed precision", args.steps, "steps after", args.warmup, "warm-up")
     t_mixed, prof, info = time_engine(s, mdx.PREC_MIXED, args.steps, args.warmup, reps=reps,
                                       autotune=args.autotune)
@@ -286,6 +331,17 @@ def main():
             "neighbour_list_skin": info["skin"],
             "cells": info["cells"],
             "cell_size": round(info["cell_size"], 4),
+            "resort_every": RESORT_EVERY,
+        },
+        # the state is gathered into cell order every RESORT_EVERY list steps;
+        # a timed window of that many steps or more contains one such step
+        "resort": {
+            "extra_ms": info.get("resort_ms"),
+            "amortised_ms_per_step": (info["resort_ms"] / RESORT_EVERY
+                                      if info.get("resort_ms") is not None else None),
+            # (re-sort steps: step 1 after prime, then every RESORT_EVERY + 1)
+            "in_timed_window": (args.warmup + args.steps - 1) // (RESORT_EVERY + 1)
+                               - (args.warmup - 1) // (RESORT_EVERY + 1),
         },
         "roofline": {
             "bound": "hbm",
@@ -301,6 +357,7 @@ def main():
             "algorithmic_bytes_per_particle_step": algo["mixed"],
             "step_kernel_ms": step_ms,
             "per_kernel": per_kernel,
+            "copy_probe_gbs": copy_gbs,
         },
         "kernel_ms": prof,
     }

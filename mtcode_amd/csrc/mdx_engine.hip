@@ -299,6 +299,7 @@ template <class R> static KArgs<R> make_args(mdx_engine *e)
     a.count = (int *)e->count.p;
     a.start = (const int *)e->start.p;
     a.list = (const int *)e->list.p;
+    a.pcap = PMAXG;
     a.Fout = (double *)e->Fout.p;
     a.Tout = (double *)e->Tout.p;
     a.obs = (double *)e->obs.p;
@@ -466,9 +467,13 @@ static int launch_gather(mdx_engine *e, const double *noise_dev, int lmode = 0)
                                         bead_list);
     if (split) {
         int rc;
+        // list steps of the compacted walk hold every candidate of a position
+        // with at most kmax list entries (its candidates are among them)
+        const bool list_walk = lmode && e->use_walkc;
+        a.pcap = list_walk ? std::max(PMAXG, e->kmax) : PMAXG;
         if ((rc = dalloc(e, e->Fw, 3 * e->rsz() * (size_t)e->N)) ||
             (rc = dalloc(e, e->ncand, 4 * (size_t)e->N)) ||
-            (rc = dalloc(e, e->cand, 4 * (size_t)PMAXG * e->N)))
+            (rc = dalloc(e, e->cand, 4 * (size_t)a.pcap * e->N)))
             return rc;
         if (e->use_compact && (rc = dalloc(e, e->Tp, 3 * e->rsz() * (size_t)e->N)))
             return rc;
@@ -503,10 +508,14 @@ static int launch_gather(mdx_engine *e, const double *noise_dev, int lmode = 0)
             HIPCHK(hipGetLastError());
             return MDX_OK;
         }
-        if (lmode && e->use_walkc)
+        if (lmode && e->use_walkc) {
+            // no overflow kernel after the compacted form's patch kernel: a
+            // position with more candidates than kmax stops the run
+            KArgs<R> w = a;
+            w.ovf_fatal = e->use_compact ? 1 : 0;
             hipLaunchKernelGGL((k_walkc<R, MODE, false>), dim3(nblk(e->N)), dim3(256), 0, e->stream,
-                               a);
-        else
+                               w);
+        } else
             hipLaunchKernelGGL((k_walk<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
         if (e->prof_mid)
             HIPCHK(hipEventRecord(e->prof_mid, e->stream));
@@ -519,8 +528,10 @@ static int launch_gather(mdx_engine *e, const double *noise_dev, int lmode = 0)
         } else if (e->use_compact) {
             hipLaunchKernelGGL((k_patch<R, MODE, false>), dim3(nblk(e->N, pblk)), dim3(256), 0,
                                e->stream, a, pblk);
-            hipLaunchKernelGGL((k_patch_ovf<R, MODE>), dim3(std::min(nblk(e->N), 512u)), dim3(256),
-                               0, e->stream, a);
+            // (list steps hold every candidate: no overflow kernel)
+            if (!list_walk)
+                hipLaunchKernelGGL((k_patch_ovf<R, MODE>), dim3(std::min(nblk(e->N), 512u)),
+                                   dim3(256), 0, e->stream, a);
             if (e->prof_mid2)
                 HIPCHK(hipEventRecord(e->prof_mid2, e->stream));
             hipLaunchKernelGGL((k_integrate<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream,
@@ -578,6 +589,10 @@ static int status_to_rc(unsigned int st, long long step = -1)
         snprintf(where, sizeof where, " in the carried-force evaluation");
     else if (step > 0)
         snprintf(where, sizeof where, " at step %lld of the call (no particle moved after it)", step);
+    if (st & 4u)
+        return set_err(MDX_ERR_CAPACITY,
+                       "a particle has more patch candidates than the neighbour list holds (kmax): "
+                       "raise kmax with mdx_set_neighbor_list%s", where);
     if (st & 2u)
         return set_err(MDX_ERR_NAN, "non-finite force, torque or updated state%s", where);
     if (st & 1u)

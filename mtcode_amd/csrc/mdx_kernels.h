@@ -38,13 +38,13 @@ struct DevRep {
 struct StepCtl {
     unsigned long long epoch;     // force evaluations so far (noise counter)
     unsigned long long call_base; // epoch at the start of the current call
-    unsigned int status;          // bit0 force guard, bit1 non-finite
+    unsigned int status;          // bit0 force guard, bit1 non-finite, bit2 candidate capacity
     unsigned int pad;
     // neighbour list: rebuild flag, displacement bound since the last build
     unsigned int rebuild; // 1: the next step rebuilds the list before its walk
     unsigned int nbuilds; // list builds so far
     double D;             // sum over steps of the max drift since the last build
-    // a walk queued more than PMAXG patch candidates for some position (the
+    // a walk queued more than pcap patch candidates for some position (the
     // overflow kernel's gate; cleared by the integrator kernel after it)
     unsigned int ovf;
     // the first force evaluation (epoch) that raised a status bit; ~0 while
@@ -81,7 +81,12 @@ template <class R> struct KArgs {
     double *Fout, *Tout, *obs;
     // split walk -> finish: WCA force and patch candidates per cell-list position
     R *Fw;
-    int *ncand, *cand; // cand[q * N + s], q < PMAXG
+    int *ncand, *cand; // cand[q * N + s], q < pcap
+    // patch candidates held per position (PMAXG, or kmax in list steps,
+    // where a position's candidates are among its list entries);
+    // ovf_fatal: no overflow kernel follows the walk -- a position with more
+    // candidates than that (its list overflowed too) raises status bit 2
+    int pcap, ovf_fatal;
     R *Tp;             // patch torque per position (compacted patch kernel)
     int nrep; // replicas (1: the replica record is loaded uniformly)
     // neighbour list (lmode): slot s walks nbr[k * N + s], k < nn[s] (<= kmax),
@@ -743,6 +748,19 @@ __device__ __forceinline__ bool own_screen(const KArgs<R> &a, uint32_t oi, int t
     return pass;
 }
 
+// the moved position of slot s into the next buffer (the whole record: a
+// write of x, y, z alone measured no faster)
+template <class R>
+__device__ __forceinline__ void store_position(const KArgs<R> &a, int s, const PosRec &xi,
+                                               const double xn[3])
+{
+    PosRec o = xi;
+    o.x = xn[0];
+    o.y = xn[1];
+    o.z = xn[2];
+    a.Xn[s] = o;
+}
+
 // Force (lab) and patch torque (lab) of one particle, then the mode's
 // integrator epilogue.  Full neighbour list: every pair is evaluated from
 // both sides, each side keeps its own share; no atomics.  The neighbour loop
@@ -811,10 +829,10 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
         for (int k = 0; k < 3; k++)
             acc.F[k] = a.Fw[3 * (size_t)s + k];
         const int npc = a.ncand[s];
-        const int nq = npc < PMAXG ? npc : PMAXG;
+        const int nq = npc < a.pcap ? npc : a.pcap;
         for (int q = 0; q < nq; q++)
             patch_pair<R, MODE>(a, rp, xi, ti, qi, a.cand[(size_t)q * a.N + s], acc);
-        if (npc > PMAXG) {
+        if (npc > a.pcap) {
             int seen = 0;
             trav(i, rp, [&](const PosRec &xj, int j) {
                 double d[3];
@@ -824,7 +842,7 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
                 if (!(r2s < a.rmax2) ||
                     !(MDX_WALK_SCREEN ? is_listed_candidate(xj, js, d, r2s) : is_patch_candidate(xj, r2s)))
                     return;
-                if (seen++ >= PMAXG)
+                if (seen++ >= a.pcap)
                     patch_pair<R, MODE>(a, rp, xi, ti, qi, js, acc);
             });
         }
@@ -856,7 +874,7 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
             if (PH == PH_WALK) {
                 const int js = j >= 0 ? j : a.slot[xj.id];
                 if (MDX_WALK_SCREEN ? is_listed_candidate(xj, js, d, r2s) : is_patch_candidate(xj, r2s)) {
-                    if (npc < PMAXG)
+                    if (npc < a.pcap)
                         a.cand[(size_t)npc * a.N + s] = js;
                     npc++;
                 }
@@ -870,7 +888,7 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
             for (int k = 0; k < 3; k++)
                 a.Fw[3 * (size_t)s + k] = acc.F[k];
             a.ncand[s] = npc;
-            if (npc > PMAXG)
+            if (npc > a.pcap)
                 atomicOr(&a.ctl->ovf, 1u);
             return;
         }
@@ -1058,11 +1076,7 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
         }
         for (int k = 0; k < 3; k++)
             a.pn[3 * (size_t)s + k] = pk[k];
-        PosRec o = xi;
-        o.x = xn[0];
-        o.y = xn[1];
-        o.z = xn[2];
-        a.Xn[s] = o;
+        store_position(a, s, xi, xn);
         if (a.lmode)
             drift = (float)sqrt(dd);
         else
@@ -1175,11 +1189,7 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
         store_orient(a.Qn, (size_t)s, qi);
     else
         copy_orient(a.Qn, (size_t)s, a.Q, (size_t)i);
-    PosRec o = xi;
-    o.x = xn[0];
-    o.y = xn[1];
-    o.z = xn[2];
-    a.Xn[s] = o;
+    store_position(a, s, xi, xn);
     if (a.lmode)
         drift = (float)sqrt(dd);
     else
@@ -1379,7 +1389,7 @@ __global__ void __launch_bounds__(256, (sizeof(R) == 4 && !INT) ? 6 : 1) k_walkc
                 F[k] += pf[3 * (e2 - r0) + k];
             const int jc = pj[e2 - r0];
             if (jc >= 0) {
-                if (npc < PMAXG)
+                if (npc < a.pcap)
                     a.cand[(size_t)npc * a.N + s] = jc;
                 npc++;
             }
@@ -1395,7 +1405,7 @@ __global__ void __launch_bounds__(256, (sizeof(R) == 4 && !INT) ? 6 : 1) k_walkc
             pa.bad = false;
             for_each_in_cells(a, rp, a.bcell[s], s, [&](const PosRec &xj, int j) {
                 if (visit(xi, sq + 9 * tid, sexc[2 * tid], sexc[2 * tid + 1], spl[tid], xj, pa)) {
-                    if (npc < PMAXG)
+                    if (npc < a.pcap)
                         a.cand[(size_t)npc * a.N + s] = j;
                     npc++;
                 }
@@ -1404,8 +1414,12 @@ __global__ void __launch_bounds__(256, (sizeof(R) == 4 && !INT) ? 6 : 1) k_walkc
                 F[k] = pa.F[k];
         }
         a.ncand[s] = npc;
-        if (npc > PMAXG)
-            atomicOr(&a.ctl->ovf, 1u);
+        if (npc > a.pcap) {
+            if (a.ovf_fatal)
+                raise_status(a, 4u);
+            else
+                atomicOr(&a.ctl->ovf, 1u);
+        }
         if (!(INT && npc == 0))
             for (int k = 0; k < 3; k++)
                 a.Fw[3 * (size_t)s + k] = F[k];
@@ -1455,9 +1469,9 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_finish(KAr
         drift_bound(a, drift);
 }
 
-// the patch candidates of slot so beyond the PMAXG handed over by the walk:
+// the patch candidates of slot so beyond the pcap handed over by the walk:
 // the neighbours are walked again in walk order and the terms after the
-// first PMAXG candidates added to Fw / Tp (k_patch has added the others)
+// first pcap candidates added to Fw / Tp (k_patch has added the others)
 template <class R, int MODE>
 __device__ __forceinline__ void patch_overflow(const KArgs<R> &a, int so)
 {
@@ -1494,7 +1508,7 @@ __device__ __forceinline__ void patch_overflow(const KArgs<R> &a, int so)
         if (MDX_WALK_SCREEN && !own_screen(a, xi.id, ti, qs, xj, d, r2s))
             return;
         const int js = j >= 0 ? j : a.slot[xj.id];
-        if (seen++ >= PMAXG)
+        if (seen++ >= a.pcap)
             patch_pair<R, MODE>(a, rp, xi, ti, qi, js, acc);
     });
     for (int k2 = 0; k2 < 3; k2++) {
@@ -1525,13 +1539,13 @@ __global__ void __launch_bounds__(256) k_patch(KArgs<R> a, int pblk)
     __shared__ int wsum[4];
     const int tid = threadIdx.x;
     const int base = xcd_block(blockIdx.x, gridDim.x) * pblk;
-    // exclusive scan of min(ncand, PMAXG) in position order (position
+    // exclusive scan of min(ncand, pcap) in position order (position
     // k * 256 + tid of the block, k < PER)
     int carry = 0;
     for (int k = 0; k < PER; k++) {
         const int p = base + k * 256 + tid;
         const int nc = p < a.N ? a.ncand[p] : 0;
-        const int nq = nc < PMAXG ? nc : PMAXG;
+        const int nq = nc < a.pcap ? nc : a.pcap;
         int v = nq;
         for (int o = 1; o < 64; o <<= 1) {
             const int t = __shfl_up(v, o);
@@ -1613,7 +1627,7 @@ __global__ void __launch_bounds__(256) k_patch(KArgs<R> a, int pblk)
     if (INT)
         for (int k = 0; k < PER; k++) {
             const int so = base + k * 256 + tid;
-            if (so < a.N && a.ncand[so] > PMAXG)
+            if (so < a.N && a.ncand[so] > a.pcap)
                 patch_overflow<R, MODE>(a, so);
         }
     if (INT) {
@@ -1652,7 +1666,7 @@ __global__ void __launch_bounds__(256) k_patch(KArgs<R> a, int pblk)
     }
 }
 
-// the candidates beyond PMAXG of the separate form's positions (after k_patch;
+// the candidates beyond pcap of the separate form's positions (after k_patch;
 // grid-stride over a small grid; almost every step has none, and then the
 // walk left the gate closed)
 template <class R, int MODE> __global__ void __launch_bounds__(256) k_patch_ovf(KArgs<R> a)
@@ -1660,7 +1674,7 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_patch_ovf(
     if (!a.ctl->ovf)
         return;
     for (int so = blockIdx.x * blockDim.x + threadIdx.x; so < a.N; so += gridDim.x * blockDim.x)
-        if (a.ncand[so] > PMAXG)
+        if (a.ncand[so] > a.pcap)
             patch_overflow<R, MODE>(a, so);
 }
 
