@@ -99,6 +99,9 @@ struct mdx_engine {
     // positions per block of the compacted patch kernel (256 .. PBLK, a
     // multiple of 256; 0: chosen from N) (MDX_PBLK)
     int pblk = 0;
+    // LDS staging of the list build: -1 chosen from the particles per x-row,
+    // 0 the smaller, 1 the larger (MDX_NBDENSE)
+    int nb_dense = -1;
     // walk / patch / integrator kernels (1) or one kernel per step (0); by
     // default the one-kernel form up to SPLIT_MIN_N particles, where launches
     // dominate the step (MDX_SPLIT=0/1 forces either)
@@ -669,7 +672,22 @@ template <class R> static int build_list(mdx_engine *e, const int *cells, int ga
 {
     KArgs<R> a = make_args<R>(e);
     a.lmode = 1;
-    hipLaunchKernelGGL(k_nbuild<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, a, cells, gated);
+    // the larger LDS staging where an x-row of cells holds many particles
+    // (about 3 + 256 / row rows of particles staged per z offset of a block)
+    long long rows = 0;
+    for (const DevRep &r : e->reps)
+        rows += (long long)r.nc[1] * r.nc[2];
+    const double per_row = rows > 0 ? (double)e->N / (double)rows : 0.0;
+    const bool dense =
+        e->nb_dense >= 0
+            ? e->nb_dense != 0
+            : per_row > 0 && (std::ceil(256.0 / per_row) + 3.0) * per_row > 0.9 * NB_CAP_S;
+    if (dense)
+        hipLaunchKernelGGL((k_nbuild<R, NB_CAP_D>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a,
+                           cells, gated);
+    else
+        hipLaunchKernelGGL((k_nbuild<R, NB_CAP_S>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a,
+                           cells, gated);
     HIPCHK(hipGetLastError());
     return MDX_OK;
 }
@@ -1041,6 +1059,8 @@ int mdx_create(const mdx_desc *d, mdx_engine **out)
         e->use_tiles = std::atoi(t) != 0;
     if (const char *t = std::getenv("MDX_PBLK"))
         e->pblk = std::min(PBLK, std::max(0, std::atoi(t) / 256 * 256));
+    if (const char *t = std::getenv("MDX_NBDENSE"))
+        e->nb_dense = std::atoi(t) != 0;
     if (const char *t = std::getenv("MDX_SPLIT"))
         e->use_split = std::atoi(t) != 0;
     if (const char *t = std::getenv("MDX_FUSE"))

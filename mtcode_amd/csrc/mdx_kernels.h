@@ -1712,9 +1712,12 @@ template <class R, int MODE> __global__ void __launch_bounds__(256) k_integrate(
 // in order.  A block whose particles span more than NB_W rows or two
 // replicas, a grid with fewer than 3 cells on a side, and a z offset whose
 // rows hold more than NB_CAP particles or NB_ROWCAP cell offsets walk global
-// memory instead.
+// memory instead.  NB_CAP: 800 (38 KB of LDS, 4 blocks per CU) for sparse
+// systems, 1152 (50 KB, 3 blocks) where an x-row of cells holds many
+// particles (the host picks it from the mean particles per row).
 constexpr int NB_W = 16;        // own x-rows of a block walked from LDS
-constexpr int NB_CAP = 800;     // staged particles per z offset
+constexpr int NB_CAP_S = 800;   // staged particles per z offset (sparse)
+constexpr int NB_CAP_D = 1152;  // (dense)
 constexpr int NB_ROWCAP = 1152; // staged cell offsets per z offset
 
 // the list ranges [r0, r1) and [r2, r3) of the cells x - 1, x, x + 1 about
@@ -1746,7 +1749,7 @@ __device__ __forceinline__ void row_ranges(const T *off, int n0, int cx, int &r0
     }
 }
 
-template <class R>
+template <class R, int NB_CAP>
 __global__ void __launch_bounds__(256) k_nbuild(KArgs<R> a, const int *cells, int gated)
 {
     if (gated && !a.ctl->rebuild)
@@ -1874,8 +1877,26 @@ __global__ void __launch_bounds__(256) k_nbuild(KArgs<R> a, const int *cells, in
     };
 
     if (binfo[0] < 0) {
-        if (live)
-            for_each_in_cells(a, rp, c, s, [&](const PosRec &xj, int j) { consider(xj, j); });
+        // the 27 cells from global memory, in the same order
+        if (live) {
+            const int cl = c - rp.coff;
+            const int n0 = rp.nc[0], n1 = rp.nc[1], n2 = rp.nc[2];
+            const int cx = cl % n0, cy = (cl / n0) % n1, cz = cl / (n0 * n1);
+            const int ylo = n1 >= 3 ? -1 : 0, yhi = n1 >= 2 ? 1 : 0;
+            const int zlo = n2 >= 3 ? -1 : 0, zhi = n2 >= 2 ? 1 : 0;
+            for (int oz = zlo; oz <= zhi; oz++) {
+                int z = cz + oz;
+                z = z < 0 ? z + n2 : (z >= n2 ? z - n2 : z);
+                for (int oy = ylo; oy <= yhi; oy++) {
+                    int y = cy + oy;
+                    y = y < 0 ? y + n1 : (y >= n1 ? y - n1 : y);
+                    int r0, r1, r2, r3;
+                    row_ranges(a.start + rp.coff + n0 * (y + n1 * z), n0, cx, r0, r1, r2, r3);
+                    global_range(r0, r1);
+                    global_range(r2, r3);
+                }
+            }
+        }
     } else {
         const DevRep &rb = a.reps[binfo[0]];
         const int n0 = rb.nc[0], n1 = rb.nc[1], n2 = rb.nc[2];

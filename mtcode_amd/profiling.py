@@ -32,7 +32,8 @@ COPY_BYTES = 256 << 20
 # the kernels of one list step of the rotor integrator (mode 3: M_FUSED) and
 # the phase each belongs to
 PHASES = {
-    "bins": ("k_bin<float>", "k_scan1", "k_scan3", "k_fill", "k_cellsort", "k_nbuild<float>"),
+    "bins": ("k_bin<float>", "k_scan1", "k_scan3", "k_fill", "k_cellsort", "k_nbuild<float, 800>",
+             "k_nbuild<float, 1152>"),
     "walk": ("k_walkc<float, 3, false>",),
     "patch": ("k_patch<float, 3, false>", "k_patch_ovf<float, 3>"),
     "finish": ("k_integrate<float, 3>", "k_decide"),

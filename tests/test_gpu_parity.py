@@ -626,7 +626,7 @@ def test_full_size_momentum_conserved_without_thermostat(monkeypatch, name, buil
     kernels, compacted walk, gated list rebuilds; fuse "1" also integrates
     inside the walk) in FP64 with gamma = gammar = 0: the 1M two-species
     condensate (WCA + patches), the 416,000-bead microtubule bundle (WCA +
-    harmonic bonds + harmonic angles, bead integrator) and the 2.1M-bead
+    harmonic bonds + cosine bending, bead integrator) and the 2.1M-bead
     polymer melt with its propulsion switched off (fa = 0: WCA + bonds +
     bending + patchy chain-end rotors).  The noise amplitude
     sqrt(2 gamma kT / dt) is then zero and there is no friction, so each half
