@@ -208,6 +208,36 @@ int mdx_patch_bonds(mdx_engine *e, int *bonds);
 int mdx_clusters(mdx_engine *e, int replica, const int n[3], double cut, int ex_lo, int ex_hi,
                  int *labels, int *nclusters);
 
+/* One-bond-per-patch binding (the reference's run_singlebond drivers,
+ * Condensate.cpp:422-618).  Patches are numbered per particle in original
+ * index order, each particle's patches in its type's order: the numbering of
+ * GeneralPatch::which_patch (GeneralPatch.cpp:292-307) when particles are
+ * sorted by type.  npatches (nullable) receives the number of patches.
+ * BindingModelSingle(on_rate, off_rate) (Bindings/BindingModelSingle.cpp:1-8);
+ * every patch starts unbound with boundto 0 (the BinaryBindStore of
+ * Condensate.cpp:434-441).  Call after mdx_set_patches (which clears it). */
+int mdx_set_binding(mdx_engine *e, double on_rate, double off_rate, int *npatches);
+/* the bound state, per patch: isbound (0/1) and boundto (the partner patch;
+ * kept, like the reference's, when the patch is unbound) --
+ * BinaryBindStore (Bindings/BinaryBindStore.h:7-11) */
+int mdx_set_bindings(mdx_engine *e, const int *isbound, const int *boundto);
+int mdx_get_bindings(mdx_engine *e, int *isbound, int *boundto);
+/* LangevinNVTR::calculate_forces_and_torques3D_onlyone_nonlets
+ * (Rotational/LangevinRforce.cpp:2874-4136) at the current state: the
+ * candidate patch pairs (both cones; a distance bound of 1, 0.7 or 0.5 x the
+ * interaction distance by bound state; the reference's score), bonds to
+ * non-candidates broken, each component of the candidate graph decided (1
+ * patch: unbound; 2, 3: the model's doublet and triplet rules; more: the
+ * highest-scored edge dropped until no part exceeds 3), the bound state
+ * updated on the device, and F, T (N x 3 each, nullable, original order)
+ * set to the Kern-Frenkel force and torques of the bound pairs alone (the
+ * reference adds them to its F, T).  The random numbers are Philox4x32-10
+ * keyed by the engine seed, at {smallest patch of the component, draw lo,
+ * draw hi, k} (the reference draws rand() per component in the same order
+ * of smallest patches).  MDX_ERR_CAPACITY when a patch has more than 8
+ * candidates or a component more than 64 patches. */
+int mdx_bind_forces(mdx_engine *e, unsigned long long draw, double *F, double *T);
+
 /* Time stepping. */
 /* Rotor integrator: computes the carried F, T at the current state
  * (Condensate.cpp:318-331).  noise0 (N x 6, nullable: Philox) */

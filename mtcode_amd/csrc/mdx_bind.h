@@ -1,0 +1,552 @@
+// One-bond-per-patch binding: the update of the bound state and the forces
+// of the bound pairs, LangevinNVTR::calculate_forces_and_torques3D_onlyone_nonlets
+// (Rotational/LangevinRforce.cpp:2874-4136) with BindingModelSingle
+// (Bindings/BindingModelSingle.cpp:1-119).
+//
+// Patches are numbered per particle in original-index order (poff[i] is the
+// number of the first patch of particle i, its patches follow in type order),
+// the numbering of GeneralPatch::which_patch (GeneralPatch.cpp:292-307) for
+// particles sorted by type.  Three kernels per call:
+//   k_bind_edges   one thread per particle: its candidate patch pairs within
+//                  the largest interaction distance (both cones, a distance
+//                  bound of 1, 0.7 or 0.5 x that distance by bound state, a
+//                  score), each appended to the row of the thread's own
+//                  patch -- no atomics, both particles of a pair compute the
+//                  same FP64 values -- and each row sorted by partner;
+//   k_bind_decide  one thread per patch: a search of its component of the
+//                  candidate graph that stops at the first smaller patch, so
+//                  only the component's smallest patch goes on; it breaks
+//                  its members' bonds to non-candidates and decides the
+//                  component (1 patch: unbound; 2 and 3: the model's doublet
+//                  and triplet rules; more: the highest-scored edge removed
+//                  until no part exceeds 3, each part decided in order of its
+//                  smallest patch) with Philox draws keyed by the smallest
+//                  patch, the call's draw counter and the decision's index;
+//   k_bind_forces  one thread per particle: the Kern-Frenkel force and
+//                  torques of its bound patches on itself.
+// Every cut-off, cone and bound-state decision is taken in FP64 on the
+// reference's own expressions.
+#pragma once
+
+#include "mdx_kernels.h"
+
+namespace mdx {
+
+constexpr int BIND_DEG = 8;  // candidate partners kept per patch
+constexpr int BIND_CMAX = 64; // patches per component
+
+struct BindArgs {
+    const int *poff;  // first patch of each original index (N + 1)
+    const int *owner; // original index of each patch
+    int *isbound, *boundto;
+    int *adj_n;       // candidates per patch
+    int *adj_p;       // partner patches, BIND_DEG per patch, ascending
+    double *adj_s;    // their scores
+    int nh;
+    double mk2;       // squared largest interaction distance (max_check)
+    double on_rate, off_rate;
+    uint32_t key0, key1;
+    unsigned long long draw;
+    StepCtl *ctl;
+};
+
+__device__ __forceinline__ void bind_status(const BindArgs &b, unsigned bit)
+{
+    atomicOr(&b.ctl->status, bit);
+}
+
+// 53-bit uniform in [0, 1) of Philox4x32-10 at {root, draw lo, draw hi, k}
+__device__ __forceinline__ double bind_uniform(const BindArgs &b, int root, int k)
+{
+    const U4 o = philox4x32_10(
+        U4{(uint32_t)root, (uint32_t)b.draw, (uint32_t)(b.draw >> 32), (uint32_t)k}, b.key0, b.key1);
+    return ((double)(o.a >> 5) * 67108864.0 + (double)(o.b >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+// candidate patch pairs (LangevinRforce.cpp:2914-3075)
+template <class R> __global__ void __launch_bounds__(256) k_bind_edges(KArgs<R> a, BindArgs b)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.N)
+        return;
+    const int i = a.list[s];
+    const PosRec xi = a.X[i];
+    const int ti = xi.meta & 255;
+    const int npi = a.np[ti];
+    if (npi == 0)
+        return;
+    const int base = b.poff[xi.id];
+    int deg[MAXP];
+    for (int k = 0; k < npi; k++)
+        deg[k] = 0;
+    const DevRep rp = rep_rec(a, xi.meta);
+    R qi[9];
+    load_orient(a.Q, (size_t)i, qi);
+    bool over = false;
+    for_each_neighbor(a, rp, i, [&](const PosRec &xj, int j) {
+        double d[3];
+        min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
+        const double r2s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (!(r2s < b.mk2))
+            return;
+        const int tj = xj.meta & 255;
+        if (a.np[tj] == 0)
+            return;
+        int ta, tc;
+        const bool i_is_a = patch_roles(xi.id, ti, xj.id, tj, ta, tc);
+        const double dis = sqrt(r2s);
+        const double sg = i_is_a ? 1.0 : -1.0;
+        const double u[3] = {sg * d[0] / dis, sg * d[1] / dis, sg * d[2] / dis};
+        R qj[9];
+        load_orient(a.Q, (size_t)j, qj);
+        const R *qa = i_is_a ? qi : qj;
+        const R *qc = i_is_a ? qj : qi;
+        const int ida = i_is_a ? xi.id : xj.id, idc = i_is_a ? xj.id : xi.id;
+        const int pa = b.poff[ida], pc = b.poff[idc];
+        const int npa = a.np[ta], npc = a.np[tc];
+        const double *da = a.pdirs + 3 * a.pstart[ta];
+        const double *dc = a.pdirs + 3 * a.pstart[tc];
+        const int rowbase = rp.prow + a.rstart[ta * MAXT + tc];
+        for (int ia = 0; ia < npa; ia++) {
+            double n1[3];
+            lab_normal_exact(qa, da + 3 * ia, n1);
+            const double argthetai = -(n1[0] * u[0] + n1[1] * u[1] + n1[2] * u[2]);
+            for (int ic = 0; ic < npc; ic++) {
+                const PatchRow pr = a.prows[rowbase + ia * npc + ic];
+                double n2[3];
+                lab_normal_exact(qc, dc + 3 * ic, n2);
+                const double argthetaj = (n2[0] * u[0] + n2[1] * u[1] + n2[2] * u[2]);
+                const int wp1 = pa + ia, wp2 = pc + ic;
+                const bool cond1 = b.boundto[wp1] == wp2 && b.boundto[wp2] == wp1;
+                const bool b1 = b.isbound[wp1] != 0, b2 = b.isbound[wp2] != 0;
+                const double disp = pr.idist;
+                double disp2;
+                if (b1 && b2 && cond1)
+                    disp2 = disp;
+                else if (b1 && b2 && !cond1)
+                    disp2 = 0.5 * disp;
+                else if (!b1 != !b2)
+                    disp2 = 0.7 * disp;
+                else
+                    disp2 = disp;
+                if (argthetai > pr.costm && argthetaj > pr.costm && dis < disp2) {
+                    const double scr = (1 - (argthetai - pr.costm)) + (1 - (argthetaj - pr.costm)) +
+                                       2 * (dis / disp2);
+                    const int own = i_is_a ? ia : ic;
+                    const int partner = i_is_a ? wp2 : wp1;
+                    if (deg[own] < BIND_DEG) {
+                        const size_t at = (size_t)(base + own) * BIND_DEG + deg[own];
+                        b.adj_p[at] = partner;
+                        b.adj_s[at] = scr;
+                        deg[own]++;
+                    } else {
+                        over = true;
+                    }
+                }
+            }
+        }
+    });
+    if (over)
+        bind_status(b, 4u);
+    for (int k = 0; k < npi; k++) {
+        const size_t r0 = (size_t)(base + k) * BIND_DEG;
+        for (int q = 1; q < deg[k]; q++) // insertion sort by partner
+            for (int t = q; t > 0 && b.adj_p[r0 + t - 1] > b.adj_p[r0 + t]; t--) {
+                const int tp = b.adj_p[r0 + t];
+                b.adj_p[r0 + t] = b.adj_p[r0 + t - 1];
+                b.adj_p[r0 + t - 1] = tp;
+                const double ts = b.adj_s[r0 + t];
+                b.adj_s[r0 + t] = b.adj_s[r0 + t - 1];
+                b.adj_s[r0 + t - 1] = ts;
+            }
+        b.adj_n[base + k] = deg[k];
+    }
+}
+
+__device__ __forceinline__ void bind_pair(const BindArgs &b, int i1, int i2)
+{
+    b.boundto[i1] = i2;
+    b.boundto[i2] = i1;
+    b.isbound[i1] = 1;
+    b.isbound[i2] = 1;
+}
+
+// BindingModelSingle::doublet (BindingModelSingle.cpp:10-25) on i1 < i2
+// (LangevinRforce.cpp:3262-3302)
+__device__ __forceinline__ void bind_decide2(const BindArgs &b, int i1, int i2, double ran)
+{
+    const double totalr = b.off_rate + b.on_rate;
+    const double rr = totalr * ran;
+    if (!(rr < b.off_rate)) {
+        bind_pair(b, i1, i2);
+    } else {
+        b.isbound[i1] = 0;
+        b.isbound[i2] = 0;
+    }
+}
+
+// BindingModelSingle::triplet (BindingModelSingle.cpp:40-119) on
+// i1 < i2 < i3 with connectivity c (LangevinRforce.cpp:3303-3497)
+__device__ __forceinline__ void bind_decide3(const BindArgs &b, int i1, int i2, int i3, bool c12,
+                                             bool c23, bool c13, double ran)
+{
+    const int *bt = b.boundto, *ib = b.isbound;
+    const bool b12 = bt[i1] == i2 && bt[i2] == i1 && ib[i1] && ib[i2];
+    const bool b23 = bt[i2] == i3 && bt[i3] == i2 && ib[i2] && ib[i3];
+    const bool b13 = bt[i1] == i3 && bt[i3] == i1 && ib[i1] && ib[i3];
+    const double on = b.on_rate, off = b.off_rate;
+    double r1, r2, r3, r4;
+    if (b12) {
+        r1 = c12 * on;
+        r2 = c23 * ((off / 3.) + 0.001);
+        r3 = c13 * ((off / 3.) + 0.001);
+        r4 = (off / 3.) + 0.001;
+    } else if (b23) {
+        r1 = c12 * ((off / 3.) + 0.001);
+        r2 = c23 * (on);
+        r3 = c13 * ((off / 3.) + 0.001);
+        r4 = ((off / 3.) + 0.001);
+    } else if (b13) {
+        r1 = c12 * ((off / 3.) + 0.001);
+        r2 = c23 * ((off / 3.) + 0.001);
+        r3 = c13 * on;
+        r4 = (off / 3.) + 0.001;
+    } else {
+        r1 = c12 * on / 3.;
+        r2 = c23 * on / 3.;
+        r3 = c13 * on / 3.;
+        r4 = off;
+    }
+    const double totalr = r1 + r2 + r3 + r4;
+    const double rr = totalr * ran;
+    if (rr < r1) {
+        bind_pair(b, i1, i2);
+        b.isbound[i3] = 0;
+    } else if (rr >= r1 && rr < r1 + r2) {
+        bind_pair(b, i2, i3);
+        b.isbound[i1] = 0;
+    } else if (rr >= r1 + r2 && rr < r1 + r2 + r3) {
+        bind_pair(b, i1, i3);
+        b.isbound[i2] = 0;
+    } else {
+        b.isbound[i1] = 0;
+        b.isbound[i2] = 0;
+        b.isbound[i3] = 0;
+    }
+}
+
+// connectivity of a three-patch component from the first one's degree and
+// first neighbour and the second one's degree; false if malformed
+__device__ __forceinline__ bool bind_conn3(int nb1, int first1, int nb2, int i2, int i3, bool &c12,
+                                           bool &c23, bool &c13)
+{
+    if (nb1 == 1) {
+        if (first1 == i2) {
+            c12 = true;
+            c13 = false;
+            c23 = true;
+        } else if (first1 == i3) {
+            c13 = true;
+            c12 = false;
+            c23 = true;
+        } else {
+            return false;
+        }
+    } else if (nb1 == 2) {
+        c12 = true;
+        c13 = true;
+        c23 = nb2 != 1;
+    } else {
+        return false;
+    }
+    return true;
+}
+
+__device__ __forceinline__ int uf_find(unsigned char *par, int x)
+{
+    while (par[x] != x)
+        x = par[x] = par[par[x]];
+    return x;
+}
+
+// a component of more than three patches (LangevinRforce.cpp:3531-3870):
+// mem (ascending, s of them) are its patches, root the smallest
+__device__ void bind_large(const BindArgs &b, const int *mem, int s, int root)
+{
+    constexpr int EMAX = BIND_CMAX * BIND_DEG / 2;
+    unsigned char ea[EMAX], eb[EMAX];
+    double es[EMAX];
+    int ne = 0;
+    // the component's edges in local indices (RecapitulateEdges,
+    // LangevinRforceUtil.cpp:508-575): members in order, each one's partners
+    // in order, each unordered edge once
+    for (int j = 0; j < s; j++) {
+        const int g = mem[j];
+        const int dg = b.adj_n[g];
+        for (int k = 0; k < dg; k++) {
+            const int pg = b.adj_p[(size_t)g * BIND_DEG + k];
+            int lo = 0, hi = s - 1, f2 = -1;
+            while (lo <= hi) {
+                const int mid = (lo + hi) >> 1;
+                if (mem[mid] == pg) {
+                    f2 = mid;
+                    break;
+                }
+                if (mem[mid] < pg)
+                    lo = mid + 1;
+                else
+                    hi = mid - 1;
+            }
+            if (f2 < 0) // (every partner of a member is a member)
+                continue;
+            const int x = j < f2 ? j : f2, y = j < f2 ? f2 : j;
+            bool already = false;
+            for (int q = 0; q < ne; q++)
+                if (ea[q] == x && eb[q] == y) {
+                    already = true;
+                    break;
+                }
+            if (!already && ne < EMAX) {
+                ea[ne] = (unsigned char)x;
+                eb[ne] = (unsigned char)y;
+                es[ne] = b.adj_s[(size_t)g * BIND_DEG + k];
+                ne++;
+            }
+        }
+    }
+    unsigned char par[BIND_CMAX];
+    for (;;) {
+        // drop the first highest-scored edge (std::max_element), once before
+        // the first test and then while a part exceeds 3 patches
+        int best = 0;
+        for (int q = 1; q < ne; q++)
+            if (es[best] < es[q])
+                best = q;
+        for (int q = best; q + 1 < ne; q++) {
+            ea[q] = ea[q + 1];
+            eb[q] = eb[q + 1];
+            es[q] = es[q + 1];
+        }
+        ne--;
+        for (int j = 0; j < s; j++)
+            par[j] = (unsigned char)j;
+        for (int q = 0; q < ne; q++) {
+            const int r1 = uf_find(par, ea[q]), r2 = uf_find(par, eb[q]);
+            if (r1 < r2)
+                par[r2] = (unsigned char)r1;
+            else if (r2 < r1)
+                par[r1] = (unsigned char)r2;
+        }
+        unsigned char cnt[BIND_CMAX];
+        for (int j = 0; j < s; j++)
+            cnt[j] = 0;
+        int big = 0;
+        for (int j = 0; j < s; j++) {
+            const int r = uf_find(par, j);
+            cnt[r]++;
+            big = cnt[r] > big ? cnt[r] : big;
+        }
+        if (big <= 3)
+            break;
+    }
+    // local degrees and first neighbours of what is left (PairHistogram)
+    unsigned char lens[BIND_CMAX], first_nb[BIND_CMAX];
+    for (int j = 0; j < s; j++)
+        lens[j] = 0;
+    for (int q = 0; q < ne; q++) {
+        if (lens[ea[q]]++ == 0)
+            first_nb[ea[q]] = eb[q];
+        if (lens[eb[q]]++ == 0)
+            first_nb[eb[q]] = ea[q];
+    }
+    // bonds to patches that are no longer neighbours break
+    for (int k1 = 0; k1 < s; k1++) {
+        const int g = mem[k1];
+        if (b.isbound[g]) {
+            bool outside = true;
+            for (int q = 0; q < ne; q++)
+                if ((ea[q] == k1 && mem[eb[q]] == b.boundto[g]) ||
+                    (eb[q] == k1 && mem[ea[q]] == b.boundto[g])) {
+                    outside = false;
+                    break;
+                }
+            if (outside)
+                b.isbound[g] = 0;
+        }
+    }
+    // the parts in order of their smallest local index
+    int it = 0;
+    for (int j = 0; j < s; j++) {
+        if (uf_find(par, j) != j)
+            continue;
+        int m[3], z = 0;
+        for (int q = j; q < s && z < 3; q++)
+            if (uf_find(par, q) == j)
+                m[z++] = q;
+        if (z == 1) {
+            b.isbound[mem[m[0]]] = 0;
+        } else if (z == 2) {
+            bind_decide2(b, mem[m[0]], mem[m[1]], bind_uniform(b, root, 1 + it));
+            it++;
+        } else {
+            bool c12 = false, c23 = false, c13 = false;
+            const int ii1 = m[0], ii2 = m[1], ii3 = m[2];
+            if (!bind_conn3(lens[ii1], lens[ii1] > 0 ? first_nb[ii1] : -1, lens[ii2], ii2, ii3,
+                            c12, c23, c13)) {
+                bind_status(b, 4u);
+                return;
+            }
+            bind_decide3(b, mem[ii1], mem[ii2], mem[ii3], c12, c23, c13,
+                         bind_uniform(b, root, 1 + it));
+            it++;
+        }
+    }
+}
+
+// bound-state decisions per component (LangevinRforce.cpp:3098-3870)
+__global__ void __launch_bounds__(256) k_bind_decide(BindArgs b)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= b.nh)
+        return;
+    if (b.adj_n[p] == 0) { // a component of one patch
+        b.isbound[p] = 0;
+        return;
+    }
+    int mem[BIND_CMAX];
+    int cnt = 1, head = 0;
+    mem[0] = p;
+    while (head < cnt) {
+        const int u = mem[head++];
+        const int dg = b.adj_n[u];
+        for (int k = 0; k < dg; k++) {
+            const int v = b.adj_p[(size_t)u * BIND_DEG + k];
+            if (v < p)
+                return; // not the component's smallest patch
+            bool seen = false;
+            for (int q = 0; q < cnt; q++)
+                if (mem[q] == v) {
+                    seen = true;
+                    break;
+                }
+            if (seen)
+                continue;
+            if (cnt == BIND_CMAX) {
+                bind_status(b, 4u);
+                return;
+            }
+            mem[cnt++] = v;
+        }
+    }
+    for (int q = 1; q < cnt; q++) // ascending
+        for (int t = q; t > 0 && mem[t - 1] > mem[t]; t--) {
+            const int tp = mem[t];
+            mem[t] = mem[t - 1];
+            mem[t - 1] = tp;
+        }
+    // bonds whose partner is not a candidate break (LangevinRforce.cpp:3098-3125)
+    for (int q = 0; q < cnt; q++) {
+        const int g = mem[q];
+        if (b.isbound[g]) {
+            bool outside = true;
+            const int dg = b.adj_n[g];
+            for (int k = 0; k < dg; k++)
+                if (b.adj_p[(size_t)g * BIND_DEG + k] == b.boundto[g]) {
+                    outside = false;
+                    break;
+                }
+            if (outside)
+                b.isbound[g] = 0;
+        }
+    }
+    if (cnt == 2) {
+        bind_decide2(b, mem[0], mem[1], bind_uniform(b, p, 0));
+    } else if (cnt == 3) {
+        bool c12 = false, c23 = false, c13 = false;
+        const int i1 = mem[0], i2 = mem[1], i3 = mem[2];
+        if (!bind_conn3(b.adj_n[i1], b.adj_p[(size_t)i1 * BIND_DEG], b.adj_n[i2], i2, i3, c12, c23,
+                        c13)) {
+            bind_status(b, 4u);
+            return;
+        }
+        bind_decide3(b, i1, i2, i3, c12, c23, c13, bind_uniform(b, p, 0));
+    } else {
+        bind_large(b, mem, cnt, p);
+    }
+}
+
+// forces and torques of the bound pairs (LangevinRforce.cpp:4030-4095,
+// KernFrenkelOnePatch2::force_and_torque, potentialtheta.h:517-750), each
+// particle its own share, written to Fb, Tb (3 per original index, lab frame)
+template <class R>
+__global__ void __launch_bounds__(256) k_bind_forces(KArgs<R> a, BindArgs b, double *Fb, double *Tb)
+{
+    constexpr bool FP64 = sizeof(R) == 8;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.N)
+        return;
+    const PosRec xi = a.X[i];
+    const int ti = xi.meta & 255;
+    const int npi = a.np[ti];
+    double F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
+    if (npi > 0) {
+        const DevRep rp = rep_rec(a, xi.meta);
+        const int base = b.poff[xi.id];
+        R qi[9];
+        load_orient(a.Q, (size_t)i, qi);
+        for (int k = 0; k < npi; k++) {
+            if (!b.isbound[base + k])
+                continue;
+            const int wq = b.boundto[base + k];
+            const int idj = b.owner[wq];
+            const int j = a.slot[idj];
+            const PosRec xj = a.X[j];
+            const int tj = xj.meta & 255;
+            double d[3];
+            min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
+            const double rij = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            int ta, tc;
+            const bool i_is_a = patch_roles(xi.id, ti, xj.id, tj, ta, tc);
+            const double sg = i_is_a ? 1.0 : -1.0;
+            const double u[3] = {sg * d[0] / rij, sg * d[1] / rij, sg * d[2] / rij};
+            R qj[9];
+            load_orient(a.Q, (size_t)j, qj);
+            const R *qa = i_is_a ? qi : qj;
+            const R *qc = i_is_a ? qj : qi;
+            const int ia = i_is_a ? k : wq - b.poff[idj];
+            const int ic = i_is_a ? wq - b.poff[idj] : k;
+            const int npc = a.np[tc];
+            const PatchRow pr = a.prows[rp.prow + a.rstart[ta * MAXT + tc] + ia * npc + ic];
+            if (rij > pr.idist)
+                continue;
+            double n1[3], n2[3];
+            lab_normal_exact(qa, a.pdirs + 3 * (a.pstart[ta] + ia), n1);
+            lab_normal_exact(qc, a.pdirs + 3 * (a.pstart[tc] + ic), n2);
+            const double a1 = -(n1[0] * u[0] + n1[1] * u[1] + n1[2] * u[2]);
+            const double a2 = (n2[0] * u[0] + n2[1] * u[1] + n2[2] * u[2]);
+            if (!(a1 > pr.costm && a2 > pr.costm))
+                continue;
+            R f[3] = {0, 0, 0}, t1[3] = {0, 0, 0}, t2[3] = {0, 0, 0};
+            if constexpr (FP64) {
+                kf2_pair_fp64(u, rij, n1, n2, a1, a2, pr, f, t1, t2);
+            } else {
+                const float uf[3] = {(float)u[0], (float)u[1], (float)u[2]};
+                const float n1f[3] = {(float)n1[0], (float)n1[1], (float)n1[2]};
+                const float n2f[3] = {(float)n2[0], (float)n2[1], (float)n2[2]};
+                kf2_pair_fp32(uf, (float)rij, n1f, n2f, (float)a1, (float)a2,
+                              fabs(a1 * a1 - 1) < 1E-5, fabs(a2 * a2 - 1) < 1E-5, (float)pr.eps,
+                              (float)pr.d, (float)pr.kk, f, t1, t2);
+            }
+            for (int q = 0; q < 3; q++) {
+                F[q] += i_is_a ? (double)f[q] : -(double)f[q];
+                T[q] += i_is_a ? (double)t1[q] : (double)t2[q];
+            }
+        }
+    }
+    for (int q = 0; q < 3; q++) {
+        Fb[3 * (size_t)xi.id + q] = F[q];
+        Tb[3 * (size_t)xi.id + q] = T[q];
+    }
+}
+
+} // namespace mdx

@@ -11,6 +11,7 @@
 //
 // Reference paths in comments are relative to the reference's Code/ folder.
 #include "mdx_kernels.h"
+#include "mdx_bind.h"
 #include "../../include/mdx.h"
 
 #include <hip/hip_runtime.h>
@@ -152,6 +153,12 @@ struct mdx_engine {
     hipEvent_t prof_mid = nullptr;  // recorded between walk and finish while profiling
     hipEvent_t prof_mid2 = nullptr; // recorded between patch and integrate while profiling
     bool last_split = false;       // the last gather launch used the split form
+    // one-bond-per-patch binding (mdx_set_binding): patch numbering, bound
+    // state, candidate rows, the bound-pair forces by original index
+    int bind_on = 0;
+    double bind_on_rate = 0, bind_off_rate = 0, bind_mk = 0;
+    int nh = 0;
+    Buf bpoff, bowner, bisbound, bboundto, badj_n, badj_p, badj_s, bF, bT;
 
     size_t rsz() const { return prec == MDX_PREC_FP64 ? 8 : 4; }
 };
@@ -218,6 +225,10 @@ static double reach(const mdx_engine *e)
                 dmax = std::max(dmax, 1.4 * e->prow_raw[3 * k + 1]);
         r = std::max(r, std::min(std::sqrt(e->maxd2), dmax));
     }
+    // binding candidates lie within the largest interaction distance of any
+    // row (max_check), whatever max_distance_check
+    if (e->bind_on)
+        r = std::max(r, e->bind_mk);
     return r;
 }
 
@@ -1139,7 +1150,8 @@ void mdx_destroy(mdx_engine *e)
                    &e->act_next, &e->noise,  &e->ctl,    &e->partial, &e->tmpd,   &e->tmpi,
                    &e->tmpi2,   &e->tmpi3,   &e->tmpll,  &e->tmpx,    &e->Fw,
                    &e->ncand,   &e->cand,    &e->nbr,    &e->nbrc,   &e->nn,     &e->bcell,
-                   &e->dslot,   &e->Tp};
+                   &e->dslot,   &e->Tp,      &e->bpoff,  &e->bowner, &e->bisbound, &e->bboundto,
+                   &e->badj_n,  &e->badj_p,  &e->badj_s, &e->bF,     &e->bT};
     for (Buf *b : bufs)
         dfree(*b);
     for (GraphSlot &g : e->graphs)
@@ -1391,6 +1403,7 @@ int mdx_set_patches(mdx_engine *e, int ntypes, const int *type_of_particle,
         if (!(params[3 * k + 1] > 0) || !(params[3 * k + 2] > 0))
             return set_err(MDX_ERR_ARG, "patch row %d: d and theta_m must be > 0", k);
     e->ntypes = ntypes;
+    e->bind_on = 0; // the patch numbering changes: mdx_set_binding again
     e->np.assign(npatch_per_type, npatch_per_type + ntypes);
     e->pstart.assign(ntypes, 0);
     for (int t = 1; t < ntypes; t++)
@@ -1909,6 +1922,170 @@ int mdx_compute_forces(mdx_engine *e, double *F, double *T)
     HIPCHK(hipSetDevice(e->device));
     return e->prec == MDX_PREC_FP64 ? compute_forces_t<double>(e, F, T)
                                     : compute_forces_t<float>(e, F, T);
+}
+
+// ------------------------------------------------------------ binding
+
+int mdx_set_binding(mdx_engine *e, double on_rate, double off_rate, int *npatches)
+{
+    if (!e)
+        return set_err(MDX_ERR_ARG, "null engine");
+    if (e->integ != MDX_INTEG_ROTOR || e->ntypes == 0)
+        return set_err(MDX_ERR_STATE, "binding needs the rotor integrator and patches");
+    if (!(on_rate >= 0) || !(off_rate >= 0) || !(on_rate + off_rate > 0))
+        return set_err(MDX_ERR_ARG, "rates must be >= 0 with a positive sum");
+    HIPCHK(hipSetDevice(e->device));
+    std::vector<int> poff(e->N + 1, 0);
+    for (int i = 0; i < e->N; i++)
+        poff[i + 1] = poff[i] + e->np[e->meta[i] & 255];
+    const int nh = poff[e->N];
+    if (nh == 0)
+        return set_err(MDX_ERR_STATE, "no particle carries patches");
+    std::vector<int> owner(nh);
+    for (int i = 0; i < e->N; i++)
+        for (int k = poff[i]; k < poff[i + 1]; k++)
+            owner[k] = i;
+    // ComboPatch::max_check (combopatch.h:15, GeneralPatch.cpp:93-94): 1.0
+    // raised to the largest 1.4 d of any row (potentialtheta.h:417)
+    double mk = 1.0;
+    for (size_t k = 0; k < e->prow_raw.size() / 3; k++)
+        mk = std::max(mk, 1.4 * e->prow_raw[3 * k + 1]);
+    int rc;
+    if ((rc = upload(e, e->bpoff, poff.data(), poff.size())) ||
+        (rc = upload(e, e->bowner, owner.data(), owner.size())) ||
+        (rc = dalloc(e, e->bisbound, 4 * (size_t)nh)) ||
+        (rc = dalloc(e, e->bboundto, 4 * (size_t)nh)) ||
+        (rc = dalloc(e, e->badj_n, 4 * (size_t)nh)) ||
+        (rc = dalloc(e, e->badj_p, 4 * (size_t)nh * BIND_DEG)) ||
+        (rc = dalloc(e, e->badj_s, 8 * (size_t)nh * BIND_DEG)) ||
+        (rc = dalloc(e, e->bF, 24 * (size_t)e->N)) || (rc = dalloc(e, e->bT, 24 * (size_t)e->N)))
+        return rc;
+    HIPCHK(hipMemsetAsync(e->bisbound.p, 0, 4 * (size_t)nh, e->stream));
+    HIPCHK(hipMemsetAsync(e->bboundto.p, 0, 4 * (size_t)nh, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (!e->bind_on || e->bind_mk != mk) {
+        e->grid_dirty = true;
+        e->binned = false;
+        e->list_valid = false;
+    }
+    e->bind_on = 1;
+    e->bind_on_rate = on_rate;
+    e->bind_off_rate = off_rate;
+    e->bind_mk = mk;
+    e->nh = nh;
+    if (npatches)
+        *npatches = nh;
+    return MDX_OK;
+}
+
+int mdx_set_bindings(mdx_engine *e, const int *isbound, const int *boundto)
+{
+    if (!e || !isbound || !boundto)
+        return set_err(MDX_ERR_ARG, "bad arguments");
+    if (!e->bind_on)
+        return set_err(MDX_ERR_STATE, "call mdx_set_binding first");
+    for (int k = 0; k < e->nh; k++)
+        if (boundto[k] < 0 || boundto[k] >= e->nh)
+            return set_err(MDX_ERR_ARG, "boundto[%d] = %d outside 0..%d", k, boundto[k], e->nh - 1);
+    HIPCHK(hipSetDevice(e->device));
+    std::vector<int> ib(isbound, isbound + e->nh);
+    for (int &v : ib)
+        v = v != 0;
+    int rc;
+    if ((rc = upload(e, e->bisbound, ib.data(), ib.size())) ||
+        (rc = upload(e, e->bboundto, boundto, (size_t)e->nh)))
+        return rc;
+    return MDX_OK;
+}
+
+int mdx_get_bindings(mdx_engine *e, int *isbound, int *boundto)
+{
+    if (!e || !isbound || !boundto)
+        return set_err(MDX_ERR_ARG, "bad arguments");
+    if (!e->bind_on)
+        return set_err(MDX_ERR_STATE, "call mdx_set_binding first");
+    HIPCHK(hipSetDevice(e->device));
+    int rc;
+    if ((rc = download(e, e->bisbound, isbound, (size_t)e->nh)) ||
+        (rc = download(e, e->bboundto, boundto, (size_t)e->nh)))
+        return rc;
+    return MDX_OK;
+}
+
+template <class R> static BindArgs bind_args(mdx_engine *e, unsigned long long draw)
+{
+    BindArgs b;
+    b.poff = (const int *)e->bpoff.p;
+    b.owner = (const int *)e->bowner.p;
+    b.isbound = (int *)e->bisbound.p;
+    b.boundto = (int *)e->bboundto.p;
+    b.adj_n = (int *)e->badj_n.p;
+    b.adj_p = (int *)e->badj_p.p;
+    b.adj_s = (double *)e->badj_s.p;
+    b.nh = e->nh;
+    b.mk2 = e->bind_mk * e->bind_mk;
+    b.on_rate = e->bind_on_rate;
+    b.off_rate = e->bind_off_rate;
+    // the noise key with its second word changed: binding draws and the
+    // Langevin noise are different streams
+    b.key0 = (uint32_t)e->seed;
+    b.key1 = (uint32_t)(e->seed >> 32) ^ 0xB1D0B1D0u;
+    b.draw = draw;
+    b.ctl = (StepCtl *)e->ctl.p;
+    return b;
+}
+
+// the three binding kernels on the current state (cell list built)
+template <class R> static int launch_bind(mdx_engine *e, unsigned long long draw)
+{
+    KArgs<R> a = make_args<R>(e);
+    BindArgs b = bind_args<R>(e, draw);
+    hipLaunchKernelGGL(k_bind_edges<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, a, b);
+    hipLaunchKernelGGL(k_bind_decide, dim3(nblk(e->nh)), dim3(256), 0, e->stream, b);
+    hipLaunchKernelGGL(k_bind_forces<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, a, b,
+                       (double *)e->bF.p, (double *)e->bT.p);
+    HIPCHK(hipGetLastError());
+    return MDX_OK;
+}
+
+static int bind_status_rc(unsigned int st)
+{
+    if (st & 4u)
+        return set_err(MDX_ERR_CAPACITY,
+                       "a patch has more than %d binding candidates or a candidate component more "
+                       "than %d patches", BIND_DEG, BIND_CMAX);
+    return status_to_rc(st);
+}
+
+template <class R> static int bind_forces_t(mdx_engine *e, unsigned long long draw, double *F,
+                                            double *T)
+{
+    int rc;
+    if (!e->binned && (rc = bin_current<R>(e)))
+        return rc;
+    if ((rc = launch_bind<R>(e, draw)))
+        return rc;
+    unsigned int st;
+    if ((rc = read_status(e, &st)))
+        return rc;
+    if ((rc = bind_status_rc(st)))
+        return rc;
+    if (F && (rc = download(e, e->bF, F, 3 * (size_t)e->N)))
+        return rc;
+    if (T && (rc = download(e, e->bT, T, 3 * (size_t)e->N)))
+        return rc;
+    return MDX_OK;
+}
+
+int mdx_bind_forces(mdx_engine *e, unsigned long long draw, double *F, double *T)
+{
+    if (!e)
+        return set_err(MDX_ERR_ARG, "null engine");
+    if (!e->bind_on)
+        return set_err(MDX_ERR_STATE, "call mdx_set_binding first");
+    HIPCHK(hipSetDevice(e->device));
+    return e->prec == MDX_PREC_FP64 ? bind_forces_t<double>(e, draw, F, T)
+                                    : bind_forces_t<float>(e, draw, F, T);
 }
 
 int mdx_prime(mdx_engine *e, const double *noise0)

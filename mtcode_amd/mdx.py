@@ -101,6 +101,10 @@ def lib():
     L.mdx_list_pairs.argtypes = [C.c_void_p, _ip, C.c_longlong, C.POINTER(C.c_longlong), _dp]
     L.mdx_stress.argtypes = [C.c_void_p, C.c_int, _dp]
     L.mdx_stress_noalloc.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.mdx_set_binding.argtypes = [C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_int)]
+    L.mdx_set_bindings.argtypes = [C.c_void_p, _ip, _ip]
+    L.mdx_get_bindings.argtypes = [C.c_void_p, _ip, _ip]
+    L.mdx_bind_forces.argtypes = [C.c_void_p, C.c_ulonglong, _dp, _dp]
     _lib = L
     return L
 
@@ -363,6 +367,33 @@ class Engine:
         out = np.zeros(self.N, dtype=np.int32)
         _check(lib().mdx_patch_bonds(self.h, out.ctypes.data_as(_ip)))
         return out
+
+    # ------------------------------------------------- binding
+    def set_binding(self, on_rate, off_rate):
+        """BindingModelSingle(on_rate, off_rate); returns the number of patches"""
+        nh = C.c_int(0)
+        _check(lib().mdx_set_binding(self.h, float(on_rate), float(off_rate), C.byref(nh)))
+        self.nh = nh.value
+        return self.nh
+
+    def set_bindings(self, isbound, boundto):
+        ib = np.ascontiguousarray(isbound, dtype=np.int32)
+        bt = np.ascontiguousarray(boundto, dtype=np.int32)
+        _check(lib().mdx_set_bindings(self.h, ib.ctypes.data_as(_ip), bt.ctypes.data_as(_ip)))
+
+    def get_bindings(self):
+        ib = np.zeros(self.nh, dtype=np.int32)
+        bt = np.zeros(self.nh, dtype=np.int32)
+        _check(lib().mdx_get_bindings(self.h, ib.ctypes.data_as(_ip), bt.ctypes.data_as(_ip)))
+        return ib, bt
+
+    def bind_forces(self, draw=0):
+        """one call of the binding routine: (F, T) of the bound pairs"""
+        F = np.zeros((self.N, 3))
+        T = np.zeros((self.N, 3))
+        _check(lib().mdx_bind_forces(self.h, int(draw), F.ctypes.data_as(_dp),
+                                     T.ctypes.data_as(_dp)))
+        return F, T
 
     # ------------------------------------------------- single stages
     def stage_halfkick(self, F, T=None):

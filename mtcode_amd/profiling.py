@@ -26,7 +26,7 @@ import sys
 from collections import defaultdict
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-KERNEL_SOURCES = ("mdx_kernels.h", "mdx_device.h", "mdx_engine.hip")
+KERNEL_SOURCES = ("mdx_kernels.h", "mdx_device.h", "mdx_bind.h", "mdx_engine.hip")
 COPY_BYTES = 256 << 20
 
 # the kernels of one list step of the rotor integrator (mode 3: M_FUSED) and

@@ -1152,3 +1152,596 @@ long long orc_dgraph_labels(const orc_box *b, int N, const double *x, const int 
     free(ij);
     return np;
 }
+
+/* ------------------------------------------------- one-bond-per-patch binding */
+
+typedef struct {
+    int a, b;
+    double s;
+} bind_edge;
+
+/* 53-bit uniform in [0, 1) from two Philox words */
+static double bind_u53(const uint32_t o[4])
+{
+    return ((double)(o[0] >> 5) * 67108864.0 + (double)(o[1] >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+double orc_bind_uniform(const uint32_t key[2], uint64_t draw, int root, int k)
+{
+    const uint32_t ctr[4] = {(uint32_t)root, (uint32_t)draw, (uint32_t)(draw >> 32), (uint32_t)k};
+    uint32_t o[4];
+    orc_philox4x32_10(ctr, key, o);
+    return bind_u53(o);
+}
+
+/* BindingModelSingle::doublet (Bindings/BindingModelSingle.cpp:10-25) */
+static int bind_doublet(const orc_bind_model *m, double ran)
+{
+    double totalr = m->off_rate + m->on_rate; /* tr, BindingModelSingle.cpp:5-8 */
+    double rr = totalr * ran;
+    return !(rr < m->off_rate);
+}
+
+/* BindingModelSingle::triplet (BindingModelSingle.cpp:40-119): which pair
+ * ends bound (1: 12, 2: 23, 3: 13, 0: none) */
+static int bind_triplet(const orc_bind_model *m, int b12, int b23, int b13, int c12, int c23,
+                        int c13, double ran)
+{
+    const double on = m->on_rate, off = m->off_rate;
+    double r1, r2, r3, r4;
+    if (b12) {
+        r1 = c12 * on;
+        r2 = c23 * ((off / 3.) + 0.001);
+        r3 = c13 * ((off / 3.) + 0.001);
+        r4 = (off / 3.) + 0.001;
+    } else if (b23) {
+        r1 = c12 * ((off / 3.) + 0.001);
+        r2 = c23 * (on);
+        r3 = c13 * ((off / 3.) + 0.001);
+        r4 = ((off / 3.) + 0.001);
+    } else if (b13) {
+        r1 = c12 * ((off / 3.) + 0.001);
+        r2 = c23 * ((off / 3.) + 0.001);
+        r3 = c13 * on;
+        r4 = (off / 3.) + 0.001;
+    } else {
+        r1 = c12 * on / 3.;
+        r2 = c23 * on / 3.;
+        r3 = c13 * on / 3.;
+        r4 = off;
+    }
+    double totalr = r1 + r2 + r3 + r4;
+    double rr = totalr * ran;
+    if (rr < r1)
+        return 1;
+    if (rr >= r1 && rr < r1 + r2)
+        return 2;
+    if (rr >= r1 + r2 && rr < r1 + r2 + r3)
+        return 3;
+    return 0;
+}
+
+typedef struct {
+    int *isbound, *boundto;
+    int *pairs; /* bound pairs in decision order */
+    long long npairs;
+} bind_out;
+
+static void bind_set_pair(bind_out *o, int i1, int i2)
+{
+    o->boundto[i1] = i2;
+    o->boundto[i2] = i1;
+    o->isbound[i1] = 1;
+    o->isbound[i2] = 1;
+    o->pairs[2 * o->npairs] = i1;
+    o->pairs[2 * o->npairs + 1] = i2;
+    o->npairs++;
+}
+
+static void sort3(int t1, int t2, int t3, int *i1, int *i2, int *i3)
+{
+    int v[3] = {t1, t2, t3};
+    for (int a = 0; a < 2; a++)
+        for (int c = 0; c < 2 - a; c++)
+            if (v[c] > v[c + 1]) {
+                int t = v[c];
+                v[c] = v[c + 1];
+                v[c + 1] = t;
+            }
+    *i1 = v[0];
+    *i2 = v[1];
+    *i3 = v[2];
+}
+
+/* a doublet i1 < i2 (LangevinRforce.cpp:3262-3302) */
+static void bind_decide2(const orc_bind_model *m, bind_out *o, int i1, int i2, double ran)
+{
+    if (bind_doublet(m, ran))
+        bind_set_pair(o, i1, i2);
+    else {
+        o->isbound[i1] = 0;
+        o->isbound[i2] = 0;
+    }
+}
+
+/* a triplet i1 < i2 < i3 with connectivity c (LangevinRforce.cpp:3303-3497) */
+static void bind_decide3(const orc_bind_model *m, bind_out *o, int i1, int i2, int i3, int c12,
+                         int c23, int c13, double ran)
+{
+    const int *bt = o->boundto, *ib = o->isbound;
+    int b12 = bt[i1] == i2 && bt[i2] == i1 && ib[i1] && ib[i2];
+    int b23 = bt[i2] == i3 && bt[i3] == i2 && ib[i2] && ib[i3];
+    int b13 = bt[i1] == i3 && bt[i3] == i1 && ib[i1] && ib[i3];
+    int w = bind_triplet(m, b12, b23, b13, c12, c23, c13, ran);
+    if (w == 1) {
+        bind_set_pair(o, i1, i2);
+        o->isbound[i3] = 0;
+    } else if (w == 2) {
+        bind_set_pair(o, i2, i3);
+        o->isbound[i1] = 0;
+    } else if (w == 3) {
+        bind_set_pair(o, i1, i3);
+        o->isbound[i2] = 0;
+    } else {
+        o->isbound[i1] = 0;
+        o->isbound[i2] = 0;
+        o->isbound[i3] = 0;
+    }
+}
+
+/* connectivity of a three-node component from node 1's degree and
+ * neighbours (LangevinRforce.cpp:3333-3402 / 3768-3819); -1 if malformed */
+static int bind_conn3(int nb1, int first1, int nb2, int i2, int i3, int *c12, int *c23, int *c13)
+{
+    if (nb1 == 1) {
+        if (first1 == i2) {
+            *c12 = 1;
+            *c13 = 0;
+            *c23 = 1;
+        } else if (first1 == i3) {
+            *c13 = 1;
+            *c12 = 0;
+            *c23 = 1;
+        } else
+            return -1;
+    } else if (nb1 == 2) {
+        *c12 = 1;
+        *c13 = 1;
+        *c23 = nb2 == 1 ? 0 : 1;
+    } else
+        return -1;
+    return 0;
+}
+
+/* DFUtilstore / ConnectedComponents (Bindings/GraphAlgorithms.cpp:467-531) */
+static void bind_dfs(int i, const int *adj, const int *lens, int depth, char *seen, int *ind,
+                     int *iter)
+{
+    seen[i] = 1;
+    ind[(*iter)++] = i;
+    for (int j = 0; j < lens[i]; j++) {
+        int p = adj[i * depth + j];
+        if (!seen[p])
+            bind_dfs(p, adj, lens, depth, seen, ind, iter);
+    }
+}
+
+/* PairHistogram of local edges (GraphAlgorithms.cpp:225-240) */
+static void bind_hist(const bind_edge *E, int ne, int n, int depth, int *adj, int *lens)
+{
+    for (int i = 0; i < n; i++)
+        lens[i] = 0;
+    for (int k = 0; k < ne; k++) {
+        adj[E[k].a * depth + lens[E[k].a]++] = E[k].b;
+        adj[E[k].b * depth + lens[E[k].b]++] = E[k].a;
+    }
+}
+
+/* component boundaries dem[0..nc] and the member order ind; returns nc */
+static int bind_components(const bind_edge *E, int ne, int n, int *adj, int *lens, char *seen,
+                           int *ind, int *dem)
+{
+    bind_hist(E, ne, n, n, adj, lens);
+    for (int i = 0; i < n; i++)
+        seen[i] = 0;
+    int iter = 0, nc = 0;
+    dem[0] = 0;
+    for (int i = 0; i < n; i++)
+        if (!seen[i]) {
+            bind_dfs(i, adj, lens, n, seen, ind, &iter);
+            dem[++nc] = iter;
+        }
+    return nc;
+}
+
+/* std::max_element with mdpairwdCompareScore: the first highest score */
+static void bind_drop_max(bind_edge *E, int *ne)
+{
+    int best = 0;
+    for (int k = 1; k < *ne; k++)
+        if (E[best].s < E[k].s)
+            best = k;
+    for (int k = best; k + 1 < *ne; k++)
+        E[k] = E[k + 1];
+    (*ne)--;
+}
+
+static int bind_local(const int *mem, int s, int g)
+{
+    int lo = 0, hi = s - 1;
+    while (lo <= hi) {
+        int mid = (lo + hi) / 2;
+        if (mem[mid] == g)
+            return mid;
+        if (mem[mid] < g)
+            lo = mid + 1;
+        else
+            hi = mid - 1;
+    }
+    return -1;
+}
+
+int orc_bind_forces(const orc_box *b, int N, const double *x, const double *Q,
+                    const orc_patchset *ps, const orc_bind_model *m, int *isbound, int *boundto,
+                    double *F, double *T, long long *stats)
+{
+    int pstart[ORC_MAXTYPES], rstart[ORC_MAXTYPES][ORC_MAXTYPES];
+    patch_layout(ps, pstart, rstart);
+    int *poff = (int *)malloc(sizeof(int) * (size_t)(N + 1));
+    poff[0] = 0;
+    for (int i = 0; i < N; i++)
+        poff[i + 1] = poff[i] + ps->npatch[ps->type[i]];
+    const int nh = poff[N];
+    int *owner = (int *)malloc(sizeof(int) * (size_t)(nh + 1));
+    for (int i = 0; i < N; i++)
+        for (int k = poff[i]; k < poff[i + 1]; k++)
+            owner[k] = i;
+    /* ComboPatch::max_check (combopatch.h:15, GeneralPatch.cpp:93-94):
+     * 1.0, raised to the largest interaction_distance = 1.4 d
+     * (potentialtheta.h:417) */
+    double mk = 1.0;
+    int nrows = 0;
+    for (int i = 0; i < ps->ntypes; i++)
+        for (int j = i; j < ps->ntypes; j++)
+            nrows += ps->npatch[i] * ps->npatch[j];
+    for (int r = 0; r < nrows; r++)
+        if (1.4 * ps->params[3 * r + 1] > mk)
+            mk = 1.4 * ps->params[3 * r + 1];
+
+    /* candidate edges (LangevinRforce.cpp:2914-3075), pairs p1 < p2 */
+    long long ecap = 1024, ne = 0;
+    bind_edge *edges = (bind_edge *)malloc(sizeof(bind_edge) * (size_t)ecap);
+    for (int p1 = 0; p1 < N; p1++)
+        for (int p2 = p1 + 1; p2 < N; p2++) {
+            double un[3], dis;
+            orc_distance_vector(b, x + 3 * p1, x + 3 * p2, un, &dis);
+            if (!(dis < mk * mk))
+                continue;
+            dis = sqrt(dis);
+            for (int k = 0; k < 3; k++)
+                un[k] /= dis;
+            int a = p1, c = p2;
+            if (ps->type[p1] > ps->type[p2]) { /* roles exchanged, as in patch_pair */
+                a = p2;
+                c = p1;
+                for (int k = 0; k < 3; k++)
+                    un[k] = -un[k];
+            }
+            const int k1 = ps->type[a], k2 = ps->type[c];
+            const int r0 = rstart[k1][k2];
+            for (int ia = 0; ia < ps->npatch[k1]; ia++)
+                for (int ic = 0; ic < ps->npatch[k2]; ic++) {
+                    const double *prm = ps->params + 3 * (r0 + ia * ps->npatch[k2] + ic);
+                    double n1[3], n2[3];
+                    lab_patch(Q + 9 * a, ps->dirs + 3 * (pstart[k1] + ia), n1);
+                    lab_patch(Q + 9 * c, ps->dirs + 3 * (pstart[k2] + ic), n2);
+                    double argthetai = -(n1[0] * un[0] + n1[1] * un[1] + n1[2] * un[2]);
+                    double argthetaj = (n2[0] * un[0] + n2[1] * un[1] + n2[2] * un[2]);
+                    double disp = 1.4 * prm[1], thetam = prm[2];
+                    double ctm = cos(thetam);
+                    const int wp1 = poff[a] + ia, wp2 = poff[c] + ic;
+                    int cond1 = boundto[wp1] == wp2 && boundto[wp2] == wp1;
+                    int b1 = isbound[wp1] != 0, b2 = isbound[wp2] != 0;
+                    double disp2;
+                    if (b1 && b2 && cond1)
+                        disp2 = disp;
+                    else if (b1 && b2 && !cond1)
+                        disp2 = 0.5 * disp;
+                    else if (!b1 != !b2)
+                        disp2 = 0.7 * disp;
+                    else
+                        disp2 = disp;
+                    if (argthetai > ctm && argthetaj > ctm && dis < disp2) {
+                        double scr1 = 1 - (argthetai - cos(thetam));
+                        double scr2 = 1 - (argthetaj - cos(thetam));
+                        double scr3 = 2 * (dis / disp2);
+                        if (ne == ecap) {
+                            ecap *= 2;
+                            edges = (bind_edge *)realloc(edges, sizeof(bind_edge) * (size_t)ecap);
+                        }
+                        edges[ne].a = wp1;
+                        edges[ne].b = wp2;
+                        edges[ne].s = scr1 + scr2 + scr3;
+                        ne++;
+                    }
+                }
+        }
+
+    /* per-patch adjacency (PairHistogramExtended, GraphAlgorithms.cpp:264-351),
+     * each patch's partners in ascending order */
+    int *deg = (int *)calloc((size_t)nh + 1, sizeof(int));
+    int *aoff = (int *)malloc(sizeof(int) * (size_t)(nh + 1));
+    for (long long k = 0; k < ne; k++) {
+        deg[edges[k].a]++;
+        deg[edges[k].b]++;
+    }
+    aoff[0] = 0;
+    for (int i = 0; i < nh; i++)
+        aoff[i + 1] = aoff[i] + deg[i];
+    int *apart = (int *)malloc(sizeof(int) * (size_t)(aoff[nh] + 1));
+    double *ascr = (double *)malloc(sizeof(double) * (size_t)(aoff[nh] + 1));
+    int *fill = (int *)calloc((size_t)nh + 1, sizeof(int));
+    for (long long k = 0; k < ne; k++) {
+        int u = edges[k].a, v = edges[k].b;
+        apart[aoff[u] + fill[u]] = v;
+        ascr[aoff[u] + fill[u]++] = edges[k].s;
+        apart[aoff[v] + fill[v]] = u;
+        ascr[aoff[v] + fill[v]++] = edges[k].s;
+    }
+    for (int i = 0; i < nh; i++) /* insertion sort by partner */
+        for (int k = aoff[i] + 1; k < aoff[i + 1]; k++)
+            for (int q = k; q > aoff[i] && apart[q - 1] > apart[q]; q--) {
+                int tp = apart[q];
+                apart[q] = apart[q - 1];
+                apart[q - 1] = tp;
+                double ts = ascr[q];
+                ascr[q] = ascr[q - 1];
+                ascr[q - 1] = ts;
+            }
+
+    /* bonds whose partner is no longer a candidate break
+     * (LangevinRforce.cpp:3098-3125) */
+    for (int i = 0; i < nh; i++)
+        if (isbound[i]) {
+            int outside = 1;
+            for (int k = aoff[i]; k < aoff[i + 1]; k++)
+                if (apart[k] == boundto[i]) {
+                    outside = 0;
+                    break;
+                }
+            if (outside)
+                isbound[i] = 0;
+        }
+
+    /* components labelled by their smallest patch (ConnectedComponentsParallel_Old,
+     * GraphAlgorithms.cpp:672-760, converges to the component minimum), sizes
+     * and members in ascending order (SingleHistogram, GraphAlgorithms.cpp:2-49) */
+    int *lab = (int *)malloc(sizeof(int) * (size_t)(nh + 1));
+    for (int i = 0; i < nh; i++)
+        lab[i] = i;
+    for (long long k = 0; k < ne; k++) {
+        int r1 = uf_root(lab, edges[k].a), r2 = uf_root(lab, edges[k].b);
+        if (r1 < r2)
+            lab[r2] = r1;
+        else if (r2 < r1)
+            lab[r1] = r2;
+    }
+    int *ccs = (int *)calloc((size_t)nh + 1, sizeof(int));
+    for (int i = 0; i < nh; i++) {
+        lab[i] = uf_root(lab, i);
+        ccs[lab[i]]++;
+    }
+    int *moff = (int *)malloc(sizeof(int) * (size_t)(nh + 1));
+    moff[0] = 0;
+    for (int i = 0; i < nh; i++)
+        moff[i + 1] = moff[i] + ccs[i];
+    int *mem = (int *)malloc(sizeof(int) * (size_t)(nh + 1));
+    for (int i = 0; i < nh; i++)
+        fill[i] = 0;
+    for (int i = 0; i < nh; i++)
+        mem[moff[lab[i]] + fill[lab[i]]++] = i;
+
+    /* random numbers: the reference's rand() draws (Generate_Random_Numbers_Needed,
+     * GraphAlgorithms.cpp:431-438, then generate_random_matrix,
+     * matrix2.cpp:1788-1805, for the large clusters) or Philox per cluster */
+    int smax = 0, nlarge = 0, nmulti = 0;
+    for (int i = 0; i < nh; i++) {
+        if (ccs[i] > smax)
+            smax = ccs[i];
+        if (ccs[i] > 1)
+            nmulti++;
+        if (ccs[i] > 3)
+            nlarge++;
+    }
+    const int ncols = smax / 2 + 1;
+    double *rsmall = NULL, *rlarge = NULL;
+    if (m->rng == 0) {
+        srand(m->seed);
+        rsmall = (double *)calloc((size_t)nh + 1, sizeof(double));
+        for (int i = 0; i < nh; i++)
+            if (ccs[i] > 1)
+                rsmall[i] = (double)(rand()) / ((double)(RAND_MAX));
+        rlarge = (double *)malloc(sizeof(double) * ((size_t)nlarge * ncols + 1));
+        for (long long k = 0; k < (long long)nlarge * ncols; k++)
+            rlarge[k] = (((double)rand() / (RAND_MAX)));
+    }
+
+    bind_out o = {isbound, boundto, (int *)malloc(sizeof(int) * (size_t)(nh + 2)), 0};
+    /* clusters of 1, 2 and 3 patches (LangevinRforce.cpp:3245-3512) */
+    for (int r = 0; r < nh; r++) {
+        const int s = ccs[r];
+        if (s == 0 || s > 3) {
+            continue;
+        }
+        const int *mm = mem + moff[r];
+        if (s == 1) {
+            isbound[mm[0]] = 0;
+            continue;
+        }
+        double ran = m->rng == 0 ? rsmall[r] : orc_bind_uniform(m->key, m->draw, r, 0);
+        if (s == 2) {
+            bind_decide2(m, &o, mm[0], mm[1], ran);
+        } else {
+            int i1 = mm[0], i2 = mm[1], i3 = mm[2], c12 = 0, c23 = 0, c13 = 0;
+            if (bind_conn3(deg[i1], deg[i1] > 0 ? apart[aoff[i1]] : -1, deg[i2], i2, i3, &c12,
+                           &c23, &c13) < 0)
+                return -2;
+            bind_decide3(m, &o, i1, i2, i3, c12, c23, c13, ran);
+        }
+    }
+    /* larger clusters (LangevinRforce.cpp:3531-3870): drop the highest-scored
+     * edge until no component exceeds 3 patches, then decide each component */
+    int allc = 0;
+    for (int r = 0; r < nh; r++) {
+        const int s = ccs[r];
+        if (s <= 3)
+            continue;
+        const int *mm = mem + moff[r];
+        int emax = 0;
+        for (int j = 0; j < s; j++)
+            emax += deg[mm[j]];
+        bind_edge *E = (bind_edge *)malloc(sizeof(bind_edge) * (size_t)(emax + 1));
+        int nE = 0;
+        for (int j = 0; j < s; j++) /* RecapitulateEdges (LangevinRforceUtil.cpp:508-575) */
+            for (int k = aoff[mm[j]]; k < aoff[mm[j] + 1]; k++) {
+                int f1 = j, f2 = bind_local(mm, s, apart[k]);
+                bind_edge e1;
+                e1.a = f1 < f2 ? f1 : f2;
+                e1.b = f1 < f2 ? f2 : f1;
+                e1.s = ascr[k];
+                int already = 0;
+                for (int q = 0; q < nE; q++)
+                    if (E[q].a == e1.a && E[q].b == e1.b) {
+                        already = 1;
+                        break;
+                    }
+                if (!already)
+                    E[nE++] = e1;
+            }
+        int *adj = (int *)malloc(sizeof(int) * (size_t)s * s), *lens = (int *)malloc(sizeof(int) * s);
+        int *ind = (int *)malloc(sizeof(int) * s), *dem = (int *)malloc(sizeof(int) * (s + 1));
+        char *seen = (char *)malloc((size_t)s);
+        bind_drop_max(E, &nE);
+        int nc = bind_components(E, nE, s, adj, lens, seen, ind, dem);
+        for (;;) {
+            int big = 0;
+            for (int j = 0; j < nc; j++)
+                if (dem[j + 1] - dem[j] > big)
+                    big = dem[j + 1] - dem[j];
+            if (big <= 3)
+                break;
+            bind_drop_max(E, &nE);
+            nc = bind_components(E, nE, s, adj, lens, seen, ind, dem);
+        }
+        bind_hist(E, nE, s, s, adj, lens); /* PairHistogram -> boindicesx, lensx */
+        for (int k1 = 0; k1 < s; k1++) {
+            const int g = mm[k1];
+            if (isbound[g]) {
+                int outside = 1;
+                for (int k = 0; k < lens[k1]; k++)
+                    if (mm[adj[k1 * s + k]] == boundto[g]) {
+                        outside = 0;
+                        break;
+                    }
+                if (outside)
+                    isbound[g] = 0;
+            }
+        }
+        int it = 0;
+        for (int j = 0; j < nc; j++) {
+            const int z = dem[j + 1] - dem[j];
+            if (z == 1) {
+                isbound[mm[ind[dem[j]]]] = 0;
+            } else if (z == 2) {
+                int n1 = ind[dem[j]], n2 = ind[dem[j] + 1];
+                int i1 = mm[n1 < n2 ? n1 : n2], i2 = mm[n1 < n2 ? n2 : n1];
+                double ran = m->rng == 0 ? rlarge[(size_t)allc * ncols + it]
+                                         : orc_bind_uniform(m->key, m->draw, r, 1 + it);
+                it++;
+                bind_decide2(m, &o, i1, i2, ran);
+            } else {
+                int ii1, ii2, ii3, c12 = 0, c23 = 0, c13 = 0;
+                sort3(ind[dem[j]], ind[dem[j] + 1], ind[dem[j] + 2], &ii1, &ii2, &ii3);
+                if (bind_conn3(lens[ii1], lens[ii1] > 0 ? adj[ii1 * s] : -1, lens[ii2], ii2, ii3,
+                               &c12, &c23, &c13) < 0)
+                    return -3;
+                double ran = m->rng == 0 ? rlarge[(size_t)allc * ncols + it]
+                                         : orc_bind_uniform(m->key, m->draw, r, 1 + it);
+                it++;
+                bind_decide3(m, &o, mm[ii1], mm[ii2], mm[ii3], c12, c23, c13, ran);
+            }
+        }
+        free(E);
+        free(adj);
+        free(lens);
+        free(ind);
+        free(dem);
+        free(seen);
+        allc++;
+    }
+
+    /* forces and torques of the bound pairs (LangevinRforce.cpp:4030-4095) */
+    for (long long q = 0; q < o.npairs; q++) {
+        const int w1 = o.pairs[2 * q], w2 = o.pairs[2 * q + 1];
+        int p1 = owner[w1], p2 = owner[w2];
+        if (p2 < p1) {
+            int t = p1;
+            p1 = p2;
+            p2 = t;
+        }
+        double un[3], dis;
+        orc_distance_vector(b, x + 3 * p1, x + 3 * p2, un, &dis);
+        dis = sqrt(dis);
+        for (int k = 0; k < 3; k++)
+            un[k] /= dis;
+        int a = p1, c = p2;
+        if (ps->type[p1] > ps->type[p2]) {
+            a = p2;
+            c = p1;
+            for (int k = 0; k < 3; k++)
+                un[k] = -un[k];
+        }
+        const int wa = owner[w1] == a ? w1 : w2, wc = owner[w1] == a ? w2 : w1;
+        const int k1 = ps->type[a], k2 = ps->type[c];
+        const int ia = wa - poff[a], ic = wc - poff[c];
+        const double *prm = ps->params + 3 * (rstart[k1][k2] + ia * ps->npatch[k2] + ic);
+        double f[9];
+        orc_kf2_force_torque(un, dis, Q + 9 * a, Q + 9 * c, ps->dirs + 3 * (pstart[k1] + ia),
+                             ps->dirs + 3 * (pstart[k2] + ic), prm[0], prm[1], prm[2], f);
+        F[3 * a + 0] += f[0];
+        F[3 * a + 1] += f[1];
+        F[3 * a + 2] += f[2];
+        F[3 * c + 0] += -f[0];
+        F[3 * c + 1] += -f[1];
+        F[3 * c + 2] += -f[2];
+        T[3 * a + 0] += f[3];
+        T[3 * a + 1] += f[4];
+        T[3 * a + 2] += f[5];
+        T[3 * c + 0] += f[6];
+        T[3 * c + 1] += f[7];
+        T[3 * c + 2] += f[8];
+    }
+    if (stats) {
+        stats[0] = ne;
+        stats[1] = nmulti;
+        stats[2] = nlarge;
+        stats[3] = smax;
+        stats[4] = o.npairs;
+    }
+    free(poff);
+    free(owner);
+    free(edges);
+    free(deg);
+    free(aoff);
+    free(apart);
+    free(ascr);
+    free(fill);
+    free(lab);
+    free(ccs);
+    free(moff);
+    free(mem);
+    free(rsmall);
+    free(rlarge);
+    free(o.pairs);
+    return 0;
+}

@@ -32,6 +32,9 @@
  *   orc_philox4x32_10            : the Random123 known-answer vectors
  *   orc_noise_uniform            : moments / support (its own definition)
  *   orc_calcg1                   : golden fixture from the compiled reference
+ *   orc_bind_forces              : golden fixtures from the compiled reference
+ *                                  (bound sets, forces, torques after one
+ *                                  call with the reference's rand() draws)
  *   orc_dgraph_labels            : golden fixture from the compiled reference
  *                                  (distance_graph); scipy's connected
  *                                  components in the tests
@@ -266,6 +269,44 @@ int orc_energy_virial(const orc_box *b, int N, const double *x, const double *Q,
                       const orc_interactions *it, double *U, double W[9]);
 
 /* measured_temperature (LangevinR.cpp:170-193): out[0..2] <p^2/2m>, out[3..5] <L^2/2I0> */
+/* ------------------------------------------------ one-bond-per-patch binding */
+/* LangevinNVTR::calculate_forces_and_torques3D_onlyone_nonlets
+ * (Rotational/LangevinRforce.cpp:2874-4136) with BindingModelSingle
+ * (Bindings/BindingModelSingle.cpp:1-119) on a frozen configuration: the
+ * candidate patch pairs (both cones, a distance bound of 1, 0.7 or 0.5 x
+ * the interaction distance by bound state, a score), the bonds whose partner
+ * is no longer a candidate broken, the candidate graph's components (labelled
+ * by their smallest patch), each component of 1-3 patches decided by the
+ * model's doublet / triplet rule, larger ones cut by removing their
+ * highest-scored edge until no part exceeds 3, and the Kern-Frenkel force
+ * and torques of every bound pair added to F, T.  Patches are numbered per
+ * particle in particle order, each particle's patches in type order
+ * (GeneralPatch::which_patch, GeneralPatch.cpp:292-307, for particles sorted
+ * by type).  Each patch's candidates are kept in ascending partner order (the
+ * reference keeps its pair-list order; the outcome differs only where two
+ * edges of a large cluster have exactly equal scores).
+ * rng 0: the reference's draws -- srand(seed), one rand()/RAND_MAX per
+ * component of 2 or more patches in ascending order of its smallest patch,
+ * then a (large clusters) x (largest size / 2 + 1) matrix
+ * (Generate_Random_Numbers_Needed, GraphAlgorithms.cpp:431-438;
+ * generate_random_matrix, matrix2.cpp:1788-1805).  rng 1: Philox4x32-10,
+ * counter {smallest patch, draw lo, draw hi, k} (k = 0 for a component of
+ * 2-3 patches, 1 + i for the i-th decision inside a larger one), the 53-bit
+ * uniform of orc_bind_uniform.
+ * isbound, boundto: per patch, updated in place.  stats (5, or NULL):
+ * candidate edges, components of 2+ patches, of 4+, the largest, bound
+ * pairs.  Returns 0, or < 0 for a malformed three-patch component. */
+typedef struct {
+    double on_rate, off_rate;
+    int rng;
+    unsigned int seed;
+    uint32_t key[2];
+    uint64_t draw;
+} orc_bind_model;
+double orc_bind_uniform(const uint32_t key[2], uint64_t draw, int root, int k);
+int orc_bind_forces(const orc_box *b, int N, const double *x, const double *Q,
+                    const orc_patchset *ps, const orc_bind_model *m, int *isbound, int *boundto,
+                    double *F, double *T, long long *stats);
 void orc_measured_temperature(const orc_params *P, int N, const double *p, const double *L,
                               double out[6]);
 

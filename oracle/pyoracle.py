@@ -34,6 +34,11 @@ class PatchSet(C.Structure):
                 ("dirs", _dp), ("params", _dp)]
 
 
+class BindModel(C.Structure):
+    _fields_ = [("on_rate", C.c_double), ("off_rate", C.c_double), ("rng", C.c_int),
+                ("seed", C.c_uint), ("key", C.c_uint32 * 2), ("draw", C.c_uint64)]
+
+
 class Interactions(C.Structure):
     _fields_ = [
         ("wca_on", C.c_int), ("wca_eps", C.c_double), ("wca_sigma", C.c_double),
@@ -120,6 +125,12 @@ def lib():
         L.orc_energy_virial.restype = C.c_int
         L.orc_energy_virial.argtypes = [C.POINTER(Box), C.c_int, _dp, _dp,
                                         C.POINTER(Interactions), _dp, _dp]
+        L.orc_bind_uniform.restype = C.c_double
+        L.orc_bind_uniform.argtypes = [C.POINTER(C.c_uint32), C.c_uint64, C.c_int, C.c_int]
+        L.orc_bind_forces.restype = C.c_int
+        L.orc_bind_forces.argtypes = [C.POINTER(Box), C.c_int, _dp, _dp, C.POINTER(PatchSet),
+                                      C.POINTER(BindModel), _ip, _ip, _dp, _dp,
+                                      C.POINTER(C.c_longlong)]
         _lib = L
     return _lib
 
@@ -357,6 +368,45 @@ def dgraph_labels(box, x, n, cut, lo=0, hi=-1, periodic=True):
     if np_ < 0:
         raise MemoryError("orc_dgraph_labels")
     return out
+
+
+def bind_forces(s, isbound, boundto, on_rate, off_rate, seed=None, key=None, draw=0, x=None,
+                Q=None):
+    """One call of the one-bond-per-patch routine (orc_bind_forces): the
+    reference's rand() draws after srand(seed) when seed is given, else Philox
+    with key (2 words) and draw.  Returns (F, T, isbound, boundto, stats)."""
+    keep = _Keep()
+    x = s["x"] if x is None else x
+    Q = s["Q"] if Q is None else Q
+    N = len(x)
+    ps = make_patchset(s["patches"], keep)
+    m = BindModel()
+    m.on_rate, m.off_rate = float(on_rate), float(off_rate)
+    if seed is not None:
+        m.rng, m.seed = 0, int(seed)
+    else:
+        m.rng = 1
+        m.key[0], m.key[1] = int(key[0]), int(key[1])
+        m.draw = int(draw)
+    ib, ibp = _i(np.array(isbound, dtype=np.int32))
+    bt, btp = _i(np.array(boundto, dtype=np.int32))
+    F = np.zeros((N, 3))
+    T = np.zeros((N, 3))
+    st = np.zeros(5, dtype=np.int64)
+    b = make_box(s["box"], s.get("periodic", True))
+    xa, xp = _d(x)
+    qa, qp = _d(Q)
+    rc = lib().orc_bind_forces(C.byref(b), N, xp, qp, C.byref(ps), C.byref(m), ibp, btp,
+                               F.ctypes.data_as(_dp), T.ctypes.data_as(_dp),
+                               st.ctypes.data_as(C.POINTER(C.c_longlong)))
+    if rc != 0:
+        raise RuntimeError(f"orc_bind_forces: {rc}")
+    return F, T, ib, bt, st
+
+
+def bind_uniform(key, draw, root, k):
+    kk = (C.c_uint32 * 2)(int(key[0]), int(key[1]))
+    return lib().orc_bind_uniform(kk, int(draw), int(root), int(k))
 
 
 def noise(seed, first, count):

@@ -28,6 +28,8 @@
  *   temperature p, L -> LangevinNVTR::measured_temperature
  *   pairpot   r    -> WCAPotential::energy, HSPotential::force and force2mdx
  *   stress    x    -> MD::calculatestress with an HSPotential interaction
+ *   bind      x, Q, isbound, boundto -> one call of the one-bond-per-patch
+ *              routine (calculate_forces_and_torques3D_onlyone_nonlets)
  */
 #include <stdlib.h>
 #include <iostream>
@@ -722,6 +724,58 @@ static int cmd_stress()
     return 0;
 }
 
+/* bind: x, Q, isbound(nh), boundto(nh) -> F, T, isbound, boundto after one
+ * LangevinNVTR::calculate_forces_and_torques3D_onlyone_nonlets with
+ * BindingModelSingle(on_rate, off_rate) (F, T start at zero: the binding
+ * forces alone), srand(seed) first; the patch list from
+ * calculate_patch_list over MD::calculatepairs at `cut` (Condensate.cpp:452-463).
+ * Run with one OpenMP thread, so that the routine's lists are in its serial
+ * order. */
+static int cmd_bind()
+{
+    int N = (int)ji("N");
+    cube geo = make_cube();
+    LangevinNVTR b(geo);
+    matrix<double> x = to_mat(rd<double>("x", (size_t)N * 3), N, 3);
+    matrix<double> Q = to_mat(rd<double>("Q", (size_t)N * 9), N, 9);
+    matrix<double> p0(N, 3), L0(N, 3);
+    b.initialize(x, p0, Q, L0);
+    ComboPatch *pots = make_patches();
+    if (!pots)
+        die("bind needs patches");
+    int nh = pots->get_total_patches(N);
+    vector<int> ib = rd<int>("isbound", (size_t)nh), bt = rd<int>("boundto", (size_t)nh);
+    BinaryBindStore bbs;
+    vector1<bool> isbound(nh);
+    vector1<int> boundto(nh);
+    for (int i = 0; i < nh; i++) {
+        isbound[i] = ib[i] != 0;
+        boundto[i] = bt[i];
+    }
+    bbs.isbound = isbound;
+    bbs.boundto = boundto;
+    BindingModelSingle bm(jd("on_rate"), jd("off_rate"));
+    matrix<int> *pairs = make_pairs(b, geo, (int)ji("n"), jd("cut"));
+    vector<patchint> opairs = b.calculate_patch_list(*pairs, *pots);
+    vector<int> runs_diff = adjacency(opairs);
+    matrix<double> F(N, 3), T(N, 3);
+    srand((unsigned int)ji("seed"));
+    b.calculate_forces_and_torques3D_onlyone_nonlets(opairs, runs_diff, *pots, bbs, bm, F, T);
+    for (int i = 0; i < nh; i++) {
+        ib[i] = bbs.isbound[i] ? 1 : 0;
+        bt[i] = bbs.boundto[i];
+    }
+    vector<double> Fv = from_mat(F), Tv = from_mat(T);
+    wr("F", Fv.data(), Fv.size());
+    wr("T", Tv.data(), Tv.size());
+    wr("isbound_out", ib.data(), ib.size());
+    wr("boundto_out", bt.data(), bt.size());
+    int np = (int)opairs.size();
+    wr("npatchpairs", &np, 1);
+    delete pairs;
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc != 3)
@@ -757,6 +811,8 @@ int main(int argc, char **argv)
         return cmd_pairpot();
     if (cmd == "stress")
         return cmd_stress();
+    if (cmd == "bind")
+        return cmd_bind();
     die("unknown command " + cmd);
     return 2;
 }

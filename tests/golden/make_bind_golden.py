@@ -1,0 +1,120 @@
+"""Writes tests/golden/bind.npz: inputs and outputs of one call of the
+reference's one-bond-per-patch routine
+(LangevinNVTR::calculate_forces_and_torques3D_onlyone_nonlets with
+BindingModelSingle), run by oracle/_ref/mtref (``make -C oracle ref``) on one
+OpenMP thread, after srand(seed).
+
+Run from the repository root:  python tests/golden/make_bind_golden.py
+
+Cases (x, Q, types and patch rows as in mtcode_amd.workloads):
+  ab0   two-species condensate (A and B halves), 1728 particles on a jittered
+        cubic lattice of spacing 1.15, all patches unbound
+  ab1   the same configuration starting from ab0's bound state
+  wide0 one type with 4 tetrahedral patches and wide cones (theta_m = 1.0,
+        d = 1.1), 1331 particles on the same lattice: components of many
+        patches
+  wide1 the same starting from wide0's state with a tenth of the bonds
+        flipped and a twentieth pointed at random patches
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import pyoracle as po  # noqa: E402
+from mtcode_amd import workloads as wl  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+WORK = "/tmp/mtcode_golden_bind"
+ON, OFF = 0.99999, 1.0 - 0.99999   # mainPatchy.cpp:97-101
+
+
+def general_job(s):
+    ps = s["patches"]
+    job = {"patch_kind": "general", "ntypes": len(ps["npatch"])}
+    cum = 0
+    for t, npt in enumerate(ps["npatch"]):
+        cnt = int(np.sum(ps["types"] == t))
+        assert np.all(ps["types"][cum:cum + cnt] == t)   # GeneralPatch type ranges
+        cum += cnt
+        job[f"npatch_{t}"] = npt
+        job[f"numpertype_{t}"] = cum
+    return job
+
+
+def lattice(s, m, a, seed):
+    """m^3 sites of a simple cubic lattice of spacing a, jittered by 0.05 a
+    (dense enough for many candidate pairs)"""
+    g = np.arange(m) * a
+    x = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
+    x = x + np.random.default_rng(seed).uniform(-0.05 * a, 0.05 * a, size=x.shape) + 0.5 * a
+    s["x"] = x
+    s["box"] = np.full(3, m * a)
+    return s
+
+
+def systems():
+    ab = lattice(wl.condensate_ab(12 ** 3, phi=0.05, seed_pos=11, seed_q=12), 12, 1.15, 31)
+    wide = lattice(wl.patchy_tetra(11 ** 3, phi=0.05, seed_pos=13, seed_q=14), 11, 1.15, 32)
+    wide["patches"]["params"][:, 1] = 1.1
+    wide["patches"]["params"][:, 2] = 1.0
+    return ab, wide
+
+
+def run(name, s, isbound, boundto, seed):
+    d = os.path.join(WORK, name)
+    shutil.rmtree(d, ignore_errors=True)
+    os.makedirs(d)
+    L = float(s["box"][0])
+    mk = max(1.0, 1.4 * float(np.max(s["patches"]["params"][:, 1])))
+    cut = mk + 0.05
+    job = {"N": s["N"], "L": repr(L), "periodic": 1, "n": int(L // cut), "cut": repr(cut),
+           "seed": seed, "on_rate": repr(ON), "off_rate": repr(OFF)}
+    job.update(general_job(s))
+    ps = s["patches"]
+    po.run_ref("bind", d, job, {"x": s["x"], "Q": s["Q"], "params": ps["params"],
+                                "dirs": np.asarray(ps["dirs"]).reshape(-1, 3),
+                                "isbound": isbound.astype(np.int32),
+                                "boundto": boundto.astype(np.int32)})
+    return {"F": po.read_ref(d, "F").reshape(-1, 3), "T": po.read_ref(d, "T").reshape(-1, 3),
+            "isbound": po.read_ref(d, "isbound_out", np.int32),
+            "boundto": po.read_ref(d, "boundto_out", np.int32)}
+
+
+def main():
+    if not po.have_ref():
+        raise SystemExit("build oracle/_ref first: make -C oracle ref")
+    ab, wide = systems()
+    out = {}
+    rng = np.random.default_rng(21)
+    for tag, s in (("ab", ab), ("wide", wide)):
+        nh = int(sum(s["patches"]["npatch"][t] for t in s["patches"]["types"]))
+        ib0 = np.zeros(nh, np.int32)
+        bt0 = np.zeros(nh, np.int32)
+        r0 = run(tag + "0", s, ib0, bt0, seed=101)
+        ib1, bt1 = r0["isbound"].copy(), r0["boundto"].copy()
+        if tag == "wide":
+            flip = rng.random(nh) < 0.1
+            ib1[flip] ^= 1
+            moved = rng.random(nh) < 0.05
+            bt1[moved] = rng.integers(0, nh, size=int(moved.sum()))
+        r1 = run(tag + "1", s, ib1, bt1, seed=202)
+        for k, ib, bt, seed, r in ((tag + "0", ib0, bt0, 101, r0), (tag + "1", ib1, bt1, 202, r1)):
+            out[k + "_isbound_in"] = ib
+            out[k + "_boundto_in"] = bt
+            out[k + "_seed"] = np.int64(seed)
+            for q, v in r.items():
+                out[f"{k}_{q}_out"] = v
+            print(k, "bound patches", int(r["isbound"].sum()), "of", nh)
+    np.savez_compressed(os.path.join(OUT, "bind.npz"), **out)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,141 @@
+"""The one-bond-per-patch binding on the GPU (mdx_bind_forces, through the C
+ABI) against the oracle's restatement of
+calculate_forces_and_torques3D_onlyone_nonlets (Rotational/LangevinRforce.cpp:
+2874-4136), which tests/test_oracle_bind.py pins to the compiled reference.
+Both draw Philox4x32-10 at {smallest patch of the component, draw, k} under
+the engine's binding key.  Bars: bound state (isbound, boundto) bit-exact;
+binding forces and torques within 1e-12 (FP64) / 1e-5 (mixed) of the oracle,
+relative to the largest value."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from mtcode_amd import mdx
+from mtcode_amd import workloads as wl
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+import make_bind_golden as g  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+SEED = 77
+KEY = (SEED & 0xFFFFFFFF, (SEED >> 32) ^ 0xB1D0B1D0)   # the engine's binding key
+Z = np.load(os.path.join(HERE, "golden", "bind.npz"))
+AB, WIDE = g.systems()
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(1.0, np.max(np.abs(b))))
+
+
+def bind_engine(s, prec, on=g.ON, off=g.OFF):
+    e = mdx.engine_from_system(s, precision=prec, seed=SEED)
+    nh = e.set_binding(on, off)
+    return e, nh
+
+
+@pytest.mark.parametrize("case", ["ab0", "ab1", "wide0", "wide1"])
+def test_bind_fp64_vs_oracle_two_calls(case):
+    s = AB if case.startswith("ab") else WIDE
+    say("bind fp64", case)
+    e, nh = bind_engine(s, mdx.PREC_FP64)
+    ib0, bt0 = Z[case + "_isbound_in"], Z[case + "_boundto_in"]
+    e.set_bindings(ib0, bt0)
+    F, T = e.bind_forces(draw=5)
+    ib, bt = e.get_bindings()
+    Fo, To, ibo, bto, st = po.bind_forces(s, ib0, bt0, g.ON, g.OFF, key=KEY, draw=5)
+    assert np.array_equal(ib, ibo) and np.array_equal(bt, bto)
+    assert rel(F, Fo) < 1e-12 and rel(T, To) < 1e-12
+    # the next call starts from the state the device holds
+    F2, T2 = e.bind_forces(draw=6)
+    ib2, bt2 = e.get_bindings()
+    Fo2, To2, ibo2, bto2, _ = po.bind_forces(s, ibo, bto, g.ON, g.OFF, key=KEY, draw=6)
+    assert np.array_equal(ib2, ibo2) and np.array_equal(bt2, bto2)
+    assert rel(F2, Fo2) < 1e-12 and rel(T2, To2) < 1e-12
+    if case.startswith("wide"):
+        assert st[2] > 0   # components of 4+ patches
+
+
+@pytest.mark.parametrize("case", ["ab1", "wide1"])
+def test_bind_mixed_vs_oracle(case):
+    """mixed precision: the decisions are FP64 on the stored (FP32
+    quaternion) orientations; on these configurations no decision sits
+    within that rounding of its threshold, so the bound sets are equal"""
+    s = AB if case.startswith("ab") else WIDE
+    say("bind mixed", case)
+    e, nh = bind_engine(s, mdx.PREC_MIXED)
+    ib0, bt0 = Z[case + "_isbound_in"], Z[case + "_boundto_in"]
+    e.set_bindings(ib0, bt0)
+    F, T = e.bind_forces(draw=9)
+    ib, bt = e.get_bindings()
+    Fo, To, ibo, bto, _ = po.bind_forces(s, ib0, bt0, g.ON, g.OFF, key=KEY, draw=9)
+    assert np.array_equal(ib, ibo) and np.array_equal(bt, bto)
+    assert rel(F, Fo) < 1e-5 and rel(T, To) < 1e-5
+
+
+def lattice_system(m, thetam, d, seed):
+    s = g.lattice(wl.patchy_tetra(m ** 3, phi=0.05, seed_pos=seed, seed_q=seed + 1), m, 1.15,
+                  seed + 2)
+    s["patches"]["params"][:, 1] = d
+    s["patches"]["params"][:, 2] = thetam
+    return s
+
+
+def test_bind_32k_vs_oracle_and_rates():
+    """32,768 particles (131,072 patches), wide cones: the same bound state
+    and forces as the oracle over three calls, with on/off rates 0.6/0.4"""
+    s = lattice_system(32, 1.0, 1.1, 41)
+    say("bind 32k")
+    e, nh = bind_engine(s, mdx.PREC_FP64, 0.6, 0.4)
+    ibo = np.zeros(nh, np.int32)
+    bto = np.zeros(nh, np.int32)
+    for draw in range(3):
+        F, T = e.bind_forces(draw=draw)
+        ib, bt = e.get_bindings()
+        Fo, To, ibo, bto, st = po.bind_forces(s, ibo, bto, 0.6, 0.4, key=KEY, draw=draw)
+        say("  call", draw, "edges", st[0], "large", st[2], "bound pairs", st[4])
+        assert np.array_equal(ib, ibo) and np.array_equal(bt, bto)
+        assert rel(F, Fo) < 1e-12 and rel(T, To) < 1e-12
+
+
+def test_bind_full_size_properties():
+    """the 1,048,576-particle condensate (the bench system) at its own
+    packing: every bond mutual and between different particles, the bound
+    pair forces sum to zero (third law), a doublet binds with probability
+    on / (on + off) (Philox), and a second call keeps the invariants"""
+    s = wl.condensate_ab()
+    say("bind full size")
+    e, nh = bind_engine(s, mdx.PREC_MIXED, 0.5, 0.5)
+    owner = np.repeat(np.arange(s["N"]), [s["patches"]["npatch"][t] for t in s["patches"]["types"]])
+    for draw in range(2):
+        F, T = e.bind_forces(draw=draw)
+        ib, bt = e.get_bindings()
+        b = np.nonzero(ib)[0]
+        assert np.all(ib[bt[b]] == 1) and np.all(bt[bt[b]] == b)
+        assert np.all(owner[b] != owner[bt[b]])
+        assert np.abs(F.sum(axis=0)).max() < 1e-6 * max(1.0, np.abs(F).max()) * np.sqrt(len(b) + 1)
+        say("  call", draw, "bound patches", len(b))
+    assert len(b) > 0
+
+
+def test_bind_component_capacity_is_an_error():
+    """cones of 86 degrees on a dense lattice join more than 64 patches into
+    one component: the call returns MDX_ERR_CAPACITY and the engine stays
+    usable"""
+    s = lattice_system(8, 1.5, 1.1, 51)
+    say("bind capacity")
+    e, nh = bind_engine(s, mdx.PREC_FP64)
+    with pytest.raises(mdx.MdxError) as ex:
+        e.bind_forces(draw=0)
+    assert ex.value.code == mdx.MDX_ERR_CAPACITY
+    F = e.compute_forces()[0]
+    assert np.all(np.isfinite(F))

@@ -1,0 +1,88 @@
+"""The oracle's one-bond-per-patch routine (orc_bind_forces) against the
+compiled reference's calculate_forces_and_torques3D_onlyone_nonlets
+(Rotational/LangevinRforce.cpp:2874-4136, BindingModelSingle) on the fixtures
+of tests/golden/make_bind_golden.py: bound sets and partners bit-exact, forces
+and torques to 1e-12, with the reference's own rand() draws."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+import make_bind_golden as g  # noqa: E402
+import pyoracle as po  # noqa: E402
+
+Z = np.load(os.path.join(HERE, "golden", "bind.npz"))
+AB, WIDE = g.systems()
+
+
+@pytest.mark.parametrize("case", ["ab0", "ab1", "wide0", "wide1"])
+def test_oracle_bind_matches_reference(case):
+    s = AB if case.startswith("ab") else WIDE
+    F, T, ib, bt, st = po.bind_forces(s, Z[case + "_isbound_in"], Z[case + "_boundto_in"], g.ON,
+                                      g.OFF, seed=int(Z[case + "_seed"]))
+    assert np.array_equal(ib, Z[case + "_isbound_out"])
+    assert np.array_equal(bt, Z[case + "_boundto_out"])
+    Fr, Tr = Z[case + "_F_out"], Z[case + "_T_out"]
+    assert np.abs(F - Fr).max() <= 1e-12 * max(1.0, np.abs(Fr).max())
+    assert np.abs(T - Tr).max() <= 1e-12 * max(1.0, np.abs(Tr).max())
+    if case.startswith("wide"):
+        assert st[2] > 0 and st[3] > 3   # components of 4+ patches were cut
+
+
+def test_oracle_bind_fixture_has_every_path():
+    """the wide-cone fixture reaches the large-component cut (sizes to 27),
+    breaks bonds and re-forms them"""
+    s = WIDE
+    _, _, ib, bt, st = po.bind_forces(s, Z["wide0_isbound_in"], Z["wide0_boundto_in"], g.ON,
+                                      g.OFF, seed=101)
+    assert st[2] > 100 and st[3] >= 10
+    assert (Z["wide1_isbound_in"] != Z["wide1_isbound_out"]).any()
+
+
+def bound_pairs(ib, bt):
+    return {(i, int(bt[i])) for i in np.nonzero(ib)[0]}
+
+
+@pytest.mark.parametrize("case", ["ab1", "wide1"])
+def test_oracle_bind_philox_consistent(case):
+    """Philox draws: every bond is mutual and is a candidate pair, each patch
+    holds at most one bond, and the same key and draw give the same result"""
+    s = AB if case.startswith("ab") else WIDE
+    key = (0x1234, 0xBEEF)
+    r1 = po.bind_forces(s, Z[case + "_isbound_in"], Z[case + "_boundto_in"], g.ON, g.OFF,
+                        key=key, draw=7)
+    r2 = po.bind_forces(s, Z[case + "_isbound_in"], Z[case + "_boundto_in"], g.ON, g.OFF,
+                        key=key, draw=7)
+    for a, b in zip(r1[:4], r2[:4]):
+        assert np.array_equal(a, b)
+    ib, bt = r1[2], r1[3]
+    for i, j in bound_pairs(ib, bt):
+        assert ib[j] and bt[j] == i and i != j
+    F = r1[0]
+    assert np.abs(F.sum(axis=0)).max() < 1e-9 * max(1.0, np.abs(F).max())
+
+
+def test_oracle_bind_uniform_is_philox():
+    """the per-component uniform is Philox4x32-10 (pinned by the Random123
+    known-answer vectors elsewhere) of {root, draw lo, draw hi, k}"""
+    key, draw = (3, 0xA5A5A5A5), (5 << 32) + 9
+    o = po.philox([17, draw & 0xFFFFFFFF, draw >> 32, 2], key)
+    want = ((int(o[0]) >> 5) * 67108864.0 + (int(o[1]) >> 6)) / 2.0 ** 53
+    assert po.bind_uniform(key, draw, 17, 2) == want
+    u = np.array([po.bind_uniform(key, 0, r, 0) for r in range(2000)])
+    assert 0.0 <= u.min() and u.max() < 1.0 and abs(u.mean() - 0.5) < 0.03
+
+
+def test_oracle_bind_on_off_limits():
+    """from an unbound start, on_rate 0 and off_rate 1 bind nothing, and
+    on_rate 1, off_rate 0 bind (BindingModelSingle's doublet and triplet
+    rules)"""
+    s = WIDE
+    ib0, bt0 = Z["wide0_isbound_in"], Z["wide0_boundto_in"]
+    _, _, ib, bt, st = po.bind_forces(s, ib0, bt0, 0.0, 1.0, key=(1, 2), draw=0)
+    assert ib.sum() == 0
+    _, _, ib, bt, st = po.bind_forces(s, ib0, bt0, 1.0, 0.0, key=(1, 2), draw=0)
+    assert st[4] > 0 and ib.sum() == 2 * st[4]
