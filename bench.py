@@ -53,7 +53,11 @@ RESORT_EVERY = int(os.environ.get("MDX_RESORT", "200"))
 #                  (2/256 of 96 B), type 4 + chain neighbours 8 = 109
 # The engine stores less (x 24, p 12, Q as a 16 B quaternion, L 12: 64 B per
 # rotor), so its own traffic can sit below these figures.
+#   binding (condensate_bind): the patchy figure plus the bound state, 8 B
+#                  per patch (isbound, boundto) read and written, 2.5 patches
+#                  per particle -> +40
 ALGO_BYTES = {"mixed": 196, "fp64": 292}
+BIND_BYTES = {"mixed": 236, "fp64": 332}
 BEAD_BYTES = {"mixed": 124, "fp64": 124}
 POLYMER_BYTES = {"mixed": 109, "fp64": 109}
 # kernels of one step and the phases of mdx_profile_step that time them
@@ -95,6 +99,10 @@ WORKLOADS = {
     "mt_bundle": ("416,000-bead microtubule bundle: 64 tubes x 13 protofilaments x 500 beads, "
                   "harmonic bonds k=100 r0=1, cosine bending kappa=50, WCA between non-bonded "
                   "beads, box 200x200x600, dt=0.002"),
+    "condensate_bind": ("the condensate_ab system under the one-bond-per-patch binding model: "
+                        "BindingModelSingle(on 0.99999, off 1e-5) (mainPatchy.cpp, "
+                        "run_singlebond), WCA plus the bound pairs' Kern-Frenkel terms, bound "
+                        "state updated every step"),
     "active_polymer": ("2,097,152-bead active polymer melt: 8192 chains x 256 beads, harmonic "
                        "k=200, cosine bending kappa=20, tangential f_a=1, patchy rotor end "
                        "beads (2 patches, eps_p=5), packing fraction 0.15"),
@@ -111,6 +119,8 @@ def build_workload(name):
         return reps[0], reps
     if name == "condensate_ab":
         s = wl.condensate_ab()
+    elif name == "condensate_bind":
+        s = wl.condensate_bind()
     elif name == "patchy_tetra":
         s = wl.patchy_tetra()
     elif name == "mt_bundle":
@@ -292,7 +302,7 @@ def main():
     s, reps = build_workload(args.workload)
     N = sum(r["N"] for r in reps) if reps else s["N"]
     rotor = s["integrator"] == "rotor"
-    algo = (ALGO_BYTES if rotor else
+    algo = (BIND_BYTES if s.get("binding") else ALGO_BYTES if rotor else
             POLYMER_BYTES if args.workload == "active_polymer" else BEAD_BYTES)
     log("timing mixed precision", args.steps, "steps after", args.warmup, "warm-up")
     t_mixed, prof, info = time_engine(s, mdx.PREC_MIXED, args.steps, args.warmup, reps=reps,
@@ -350,7 +360,10 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": ("one step: k_walkc + k_patch + k_integrate + list upkeep (k_bin, k_scan1, "
+            "kernel": ("one step: k_walkc (WCA) + k_bind_edges + k_bind_decide + k_bind_forces "
+                       "+ k_integrate + list upkeep (gated; amortised) + k_decide"
+                       if s.get("binding") else
+                       "one step: k_walkc + k_patch + k_integrate + list upkeep (k_bin, k_scan1, "
                        "k_scan3, k_fill, k_cellsort, k_nbuild, gated; amortised) + k_decide"
                        if "walk" in prof else
                        "one step: k_gather + list upkeep (gated; amortised) + k_decide"),

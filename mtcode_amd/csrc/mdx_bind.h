@@ -47,19 +47,24 @@ struct BindArgs {
     double on_rate, off_rate;
     uint32_t key0, key1;
     unsigned long long draw;
+    int draw_epoch; // 1: the draw counter is the force-evaluation count (steps)
     StepCtl *ctl;
 };
 
+// status bit 3: a patch with more than BIND_DEG candidates, or a component
+// of more than BIND_CMAX patches
 __device__ __forceinline__ void bind_status(const BindArgs &b, unsigned bit)
 {
     atomicOr(&b.ctl->status, bit);
+    atomicMin(&b.ctl->fail_epoch, b.ctl->epoch);
 }
 
 // 53-bit uniform in [0, 1) of Philox4x32-10 at {root, draw lo, draw hi, k}
 __device__ __forceinline__ double bind_uniform(const BindArgs &b, int root, int k)
 {
+    const unsigned long long draw = b.draw_epoch ? b.ctl->epoch : b.draw;
     const U4 o = philox4x32_10(
-        U4{(uint32_t)root, (uint32_t)b.draw, (uint32_t)(b.draw >> 32), (uint32_t)k}, b.key0, b.key1);
+        U4{(uint32_t)root, (uint32_t)draw, (uint32_t)(draw >> 32), (uint32_t)k}, b.key0, b.key1);
     return ((double)(o.a >> 5) * 67108864.0 + (double)(o.b >> 6)) * (1.0 / 9007199254740992.0);
 }
 
@@ -69,7 +74,7 @@ template <class R> __global__ void __launch_bounds__(256) k_bind_edges(KArgs<R> 
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.N)
         return;
-    const int i = a.list[s];
+    const int i = a.lmode ? s : a.list[s];
     const PosRec xi = a.X[i];
     const int ti = xi.meta & 255;
     const int npi = a.np[ti];
@@ -83,7 +88,8 @@ template <class R> __global__ void __launch_bounds__(256) k_bind_edges(KArgs<R> 
     R qi[9];
     load_orient(a.Q, (size_t)i, qi);
     bool over = false;
-    for_each_neighbor(a, rp, i, [&](const PosRec &xj, int j) {
+    // (the neighbour list in list steps, else the 27 cells)
+    for_each_near(a, rp, i, [&](const PosRec &xj, int j) {
         double d[3];
         min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
         const double r2s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
@@ -147,7 +153,7 @@ template <class R> __global__ void __launch_bounds__(256) k_bind_edges(KArgs<R> 
         }
     });
     if (over)
-        bind_status(b, 4u);
+        bind_status(b, 8u);
     for (int k = 0; k < npi; k++) {
         const size_t r0 = (size_t)(base + k) * BIND_DEG;
         for (int q = 1; q < deg[k]; q++) // insertion sort by partner
@@ -393,7 +399,7 @@ __device__ void bind_large(const BindArgs &b, const int *mem, int s, int root)
             const int ii1 = m[0], ii2 = m[1], ii3 = m[2];
             if (!bind_conn3(lens[ii1], lens[ii1] > 0 ? first_nb[ii1] : -1, lens[ii2], ii2, ii3,
                             c12, c23, c13)) {
-                bind_status(b, 4u);
+                bind_status(b, 8u);
                 return;
             }
             bind_decide3(b, mem[ii1], mem[ii2], mem[ii3], c12, c23, c13,
@@ -432,7 +438,7 @@ __global__ void __launch_bounds__(256) k_bind_decide(BindArgs b)
             if (seen)
                 continue;
             if (cnt == BIND_CMAX) {
-                bind_status(b, 4u);
+                bind_status(b, 8u);
                 return;
             }
             mem[cnt++] = v;
@@ -466,7 +472,7 @@ __global__ void __launch_bounds__(256) k_bind_decide(BindArgs b)
         const int i1 = mem[0], i2 = mem[1], i3 = mem[2];
         if (!bind_conn3(b.adj_n[i1], b.adj_p[(size_t)i1 * BIND_DEG], b.adj_n[i2], i2, i3, c12, c23,
                         c13)) {
-            bind_status(b, 4u);
+            bind_status(b, 8u);
             return;
         }
         bind_decide3(b, i1, i2, i3, c12, c23, c13, bind_uniform(b, p, 0));

@@ -328,7 +328,10 @@ template <class R> static KArgs<R> make_args(mdx_engine *e)
     a.wca_pl = e->wca_pl;
     a.wca_pl2 = e->wca_pl * e->wca_pl * (1 + 1e-12);
     a.nrep = e->nrep;
-    a.has_patches = e->ntypes > 0 && e->integ == MDX_INTEG_ROTOR && e->active_mask != 0;
+    // with binding the pair walk carries no Kern-Frenkel terms: the bound
+    // pairs' forces come from the binding kernels (Condensate.cpp:468-473)
+    a.has_patches =
+        e->ntypes > 0 && e->integ == MDX_INTEG_ROTOR && e->active_mask != 0 && !e->bind_on;
     for (int t = 0; t < MAXT; t++) {
         a.np[t] = t < e->ntypes ? e->np[t] : 0;
         a.pstart[t] = t < e->ntypes ? e->pstart[t] : 0;
@@ -413,6 +416,13 @@ template <class R> static KArgs<R> make_args(mdx_engine *e)
             all_active &= (e->active_mask >> (ta * MAXT + tc)) & 1ull;
     const double rw = e->wca ? e->wca_pl * (1 + 1e-12) + e->skin : 0.0;
     a.rl2w = all_active ? a.rl2 : (e->wca ? std::min(rw * rw, a.rl2) : 0.0);
+    a.bF = a.bT = nullptr;
+    if (e->bind_on) {
+        // every type pair can bind: one list radius (max_check + skin)
+        a.rl2w = a.rl2;
+        a.bF = (const double *)e->bF.p;
+        a.bT = (const double *)e->bT.p;
+    }
     a.skin = e->skin;
     a.dslot = (unsigned int *)e->dslot.p;
     return a;
@@ -468,8 +478,13 @@ template <class R> static int bin_current(mdx_engine *e)
 }
 
 template <class R, int MODE>
+static int launch_bind_step(mdx_engine *e, const double *noise_dev, int lmode);
+
+template <class R, int MODE>
 static int launch_gather(mdx_engine *e, const double *noise_dev, int lmode = 0)
 {
+    if (e->bind_on && (MODE == M_PRIME || MODE == M_STEP || MODE == M_FUSED))
+        return launch_bind_step<R, MODE>(e, noise_dev, lmode);
     KArgs<R> a = make_args<R>(e);
     a.noise = noise_dev;
     a.lmode = lmode;
@@ -603,6 +618,10 @@ static int status_to_rc(unsigned int st, long long step = -1)
         snprintf(where, sizeof where, " in the carried-force evaluation");
     else if (step > 0)
         snprintf(where, sizeof where, " at step %lld of the call (no particle moved after it)", step);
+    if (st & 8u)
+        return set_err(MDX_ERR_CAPACITY,
+                       "a patch has more than %d binding candidates or a candidate component more "
+                       "than %d patches%s", BIND_DEG, BIND_CMAX, where);
     if (st & 4u)
         return set_err(MDX_ERR_CAPACITY,
                        "a particle has more patch candidates than the neighbour list holds (kmax): "
@@ -752,9 +771,11 @@ template <class R, int MODE> static int list_step_graph(mdx_engine *e)
     // the step launches)
     const int form[7] = {e->ncells,      e->N,         e->use_split, e->use_compact,
                          e->use_walkc,   e->use_fuse,  e->use_tiles};
-    std::vector<char> sig(sizeof a + sizeof form);
+    const double rates[2] = {e->bind_on_rate, e->bind_off_rate};
+    std::vector<char> sig(sizeof a + sizeof form + sizeof rates);
     memcpy(sig.data(), &a, sizeof a);
     memcpy(sig.data() + sizeof a, form, sizeof form);
+    memcpy(sig.data() + sizeof a + sizeof form, rates, sizeof rates);
     GraphSlot &g = e->graphs[e->cur];
     if (!g.exec || g.sig != sig) {
         if (g.exec) {
@@ -1896,6 +1917,8 @@ int mdx_clusters(mdx_engine *e, int replica, const int n[3], double cut, int ex_
     return MDX_OK;
 }
 
+template <class R> static BindArgs bind_args(mdx_engine *e, unsigned long long draw);
+
 template <class R> static int compute_forces_t(mdx_engine *e, double *F, double *T)
 {
     int rc;
@@ -1909,6 +1932,23 @@ template <class R> static int compute_forces_t(mdx_engine *e, double *F, double 
         return rc;
     if (T && (rc = download(e, e->Tout, T, 3 * (size_t)e->N)))
         return rc;
+    if (e->bind_on) {
+        // with binding: the bound pairs' terms of the current bound state
+        // (no update), as the step adds them
+        hipLaunchKernelGGL(k_bind_forces<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream,
+                           make_args<R>(e), bind_args<R>(e, 0), (double *)e->bF.p,
+                           (double *)e->bT.p);
+        HIPCHK(hipGetLastError());
+        std::vector<double> fb(3 * (size_t)e->N), tb(3 * (size_t)e->N);
+        if ((rc = download(e, e->bF, fb.data(), fb.size())) ||
+            (rc = download(e, e->bT, tb.data(), tb.size())))
+            return rc;
+        for (size_t k = 0; k < fb.size(); k++) {
+            F[k] += fb[k];
+            if (T)
+                T[k] += tb[k];
+        }
+    }
     unsigned int st;
     if ((rc = read_status(e, &st)))
         return rc;
@@ -2035,11 +2075,14 @@ template <class R> static BindArgs bind_args(mdx_engine *e, unsigned long long d
     return b;
 }
 
-// the three binding kernels on the current state (cell list built)
-template <class R> static int launch_bind(mdx_engine *e, unsigned long long draw)
+// the three binding kernels on the current state (the cell list built, or
+// the neighbour list in list steps: a.lmode); draw_epoch: the draw counter
+// is the force-evaluation count held on the device
+template <class R>
+static int launch_bind(mdx_engine *e, const KArgs<R> &a, unsigned long long draw, int draw_epoch)
 {
-    KArgs<R> a = make_args<R>(e);
     BindArgs b = bind_args<R>(e, draw);
+    b.draw_epoch = draw_epoch;
     hipLaunchKernelGGL(k_bind_edges<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, a, b);
     hipLaunchKernelGGL(k_bind_decide, dim3(nblk(e->nh)), dim3(256), 0, e->stream, b);
     hipLaunchKernelGGL(k_bind_forces<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, a, b,
@@ -2048,32 +2091,61 @@ template <class R> static int launch_bind(mdx_engine *e, unsigned long long draw
     return MDX_OK;
 }
 
-static int bind_status_rc(unsigned int st)
-{
-    if (st & 4u)
-        return set_err(MDX_ERR_CAPACITY,
-                       "a patch has more than %d binding candidates or a candidate component more "
-                       "than %d patches", BIND_DEG, BIND_CMAX);
-    return status_to_rc(st);
-}
-
 template <class R> static int bind_forces_t(mdx_engine *e, unsigned long long draw, double *F,
                                             double *T)
 {
     int rc;
     if (!e->binned && (rc = bin_current<R>(e)))
         return rc;
-    if ((rc = launch_bind<R>(e, draw)))
+    if ((rc = launch_bind<R>(e, make_args<R>(e), draw, 0)))
         return rc;
     unsigned int st;
     if ((rc = read_status(e, &st)))
         return rc;
-    if ((rc = bind_status_rc(st)))
+    if ((rc = status_to_rc(st)))
         return rc;
     if (F && (rc = download(e, e->bF, F, 3 * (size_t)e->N)))
         return rc;
     if (T && (rc = download(e, e->bT, T, 3 * (size_t)e->N)))
         return rc;
+    return MDX_OK;
+}
+
+// a force evaluation of the step with binding (the loop of Condensate::
+// run_singlebond, Condensate.cpp:464-478): the pair walk (WCA, no patch
+// terms), the binding kernels at the same positions, then the integrator
+// kernel, which adds the bound pairs' terms (never the fused forms)
+template <class R, int MODE>
+static int launch_bind_step(mdx_engine *e, const double *noise_dev, int lmode)
+{
+    KArgs<R> a = make_args<R>(e);
+    a.noise = noise_dev;
+    a.lmode = lmode;
+    a.pcap = PMAXG;
+    int rc;
+    if ((rc = dalloc(e, e->Fw, 3 * e->rsz() * (size_t)e->N)) ||
+        (rc = dalloc(e, e->ncand, 4 * (size_t)e->N)) ||
+        (rc = dalloc(e, e->cand, 4 * (size_t)a.pcap * e->N)) ||
+        (rc = dalloc(e, e->Tp, 3 * e->rsz() * (size_t)e->N)))
+        return rc;
+    a.Fw = (R *)e->Fw.p;
+    a.ncand = (int *)e->ncand.p;
+    a.cand = (int *)e->cand.p;
+    a.Tp = (R *)e->Tp.p;
+    if (lmode && e->use_walkc)
+        hipLaunchKernelGGL((k_walkc<R, MODE, false>), dim3(nblk(e->N)), dim3(256), 0, e->stream,
+                           a);
+    else
+        hipLaunchKernelGGL((k_walk<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
+    if (e->prof_mid)
+        HIPCHK(hipEventRecord(e->prof_mid, e->stream));
+    e->last_split = true;
+    if ((rc = launch_bind<R>(e, a, 0, 1)))
+        return rc;
+    if (e->prof_mid2)
+        HIPCHK(hipEventRecord(e->prof_mid2, e->stream));
+    hipLaunchKernelGGL((k_integrate<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
+    HIPCHK(hipGetLastError());
     return MDX_OK;
 }
 

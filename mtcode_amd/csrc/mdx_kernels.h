@@ -38,7 +38,8 @@ struct DevRep {
 struct StepCtl {
     unsigned long long epoch;     // force evaluations so far (noise counter)
     unsigned long long call_base; // epoch at the start of the current call
-    unsigned int status;          // bit0 force guard, bit1 non-finite, bit2 candidate capacity
+    unsigned int status;          // bit0 force guard, bit1 non-finite, bit2 candidate capacity,
+                                  // bit3 binding capacity
     unsigned int pad;
     // neighbour list: rebuild flag, displacement bound since the last build
     unsigned int rebuild; // 1: the next step rebuilds the list before its walk
@@ -138,6 +139,9 @@ template <class R> struct KArgs {
     const double *noise; // injected rows (ncomp per particle, original order) or NULL
     StepCtl *ctl;
     double fguard;
+    // one-bond-per-patch binding: the bound-pair force and torque (lab) by
+    // original index, added by the integrator (null: no binding)
+    const double *bF, *bT;
 };
 
 // ------------------------------------------------------- orientation store
@@ -855,6 +859,13 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
             acc.F[k] = a.Fw[3 * (size_t)s + k];
             acc.T[k] = pt ? a.Tp[3 * (size_t)s + k] : (R)0;
         }
+        // the bound pairs' terms after the pair forces (the order of
+        // Condensate.cpp:471-473: F = WCA, then the binding routine adds)
+        if (a.bF)
+            for (int k = 0; k < 3; k++) {
+                acc.F[k] += (R)a.bF[3 * (size_t)oi + k];
+                acc.T[k] += (R)a.bT[3 * (size_t)oi + k];
+            }
     }
     if (PH == PH_INTEGRATE_F) {
         for (int k = 0; k < 3; k++) {

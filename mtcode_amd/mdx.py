@@ -443,6 +443,8 @@ def engine_from_system(s, precision=PREC_FP64, seed=0, device=0, replicas=None):
         if len(systems) > 1:
             for r, x in enumerate(systems):
                 e.set_replica_patch_params(r, x["patches"]["params"])
+        if s0.get("binding") is not None:
+            e.set_binding(*s0["binding"])
 
     def shifted(key, width):
         return np.concatenate([x[key] + offs[r] for r, x in enumerate(systems)]).reshape(-1, width)

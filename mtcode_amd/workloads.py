@@ -262,6 +262,16 @@ def condensate_ab(n: int = 1 << 20, phi: float = 0.05, seed_pos: int = 3, seed_q
     return s
 
 
+def condensate_bind(n: int = 1 << 20, phi: float = 0.05) -> dict:
+    """The two-species condensate under the one-bond-per-patch binding model
+    of the reference's patchy driver: BindingModelSingle(0.99999, 1 - 0.99999)
+    (mainPatchy.cpp:97-101, run_singlebond); all patches start unbound."""
+    s = condensate_ab(n, phi)
+    s["name"] = "condensate_bind"
+    s["binding"] = (0.99999, 1.0 - 0.99999)
+    return s
+
+
 def replicas(count: int = 256, seed: int = 5, nmin: int = 512, nmax: int = 16384,
              phi: float = 0.1, scale: float = 1.0, progress=None) -> list:
     """Independent tetrahedral systems with N log-uniform in [nmin, nmax],

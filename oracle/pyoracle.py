@@ -475,6 +475,17 @@ def sphere(s, p, L, Q, F, T, RT):
     return F, T
 
 
+def halfkick(s, p, L, F, T):
+    """LangevinNVTR::advancemom_halfstep (orc_halfkick); returns (p, L)"""
+    P = make_params(s)
+    p = np.array(p, dtype=np.float64)
+    L = np.array(L, dtype=np.float64)
+    keep = _Keep()
+    lib().orc_halfkick(C.byref(P), len(p), p.ctypes.data_as(_dp), L.ctypes.data_as(_dp),
+                       keep.d(F), keep.d(T))
+    return p, L
+
+
 def genfullmat(s, L3):
     P = make_params(s)
     up = np.zeros(9)

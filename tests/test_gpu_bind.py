@@ -139,3 +139,76 @@ def test_bind_component_capacity_is_an_error():
     assert ex.value.code == mdx.MDX_ERR_CAPACITY
     F = e.compute_forces()[0]
     assert np.all(np.isfinite(F))
+
+
+def oracle_bind_run(s, nsteps, noise, on, off, draws):
+    """the loop of Condensate::run_singlebond (Condensate.cpp:464-478) on the
+    oracle: half kick, drift, rotation, F = WCA + the binding routine's
+    bound-pair terms, the sphere's friction and noise, half kick"""
+    x, p, Q, L = (np.array(s[k], dtype=np.float64) for k in ("x", "p", "Q", "L"))
+    ps = s["patches"]
+    nh = int(sum(ps["npatch"][t] for t in ps["types"]))
+    ib, bt = np.zeros(nh, np.int32), np.zeros(nh, np.int32)
+    sw = dict(s, patches=None)
+
+    def forces(x, Q, ib, bt, draw):
+        F, T = po.compute_forces(sw, x=x, Q=Q)
+        Fb, Tb, ib, bt, _ = po.bind_forces(s, ib, bt, on, off, key=KEY, draw=draw, x=x, Q=Q)
+        return F + Fb, T + Tb, ib, bt
+
+    F, T, ib, bt = forces(x, Q, ib, bt, draws[0])
+    F, T = po.sphere(s, p, L, Q, F, T, noise[0])
+    for k in range(1, nsteps + 1):
+        p, L = po.halfkick(s, p, L, F, T)
+        x, p = po.advance_pos(s, x, p)
+        L, Q = po.rotate(s, L, Q)
+        F, T, ib, bt = forces(x, Q, ib, bt, draws[k])
+        F, T = po.sphere(s, p, L, Q, F, T, noise[k])
+        p, L = po.halfkick(s, p, L, F, T)
+    return {"x": x, "p": p, "Q": Q, "L": L}, ib, bt
+
+
+@pytest.mark.parametrize("skin", [0.0, 0.4])
+def test_bind_steps_fp64_vs_oracle(skin):
+    """20 steps of the binding loop with injected noise (cell walk, and the
+    neighbour list with its rebuilds): the bound state equal to the oracle's
+    and the state within 1e-10"""
+    s = wl.thermal_momenta(lattice_system(11, 1.0, 1.1, 61), seed=5)
+    N, nsteps = s["N"], 20
+    noise = po.noise(123, 0, (nsteps + 1) * N * 6).reshape(nsteps + 1, N, 6)
+    say("bind steps fp64 skin", skin)
+    e, nh = bind_engine(s, mdx.PREC_FP64, 0.7, 0.3)
+    e.set_neighbor_list(skin, 64, 200)
+    c0 = e.noise_counter()
+    e.prime(noise[0])
+    e.step(nsteps, noise[1:])
+    st = e.download()
+    ib, bt = e.get_bindings()
+    ref, ibo, bto = oracle_bind_run(s, nsteps, noise, 0.7, 0.3, [c0 + k for k in range(nsteps + 1)])
+    assert ibo.sum() > 0
+    assert np.array_equal(ib, ibo) and np.array_equal(bt, bto)
+    for k in ("x", "Q"):
+        assert np.max(np.abs(st[k] - ref[k])) < 1e-10, k
+    for k in ("p", "L"):
+        assert rel(st[k], ref[k]) < 1e-9, k
+
+
+def test_bind_steps_mixed_long():
+    """200 mixed-precision steps of the binding loop on 32,768 particles
+    (Philox noise, neighbour list): bonds stay mutual, the state finite, and
+    the bound fraction near the FP64 engine's"""
+    s = wl.thermal_momenta(lattice_system(32, 0.8, 1.1, 71), seed=6)
+    say("bind steps mixed 32k")
+    frac = {}
+    for prec in (mdx.PREC_MIXED, mdx.PREC_FP64):
+        e, nh = bind_engine(s, prec, 0.7, 0.3)
+        e.prime()
+        e.step(200)
+        st = e.download()
+        ib, bt = e.get_bindings()
+        b = np.nonzero(ib)[0]
+        assert np.all(ib[bt[b]] == 1) and np.all(bt[bt[b]] == b)
+        assert all(np.all(np.isfinite(v)) for v in st.values())
+        frac[prec] = len(b) / nh
+    say("  bound fraction", frac)
+    assert abs(frac[mdx.PREC_MIXED] - frac[mdx.PREC_FP64]) < 0.02
