@@ -6,13 +6,20 @@
 // Patches are numbered per particle in original-index order (poff[i] is the
 // number of the first patch of particle i, its patches follow in type order),
 // the numbering of GeneralPatch::which_patch (GeneralPatch.cpp:292-307) for
-// particles sorted by type.  Three kernels per call:
-//   k_bind_edges   one thread per particle: its candidate patch pairs within
-//                  the largest interaction distance (both cones, a distance
-//                  bound of 1, 0.7 or 0.5 x that distance by bound state, a
-//                  score), each appended to the row of the thread's own
-//                  patch -- no atomics, both particles of a pair compute the
-//                  same FP64 values -- and each row sorted by partner;
+// particles sorted by type.  Four kernels per call (after a clear of the
+// per-patch candidate counts):
+//   k_bind_screen  one thread per particle: the neighbours within the largest
+//                  interaction distance whose patches pass an FP32 cone
+//                  screen on both sides (a superset of the candidates) --
+//                  in the step the pair walk collects them instead (its own
+//                  side's screen);
+//   k_bind_edges   one thread per particle with screened neighbours: its
+//                  candidate patch pairs (both cones, a distance bound of 1,
+//                  0.7 or 0.5 x the interaction distance by bound state, a
+//                  score) decided in FP64, each appended to the row of the
+//                  thread's own patch -- no atomics, both particles of a pair
+//                  compute the same FP64 values -- and each row sorted by
+//                  partner;
 //   k_bind_decide  one thread per patch: a search of its component of the
 //                  candidate graph that stops at the first smaller patch, so
 //                  only the component's smallest patch goes on; it breaks
@@ -44,6 +51,7 @@ struct BindArgs {
     double *adj_s;    // their scores
     int nh;
     double mk2;       // squared largest interaction distance (max_check)
+    double cmin;      // smallest cos(theta_m) of any row (the cone screen)
     double on_rate, off_rate;
     uint32_t key0, key1;
     unsigned long long draw;
@@ -68,28 +76,37 @@ __device__ __forceinline__ double bind_uniform(const BindArgs &b, int root, int 
     return ((double)(o.a >> 5) * 67108864.0 + (double)(o.b >> 6)) * (1.0 / 9007199254740992.0);
 }
 
-// candidate patch pairs (LangevinRforce.cpp:2914-3075)
-template <class R> __global__ void __launch_bounds__(256) k_bind_edges(KArgs<R> a, BindArgs b)
+// the patches of a particle (orientation q, type t) whose cone value
+// sign * (n . u) passes the FP32 screen: n . u = nb . (Q u), against the
+// smallest cos(theta_m) of any row less 1e-5 (well above the FP32 error of
+// these unit-vector products), so a patch inside its own row's cone always
+// passes; sign -1 for the particle in role "a" (its value is -(n . u))
+template <class R>
+__device__ __forceinline__ uint32_t bind_cone_mask(const KArgs<R> &a, const BindArgs &b,
+                                                   const R *q, int t, const double u[3], float sign)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= a.N)
-        return;
-    const int i = a.lmode ? s : a.list[s];
-    const PosRec xi = a.X[i];
-    const int ti = xi.meta & 255;
-    const int npi = a.np[ti];
-    if (npi == 0)
-        return;
-    const int base = b.poff[xi.id];
-    int deg[MAXP];
-    for (int k = 0; k < npi; k++)
-        deg[k] = 0;
-    const DevRep rp = rep_rec(a, xi.meta);
-    R qi[9];
-    load_orient(a.Q, (size_t)i, qi);
-    bool over = false;
-    // (the neighbour list in list steps, else the 27 cells)
-    for_each_near(a, rp, i, [&](const PosRec &xj, int j) {
+    const float uf[3] = {(float)u[0], (float)u[1], (float)u[2]};
+    float v[3];
+    for (int r = 0; r < 3; r++)
+        v[r] = sign * ((float)q[3 * r] * uf[0] + (float)q[3 * r + 1] * uf[1] +
+                       (float)q[3 * r + 2] * uf[2]);
+    const float *f = a.pdirs_f + 3 * a.pstart[t];
+    const float scr_f = (float)(b.cmin - 1e-5);
+    uint32_t m = 0;
+    for (int k = 0; k < a.np[t]; k++)
+        if (f[3 * k] * v[0] + f[3 * k + 1] * v[1] + f[3 * k + 2] * v[2] > scr_f)
+            m |= 1u << k;
+    return m;
+}
+
+// the neighbours of slot i within the binding radius whose patches pass
+// the cone screen on both sides (bind_cone_mask): fn(xj, j) for each
+template <class R, class Fn>
+__device__ __forceinline__ void bind_screened(const KArgs<R> &a, const BindArgs &b, const DevRep &rp,
+                                              int i, const PosRec &xi, int ti, const R *qi,
+                                              Fn &&fn)
+{
+    auto visit = [&](const PosRec &xj, int j) {
         double d[3];
         min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
         const double r2s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
@@ -105,27 +122,128 @@ template <class R> __global__ void __launch_bounds__(256) k_bind_edges(KArgs<R> 
         const double u[3] = {sg * d[0] / dis, sg * d[1] / dis, sg * d[2] / dis};
         R qj[9];
         load_orient(a.Q, (size_t)j, qj);
+        if (bind_cone_mask(a, b, i_is_a ? qi : qj, ta, u, -1.0f) &&
+            bind_cone_mask(a, b, i_is_a ? qj : qi, tc, u, 1.0f))
+            fn(xj, j);
+    };
+    // the neighbour list in list steps (its loads batched), else the 27 cells
+    for_each_near(a, rp, i, visit);
+}
+
+constexpr int BIND_SCR = 8; // screened neighbours held per particle
+
+// the candidate counts of every patch cleared (coalesced), before the edges
+__global__ void __launch_bounds__(256) k_bind_clear(int *adj_n, int nh)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < nh)
+        adj_n[p] = 0;
+}
+
+// pass 1 (one call on a frozen state): the screened neighbours of each
+// cell-list position s into scr[q * N + s], q < BIND_SCR, their count in
+// nscr[s] (beyond BIND_SCR the edge kernel screens the neighbours again)
+template <class R>
+__global__ void __launch_bounds__(256) k_bind_screen(KArgs<R> a, BindArgs b, int *scr, int *nscr)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.N)
+        return;
+    const int i = a.lmode ? s : a.list[s];
+    const PosRec xi = a.X[i];
+    const int ti = xi.meta & 255;
+    const int npi = a.np[ti];
+    int cnt = 0;
+    if (npi > 0) {
+        const DevRep rp = rep_rec(a, xi.meta);
+        R qi[9];
+        load_orient(a.Q, (size_t)i, qi);
+        bind_screened(a, b, rp, i, xi, ti, qi, [&](const PosRec &, int j) {
+            if (cnt < BIND_SCR)
+                scr[(size_t)cnt * a.N + s] = j;
+            cnt++;
+        });
+    }
+    nscr[s] = cnt;
+}
+
+// pass 2, candidate patch pairs (LangevinRforce.cpp:2914-3075): for each
+// screened neighbour the FP64 cone values of the patches, decided on the
+// row's own threshold, the interaction distance and then the bound-state
+// distance (which never exceeds it) -- the reference's decision -- with the
+// score; each candidate appended to the row of the thread's own patch (no
+// atomics: both particles of a pair compute the same FP64 values), rows
+// sorted by partner.  Per-patch counts are packed 4 bits each.
+// scr, nscr: the screened neighbours (slots) of each position and their
+// count, cap of them held (the screen kernel's, or the candidates the pair
+// walk collected in steps, per cell-list position); beyond cap the thread
+// screens its neighbours again itself
+template <class R>
+__global__ void __launch_bounds__(256) k_bind_edges(KArgs<R> a, BindArgs b, const int *scr,
+                                                    const int *nscr, int cap)
+{
+    static_assert(MAXP * 4 <= 32 && BIND_DEG < 16, "packed candidate counts");
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.N)
+        return;
+    const int cnt = nscr[s];
+    if (cnt == 0)
+        return;
+    const int i = a.lmode ? s : a.list[s];
+    const PosRec xi = a.X[i];
+    const int ti = xi.meta & 255;
+    const int npi = a.np[ti];
+    const int base = b.poff[xi.id];
+    uint32_t degs = 0; // candidates of own patch k in bits 4k..4k+3
+    const DevRep rp = rep_rec(a, xi.meta);
+    R qi[9];
+    load_orient(a.Q, (size_t)i, qi);
+    bool over = false;
+    auto exact = [&](const PosRec &xj, int j) {
+        double d[3];
+        min_image(a, rp, xi.x, xi.y, xi.z, xj, d);
+        const double r2s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        const int tj = xj.meta & 255;
+        int ta, tc;
+        const bool i_is_a = patch_roles(xi.id, ti, xj.id, tj, ta, tc);
+        const double dis = sqrt(r2s);
+        const double sg = i_is_a ? 1.0 : -1.0;
+        const double u[3] = {sg * d[0] / dis, sg * d[1] / dis, sg * d[2] / dis};
+        R qj[9];
+        load_orient(a.Q, (size_t)j, qj);
         const R *qa = i_is_a ? qi : qj;
         const R *qc = i_is_a ? qj : qi;
-        const int ida = i_is_a ? xi.id : xj.id, idc = i_is_a ? xj.id : xi.id;
-        const int pa = b.poff[ida], pc = b.poff[idc];
         const int npa = a.np[ta], npc = a.np[tc];
+        // the FP32 screen of both sides first (as in bind_screened)
+        const uint32_t ma = bind_cone_mask(a, b, qa, ta, u, -1.0f);
+        if (!ma)
+            return;
+        const uint32_t mc = bind_cone_mask(a, b, qc, tc, u, 1.0f);
+        if (!mc)
+            return;
         const double *da = a.pdirs + 3 * a.pstart[ta];
         const double *dc = a.pdirs + 3 * a.pstart[tc];
         const int rowbase = rp.prow + a.rstart[ta * MAXT + tc];
-        for (int ia = 0; ia < npa; ia++) {
+        const int pj = b.poff[xj.id];
+        for (uint32_t m1 = ma; m1; m1 &= m1 - 1) {
+            const int ia = __builtin_ctz(m1);
             double n1[3];
             lab_normal_exact(qa, da + 3 * ia, n1);
             const double argthetai = -(n1[0] * u[0] + n1[1] * u[1] + n1[2] * u[2]);
-            for (int ic = 0; ic < npc; ic++) {
-                const PatchRow pr = a.prows[rowbase + ia * npc + ic];
+            if (!(argthetai > b.cmin))
+                continue;
+            for (uint32_t m2 = mc; m2; m2 &= m2 - 1) {
+                const int ic = __builtin_ctz(m2);
                 double n2[3];
                 lab_normal_exact(qc, dc + 3 * ic, n2);
                 const double argthetaj = (n2[0] * u[0] + n2[1] * u[1] + n2[2] * u[2]);
-                const int wp1 = pa + ia, wp2 = pc + ic;
+                const PatchRow &pr = a.prows[rowbase + ia * npc + ic];
+                const double ctm = pr.costm, disp = pr.idist;
+                if (!(argthetai > ctm && argthetaj > ctm && dis < disp))
+                    continue;
+                const int wp1 = (i_is_a ? base : pj) + ia, wp2 = (i_is_a ? pj : base) + ic;
                 const bool cond1 = b.boundto[wp1] == wp2 && b.boundto[wp2] == wp1;
                 const bool b1 = b.isbound[wp1] != 0, b2 = b.isbound[wp2] != 0;
-                const double disp = pr.idist;
                 double disp2;
                 if (b1 && b2 && cond1)
                     disp2 = disp;
@@ -135,28 +253,38 @@ template <class R> __global__ void __launch_bounds__(256) k_bind_edges(KArgs<R> 
                     disp2 = 0.7 * disp;
                 else
                     disp2 = disp;
-                if (argthetai > pr.costm && argthetaj > pr.costm && dis < disp2) {
-                    const double scr = (1 - (argthetai - pr.costm)) + (1 - (argthetaj - pr.costm)) +
-                                       2 * (dis / disp2);
-                    const int own = i_is_a ? ia : ic;
-                    const int partner = i_is_a ? wp2 : wp1;
-                    if (deg[own] < BIND_DEG) {
-                        const size_t at = (size_t)(base + own) * BIND_DEG + deg[own];
-                        b.adj_p[at] = partner;
-                        b.adj_s[at] = scr;
-                        deg[own]++;
-                    } else {
-                        over = true;
-                    }
+                if (!(dis < disp2))
+                    continue;
+                const double scr_v =
+                    (1 - (argthetai - ctm)) + (1 - (argthetaj - ctm)) + 2 * (dis / disp2);
+                const int own = i_is_a ? ia : ic;
+                const int partner = i_is_a ? wp2 : wp1;
+                const int dg = (degs >> (4 * own)) & 15u;
+                if (dg < BIND_DEG) {
+                    const size_t at = (size_t)(base + own) * BIND_DEG + dg;
+                    b.adj_p[at] = partner;
+                    b.adj_s[at] = scr_v;
+                    degs += 1u << (4 * own);
+                } else {
+                    over = true;
                 }
             }
         }
-    });
+    };
+    if (cnt <= cap) {
+        for (int q = 0; q < cnt; q++) {
+            const int j = scr[(size_t)q * a.N + s];
+            exact(a.X[j], j);
+        }
+    } else {
+        bind_screened(a, b, rp, i, xi, ti, qi, exact);
+    }
     if (over)
         bind_status(b, 8u);
     for (int k = 0; k < npi; k++) {
+        const int dg = (degs >> (4 * k)) & 15u;
         const size_t r0 = (size_t)(base + k) * BIND_DEG;
-        for (int q = 1; q < deg[k]; q++) // insertion sort by partner
+        for (int q = 1; q < dg; q++) // insertion sort by partner
             for (int t = q; t > 0 && b.adj_p[r0 + t - 1] > b.adj_p[r0 + t]; t--) {
                 const int tp = b.adj_p[r0 + t];
                 b.adj_p[r0 + t] = b.adj_p[r0 + t - 1];
@@ -165,7 +293,8 @@ template <class R> __global__ void __launch_bounds__(256) k_bind_edges(KArgs<R> 
                 b.adj_s[r0 + t] = b.adj_s[r0 + t - 1];
                 b.adj_s[r0 + t - 1] = ts;
             }
-        b.adj_n[base + k] = deg[k];
+        if (dg)
+            b.adj_n[base + k] = dg;
     }
 }
 
@@ -483,19 +612,29 @@ __global__ void __launch_bounds__(256) k_bind_decide(BindArgs b)
 
 // forces and torques of the bound pairs (LangevinRforce.cpp:4030-4095,
 // KernFrenkelOnePatch2::force_and_torque, potentialtheta.h:517-750), each
-// particle its own share, written to Fb, Tb (3 per original index, lab frame)
+// particle its own share, written to Fb, Tb (3 per slot, or per original
+// index with by_id; lab frame)
+// ncand (per cell-list position, nullable): a particle without candidate
+// neighbours has no bound patch after the decisions (an edge is seen from
+// both of its particles): by slot (the step's form) nothing is written for
+// it -- the integrator reads the terms only where ncand > 0 -- and by
+// original index zeros
 template <class R>
-__global__ void __launch_bounds__(256) k_bind_forces(KArgs<R> a, BindArgs b, double *Fb, double *Tb)
+__global__ void __launch_bounds__(256) k_bind_forces(KArgs<R> a, BindArgs b, double *Fb, double *Tb,
+                                                     int by_id, const int *ncand)
 {
     constexpr bool FP64 = sizeof(R) == 8;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.N)
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.N)
         return;
+    if (!by_id && ncand && ncand[s] == 0)
+        return;
+    const int i = a.lmode ? s : a.list[s];
     const PosRec xi = a.X[i];
     const int ti = xi.meta & 255;
     const int npi = a.np[ti];
     double F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
-    if (npi > 0) {
+    if (npi > 0 && (!ncand || ncand[s] > 0)) {
         const DevRep rp = rep_rec(a, xi.meta);
         const int base = b.poff[xi.id];
         R qi[9];
@@ -549,9 +688,10 @@ __global__ void __launch_bounds__(256) k_bind_forces(KArgs<R> a, BindArgs b, dou
             }
         }
     }
+    const size_t o = 3 * (size_t)(by_id ? (int)xi.id : i);
     for (int q = 0; q < 3; q++) {
-        Fb[3 * (size_t)xi.id + q] = F[q];
-        Tb[3 * (size_t)xi.id + q] = T[q];
+        Fb[o + q] = F[q];
+        Tb[o + q] = T[q];
     }
 }
 

@@ -156,9 +156,9 @@ struct mdx_engine {
     // one-bond-per-patch binding (mdx_set_binding): patch numbering, bound
     // state, candidate rows, the bound-pair forces by original index
     int bind_on = 0;
-    double bind_on_rate = 0, bind_off_rate = 0, bind_mk = 0;
+    double bind_on_rate = 0, bind_off_rate = 0, bind_mk = 0, bind_cmin = 1;
     int nh = 0;
-    Buf bpoff, bowner, bisbound, bboundto, badj_n, badj_p, badj_s, bF, bT;
+    Buf bpoff, bowner, bisbound, bboundto, badj_n, badj_p, badj_s, bF, bT, bscr, bnscr;
 
     size_t rsz() const { return prec == MDX_PREC_FP64 ? 8 : 4; }
 };
@@ -416,13 +416,10 @@ template <class R> static KArgs<R> make_args(mdx_engine *e)
             all_active &= (e->active_mask >> (ta * MAXT + tc)) & 1ull;
     const double rw = e->wca ? e->wca_pl * (1 + 1e-12) + e->skin : 0.0;
     a.rl2w = all_active ? a.rl2 : (e->wca ? std::min(rw * rw, a.rl2) : 0.0);
-    a.bF = a.bT = nullptr;
-    if (e->bind_on) {
-        // every type pair can bind: one list radius (max_check + skin)
-        a.rl2w = a.rl2;
-        a.bF = (const double *)e->bF.p;
-        a.bT = (const double *)e->bT.p;
-    }
+    a.bF = a.bT = nullptr; // (set by the binding step's launch)
+    a.bncand = nullptr;
+    if (e->bind_on)
+        a.rl2w = a.rl2; // every type pair can bind: one list radius (max_check + skin)
     a.skin = e->skin;
     a.dslot = (unsigned int *)e->dslot.p;
     return a;
@@ -1172,7 +1169,8 @@ void mdx_destroy(mdx_engine *e)
                    &e->tmpi2,   &e->tmpi3,   &e->tmpll,  &e->tmpx,    &e->Fw,
                    &e->ncand,   &e->cand,    &e->nbr,    &e->nbrc,   &e->nn,     &e->bcell,
                    &e->dslot,   &e->Tp,      &e->bpoff,  &e->bowner, &e->bisbound, &e->bboundto,
-                   &e->badj_n,  &e->badj_p,  &e->badj_s, &e->bF,     &e->bT};
+                   &e->badj_n,  &e->badj_p,  &e->badj_s, &e->bF,     &e->bT,      &e->bscr,
+                   &e->bnscr};
     for (Buf *b : bufs)
         dfree(*b);
     for (GraphSlot &g : e->graphs)
@@ -1461,6 +1459,8 @@ int mdx_set_patches(mdx_engine *e, int ntypes, const int *type_of_particle,
     return patch_rows_upload(e);
 }
 
+static void bind_limits(const mdx_engine *e, double &mk, double &cmin);
+
 int mdx_set_replica_patch_params(mdx_engine *e, int replica, const double *params)
 {
     if (!e || replica < 0 || replica >= e->nrep || !params || e->ntypes == 0)
@@ -1470,6 +1470,15 @@ int mdx_set_replica_patch_params(mdx_engine *e, int replica, const double *param
             return set_err(MDX_ERR_ARG, "patch row %d: d and theta_m must be > 0", k);
     std::copy(params, params + 3 * e->nrows, e->prow_raw.begin() + (size_t)replica * 3 * e->nrows);
     HIPCHK(hipSetDevice(e->device));
+    if (e->bind_on) {
+        const double mk0 = e->bind_mk;
+        bind_limits(e, e->bind_mk, e->bind_cmin);
+        if (e->bind_mk != mk0) {
+            e->grid_dirty = true;
+            e->binned = false;
+            e->list_valid = false;
+        }
+    }
     return patch_rows_upload(e);
 }
 
@@ -1937,7 +1946,7 @@ template <class R> static int compute_forces_t(mdx_engine *e, double *F, double 
         // (no update), as the step adds them
         hipLaunchKernelGGL(k_bind_forces<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream,
                            make_args<R>(e), bind_args<R>(e, 0), (double *)e->bF.p,
-                           (double *)e->bT.p);
+                           (double *)e->bT.p, 1, (const int *)nullptr);
         HIPCHK(hipGetLastError());
         std::vector<double> fb(3 * (size_t)e->N), tb(3 * (size_t)e->N);
         if ((rc = download(e, e->bF, fb.data(), fb.size())) ||
@@ -1966,6 +1975,19 @@ int mdx_compute_forces(mdx_engine *e, double *F, double *T)
 
 // ------------------------------------------------------------ binding
 
+// ComboPatch::max_check (combopatch.h:15, GeneralPatch.cpp:93-94): 1.0
+// raised to the largest 1.4 d of any row (potentialtheta.h:417); and the
+// smallest cos(theta_m) of any row (the binding kernel's cone screen)
+static void bind_limits(const mdx_engine *e, double &mk, double &cmin)
+{
+    mk = 1.0;
+    cmin = 1.0;
+    for (size_t k = 0; k < e->prow_raw.size() / 3; k++) {
+        mk = std::max(mk, 1.4 * e->prow_raw[3 * k + 1]);
+        cmin = std::min(cmin, std::cos(e->prow_raw[3 * k + 2]));
+    }
+}
+
 int mdx_set_binding(mdx_engine *e, double on_rate, double off_rate, int *npatches)
 {
     if (!e)
@@ -1985,11 +2007,8 @@ int mdx_set_binding(mdx_engine *e, double on_rate, double off_rate, int *npatche
     for (int i = 0; i < e->N; i++)
         for (int k = poff[i]; k < poff[i + 1]; k++)
             owner[k] = i;
-    // ComboPatch::max_check (combopatch.h:15, GeneralPatch.cpp:93-94): 1.0
-    // raised to the largest 1.4 d of any row (potentialtheta.h:417)
-    double mk = 1.0;
-    for (size_t k = 0; k < e->prow_raw.size() / 3; k++)
-        mk = std::max(mk, 1.4 * e->prow_raw[3 * k + 1]);
+    double mk, cmin;
+    bind_limits(e, mk, cmin);
     int rc;
     if ((rc = upload(e, e->bpoff, poff.data(), poff.size())) ||
         (rc = upload(e, e->bowner, owner.data(), owner.size())) ||
@@ -1998,7 +2017,9 @@ int mdx_set_binding(mdx_engine *e, double on_rate, double off_rate, int *npatche
         (rc = dalloc(e, e->badj_n, 4 * (size_t)nh)) ||
         (rc = dalloc(e, e->badj_p, 4 * (size_t)nh * BIND_DEG)) ||
         (rc = dalloc(e, e->badj_s, 8 * (size_t)nh * BIND_DEG)) ||
-        (rc = dalloc(e, e->bF, 24 * (size_t)e->N)) || (rc = dalloc(e, e->bT, 24 * (size_t)e->N)))
+        (rc = dalloc(e, e->bF, 24 * (size_t)e->N)) || (rc = dalloc(e, e->bT, 24 * (size_t)e->N)) ||
+        (rc = dalloc(e, e->bscr, 4 * (size_t)BIND_SCR * e->N)) ||
+        (rc = dalloc(e, e->bnscr, 4 * (size_t)e->N)))
         return rc;
     HIPCHK(hipMemsetAsync(e->bisbound.p, 0, 4 * (size_t)nh, e->stream));
     HIPCHK(hipMemsetAsync(e->bboundto.p, 0, 4 * (size_t)nh, e->stream));
@@ -2012,6 +2033,7 @@ int mdx_set_binding(mdx_engine *e, double on_rate, double off_rate, int *npatche
     e->bind_on_rate = on_rate;
     e->bind_off_rate = off_rate;
     e->bind_mk = mk;
+    e->bind_cmin = cmin;
     e->nh = nh;
     if (npatches)
         *npatches = nh;
@@ -2064,6 +2086,7 @@ template <class R> static BindArgs bind_args(mdx_engine *e, unsigned long long d
     b.adj_s = (double *)e->badj_s.p;
     b.nh = e->nh;
     b.mk2 = e->bind_mk * e->bind_mk;
+    b.cmin = e->bind_cmin;
     b.on_rate = e->bind_on_rate;
     b.off_rate = e->bind_off_rate;
     // the noise key with its second word changed: binding draws and the
@@ -2075,18 +2098,30 @@ template <class R> static BindArgs bind_args(mdx_engine *e, unsigned long long d
     return b;
 }
 
-// the three binding kernels on the current state (the cell list built, or
+// the four binding kernels on the current state (the cell list built, or
 // the neighbour list in list steps: a.lmode); draw_epoch: the draw counter
 // is the force-evaluation count held on the device
+// (cand, ncand, cap: the candidates the pair walk collected; null: the
+// screen kernel collects them)
 template <class R>
-static int launch_bind(mdx_engine *e, const KArgs<R> &a, unsigned long long draw, int draw_epoch)
+static int launch_bind(mdx_engine *e, const KArgs<R> &a, unsigned long long draw, int draw_epoch,
+                       const int *cand = nullptr, const int *ncand = nullptr, int cap = 0)
 {
     BindArgs b = bind_args<R>(e, draw);
     b.draw_epoch = draw_epoch;
-    hipLaunchKernelGGL(k_bind_edges<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, a, b);
+    hipLaunchKernelGGL(k_bind_clear, dim3(nblk(e->nh)), dim3(256), 0, e->stream, b.adj_n, e->nh);
+    if (!cand) {
+        hipLaunchKernelGGL(k_bind_screen<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, a, b,
+                           (int *)e->bscr.p, (int *)e->bnscr.p);
+        cand = (const int *)e->bscr.p;
+        ncand = (const int *)e->bnscr.p;
+        cap = BIND_SCR;
+    }
+    hipLaunchKernelGGL(k_bind_edges<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, a, b, cand,
+                       ncand, cap);
     hipLaunchKernelGGL(k_bind_decide, dim3(nblk(e->nh)), dim3(256), 0, e->stream, b);
     hipLaunchKernelGGL(k_bind_forces<R>, dim3(nblk(e->N)), dim3(256), 0, e->stream, a, b,
-                       (double *)e->bF.p, (double *)e->bT.p);
+                       (double *)e->bF.p, (double *)e->bT.p, draw_epoch ? 0 : 1, ncand);
     HIPCHK(hipGetLastError());
     return MDX_OK;
 }
@@ -2121,7 +2156,8 @@ static int launch_bind_step(mdx_engine *e, const double *noise_dev, int lmode)
     KArgs<R> a = make_args<R>(e);
     a.noise = noise_dev;
     a.lmode = lmode;
-    a.pcap = PMAXG;
+    const bool list_walk = lmode && e->use_walkc;
+    a.pcap = list_walk ? std::max(PMAXG, e->kmax) : PMAXG;
     int rc;
     if ((rc = dalloc(e, e->Fw, 3 * e->rsz() * (size_t)e->N)) ||
         (rc = dalloc(e, e->ncand, 4 * (size_t)e->N)) ||
@@ -2132,18 +2168,39 @@ static int launch_bind_step(mdx_engine *e, const double *noise_dev, int lmode)
     a.ncand = (int *)e->ncand.p;
     a.cand = (int *)e->cand.p;
     a.Tp = (R *)e->Tp.p;
-    if (lmode && e->use_walkc)
+    // the walk computes WCA and collects the binding candidates: neighbours
+    // within max_check of any patch-carrying type pair whose own patches
+    // pass an FP32 cone screen at the smallest cos(theta_m) less 1e-5 (the
+    // binding kernel decides them exactly); no Kern-Frenkel terms
+    KArgs<R> w = a;
+    w.has_patches = 1;
+    w.maxd2 = e->bind_mk * e->bind_mk;
+    w.rmax2 = std::max(a.rmax2, w.maxd2);
+    w.costm_screen_f = e->bind_cmin - 1e-5;
+    w.active_mask = 0;
+    for (int ta = 0; ta < e->ntypes; ta++)
+        for (int tc = ta; tc < e->ntypes; tc++)
+            if (e->np[ta] > 0 && e->np[tc] > 0)
+                w.active_mask |= 1ull << (ta * MAXT + tc);
+    w.ovf_fatal = 1;
+    if (list_walk)
         hipLaunchKernelGGL((k_walkc<R, MODE, false>), dim3(nblk(e->N)), dim3(256), 0, e->stream,
-                           a);
+                           w);
     else
-        hipLaunchKernelGGL((k_walk<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
+        hipLaunchKernelGGL((k_walk<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, w);
     if (e->prof_mid)
         HIPCHK(hipEventRecord(e->prof_mid, e->stream));
     e->last_split = true;
-    if ((rc = launch_bind<R>(e, a, 0, 1)))
+    if ((rc = launch_bind<R>(e, a, 0, 1, (const int *)e->cand.p, (const int *)e->ncand.p,
+                             a.pcap)))
         return rc;
     if (e->prof_mid2)
         HIPCHK(hipEventRecord(e->prof_mid2, e->stream));
+    // the integrator adds the bound pairs' terms where the walk found
+    // binding candidates
+    a.bF = (const double *)e->bF.p;
+    a.bT = (const double *)e->bT.p;
+    a.bncand = (const int *)e->ncand.p;
     hipLaunchKernelGGL((k_integrate<R, MODE>), dim3(nblk(e->N)), dim3(256), 0, e->stream, a);
     HIPCHK(hipGetLastError());
     return MDX_OK;

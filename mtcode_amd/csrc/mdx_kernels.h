@@ -140,8 +140,10 @@ template <class R> struct KArgs {
     StepCtl *ctl;
     double fguard;
     // one-bond-per-patch binding: the bound-pair force and torque (lab) by
-    // original index, added by the integrator (null: no binding)
+    // slot, added by the integrator where the position's binding candidate
+    // count (bncand) is non-zero (null: no binding)
     const double *bF, *bT;
+    const int *bncand;
 };
 
 // ------------------------------------------------------- orientation store
@@ -861,10 +863,10 @@ __device__ __forceinline__ void process_particle(const KArgs<R> &a, int s, int *
         }
         // the bound pairs' terms after the pair forces (the order of
         // Condensate.cpp:471-473: F = WCA, then the binding routine adds)
-        if (a.bF)
+        if (a.bF && a.bncand[s] > 0)
             for (int k = 0; k < 3; k++) {
-                acc.F[k] += (R)a.bF[3 * (size_t)oi + k];
-                acc.T[k] += (R)a.bT[3 * (size_t)oi + k];
+                acc.F[k] += (R)a.bF[3 * (size_t)i + k];
+                acc.T[k] += (R)a.bT[3 * (size_t)i + k];
             }
     }
     if (PH == PH_INTEGRATE_F) {
