@@ -179,16 +179,10 @@ __global__ void __launch_bounds__(256) k_bind_screen(KArgs<R> a, BindArgs b, int
 // walk collected in steps, per cell-list position); beyond cap the thread
 // screens its neighbours again itself
 template <class R>
-__global__ void __launch_bounds__(256) k_bind_edges(KArgs<R> a, BindArgs b, const int *scr,
-                                                    const int *nscr, int cap)
+__device__ __forceinline__ void bind_edges_one(const KArgs<R> &a, const BindArgs &b, const int *scr,
+                                               int cnt, int cap, int s)
 {
     static_assert(MAXP * 4 <= 32 && BIND_DEG < 16, "packed candidate counts");
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= a.N)
-        return;
-    const int cnt = nscr[s];
-    if (cnt == 0)
-        return;
     const int i = a.lmode ? s : a.list[s];
     const PosRec xi = a.X[i];
     const int ti = xi.meta & 255;
@@ -296,6 +290,37 @@ __global__ void __launch_bounds__(256) k_bind_edges(KArgs<R> a, BindArgs b, cons
         if (dg)
             b.adj_n[base + k] = dg;
     }
+}
+
+// the positions of a block with screened neighbours are compacted to its
+// leading lanes first (a ballot and a prefix over the waves), so that the
+// few that have any run together
+template <class R>
+__global__ void __launch_bounds__(256) k_bind_edges(KArgs<R> a, BindArgs b, const int *scr,
+                                                    const int *nscr, int cap)
+{
+    __shared__ int off[256], num[256];
+    __shared__ int wsum[4];
+    const int tid = threadIdx.x;
+    const int s = blockIdx.x * 256 + tid;
+    const int c = s < a.N ? nscr[s] : 0;
+    const unsigned long long m = __ballot(c > 0);
+    const int lane = tid & 63;
+    const int rank = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0)
+        wsum[tid >> 6] = __popcll(m);
+    __syncthreads();
+    int wb = 0;
+    for (int w = 0; w < (tid >> 6); w++)
+        wb += wsum[w];
+    if (c > 0) {
+        off[wb + rank] = s;
+        num[wb + rank] = c;
+    }
+    __syncthreads();
+    const int total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    for (int t = tid; t < total; t += 256)
+        bind_edges_one(a, b, scr, num[t], cap, off[t]);
 }
 
 __device__ __forceinline__ void bind_pair(const BindArgs &b, int i1, int i2)
@@ -620,21 +645,16 @@ __global__ void __launch_bounds__(256) k_bind_decide(BindArgs b)
 // it -- the integrator reads the terms only where ncand > 0 -- and by
 // original index zeros
 template <class R>
-__global__ void __launch_bounds__(256) k_bind_forces(KArgs<R> a, BindArgs b, double *Fb, double *Tb,
-                                                     int by_id, const int *ncand)
+__device__ __forceinline__ void bind_forces_one(const KArgs<R> &a, const BindArgs &b, double *Fb,
+                                                double *Tb, int by_id, int s)
 {
     constexpr bool FP64 = sizeof(R) == 8;
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= a.N)
-        return;
-    if (!by_id && ncand && ncand[s] == 0)
-        return;
     const int i = a.lmode ? s : a.list[s];
     const PosRec xi = a.X[i];
     const int ti = xi.meta & 255;
     const int npi = a.np[ti];
     double F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
-    if (npi > 0 && (!ncand || ncand[s] > 0)) {
+    if (npi > 0) {
         const DevRep rp = rep_rec(a, xi.meta);
         const int base = b.poff[xi.id];
         R qi[9];
@@ -693,6 +713,38 @@ __global__ void __launch_bounds__(256) k_bind_forces(KArgs<R> a, BindArgs b, dou
         Fb[o + q] = F[q];
         Tb[o + q] = T[q];
     }
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_bind_forces(KArgs<R> a, BindArgs b, double *Fb, double *Tb,
+                                                     int by_id, const int *ncand)
+{
+    const int tid = threadIdx.x;
+    const int s = blockIdx.x * 256 + tid;
+    if (by_id || !ncand) {
+        if (s < a.N)
+            bind_forces_one(a, b, Fb, Tb, by_id, s);
+        return;
+    }
+    // the positions with candidates compacted to the block's leading lanes
+    __shared__ int off[256];
+    __shared__ int wsum[4];
+    const bool here = s < a.N && ncand[s] > 0;
+    const unsigned long long m = __ballot(here);
+    const int lane = tid & 63;
+    const int rank = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0)
+        wsum[tid >> 6] = __popcll(m);
+    __syncthreads();
+    int wb = 0;
+    for (int w = 0; w < (tid >> 6); w++)
+        wb += wsum[w];
+    if (here)
+        off[wb + rank] = s;
+    __syncthreads();
+    const int total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    for (int t = tid; t < total; t += 256)
+        bind_forces_one(a, b, Fb, Tb, 0, off[t]);
 }
 
 } // namespace mdx
