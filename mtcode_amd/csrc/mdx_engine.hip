@@ -1641,6 +1641,7 @@ int mdx_clear_interactions(mdx_engine *e, int terms)
     if (terms & MDX_TERM_PATCHES) {
         e->ntypes = 0;
         e->active_mask = 0;
+        e->bind_on = 0; // (binding needs the patches: mdx_set_binding again)
     }
     if (terms & MDX_TERM_BONDS)
         e->nbonds = 0;

@@ -241,6 +241,15 @@ struct ComboPatch {
     std::vector<int> num_per_type; // cumulative end index of each type; empty: all type 0
     virtual ~ComboPatch() {}
     int ntypes() const { return (int)npatch.size(); }
+    // GeneralPatch::get_total_patches (GeneralPatch.cpp:284-291)
+    int get_total_patches(int N) const
+    {
+        const std::vector<int> t = types(N);
+        int nh = 0;
+        for (int i = 0; i < N; i++)
+            nh += npatch[(size_t)t[(size_t)i]];
+        return nh;
+    }
     std::vector<int> types(int N) const
     {
         std::vector<int> t((size_t)N, 0);
@@ -308,6 +317,21 @@ struct GeneralPatch : ComboPatch {
         for (int r = 0; r < nr; r++)
             rows.insert(rows.end(), {params(r, 0), params(r, 1), params(r, 2)});
     }
+};
+
+// ------------------------------------------------------------ binding
+// BinaryBindStore (Bindings/BinaryBindStore.h:7-11): per patch, bound or not
+// and the partner patch (kept while unbound)
+struct BinaryBindStore {
+    vector1<bool> isbound;
+    vector1<int> boundto;
+};
+
+// BindingModelSingle(on_rate, off_rate) (Bindings/BindingModelSingle.cpp:1-8):
+// the doublet / triplet rules of the device's binding kernels
+struct BindingModelSingle {
+    double on_rate, off_rate;
+    BindingModelSingle(double onn, double offf) : on_rate(onn), off_rate(offf) {}
 };
 
 // ------------------------------------------------------------ integrators
@@ -653,6 +677,33 @@ class LangevinNVT {
 // LangevinNVTR (MDBase/Rotational/LangevinR.h, LangevinR.cpp): rotors with
 // orientation (lab -> body) and body-frame angular momentum
 class LangevinNVTR : public LangevinNVT {
+    unsigned long long bind_draw_ = 0; // draws of the call-by-call binding routine
+    // the binding model and the bound state into the engine (after the
+    // patches are set)
+    void bind_push(const BindingModelSingle &bm, const BinaryBindStore &bo)
+    {
+        int nh = 0;
+        check(mdx_set_binding(e_, bm.on_rate, bm.off_rate, &nh));
+        if (bo.isbound.getsize() != nh || bo.boundto.getsize() != nh)
+            throw mdx_error(MDX_ERR_ARG, "BinaryBindStore size is not the number of patches");
+        std::vector<int> ib((size_t)nh), bt((size_t)nh);
+        for (int k = 0; k < nh; k++) {
+            ib[(size_t)k] = bo.isbound[k] ? 1 : 0;
+            bt[(size_t)k] = bo.boundto[k];
+        }
+        check(mdx_set_bindings(e_, ib.data(), bt.data()));
+    }
+    void bind_pull(BinaryBindStore &bo)
+    {
+        const int nh = bo.isbound.getsize();
+        std::vector<int> ib((size_t)nh), bt((size_t)nh);
+        check(mdx_get_bindings(e_, ib.data(), bt.data()));
+        for (int k = 0; k < nh; k++) {
+            bo.isbound[k] = ib[(size_t)k] != 0;
+            bo.boundto[k] = bt[(size_t)k];
+        }
+    }
+
   public:
     explicit LangevinNVTR(cube &a) : LangevinNVT(a) { integ_ = MDX_INTEG_ROTOR; }
 
@@ -721,6 +772,46 @@ class LangevinNVTR : public LangevinNVT {
         forces(t, Fp, &Tp);
         F += Fp;
         T += Tp;
+    }
+    // calculate_forces_and_torques3D_onlyone_nonlets(pairs, divs, ComboPatch&,
+    // BinaryBindStore&, AbstractBindingModel&, F, T)
+    // (Rotational/LangevinRforce.cpp:2874-4136): updates the bound state and
+    // adds the bound pairs' forces and lab torques.  The candidate pairs come
+    // from the engine's own neighbour search (the patch-list arguments keep
+    // the reference's signature and are not read); the decisions draw the
+    // engine's Philox numbers, a new draw per call.
+    template <class PL, class DV>
+    void calculate_forces_and_torques3D_onlyone_nonlets(PL &, DV &, ComboPatch &iny,
+                                                        BinaryBindStore &bo,
+                                                        const BindingModelSingle &bm,
+                                                        matrix<double> &F, matrix<double> &T)
+    {
+        StepTerms t;
+        t.patches = &iny;
+        apply_terms(t);
+        bind_push(bm, bo);
+        matrix<double> Fb(N(), 3), Tb(N(), 3);
+        check(mdx_bind_forces(e_, bind_draw_++, Fb.getdata(), Tb.getdata()));
+        bind_pull(bo);
+        F += Fb;
+        T += Tb;
+    }
+    // the loop of Condensate::run_singlebond (Condensate.cpp:464-478) on the
+    // device: the set-up evaluation and nsteps steps of WCA + the bound pairs'
+    // terms (t: the WCA and the patches), the bound state carried in bo.
+    // noise null (engine Philox) or (nsteps + 1) x N x 6, row 0 for the set-up.
+    void run_singlebond(int nsteps, const StepTerms &t, BinaryBindStore &bo,
+                        const BindingModelSingle &bm, const double *noise = nullptr)
+    {
+        if (!t.patches)
+            throw mdx_error(MDX_ERR_ARG, "run_singlebond needs the patches");
+        apply_terms(t);
+        bind_push(bm, bo);
+        check(mdx_prime(e_, noise));
+        if (nsteps > 0)
+            check(mdx_step(e_, nsteps, noise ? noise + 6 * (size_t)N() : nullptr));
+        host_stale_ = true;
+        bind_pull(bo);
     }
     // create_forces_and_torques_sphere(F, T, RT) (LangevinR.cpp:475-578)
     void create_forces_and_torques_sphere(matrix<double> &F, matrix<double> &T, matrix<double> &RT)

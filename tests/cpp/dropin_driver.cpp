@@ -18,6 +18,10 @@
 //   integ "stencil" with key n: writes the stencil table out_stencil.bin (no GPU)
 //   integ "csv" with N, M: outfunc / importcsv of x.bin (no GPU)
 //   rotor mode "output": Condensate::run(nsteps, every) with its CSV files
+//   rotor mode "bind" / "bindrun": Condensate::run_singlebond's loop
+//          (BinaryBindStore, BindingModelSingle(on_rate, off_rate),
+//          calculate_forces_and_torques3D_onlyone_nonlets) call by call or
+//          on the device; also writes out_isbound.bin, out_boundto.bin
 //   integ "dgraph" with N, L, n, cut, ex_lo, ex_hi: distance_graph labels of
 //          x.bin to out_labels.bin
 #include "mtcode_amd.hpp"
@@ -84,12 +88,62 @@ static int rotor_main()
     HSPotential wsa(1.0, 1.0);
     TetrahedralPatch pots(jd("eps_p"), jd("d"), jd("thetam"));
     const std::vector<double> nz = rd<double>("noise.bin", (size_t)(nsteps + 1) * N * 6);
-    if (g_job.at("mode") == "run") {
+    const std::string mode = g_job.at("mode");
+    if (mode == "bind" || mode == "bindrun") {
+        // Condensate::run_singlebond (Condensate.cpp:422-478) from an unbound
+        // store: call by call, or as device-resident steps
+        BindingModelSingle bm(jd("on_rate"), jd("off_rate"));
+        const int nh = pots.get_total_patches(N);
+        BinaryBindStore bbs;
+        bbs.isbound = vector1<bool>(nh);
+        bbs.boundto = vector1<int>(nh);
+        if (mode == "bindrun") {
+            StepTerms t;
+            t.wca = &wsa;
+            t.patches = &pots;
+            b.run_singlebond(nsteps, t, bbs, bm, nz.data());
+        } else {
+            int tb;
+            const int n = std::max(1, (int)(geo.l / 4.0));
+            matrix<int> boxes = geo.generate_boxes_relationships(n, tb);
+            matrix<int> *pairs = b.calculatepairs(boxes, 3.5);
+            std::vector<int> opairs, runs_diff; // (patch list: not read by the engine)
+            matrix<double> F = b.calculateforces(*pairs, wsa);
+            matrix<double> T(N, 3);
+            b.calculate_forces_and_torques3D_onlyone_nonlets(opairs, runs_diff, pots, bbs, bm, F, T);
+            matrix<double> RT = noise_row(nz, 0, N, 6);
+            b.create_forces_and_torques_sphere(F, T, RT);
+            for (int i = 0; i < nsteps; i++) {
+                if (i > 0 && i % 20 == 0) {
+                    delete pairs;
+                    pairs = b.calculatepairs(boxes, 3.5);
+                }
+                b.advancemom_halfstep(F, T);
+                b.advance_pos();
+                b.rotate();
+                F = b.calculateforces(*pairs, wsa);
+                T.reset(0.0);
+                b.calculate_forces_and_torques3D_onlyone_nonlets(opairs, runs_diff, pots, bbs, bm,
+                                                                 F, T);
+                RT = noise_row(nz, i + 1, N, 6);
+                b.create_forces_and_torques_sphere(F, T, RT);
+                b.advancemom_halfstep(F, T);
+            }
+            delete pairs;
+        }
+        std::vector<int> ib(nh), bt(nh);
+        for (int k = 0; k < nh; k++) {
+            ib[k] = bbs.isbound[k] ? 1 : 0;
+            bt[k] = bbs.boundto[k];
+        }
+        wr("out_isbound.bin", ib.data(), (size_t)nh);
+        wr("out_boundto.bin", bt.data(), (size_t)nh);
+    } else if (mode == "run") {
         StepTerms t;
         t.wca = &wsa;
         t.patches = &pots;
         b.run(nsteps, t, nz.data());
-    } else if (g_job.at("mode") == "output") {
+    } else if (mode == "output") {
         // Condensate::run(runtime, every, strbase) with its CSV output, from dir
         StepTerms t;
         t.wca = &wsa;

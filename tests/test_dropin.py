@@ -119,6 +119,45 @@ def test_rotor_condensate_loop_vs_oracle(tmp_path, mode):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["bind", "bindrun"])
+def test_singlebond_loop_vs_oracle(tmp_path, mode):
+    """Condensate::run_singlebond through the C++ classes (BinaryBindStore,
+    BindingModelSingle, calculate_forces_and_torques3D_onlyone_nonlets), call
+    by call and on the device, against the oracle's loop with the same
+    injected noise and binding draws (0, 1, 2, ...: the call-by-call counter
+    and a new engine's force-evaluation count): bound state equal, state
+    within 1e-10"""
+    import test_gpu_bind as tb
+    s = wl.thermal_momenta(tb.lattice_system(8, 1.0, 1.1, 81), seed=3)
+    N, nsteps, seed = s["N"], 25, 12
+    say("drop-in singlebond", mode, N, nsteps)
+    nz = po.noise(seed, 0, (nsteps + 1) * N * 6)
+    e, d, th = s["patches"]["params"][0]
+    job = {"integ": "rotor", "mode": mode, "N": N, "L": repr(float(s["box"][0])),
+           "nsteps": nsteps, "dt": s["dt"], "gamma": s["gamma"], "gammar": repr(float(s["gammar"])),
+           "kT": s["kT"], "precision": 0, "eps_p": repr(float(e)), "d": repr(float(d)),
+           "thetam": repr(float(th)), "on_rate": 0.7, "off_rate": 0.3}
+    s0 = dict(s, p=np.zeros((N, 3)), L=np.zeros((N, 3)))  # the driver starts from rest
+    run_driver(str(tmp_path), job, {"x": s["x"], "Q": s["Q"], "noise": nz})
+    key = (1, 0xB1D0B1D0)   # the drop-in classes' default seed 1
+    tb_key = tb.KEY
+    tb.KEY = key
+    try:
+        ref, ibo, bto = tb.oracle_bind_run(s0, nsteps, nz.reshape(nsteps + 1, N, 6), 0.7, 0.3,
+                                           list(range(nsteps + 1)))
+    finally:
+        tb.KEY = tb_key
+    nh = len(ibo)
+    assert ibo.sum() > 0
+    assert np.array_equal(np.fromfile(os.path.join(str(tmp_path), "out_isbound.bin"), np.int32), ibo)
+    assert np.array_equal(np.fromfile(os.path.join(str(tmp_path), "out_boundto.bin"), np.int32), bto)
+    for k, shape in (("x", (N, 3)), ("p", (N, 3)), ("Q", (N, 9)), ("L", (N, 3))):
+        err = relmax(rd(str(tmp_path), f"out_{k}.bin", shape), ref[k])
+        say(k, err)
+        assert err < 1e-10, k
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["stages", "run"])
 def test_bead_loop_vs_oracle(tmp_path, mode):
     s = wl.mt_bundle(2, 13, 40, spacing=6.0, box=(40.0, 40.0, 40.0))
