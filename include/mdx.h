@@ -217,6 +217,22 @@ int mdx_clusters(mdx_engine *e, int replica, const int n[3], double cut, int ex_
  * every patch starts unbound with boundto 0 (the BinaryBindStore of
  * Condensate.cpp:434-441).  Call after mdx_set_patches (which clears it). */
 int mdx_set_binding(mdx_engine *e, double on_rate, double off_rate, int *npatches);
+/* the species-table binding models in place of BindingModelSingle:
+ * BindingModelBinary (Bindings/BindingModelBinary.cpp:97-470) and
+ * BindingModelTernary (Bindings/BindingModelTernary.cpp:492-870).
+ * species[nh]: 1..3 per patch (the model's rule: index < div, or its
+ * sorting function); dtab[24]: per sorted species pair 11,12,13,22,23,33
+ * the four rates doubrXY (before unbound: to unbound, to bound; before bound:
+ * to unbound, to bound); ttab[160]: per sorted species triple
+ * 111,112,113,122,123,133,222,223,233,333 the 16 rates triprXYZ (from bound
+ * 12, 23, 13, none x to 12, 23, 13, none); tr4[40]: per triple and "from"
+ * the rate to none the model reads (its own table's, except the binary
+ * model's 122 and 222, which read 111's).  A triplet is decided in the
+ * frame of its sorted species (sort_triplet_and_save_permutation,
+ * save_to_type, basic.h:166-468).  After mdx_set_binding; mdx_set_binding
+ * returns to BindingModelSingle. */
+int mdx_set_binding_tables(mdx_engine *e, const int *species, const double *dtab,
+                           const double *ttab, const double *tr4);
 /* the bound state, per patch: isbound (0/1) and boundto (the partner patch;
  * kept, like the reference's, when the patch is unbound) --
  * BinaryBindStore (Bindings/BinaryBindStore.h:7-11) */

@@ -57,6 +57,15 @@ struct BindArgs {
     unsigned long long draw;
     int draw_epoch; // 1: the draw counter is the force-evaluation count (steps)
     StepCtl *ctl;
+    // 0: BindingModelSingle(on_rate, off_rate); 1: the species-table form of
+    // BindingModelBinary / BindingModelTernary: species[patch] in 1..3,
+    // dtab[pair * 4 + 2 * before + k] (k 0: rate to unbound, 1: to bound) by
+    // sorted species pair 11,12,13,22,23,33, ttab[triple * 16 + 4 * from + to]
+    // (from: bound 12, 23, 13, none; to: 12, 23, 13) and tr4[triple * 4 + from]
+    // (to none) by sorted species triple 111,112,113,122,123,133,222,223,233,333
+    int table;
+    const int *species;
+    const double *dtab, *ttab, *tr4;
 };
 
 // status bit 3: a patch with more than BIND_DEG candidates, or a component
@@ -331,13 +340,43 @@ __device__ __forceinline__ void bind_pair(const BindArgs &b, int i1, int i2)
     b.isbound[i2] = 1;
 }
 
-// BindingModelSingle::doublet (BindingModelSingle.cpp:10-25) on i1 < i2
-// (LangevinRforce.cpp:3262-3302)
+__device__ __forceinline__ int bind_pair_code(int a, int c) // sorted species, 1..3
+{
+    return a == 1 ? c - 1 : a == 2 ? c + 1 : 5;
+}
+
+__device__ __forceinline__ int bind_triple_code(int a, int c, int e) // sorted
+{
+    if (a == 1)
+        return c == 1 ? e - 1 : c == 2 ? (e == 2 ? 3 : 4) : 5;
+    if (a == 2)
+        return c == 2 ? (e == 2 ? 6 : 7) : 8;
+    return 9;
+}
+
+// BindingModelSingle::doublet (BindingModelSingle.cpp:10-25), or the
+// species-table form (BindingModelTernary.cpp:523-558, BindingModelBinary.cpp:
+// 97-166), on i1 < i2 (LangevinRforce.cpp:3262-3302)
 __device__ __forceinline__ void bind_decide2(const BindArgs &b, int i1, int i2, double ran)
 {
-    const double totalr = b.off_rate + b.on_rate;
-    const double rr = totalr * ran;
-    if (!(rr < b.off_rate)) {
+    bool bind;
+    if (b.table) {
+        const bool before =
+            b.boundto[i1] == i2 && b.boundto[i2] == i1 && b.isbound[i1] && b.isbound[i2];
+        const int s1 = b.species[i1], s2 = b.species[i2];
+        const double *r = b.dtab + 4 * bind_pair_code(s1 < s2 ? s1 : s2, s1 < s2 ? s2 : s1);
+        const int bb = before ? 1 : 0;
+        const double r1 = r[bb * 2 + 0];
+        const double r2 = r[bb * 2 + 1];
+        const double totalr = r1 + r2;
+        const double rr = totalr * ran;
+        bind = !(rr < r1);
+    } else {
+        const double totalr = b.off_rate + b.on_rate;
+        const double rr = totalr * ran;
+        bind = !(rr < b.off_rate);
+    }
+    if (bind) {
         bind_pair(b, i1, i2);
     } else {
         b.isbound[i1] = 0;
@@ -345,8 +384,64 @@ __device__ __forceinline__ void bind_decide2(const BindArgs &b, int i1, int i2, 
     }
 }
 
-// BindingModelSingle::triplet (BindingModelSingle.cpp:40-119) on
-// i1 < i2 < i3 with connectivity c (LangevinRforce.cpp:3303-3497)
+// sort_triplet_and_save_permutation (basic.h:166-245): the sorted species
+// and the permutation code (p1 * 10 + p2)
+__device__ __forceinline__ int bind_sort3(int t1, int t2, int t3, int &a, int &c, int &e)
+{
+    if (t1 <= t2) {
+        if (t2 <= t3) {
+            a = t1; c = t2; e = t3;
+            return 12;
+        }
+        if (t1 <= t3) {
+            a = t1; c = t3; e = t2;
+            return 13;
+        }
+        a = t3; c = t1; e = t2;
+        return 31;
+    }
+    if (t1 <= t3) {
+        a = t2; c = t1; e = t3;
+        return 21;
+    }
+    if (t2 <= t3) {
+        a = t2; c = t3; e = t1;
+        return 23;
+    }
+    a = t3; c = t2; e = t1;
+    return 32;
+}
+
+// save_to_type (basic.h:373-419): {12, 23, 13} into the sorted frame
+__device__ __forceinline__ void bind_to_type(const bool *x, int o, bool *y)
+{
+    switch (o) {
+    case 12: y[0] = x[0]; y[1] = x[1]; y[2] = x[2]; break;
+    case 13: y[0] = x[2]; y[1] = x[1]; y[2] = x[0]; break;
+    case 21: y[0] = x[0]; y[1] = x[2]; y[2] = x[1]; break;
+    case 23: y[0] = x[1]; y[2] = x[0]; y[1] = x[2]; break;
+    case 31: y[0] = x[1]; y[1] = x[2]; y[2] = x[0]; break;
+    default: y[1] = x[0]; y[0] = x[1]; y[2] = x[2]; break;
+    }
+}
+
+// inverse_save_to_type (basic.h:421-468)
+__device__ __forceinline__ void bind_from_type(const bool *x, int o, bool *y)
+{
+    switch (o) {
+    case 12: y[0] = x[0]; y[1] = x[1]; y[2] = x[2]; break;
+    case 13: y[2] = x[0]; y[1] = x[1]; y[0] = x[2]; break;
+    case 21: y[0] = x[0]; y[2] = x[1]; y[1] = x[2]; break;
+    case 23: y[1] = x[0]; y[0] = x[2]; y[2] = x[1]; break;
+    case 31: y[2] = x[0]; y[0] = x[1]; y[1] = x[2]; break;
+    default: y[1] = x[0]; y[0] = x[1]; y[2] = x[2]; break;
+    }
+}
+
+// BindingModelSingle::triplet (BindingModelSingle.cpp:40-119), or the
+// species-table form (BindingModelTernary.cpp:791-870; the binary model's
+// is its identity-permutation case), on i1 < i2 < i3 with connectivity c
+// (LangevinRforce.cpp:3303-3497)
 __device__ __forceinline__ void bind_decide3(const BindArgs &b, int i1, int i2, int i3, bool c12,
                                              bool c23, bool c13, double ran)
 {
@@ -354,6 +449,46 @@ __device__ __forceinline__ void bind_decide3(const BindArgs &b, int i1, int i2, 
     const bool b12 = bt[i1] == i2 && bt[i2] == i1 && ib[i1] && ib[i2];
     const bool b23 = bt[i2] == i3 && bt[i3] == i2 && ib[i2] && ib[i3];
     const bool b13 = bt[i1] == i3 && bt[i3] == i1 && ib[i1] && ib[i3];
+    if (b.table) {
+        int sa, sc, se;
+        const int o = bind_sort3(b.species[i1], b.species[i2], b.species[i3], sa, sc, se);
+        const bool bb[3] = {b12, b23, b13}, cc[3] = {c12, c23, c13};
+        bool tb[3], tc[3];
+        bind_to_type(bb, o, tb);
+        bind_to_type(cc, o, tc);
+        const int f = tb[0] ? 0 : tb[1] ? 1 : tb[2] ? 2 : 3;
+        const int code = bind_triple_code(sa, sc, se);
+        const double *r = b.ttab + 16 * code;
+        const double r1 = tc[0] * r[f * 4 + 0];
+        const double r2 = tc[1] * r[f * 4 + 1];
+        const double r3 = tc[2] * r[f * 4 + 2];
+        const double r4 = b.tr4[4 * code + f];
+        const double totalr = r1 + r2 + r3 + r4;
+        const double rr = totalr * ran;
+        bool ta[3] = {false, false, false}, a[3];
+        if (rr < r1)
+            ta[0] = true;
+        else if (rr >= r1 && rr < r1 + r2)
+            ta[1] = true;
+        else if (rr >= r1 + r2 && rr < r1 + r2 + r3)
+            ta[2] = true;
+        bind_from_type(ta, o, a);
+        if (a[0]) {
+            bind_pair(b, i1, i2);
+            b.isbound[i3] = 0;
+        } else if (a[1]) {
+            bind_pair(b, i2, i3);
+            b.isbound[i1] = 0;
+        } else if (a[2]) {
+            bind_pair(b, i1, i3);
+            b.isbound[i2] = 0;
+        } else {
+            b.isbound[i1] = 0;
+            b.isbound[i2] = 0;
+            b.isbound[i3] = 0;
+        }
+        return;
+    }
     const double on = b.on_rate, off = b.off_rate;
     double r1, r2, r3, r4;
     if (b12) {

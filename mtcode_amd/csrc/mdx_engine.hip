@@ -159,6 +159,8 @@ struct mdx_engine {
     double bind_on_rate = 0, bind_off_rate = 0, bind_mk = 0, bind_cmin = 1;
     int nh = 0;
     Buf bpoff, bowner, bisbound, bboundto, badj_n, badj_p, badj_s, bF, bT, bscr, bnscr;
+    int bind_table = 0; // 1: the species-table model (mdx_set_binding_tables)
+    Buf bspecies, bdtab, bttab, btr4;
 
     size_t rsz() const { return prec == MDX_PREC_FP64 ? 8 : 4; }
 };
@@ -768,7 +770,7 @@ template <class R, int MODE> static int list_step_graph(mdx_engine *e)
     // the step launches)
     const int form[7] = {e->ncells,      e->N,         e->use_split, e->use_compact,
                          e->use_walkc,   e->use_fuse,  e->use_tiles};
-    const double rates[2] = {e->bind_on_rate, e->bind_off_rate};
+    const double rates[3] = {e->bind_on_rate, e->bind_off_rate, (double)e->bind_table};
     std::vector<char> sig(sizeof a + sizeof form + sizeof rates);
     memcpy(sig.data(), &a, sizeof a);
     memcpy(sig.data() + sizeof a, form, sizeof form);
@@ -1170,7 +1172,7 @@ void mdx_destroy(mdx_engine *e)
                    &e->ncand,   &e->cand,    &e->nbr,    &e->nbrc,   &e->nn,     &e->bcell,
                    &e->dslot,   &e->Tp,      &e->bpoff,  &e->bowner, &e->bisbound, &e->bboundto,
                    &e->badj_n,  &e->badj_p,  &e->badj_s, &e->bF,     &e->bT,      &e->bscr,
-                   &e->bnscr};
+                   &e->bnscr,   &e->bspecies, &e->bdtab, &e->bttab,  &e->btr4};
     for (Buf *b : bufs)
         dfree(*b);
     for (GraphSlot &g : e->graphs)
@@ -2031,6 +2033,7 @@ int mdx_set_binding(mdx_engine *e, double on_rate, double off_rate, int *npatche
         e->list_valid = false;
     }
     e->bind_on = 1;
+    e->bind_table = 0;
     e->bind_on_rate = on_rate;
     e->bind_off_rate = off_rate;
     e->bind_mk = mk;
@@ -2038,6 +2041,35 @@ int mdx_set_binding(mdx_engine *e, double on_rate, double off_rate, int *npatche
     e->nh = nh;
     if (npatches)
         *npatches = nh;
+    return MDX_OK;
+}
+
+int mdx_set_binding_tables(mdx_engine *e, const int *species, const double *dtab,
+                           const double *ttab, const double *tr4)
+{
+    if (!e || !species || !dtab || !ttab || !tr4)
+        return set_err(MDX_ERR_ARG, "bad arguments");
+    if (!e->bind_on)
+        return set_err(MDX_ERR_STATE, "call mdx_set_binding first");
+    for (int k = 0; k < e->nh; k++)
+        if (species[k] < 1 || species[k] > 3)
+            return set_err(MDX_ERR_ARG, "species[%d] = %d outside 1..3", k, species[k]);
+    for (int k = 0; k < 24; k++)
+        if (!(dtab[k] >= 0))
+            return set_err(MDX_ERR_ARG, "doublet rates must be >= 0");
+    for (int k = 0; k < 160; k++)
+        if (!(ttab[k] >= 0))
+            return set_err(MDX_ERR_ARG, "triplet rates must be >= 0");
+    for (int k = 0; k < 40; k++)
+        if (!(tr4[k] >= 0))
+            return set_err(MDX_ERR_ARG, "triplet rates must be >= 0");
+    HIPCHK(hipSetDevice(e->device));
+    int rc;
+    if ((rc = upload(e, e->bspecies, species, (size_t)e->nh)) ||
+        (rc = upload(e, e->bdtab, dtab, 24)) || (rc = upload(e, e->bttab, ttab, 160)) ||
+        (rc = upload(e, e->btr4, tr4, 40)))
+        return rc;
+    e->bind_table = 1;
     return MDX_OK;
 }
 
@@ -2096,6 +2128,11 @@ template <class R> static BindArgs bind_args(mdx_engine *e, unsigned long long d
     b.key1 = (uint32_t)(e->seed >> 32) ^ 0xB1D0B1D0u;
     b.draw = draw;
     b.ctl = (StepCtl *)e->ctl.p;
+    b.table = e->bind_table;
+    b.species = (const int *)e->bspecies.p;
+    b.dtab = (const double *)e->bdtab.p;
+    b.ttab = (const double *)e->bttab.p;
+    b.tr4 = (const double *)e->btr4.p;
     return b;
 }
 

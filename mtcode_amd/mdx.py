@@ -103,6 +103,7 @@ def lib():
     L.mdx_stress_noalloc.argtypes = [C.c_void_p, C.c_int, _dp]
     L.mdx_set_binding.argtypes = [C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_int)]
     L.mdx_set_bindings.argtypes = [C.c_void_p, _ip, _ip]
+    L.mdx_set_binding_tables.argtypes = [C.c_void_p, _ip, _dp, _dp, _dp]
     L.mdx_get_bindings.argtypes = [C.c_void_p, _ip, _ip]
     L.mdx_bind_forces.argtypes = [C.c_void_p, C.c_ulonglong, _dp, _dp]
     _lib = L
@@ -375,6 +376,15 @@ class Engine:
         _check(lib().mdx_set_binding(self.h, float(on_rate), float(off_rate), C.byref(nh)))
         self.nh = nh.value
         return self.nh
+
+    def set_binding_tables(self, species, dtab, ttab, tr4):
+        """the species-table model (BindingModelBinary / Ternary)"""
+        sp = np.ascontiguousarray(species, dtype=np.int32)
+        d = np.ascontiguousarray(dtab, dtype=np.float64)
+        t = np.ascontiguousarray(ttab, dtype=np.float64)
+        r4 = np.ascontiguousarray(tr4, dtype=np.float64)
+        _check(lib().mdx_set_binding_tables(self.h, sp.ctypes.data_as(_ip), d.ctypes.data_as(_dp),
+                                            t.ctypes.data_as(_dp), r4.ctypes.data_as(_dp)))
 
     def set_bindings(self, isbound, boundto):
         ib = np.ascontiguousarray(isbound, dtype=np.int32)

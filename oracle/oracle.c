@@ -1174,12 +1174,129 @@ double orc_bind_uniform(const uint32_t key[2], uint64_t draw, int root, int k)
     return bind_u53(o);
 }
 
-/* BindingModelSingle::doublet (Bindings/BindingModelSingle.cpp:10-25) */
-static int bind_doublet(const orc_bind_model *m, double ran)
+static int pair_code(int a, int b) /* sorted species a <= b, 1..3 */
 {
+    static const int c[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    return c[a - 1][b - 1];
+}
+
+static int triple_code(int a, int b, int c) /* sorted a <= b <= c */
+{
+    if (a == 1)
+        return b == 1 ? (c == 1 ? 0 : c == 2 ? 1 : 2) : b == 2 ? (c == 2 ? 3 : 4) : 5;
+    if (a == 2)
+        return b == 2 ? (c == 2 ? 6 : 7) : 8;
+    return 9;
+}
+
+/* BindingModelSingle::doublet (Bindings/BindingModelSingle.cpp:10-25), or
+ * the species-table form (BindingModelTernary.cpp:523-558) */
+static int bind_doublet(const orc_bind_model *m, int before, int i1, int i2, double ran)
+{
+    if (m->table) {
+        int s1 = m->species[i1], s2 = m->species[i2];
+        const double *r = m->dtab + 4 * pair_code(s1 < s2 ? s1 : s2, s1 < s2 ? s2 : s1);
+        const int b = before ? 1 : 0;
+        double r1 = r[b * 2 + 0];
+        double r2 = r[b * 2 + 1];
+        double totalr = r1 + r2;
+        double rr = totalr * ran;
+        return !(rr < r1);
+    }
     double totalr = m->off_rate + m->on_rate; /* tr, BindingModelSingle.cpp:5-8 */
     double rr = totalr * ran;
     return !(rr < m->off_rate);
+}
+
+/* sort_triplet_and_save_permutation (basic.h:166-245) */
+static void sort3_perm(int t1, int t2, int t3, int *i, int *p)
+{
+    if (t1 <= t2) {
+        if (t2 <= t3) {
+            i[0] = t1; i[1] = t2; i[2] = t3; p[0] = 1; p[1] = 2; p[2] = 3;
+        } else if (t1 <= t3) {
+            i[0] = t1; i[1] = t3; i[2] = t2; p[0] = 1; p[1] = 3; p[2] = 2;
+        } else {
+            i[0] = t3; i[1] = t1; i[2] = t2; p[0] = 3; p[1] = 1; p[2] = 2;
+        }
+    } else {
+        if (t1 <= t3) {
+            i[0] = t2; i[1] = t1; i[2] = t3; p[0] = 2; p[1] = 1; p[2] = 3;
+        } else if (t2 <= t3) {
+            i[0] = t2; i[1] = t3; i[2] = t1; p[0] = 2; p[1] = 3; p[2] = 1;
+        } else {
+            i[0] = t3; i[1] = t2; i[2] = t1; p[0] = 3; p[1] = 2; p[2] = 1;
+        }
+    }
+}
+
+/* save_to_type (basic.h:373-419): b = {12, 23, 13} into the sorted frame */
+static void to_type(const int *b, const int *o, int *a)
+{
+    if (o[0] == 1) {
+        if (o[1] == 2) { a[0] = b[0]; a[1] = b[1]; a[2] = b[2]; }
+        else { a[0] = b[2]; a[1] = b[1]; a[2] = b[0]; }
+    } else if (o[0] == 2) {
+        if (o[1] == 1) { a[0] = b[0]; a[1] = b[2]; a[2] = b[1]; }
+        else { a[0] = b[1]; a[2] = b[0]; a[1] = b[2]; }
+    } else {
+        if (o[1] == 1) { a[0] = b[1]; a[1] = b[2]; a[2] = b[0]; }
+        else { a[1] = b[0]; a[0] = b[1]; a[2] = b[2]; }
+    }
+}
+
+/* inverse_save_to_type (basic.h:421-468) */
+static void from_type(const int *b, const int *o, int *a)
+{
+    if (o[0] == 1) {
+        if (o[1] == 2) { a[0] = b[0]; a[1] = b[1]; a[2] = b[2]; }
+        else { a[2] = b[0]; a[1] = b[1]; a[0] = b[2]; }
+    } else if (o[0] == 2) {
+        if (o[1] == 1) { a[0] = b[0]; a[2] = b[1]; a[1] = b[2]; }
+        else { a[1] = b[0]; a[0] = b[2]; a[2] = b[1]; }
+    } else {
+        if (o[1] == 1) { a[2] = b[0]; a[0] = b[1]; a[1] = b[2]; }
+        else { a[1] = b[0]; a[0] = b[1]; a[2] = b[2]; }
+    }
+}
+
+/* the species-table triplet (BindingModelTernary.cpp:791-870; the binary
+ * model's is its identity-permutation case, BindingModelBinary.cpp:236-470) */
+static int bind_triplet_table(const orc_bind_model *m, int i1, int i2, int i3, int b12, int b23,
+                              int b13, int c12, int c23, int c13, double ran)
+{
+    int sp[3], o[3];
+    sort3_perm(m->species[i1], m->species[i2], m->species[i3], sp, o);
+    const int bb[3] = {b12, b23, b13}, cc[3] = {c12, c23, c13};
+    int tb[3], tc[3];
+    to_type(bb, o, tb);
+    to_type(cc, o, tc);
+    int f;
+    if (tb[0])
+        f = 0;
+    else if (tb[1])
+        f = 1;
+    else if (tb[2])
+        f = 2;
+    else
+        f = 3;
+    const int code = triple_code(sp[0], sp[1], sp[2]);
+    const double *r = m->ttab + 16 * code;
+    double r1 = tc[0] * r[f * 4 + 0];
+    double r2 = tc[1] * r[f * 4 + 1];
+    double r3 = tc[2] * r[f * 4 + 2];
+    double r4 = m->tr4[4 * code + f];
+    double totalr = r1 + r2 + r3 + r4;
+    double rr = totalr * ran;
+    int ta[3] = {0, 0, 0}, a[3];
+    if (rr < r1)
+        ta[0] = 1;
+    else if (rr >= r1 && rr < r1 + r2)
+        ta[1] = 1;
+    else if (rr >= r1 + r2 && rr < r1 + r2 + r3)
+        ta[2] = 1;
+    from_type(ta, o, a);
+    return a[0] ? 1 : a[1] ? 2 : a[2] ? 3 : 0;
 }
 
 /* BindingModelSingle::triplet (BindingModelSingle.cpp:40-119): which pair
@@ -1256,7 +1373,9 @@ static void sort3(int t1, int t2, int t3, int *i1, int *i2, int *i3)
 /* a doublet i1 < i2 (LangevinRforce.cpp:3262-3302) */
 static void bind_decide2(const orc_bind_model *m, bind_out *o, int i1, int i2, double ran)
 {
-    if (bind_doublet(m, ran))
+    const int before = o->boundto[i1] == i2 && o->boundto[i2] == i1 && o->isbound[i1] &&
+                       o->isbound[i2];
+    if (bind_doublet(m, before, i1, i2, ran))
         bind_set_pair(o, i1, i2);
     else {
         o->isbound[i1] = 0;
@@ -1272,7 +1391,8 @@ static void bind_decide3(const orc_bind_model *m, bind_out *o, int i1, int i2, i
     int b12 = bt[i1] == i2 && bt[i2] == i1 && ib[i1] && ib[i2];
     int b23 = bt[i2] == i3 && bt[i3] == i2 && ib[i2] && ib[i3];
     int b13 = bt[i1] == i3 && bt[i3] == i1 && ib[i1] && ib[i3];
-    int w = bind_triplet(m, b12, b23, b13, c12, c23, c13, ran);
+    int w = m->table ? bind_triplet_table(m, i1, i2, i3, b12, b23, b13, c12, c23, c13, ran)
+                     : bind_triplet(m, b12, b23, b13, c12, c23, c13, ran);
     if (w == 1) {
         bind_set_pair(o, i1, i2);
         o->isbound[i3] = 0;

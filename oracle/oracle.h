@@ -302,6 +302,19 @@ typedef struct {
     unsigned int seed;
     uint32_t key[2];
     uint64_t draw;
+    /* table: 0 BindingModelSingle(on_rate, off_rate); 1 the species-table
+     * form of BindingModelBinary / BindingModelTernary
+     * (Bindings/BindingModelBinary.cpp:97-470, BindingModelTernary.cpp:
+     * 492-870): species[patch] in 1..3; dtab[pair * 4 + 2 * before + k]
+     * (k 0: rate to unbound, 1: to bound) by sorted species pair
+     * 11,12,13,22,23,33; ttab[triple * 16 + 4 * from + to] (from: bound 12,
+     * 23, 13, none; to: 12, 23, 13) and tr4[triple * 4 + from] (the rate to
+     * none) by sorted species triple 111,112,113,122,123,133,222,223,233,333;
+     * a triplet is decided in the frame of its sorted species
+     * (sort_triplet_and_save_permutation, save_to_type, basic.h:166-468) */
+    int table;
+    const int *species;
+    const double *dtab, *ttab, *tr4;
 } orc_bind_model;
 double orc_bind_uniform(const uint32_t key[2], uint64_t draw, int root, int k);
 int orc_bind_forces(const orc_box *b, int N, const double *x, const double *Q,

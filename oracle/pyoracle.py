@@ -36,7 +36,9 @@ class PatchSet(C.Structure):
 
 class BindModel(C.Structure):
     _fields_ = [("on_rate", C.c_double), ("off_rate", C.c_double), ("rng", C.c_int),
-                ("seed", C.c_uint), ("key", C.c_uint32 * 2), ("draw", C.c_uint64)]
+                ("seed", C.c_uint), ("key", C.c_uint32 * 2), ("draw", C.c_uint64),
+                ("table", C.c_int), ("species", _ip), ("dtab", _dp), ("ttab", _dp),
+                ("tr4", _dp)]
 
 
 class Interactions(C.Structure):
@@ -371,10 +373,11 @@ def dgraph_labels(box, x, n, cut, lo=0, hi=-1, periodic=True):
 
 
 def bind_forces(s, isbound, boundto, on_rate, off_rate, seed=None, key=None, draw=0, x=None,
-                Q=None):
+                Q=None, tables=None):
     """One call of the one-bond-per-patch routine (orc_bind_forces): the
     reference's rand() draws after srand(seed) when seed is given, else Philox
-    with key (2 words) and draw.  Returns (F, T, isbound, boundto, stats)."""
+    with key (2 words) and draw; tables (species, dtab, ttab, tr4) selects the
+    species-table model.  Returns (F, T, isbound, boundto, stats)."""
     keep = _Keep()
     x = s["x"] if x is None else x
     Q = s["Q"] if Q is None else Q
@@ -388,6 +391,12 @@ def bind_forces(s, isbound, boundto, on_rate, off_rate, seed=None, key=None, dra
         m.rng = 1
         m.key[0], m.key[1] = int(key[0]), int(key[1])
         m.draw = int(draw)
+    if tables is not None:
+        m.table = 1
+        m.species = keep.i(tables["species"])
+        m.dtab = keep.d(tables["dtab"])
+        m.ttab = keep.d(tables["ttab"])
+        m.tr4 = keep.d(tables["tr4"])
     ib, ibp = _i(np.array(isbound, dtype=np.int32))
     bt, btp = _i(np.array(boundto, dtype=np.int32))
     F = np.zeros((N, 3))

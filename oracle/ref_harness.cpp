@@ -29,7 +29,8 @@
  *   pairpot   r    -> WCAPotential::energy, HSPotential::force and force2mdx
  *   stress    x    -> MD::calculatestress with an HSPotential interaction
  *   bind      x, Q, isbound, boundto -> one call of the one-bond-per-patch
- *              routine (calculate_forces_and_torques3D_onlyone_nonlets)
+ *              routine (calculate_forces_and_torques3D_onlyone_nonlets) with
+ *              BindingModelSingle, BindingModelBinary or BindingModelTernary
  */
 #include <stdlib.h>
 #include <iostream>
@@ -754,13 +755,80 @@ static int cmd_bind()
     }
     bbs.isbound = isbound;
     bbs.boundto = boundto;
-    BindingModelSingle bm(jd("on_rate"), jd("off_rate"));
     matrix<int> *pairs = make_pairs(b, geo, (int)ji("n"), jd("cut"));
     vector<patchint> opairs = b.calculate_patch_list(*pairs, *pots);
     vector<int> runs_diff = adjacency(opairs);
     matrix<double> F(N, 3), T(N, 3);
+    /* the model: single (on_rate, off_rate), binary (div, setup_equilibrium)
+     * or ternary (SortingFunctionNonUniform div1 div2 np, setup t0..t14); for
+     * the table models the species of each patch and the rate tables are
+     * written out in the oracle's layout (pairs 11,12,13,22,23,33; triples
+     * 111,112,113,122,123,133,222,223,233,333; tr4 = the rate to none each
+     * model reads) */
+    string model = g_job.count("model") ? g_job["model"] : "single";
+    vector<int> species(nh, 0);
+    vector<double> dtab(6 * 4, 0.0), ttab(10 * 16, 0.0), tr4(10 * 4, 0.0);
+    auto put_d = [&](int code, vector1<double> &r) {
+        for (int k = 0; k < 4; k++)
+            dtab[4 * code + k] = r[k];
+    };
+    auto put_t = [&](int code, vector1<double> &r, vector1<double> &r4src) {
+        for (int k = 0; k < 16; k++)
+            ttab[16 * code + k] = r[k];
+        for (int f = 0; f < 4; f++)
+            tr4[4 * code + f] = r4src[4 * f + 3];
+    };
     srand((unsigned int)ji("seed"));
-    b.calculate_forces_and_torques3D_onlyone_nonlets(opairs, runs_diff, *pots, bbs, bm, F, T);
+    if (model == "binary") {
+        const int div = (int)ji("div");
+        BindingModelBinary bm(div);
+        bm.setup_equilibrium();
+        for (int i = 0; i < nh; i++)
+            species[i] = i < div ? 1 : 2;
+        put_d(0, bm.doubr11);
+        put_d(1, bm.doubr12);
+        put_d(3, bm.doubr22);
+        put_t(0, bm.tripr111, bm.tripr111);
+        put_t(1, bm.tripr112, bm.tripr112);
+        put_t(3, bm.tripr122, bm.tripr111);
+        put_t(6, bm.tripr222, bm.tripr111);
+        b.calculate_forces_and_torques3D_onlyone_nonlets(opairs, runs_diff, *pots, bbs, bm, F, T);
+    } else if (model == "ternary") {
+        SortingFunctionNonUniform sf;
+        sf.div1 = (int)ji("div1");
+        sf.div2 = (int)ji("div2");
+        sf.np = (int)ji("np");
+        BindingModelTernary<SortingFunctionNonUniform> bm(sf);
+        double t[15];
+        for (int k = 0; k < 15; k++) {
+            char key[8];
+            snprintf(key, sizeof key, "t%d", k);
+            t[k] = jd(key);
+        }
+        bm.setup(t[0], t[1], t[2], t[3], t[4], t[5], t[6], t[7], t[8], t[9], t[10], t[11], t[12],
+                 t[13], t[14]);
+        for (int i = 0; i < nh; i++)
+            species[i] = sf(i);
+        put_d(0, bm.doubr11);
+        put_d(1, bm.doubr12);
+        put_d(2, bm.doubr13);
+        put_d(3, bm.doubr22);
+        put_d(4, bm.doubr23);
+        put_d(5, bm.doubr33);
+        vector1<double> *tt[10] = {&bm.tripr111, &bm.tripr112, &bm.tripr113, &bm.tripr122,
+                                   &bm.tripr123, &bm.tripr133, &bm.tripr222, &bm.tripr223,
+                                   &bm.tripr233, &bm.tripr333};
+        for (int c = 0; c < 10; c++)
+            put_t(c, *tt[c], *tt[c]);
+        b.calculate_forces_and_torques3D_onlyone_nonlets(opairs, runs_diff, *pots, bbs, bm, F, T);
+    } else {
+        BindingModelSingle bm(jd("on_rate"), jd("off_rate"));
+        b.calculate_forces_and_torques3D_onlyone_nonlets(opairs, runs_diff, *pots, bbs, bm, F, T);
+    }
+    wr("species", species.data(), species.size());
+    wr("dtab", dtab.data(), dtab.size());
+    wr("ttab", ttab.data(), ttab.size());
+    wr("tr4", tr4.data(), tr4.size());
     for (int i = 0; i < nh; i++) {
         ib[i] = bbs.isbound[i] ? 1 : 0;
         bt[i] = bbs.boundto[i];

@@ -15,6 +15,9 @@ Cases (x, Q, types and patch rows as in mtcode_amd.workloads):
         patches
   wide1 the same starting from wide0's state with a tenth of the bonds
         flipped and a twentieth pointed at random patches
+  bin0, bin1, ter0, ter1  the wide-cone system under BindingModelBinary and
+        BindingModelTernary (TABLE_MODELS), from unbound and from wide1's
+        start; with each model's species per patch and rate tables
 """
 from __future__ import annotations
 
@@ -68,7 +71,18 @@ def systems():
     return ab, wide
 
 
-def run(name, s, isbound, boundto, seed):
+# the table models (BindingModelBinary(div).setup_equilibrium(),
+# BindingModelTernary<SortingFunctionNonUniform>(div1, div2, np).setup(t0..t14))
+# on the wide-cone system: the job keys
+TABLE_MODELS = {
+    "bin": {"model": "binary", "div": 2662},
+    "ter": dict({"model": "ternary", "div1": 1200, "div2": 3000, "np": 4},
+                **{f"t{k}": repr(v) for k, v in enumerate(
+                    [0.9, 0.8, 0.7, 0.6, 0.5, 0.4, 0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9])}),
+}
+
+
+def run(name, s, isbound, boundto, seed, model=None):
     d = os.path.join(WORK, name)
     shutil.rmtree(d, ignore_errors=True)
     os.makedirs(d)
@@ -78,14 +92,20 @@ def run(name, s, isbound, boundto, seed):
     job = {"N": s["N"], "L": repr(L), "periodic": 1, "n": int(L // cut), "cut": repr(cut),
            "seed": seed, "on_rate": repr(ON), "off_rate": repr(OFF)}
     job.update(general_job(s))
+    if model:
+        job.update(model)
     ps = s["patches"]
     po.run_ref("bind", d, job, {"x": s["x"], "Q": s["Q"], "params": ps["params"],
                                 "dirs": np.asarray(ps["dirs"]).reshape(-1, 3),
                                 "isbound": isbound.astype(np.int32),
                                 "boundto": boundto.astype(np.int32)})
-    return {"F": po.read_ref(d, "F").reshape(-1, 3), "T": po.read_ref(d, "T").reshape(-1, 3),
-            "isbound": po.read_ref(d, "isbound_out", np.int32),
-            "boundto": po.read_ref(d, "boundto_out", np.int32)}
+    out = {"F": po.read_ref(d, "F").reshape(-1, 3), "T": po.read_ref(d, "T").reshape(-1, 3),
+           "isbound": po.read_ref(d, "isbound_out", np.int32),
+           "boundto": po.read_ref(d, "boundto_out", np.int32)}
+    if model:
+        out.update({"species": po.read_ref(d, "species", np.int32), "dtab": po.read_ref(d, "dtab"),
+                    "ttab": po.read_ref(d, "ttab"), "tr4": po.read_ref(d, "tr4")})
+    return out
 
 
 def main():
@@ -112,6 +132,21 @@ def main():
             out[k + "_seed"] = np.int64(seed)
             for q, v in r.items():
                 out[f"{k}_{q}_out"] = v
+            print(k, "bound patches", int(r["isbound"].sum()), "of", nh)
+    # the table models on the wide-cone system, from unbound and from the
+    # single model's perturbed state
+    nh = int(sum(wide["patches"]["npatch"][t] for t in wide["patches"]["types"]))
+    for tag, model in TABLE_MODELS.items():
+        for k, ib, bt, seed in ((tag + "0", np.zeros(nh, np.int32), np.zeros(nh, np.int32), 303),
+                                (tag + "1", out["wide1_isbound_in"], out["wide1_boundto_in"], 404)):
+            r = run(k, wide, ib, bt, seed, model)
+            out[k + "_isbound_in"] = ib
+            out[k + "_boundto_in"] = bt
+            out[k + "_seed"] = np.int64(seed)
+            for q in ("F", "T", "isbound", "boundto"):
+                out[f"{k}_{q}_out"] = r[q]
+            for q in ("species", "dtab", "ttab", "tr4"):
+                out[f"{tag}_{q}"] = r[q]
             print(k, "bound patches", int(r["isbound"].sum()), "of", nh)
     np.savez_compressed(os.path.join(OUT, "bind.npz"), **out)
 

@@ -212,3 +212,25 @@ def test_bind_steps_mixed_long():
         frac[prec] = len(b) / nh
     say("  bound fraction", frac)
     assert abs(frac[mdx.PREC_MIXED] - frac[mdx.PREC_FP64]) < 0.02
+
+
+@pytest.mark.parametrize("case", ["bin0", "bin1", "ter0", "ter1"])
+def test_bind_table_models_vs_oracle(case):
+    """BindingModelBinary / BindingModelTernary through
+    mdx_set_binding_tables: FP64 bound state bit-exact and forces within
+    1e-12 of the oracle's species-table form (pinned to the reference's
+    models), over two calls"""
+    tag = case[:3]
+    tabs = {q: Z[f"{tag}_{q}"] for q in ("species", "dtab", "ttab", "tr4")}
+    say("bind table model", case)
+    e, nh = bind_engine(WIDE, mdx.PREC_FP64)
+    e.set_binding_tables(tabs["species"], tabs["dtab"], tabs["ttab"], tabs["tr4"])
+    ib0, bt0 = Z[case + "_isbound_in"], Z[case + "_boundto_in"]
+    e.set_bindings(ib0, bt0)
+    for draw in (3, 4):
+        F, T = e.bind_forces(draw=draw)
+        ib, bt = e.get_bindings()
+        Fo, To, ib0, bt0, st = po.bind_forces(WIDE, ib0, bt0, 0.0, 0.0, key=KEY, draw=draw,
+                                              tables=tabs)
+        assert np.array_equal(ib, ib0) and np.array_equal(bt, bt0)
+        assert rel(F, Fo) < 1e-12 and rel(T, To) < 1e-12

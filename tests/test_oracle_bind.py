@@ -86,3 +86,35 @@ def test_oracle_bind_on_off_limits():
     assert ib.sum() == 0
     _, _, ib, bt, st = po.bind_forces(s, ib0, bt0, 1.0, 0.0, key=(1, 2), draw=0)
     assert st[4] > 0 and ib.sum() == 2 * st[4]
+
+
+@pytest.mark.parametrize("case", ["bin0", "bin1", "ter0", "ter1"])
+def test_oracle_bind_table_models_match_reference(case):
+    """BindingModelBinary (setup_equilibrium) and BindingModelTernary with
+    SortingFunctionNonUniform (species not monotone in the patch index, so
+    triplets are decided in a permuted frame): the oracle's species-table
+    form with the tables and species the harness read out of the reference's
+    models reproduces the reference's bound state bit for bit"""
+    tag = case[:3]
+    tabs = {q: Z[f"{tag}_{q}"] for q in ("species", "dtab", "ttab", "tr4")}
+    F, T, ib, bt, st = po.bind_forces(WIDE, Z[case + "_isbound_in"], Z[case + "_boundto_in"], 0.0,
+                                      0.0, seed=int(Z[case + "_seed"]), tables=tabs)
+    assert np.array_equal(ib, Z[case + "_isbound_out"])
+    assert np.array_equal(bt, Z[case + "_boundto_out"])
+    Fr, Tr = Z[case + "_F_out"], Z[case + "_T_out"]
+    assert np.abs(F - Fr).max() <= 1e-12 * max(1.0, np.abs(Fr).max())
+    assert np.abs(T - Tr).max() <= 1e-12 * max(1.0, np.abs(Tr).max())
+    if tag == "ter":
+        sp = tabs["species"]
+        assert set(np.unique(sp)) == {1, 2, 3} and np.any(np.diff(sp) < 0)
+
+
+def test_oracle_bind_table_model_differs_from_single():
+    """the table model changes the decisions (a mistaken table lookup would
+    not go unnoticed): the ternary tables bind a different set than the
+    single model from the same start and draws"""
+    tabs = {q: Z[f"ter_{q}"] for q in ("species", "dtab", "ttab", "tr4")}
+    a = po.bind_forces(WIDE, Z["ter0_isbound_in"], Z["ter0_boundto_in"], 0.0, 0.0, seed=303,
+                       tables=tabs)
+    b = po.bind_forces(WIDE, Z["ter0_isbound_in"], Z["ter0_boundto_in"], g.ON, g.OFF, seed=303)
+    assert not np.array_equal(a[2], b[2])
