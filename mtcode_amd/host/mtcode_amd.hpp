@@ -109,6 +109,23 @@ template <class T> std::ostream &write_csv(std::ostream &s, const matrix<T> &a)
     return s;
 }
 
+// the vector1 CSV operator (DataStructures/vector1.cpp:682-693): precision 10,
+// comma-separated on one line, no newline; bools as 0 / 1
+template <class T> std::ostream &write_csv(std::ostream &s, const vector1<T> &v)
+{
+    s.precision(10);
+    const int n = v.getsize();
+    for (int i = 0; i < n; ++i) {
+        if (std::is_same<T, bool>::value)
+            s << (v[i] ? 1 : 0);
+        else
+            s << v[i];
+        if (i != n - 1)
+            s << ",";
+    }
+    return s;
+}
+
 // outfunc (matrix2.cpp:1557-1565): writes <s>.csv
 template <class T> void outfunc(const matrix<T> &a, std::string s)
 {
@@ -812,6 +829,62 @@ class LangevinNVTR : public LangevinNVT {
             check(mdx_step(e_, nsteps, noise ? noise + 6 * (size_t)N() : nullptr));
         host_stale_ = true;
         bind_pull(bo);
+    }
+    // Condensate::run_singlebond(runtime, every, strbase) (Condensate.cpp:
+    // 422-618) with its output: after step i with i % every == 0 and
+    // i > int(0.9 runtime), pos<strbase>_i=<k>.csv, orientation<...>.csv (at
+    // precision 10), bindings<...>.csv (isbound, a newline, boundto) and
+    // connectedcomponents<...>.csv (distance_graph at 1.25 on
+    // floor(l / (3 size_mol)) cells per side), k = i / every zero-padded to
+    // the digits of ceil(runtime / every); the steps on the device with
+    // Philox noise
+    void run_singlebond(int runtime, int every, const std::string &strbase, const StepTerms &t,
+                        BinaryBindStore &bo, const BindingModelSingle &bm, double size_mol = 1.0)
+    {
+        if (every < 1)
+            throw mdx_error(MDX_ERR_ARG, "every must be >= 1");
+        if (!t.patches)
+            throw mdx_error(MDX_ERR_ARG, "run_singlebond needs the patches");
+        int tf = (int)std::ceil((double)runtime / (double)every);
+        int digits = 0;
+        do {
+            ++digits;
+            tf /= 10;
+        } while (tf);
+        int ccc;
+        matrix<int> boxes = geo_.generate_boxes_relationships(
+            std::max(1, (int)std::floor(geo_.l / (3. * size_mol))), ccc);
+        apply_terms(t);
+        bind_push(bm, bo);
+        check(mdx_prime(e_, nullptr));
+        int done = 0; // steps taken
+        for (int i = 0; i < runtime; i++) {
+            if (!(i % every == 0 && i > int(0.9 * runtime)))
+                continue;
+            check(mdx_step(e_, i + 1 - done, nullptr));
+            done = i + 1;
+            host_stale_ = true;
+            bind_pull(bo);
+            std::stringstream ss;
+            ss << std::setw(digits) << std::setfill('0') << (i / every);
+            const std::string k = strbase + "_i=" + ss.str() + ".csv";
+            std::ofstream fp(("pos" + k).c_str()), fo(("orientation" + k).c_str()),
+                fb(("bindings" + k).c_str()), fc(("connectedcomponents" + k).c_str());
+            if (!fp.is_open() || !fo.is_open() || !fb.is_open() || !fc.is_open())
+                throw mdx_error(MDX_ERR_ARG, "cannot open output files");
+            write_csv(fp, getdat());
+            fo << std::setprecision(10);
+            write_csv(fo, getorientation());
+            write_csv(fb, bo.isbound);
+            fb << "\n";
+            write_csv(fb, bo.boundto);
+            write_csv(fc, distance_graph(boxes, 1.25));
+        }
+        if (runtime > done) {
+            check(mdx_step(e_, runtime - done, nullptr));
+            host_stale_ = true;
+            bind_pull(bo);
+        }
     }
     // create_forces_and_torques_sphere(F, T, RT) (LangevinR.cpp:475-578)
     void create_forces_and_torques_sphere(matrix<double> &F, matrix<double> &T, matrix<double> &RT)

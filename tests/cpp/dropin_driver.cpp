@@ -18,6 +18,8 @@
 //   integ "stencil" with key n: writes the stencil table out_stencil.bin (no GPU)
 //   integ "csv" with N, M: outfunc / importcsv of x.bin (no GPU)
 //   rotor mode "output": Condensate::run(nsteps, every) with its CSV files
+//   rotor mode "bindout": Condensate::run_singlebond(nsteps, every) with its
+//          CSV files (pos, orientation, bindings, connectedcomponents)
 //   rotor mode "bind" / "bindrun": Condensate::run_singlebond's loop
 //          (BinaryBindStore, BindingModelSingle(on_rate, off_rate),
 //          calculate_forces_and_torques3D_onlyone_nonlets) call by call or
@@ -89,7 +91,21 @@ static int rotor_main()
     TetrahedralPatch pots(jd("eps_p"), jd("d"), jd("thetam"));
     const std::vector<double> nz = rd<double>("noise.bin", (size_t)(nsteps + 1) * N * 6);
     const std::string mode = g_job.at("mode");
-    if (mode == "bind" || mode == "bindrun") {
+    if (mode == "bindout") {
+        // Condensate::run_singlebond(runtime, every, strbase) with its files
+        BindingModelSingle bm(jd("on_rate"), jd("off_rate"));
+        const int nh = pots.get_total_patches(N);
+        BinaryBindStore bbs;
+        bbs.isbound = vector1<bool>(nh);
+        bbs.boundto = vector1<int>(nh);
+        StepTerms t;
+        t.wca = &wsa;
+        t.patches = &pots;
+        if (chdir(g_dir.c_str()) != 0)
+            throw mdx_error(MDX_ERR_ARG, "cannot enter " + g_dir);
+        b.run_singlebond(nsteps, ji("every"), "_test", t, bbs, bm);
+        g_dir = ".";
+    } else if (mode == "bind" || mode == "bindrun") {
         // Condensate::run_singlebond (Condensate.cpp:422-478) from an unbound
         // store: call by call, or as device-resident steps
         BindingModelSingle bm(jd("on_rate"), jd("off_rate"));

@@ -158,6 +158,57 @@ def test_singlebond_loop_vs_oracle(tmp_path, mode):
 
 
 @pytest.mark.gpu
+def test_singlebond_run_writes_reference_files(tmp_path):
+    """Condensate::run_singlebond(runtime, every, strbase) through the C++
+    classes: the reference's file set for the steps i % every == 0 with
+    i > 0.9 runtime (pos, orientation at precision 10, bindings: isbound then
+    boundto, connectedcomponents), bonds mutual, and the components equal to
+    the oracle's distance_graph labels of the written positions"""
+    import test_gpu_bind as tb
+    s = tb.lattice_system(8, 1.0, 1.1, 91)
+    N, runtime, every = s["N"], 60, 5
+    e, d, th = s["patches"]["params"][0]
+    job = {"integ": "rotor", "mode": "bindout", "N": N, "L": repr(float(s["box"][0])),
+           "nsteps": runtime, "every": every, "dt": s["dt"], "gamma": s["gamma"],
+           "gammar": repr(float(s["gammar"])), "kT": s["kT"], "precision": 0,
+           "eps_p": repr(float(e)), "d": repr(float(d)), "thetam": repr(float(th)),
+           "on_rate": 0.7, "off_rate": 0.3}
+    say("drop-in singlebond output")
+    run_driver(str(tmp_path), job, {"x": s["x"], "Q": s["Q"],
+                                    "noise": np.zeros((runtime + 1) * N * 6)})
+    steps = [i for i in range(runtime) if i % every == 0 and i > int(0.9 * runtime)]
+    names = sorted(os.listdir(str(tmp_path)))
+    L = float(s["box"][0])
+    for i in steps:
+        k = f"_test_i={i // every:02d}.csv"
+        for base in ("pos", "orientation", "bindings", "connectedcomponents"):
+            assert base + k in names, base + k
+        x = np.loadtxt(os.path.join(str(tmp_path), "pos" + k), delimiter=",")
+        q = open(os.path.join(str(tmp_path), "orientation" + k)).read().splitlines()
+        assert max(len(v) for ln in q for v in ln.split(",")) > 9   # precision 10
+        ib_s, bt_s = open(os.path.join(str(tmp_path), "bindings" + k)).read().split("\n")
+        ib = np.array([int(v) for v in ib_s.split(",")])
+        bt = np.array([int(v) for v in bt_s.split(",")])
+        assert len(ib) == 4 * N and set(np.unique(ib)) <= {0, 1} and ib.sum() > 0
+        b = np.nonzero(ib)[0]
+        assert np.all(ib[bt[b]] == 1) and np.all(bt[bt[b]] == b)
+        cc = np.array([int(v) for v in
+                       open(os.path.join(str(tmp_path), "connectedcomponents" + k)).read().split(",")])
+        # labels: each the smallest index of its component; pairs within the
+        # distance share one.  The CSV positions carry 6 digits: the oracle's
+        # labels of them must agree unless a pair sits within 1e-4 of 1.25
+        assert np.all(cc <= np.arange(N)) and np.all(cc[cc] == cc)
+        dx = x[:, None, :] - x[None, :, :]
+        dx -= L * np.round(dx / L)
+        r = np.sqrt((dx ** 2).sum(-1))
+        close = np.argwhere((r < 1.25 - 1e-4) & ~np.eye(N, dtype=bool))
+        assert np.all(cc[close[:, 0]] == cc[close[:, 1]])
+        if not np.any(np.abs(r - 1.25) < 1e-4):
+            n = max(1, int(np.floor(L / 3.0)))
+            assert np.array_equal(cc, po.dgraph_labels(L, x, n, 1.25))
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["stages", "run"])
 def test_bead_loop_vs_oracle(tmp_path, mode):
     s = wl.mt_bundle(2, 13, 40, spacing=6.0, box=(40.0, 40.0, 40.0))
