@@ -344,11 +344,134 @@ struct BinaryBindStore {
     vector1<int> boundto;
 };
 
-// BindingModelSingle(on_rate, off_rate) (Bindings/BindingModelSingle.cpp:1-8):
-// the doublet / triplet rules of the device's binding kernels
-struct BindingModelSingle {
+// AbstractBindingModel (Bindings/BindingModeAbstract.h:6-20): what a model
+// hands the engine (mdx_set_binding, and for the species-table models
+// mdx_set_binding_tables)
+struct AbstractBindingModel {
+    virtual ~AbstractBindingModel() {}
+    virtual void apply(mdx_engine *e, int *nh) const = 0;
+};
+
+// BindingModelSingle(on_rate, off_rate) (Bindings/BindingModelSingle.cpp:1-8)
+struct BindingModelSingle : AbstractBindingModel {
     double on_rate, off_rate;
     BindingModelSingle(double onn, double offf) : on_rate(onn), off_rate(offf) {}
+    void apply(mdx_engine *e, int *nh) const override
+    {
+        check(mdx_set_binding(e, on_rate, off_rate, nh));
+    }
+};
+
+// the species-table form shared by the binary and ternary models: the
+// species of each patch (1..3), the doublet rates of each sorted species
+// pair (11,12,13,22,23,33) and the triplet rates of each sorted triple
+// (111,112,113,122,123,133,222,223,233,333), with the "to none" rate each
+// triple reads
+inline void apply_tables(mdx_engine *e, int *nh, const std::vector<int> &species,
+                         const vector1<double> *const d[6], const vector1<double> *const t[10],
+                         const vector1<double> *const r4[10])
+{
+    check(mdx_set_binding(e, 1.0, 0.0, nh));
+    if ((int)species.size() != *nh)
+        throw mdx_error(MDX_ERR_ARG, "species per patch");
+    std::vector<double> dtab(24, 0.0), ttab(160, 0.0), tr4(40, 0.0);
+    for (int c = 0; c < 6; c++)
+        if (d[c])
+            for (int k = 0; k < 4; k++)
+                dtab[(size_t)(4 * c + k)] = (*d[c])[k];
+    for (int c = 0; c < 10; c++)
+        if (t[c]) {
+            for (int k = 0; k < 16; k++)
+                ttab[(size_t)(16 * c + k)] = (*t[c])[k];
+            for (int f = 0; f < 4; f++)
+                tr4[(size_t)(4 * c + f)] = (*r4[c])[4 * f + 3];
+        }
+    check(mdx_set_binding_tables(e, species.data(), dtab.data(), ttab.data(), tr4.data()));
+}
+
+// BindingModelBinary(div) (Bindings/BindingModelBinary.h:6-41): species 1
+// below patch div, 2 from it; setup_equilibrium() sets the model's own
+// tables (BindingModelBinary.cpp:8-96).  As in the reference, its 122 and
+// 222 triplets read their rate to none from the 111 table.
+struct BindingModelBinary : AbstractBindingModel {
+    vector1<double> doubr11, doubr12, doubr22, tripr111, tripr112, tripr122, tripr222;
+    int div;
+    explicit BindingModelBinary(int divv)
+        : doubr11(4), doubr12(4), doubr22(4), tripr111(16), tripr112(16), tripr122(16),
+          tripr222(16), div(divv)
+    {
+    }
+    void setup_equilibrium()
+    {
+        const double d[4] = {0.0, 1.0, 0.0, 1.0};
+        // from (bound 12, 23, 13, none) x to (12, 23, 13, none)
+        const double t[16] = {0.998, 0.001, 0.001, 0.000, 0.001, 0.998, 0.001, 0.000,
+                              0.001, 0.001, 0.998, 0.000, 0.33,  0.33,  0.33,  0.01};
+        for (int k = 0; k < 4; k++)
+            doubr11[k] = doubr12[k] = doubr22[k] = d[k];
+        for (int k = 0; k < 16; k++)
+            tripr111[k] = tripr112[k] = tripr122[k] = tripr222[k] = t[k];
+    }
+    void apply(mdx_engine *e, int *nh) const override
+    {
+        int n = 0;
+        check(mdx_set_binding(e, 1.0, 0.0, &n));
+        std::vector<int> sp((size_t)n);
+        for (int i = 0; i < n; i++)
+            sp[(size_t)i] = i < div ? 1 : 2;
+        const vector1<double> *d[6] = {&doubr11, &doubr12, nullptr, &doubr22, nullptr, nullptr};
+        const vector1<double> *t[10] = {&tripr111, &tripr112, nullptr, &tripr122, nullptr,
+                                        nullptr, &tripr222, nullptr, nullptr, nullptr};
+        const vector1<double> *r4[10] = {&tripr111, &tripr112, nullptr, &tripr111, nullptr,
+                                         nullptr, &tripr111, nullptr, nullptr, nullptr};
+        apply_tables(e, nh, sp, d, t, r4);
+    }
+};
+
+// SortingFunctionUniform / SortingFunctionNonUniform
+// (Bindings/BindingModelTernary.h:7-37): the species of a patch index
+struct SortingFunctionUniform {
+    int div1, div2;
+    int operator()(const int &i) const { return i < div1 ? 1 : i < div2 ? 2 : 3; }
+};
+struct SortingFunctionNonUniform {
+    int div1, div2, np;
+    int operator()(const int &i) const
+    {
+        if (i < div1)
+            return 1;
+        if (i < div2)
+            return 2;
+        return (i - div2) % np == np - 1 ? 3 : 1;
+    }
+};
+
+// BindingModelTernary<Q>(sorter) (Bindings/BindingModelTernary.h:38-101):
+// species by the sorting function; the caller fills the rate tables (the
+// reference's setup(...) builds them from its parameters)
+template <typename Q> struct BindingModelTernary : AbstractBindingModel {
+    vector1<double> doubr11, doubr12, doubr22, doubr13, doubr23, doubr33;
+    vector1<double> tripr111, tripr112, tripr113, tripr122, tripr123, tripr133, tripr222,
+        tripr223, tripr233, tripr333;
+    Q func;
+    explicit BindingModelTernary(Q &f)
+        : doubr11(4), doubr12(4), doubr22(4), doubr13(4), doubr23(4), doubr33(4), tripr111(16),
+          tripr112(16), tripr113(16), tripr122(16), tripr123(16), tripr133(16), tripr222(16),
+          tripr223(16), tripr233(16), tripr333(16), func(f)
+    {
+    }
+    void apply(mdx_engine *e, int *nh) const override
+    {
+        int n = 0;
+        check(mdx_set_binding(e, 1.0, 0.0, &n));
+        std::vector<int> sp((size_t)n);
+        for (int i = 0; i < n; i++)
+            sp[(size_t)i] = func(i);
+        const vector1<double> *d[6] = {&doubr11, &doubr12, &doubr13, &doubr22, &doubr23, &doubr33};
+        const vector1<double> *t[10] = {&tripr111, &tripr112, &tripr113, &tripr122, &tripr123,
+                                        &tripr133, &tripr222, &tripr223, &tripr233, &tripr333};
+        apply_tables(e, nh, sp, d, t, t);
+    }
 };
 
 // ------------------------------------------------------------ integrators
@@ -697,10 +820,10 @@ class LangevinNVTR : public LangevinNVT {
     unsigned long long bind_draw_ = 0; // draws of the call-by-call binding routine
     // the binding model and the bound state into the engine (after the
     // patches are set)
-    void bind_push(const BindingModelSingle &bm, const BinaryBindStore &bo)
+    void bind_push(const AbstractBindingModel &bm, const BinaryBindStore &bo)
     {
         int nh = 0;
-        check(mdx_set_binding(e_, bm.on_rate, bm.off_rate, &nh));
+        bm.apply(e_, &nh);
         if (bo.isbound.getsize() != nh || bo.boundto.getsize() != nh)
             throw mdx_error(MDX_ERR_ARG, "BinaryBindStore size is not the number of patches");
         std::vector<int> ib((size_t)nh), bt((size_t)nh);
@@ -800,7 +923,7 @@ class LangevinNVTR : public LangevinNVT {
     template <class PL, class DV>
     void calculate_forces_and_torques3D_onlyone_nonlets(PL &, DV &, ComboPatch &iny,
                                                         BinaryBindStore &bo,
-                                                        const BindingModelSingle &bm,
+                                                        const AbstractBindingModel &bm,
                                                         matrix<double> &F, matrix<double> &T)
     {
         StepTerms t;
@@ -818,7 +941,7 @@ class LangevinNVTR : public LangevinNVT {
     // terms (t: the WCA and the patches), the bound state carried in bo.
     // noise null (engine Philox) or (nsteps + 1) x N x 6, row 0 for the set-up.
     void run_singlebond(int nsteps, const StepTerms &t, BinaryBindStore &bo,
-                        const BindingModelSingle &bm, const double *noise = nullptr)
+                        const AbstractBindingModel &bm, const double *noise = nullptr)
     {
         if (!t.patches)
             throw mdx_error(MDX_ERR_ARG, "run_singlebond needs the patches");
@@ -839,7 +962,7 @@ class LangevinNVTR : public LangevinNVT {
     // the digits of ceil(runtime / every); the steps on the device with
     // Philox noise
     void run_singlebond(int runtime, int every, const std::string &strbase, const StepTerms &t,
-                        BinaryBindStore &bo, const BindingModelSingle &bm, double size_mol = 1.0)
+                        BinaryBindStore &bo, const AbstractBindingModel &bm, double size_mol = 1.0)
     {
         if (every < 1)
             throw mdx_error(MDX_ERR_ARG, "every must be >= 1");

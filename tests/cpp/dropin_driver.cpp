@@ -107,8 +107,37 @@ static int rotor_main()
         g_dir = ".";
     } else if (mode == "bind" || mode == "bindrun") {
         // Condensate::run_singlebond (Condensate.cpp:422-478) from an unbound
-        // store: call by call, or as device-resident steps
-        BindingModelSingle bm(jd("on_rate"), jd("off_rate"));
+        // store: call by call, or as device-resident steps; model single
+        // (on_rate, off_rate), binary (div, setup_equilibrium) or ternary
+        // (SortingFunctionNonUniform div1 div2 np, tables from dtab.bin /
+        // ttab.bin in the pair / triple order 11,12,13,22,23,33 and
+        // 111,...,333)
+        const std::string model = g_job.count("model") ? g_job.at("model") : "single";
+        BindingModelSingle single(jd("on_rate"), jd("off_rate"));
+        BindingModelBinary binary(g_job.count("div") ? ji("div") : 0);
+        binary.setup_equilibrium();
+        SortingFunctionNonUniform sf{g_job.count("div1") ? ji("div1") : 0,
+                                     g_job.count("div2") ? ji("div2") : 0,
+                                     g_job.count("np") ? ji("np") : 1};
+        BindingModelTernary<SortingFunctionNonUniform> ternary(sf);
+        if (model == "ternary") {
+            const std::vector<double> dt = rd<double>("dtab.bin", 24), tt = rd<double>("ttab.bin", 160);
+            vector1<double> *d[6] = {&ternary.doubr11, &ternary.doubr12, &ternary.doubr13,
+                                     &ternary.doubr22, &ternary.doubr23, &ternary.doubr33};
+            vector1<double> *t[10] = {&ternary.tripr111, &ternary.tripr112, &ternary.tripr113,
+                                      &ternary.tripr122, &ternary.tripr123, &ternary.tripr133,
+                                      &ternary.tripr222, &ternary.tripr223, &ternary.tripr233,
+                                      &ternary.tripr333};
+            for (int c = 0; c < 6; c++)
+                for (int k = 0; k < 4; k++)
+                    (*d[c])[k] = dt[(size_t)(4 * c + k)];
+            for (int c = 0; c < 10; c++)
+                for (int k = 0; k < 16; k++)
+                    (*t[c])[k] = tt[(size_t)(16 * c + k)];
+        }
+        const AbstractBindingModel &bm = model == "binary"    ? (const AbstractBindingModel &)binary
+                                         : model == "ternary" ? (const AbstractBindingModel &)ternary
+                                                              : (const AbstractBindingModel &)single;
         const int nh = pots.get_total_patches(N);
         BinaryBindStore bbs;
         bbs.isbound = vector1<bool>(nh);

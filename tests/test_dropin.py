@@ -119,14 +119,17 @@ def test_rotor_condensate_loop_vs_oracle(tmp_path, mode):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", ["bind", "bindrun"])
-def test_singlebond_loop_vs_oracle(tmp_path, mode):
+@pytest.mark.parametrize("mode,model", [("bind", "single"), ("bindrun", "single"),
+                                        ("bind", "binary"), ("bindrun", "ternary")])
+def test_singlebond_loop_vs_oracle(tmp_path, mode, model):
     """Condensate::run_singlebond through the C++ classes (BinaryBindStore,
-    BindingModelSingle, calculate_forces_and_torques3D_onlyone_nonlets), call
-    by call and on the device, against the oracle's loop with the same
-    injected noise and binding draws (0, 1, 2, ...: the call-by-call counter
-    and a new engine's force-evaluation count): bound state equal, state
-    within 1e-10"""
+    BindingModelSingle / BindingModelBinary / BindingModelTernary,
+    calculate_forces_and_torques3D_onlyone_nonlets), call by call and on the
+    device, against the oracle's loop with the same injected noise and
+    binding draws (0, 1, 2, ...: the call-by-call counter and a new engine's
+    force-evaluation count): bound state equal, state within 1e-10.  The
+    binary model's tables come from the classes' own setup_equilibrium, the
+    oracle's from the reference's (tests/golden/bind.npz)"""
     import test_gpu_bind as tb
     s = wl.thermal_momenta(tb.lattice_system(8, 1.0, 1.1, 81), seed=3)
     N, nsteps, seed = s["N"], 25, 12
@@ -138,13 +141,27 @@ def test_singlebond_loop_vs_oracle(tmp_path, mode):
            "kT": s["kT"], "precision": 0, "eps_p": repr(float(e)), "d": repr(float(d)),
            "thetam": repr(float(th)), "on_rate": 0.7, "off_rate": 0.3}
     s0 = dict(s, p=np.zeros((N, 3)), L=np.zeros((N, 3)))  # the driver starts from rest
-    run_driver(str(tmp_path), job, {"x": s["x"], "Q": s["Q"], "noise": nz})
+    nh = 4 * N
+    arrays = {"x": s["x"], "Q": s["Q"], "noise": nz}
+    tables = None
+    if model == "binary":
+        job.update({"model": "binary", "div": nh // 2})
+        tables = {q: tb.Z[f"bin_{q}"] for q in ("dtab", "ttab", "tr4")}
+        tables["species"] = np.where(np.arange(nh) < nh // 2, 1, 2).astype(np.int32)
+    elif model == "ternary":
+        job.update({"model": "ternary", "div1": 400, "div2": 1200, "np": 4})
+        tables = {q: tb.Z[f"ter_{q}"] for q in ("dtab", "ttab", "tr4")}
+        i = np.arange(nh)
+        tables["species"] = np.where(i < 400, 1, np.where(i < 1200, 2,
+                                     np.where((i - 1200) % 4 == 3, 3, 1))).astype(np.int32)
+        arrays.update({"dtab": tables["dtab"], "ttab": tables["ttab"]})
+    run_driver(str(tmp_path), job, arrays)
     key = (1, 0xB1D0B1D0)   # the drop-in classes' default seed 1
     tb_key = tb.KEY
     tb.KEY = key
     try:
         ref, ibo, bto = tb.oracle_bind_run(s0, nsteps, nz.reshape(nsteps + 1, N, 6), 0.7, 0.3,
-                                           list(range(nsteps + 1)))
+                                           list(range(nsteps + 1)), tables=tables)
     finally:
         tb.KEY = tb_key
     nh = len(ibo)

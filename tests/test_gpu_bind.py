@@ -141,7 +141,7 @@ def test_bind_component_capacity_is_an_error():
     assert np.all(np.isfinite(F))
 
 
-def oracle_bind_run(s, nsteps, noise, on, off, draws):
+def oracle_bind_run(s, nsteps, noise, on, off, draws, tables=None):
     """the loop of Condensate::run_singlebond (Condensate.cpp:464-478) on the
     oracle: half kick, drift, rotation, F = WCA + the binding routine's
     bound-pair terms, the sphere's friction and noise, half kick"""
@@ -153,7 +153,8 @@ def oracle_bind_run(s, nsteps, noise, on, off, draws):
 
     def forces(x, Q, ib, bt, draw):
         F, T = po.compute_forces(sw, x=x, Q=Q)
-        Fb, Tb, ib, bt, _ = po.bind_forces(s, ib, bt, on, off, key=KEY, draw=draw, x=x, Q=Q)
+        Fb, Tb, ib, bt, _ = po.bind_forces(s, ib, bt, on, off, key=KEY, draw=draw, x=x, Q=Q,
+                                           tables=tables)
         return F + Fb, T + Tb, ib, bt
 
     F, T, ib, bt = forces(x, Q, ib, bt, draws[0])
