@@ -18,6 +18,12 @@ Cases (x, Q, types and patch rows as in mtcode_amd.workloads):
   bin0, bin1, ter0, ter1  the wide-cone system under BindingModelBinary and
         BindingModelTernary (TABLE_MODELS), from unbound and from wide1's
         start; with each model's species per patch and rate tables
+
+    python tests/golden/make_bind_golden.py wall
+
+writes tests/golden/bind_wall.npz: wall0, wall1, the wide-cone system of
+wide0 / wide1 in a box with reflecting walls (cube periodicity false, the
+default of Condensate.cpp:17-18): no candidate pair across a face.
 """
 from __future__ import annotations
 
@@ -82,6 +88,11 @@ TABLE_MODELS = {
 }
 
 
+def walled(s):
+    """the same system in a box with reflecting walls"""
+    return dict(s, periodic=False)
+
+
 def run(name, s, isbound, boundto, seed, model=None):
     d = os.path.join(WORK, name)
     shutil.rmtree(d, ignore_errors=True)
@@ -89,7 +100,8 @@ def run(name, s, isbound, boundto, seed, model=None):
     L = float(s["box"][0])
     mk = max(1.0, 1.4 * float(np.max(s["patches"]["params"][:, 1])))
     cut = mk + 0.05
-    job = {"N": s["N"], "L": repr(L), "periodic": 1, "n": int(L // cut), "cut": repr(cut),
+    job = {"N": s["N"], "L": repr(L), "periodic": 1 if s.get("periodic", True) else 0,
+           "n": int(L // cut), "cut": repr(cut),
            "seed": seed, "on_rate": repr(ON), "off_rate": repr(OFF)}
     job.update(general_job(s))
     if model:
@@ -151,5 +163,27 @@ def main():
     np.savez_compressed(os.path.join(OUT, "bind.npz"), **out)
 
 
+def main_wall():
+    if not po.have_ref():
+        raise SystemExit("build oracle/_ref first: make -C oracle ref")
+    _, wide = systems()
+    s = walled(wide)
+    Z = np.load(os.path.join(OUT, "bind.npz"))
+    out = {}
+    for k, src, seed in (("wall0", "wide0", 505), ("wall1", "wide1", 606)):
+        ib, bt = Z[src + "_isbound_in"], Z[src + "_boundto_in"]
+        r = run(k, s, ib, bt, seed)
+        out[k + "_isbound_in"] = ib
+        out[k + "_boundto_in"] = bt
+        out[k + "_seed"] = np.int64(seed)
+        for q, v in r.items():
+            out[f"{k}_{q}_out"] = v
+        print(k, "bound patches", int(r["isbound"].sum()), "of", len(ib))
+    np.savez_compressed(os.path.join(OUT, "bind_wall.npz"), **out)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["wall"]:
+        main_wall()
+    else:
+        main()

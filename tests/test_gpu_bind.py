@@ -82,6 +82,27 @@ def test_bind_mixed_vs_oracle(case):
     assert rel(F, Fo) < 1e-5 and rel(T, To) < 1e-5
 
 
+W = np.load(os.path.join(HERE, "golden", "bind_wall.npz"))
+
+
+@pytest.mark.parametrize("case", ["wall0", "wall1"])
+def test_bind_walls_fp64_vs_oracle(case):
+    """reflecting walls (the condensate's default box; the oracle is pinned
+    to the reference on it): FP64 bound state bit-exact and forces within
+    1e-12 over two calls"""
+    s = g.walled(WIDE)
+    say("bind walls fp64", case)
+    e, nh = bind_engine(s, mdx.PREC_FP64)
+    ib0, bt0 = W[case + "_isbound_in"], W[case + "_boundto_in"]
+    e.set_bindings(ib0, bt0)
+    for draw in (11, 12):
+        F, T = e.bind_forces(draw=draw)
+        ib, bt = e.get_bindings()
+        Fo, To, ib0, bt0, _ = po.bind_forces(s, ib0, bt0, g.ON, g.OFF, key=KEY, draw=draw)
+        assert np.array_equal(ib, ib0) and np.array_equal(bt, bt0)
+        assert rel(F, Fo) < 1e-12 and rel(T, To) < 1e-12
+
+
 def lattice_system(m, thetam, d, seed):
     s = g.lattice(wl.patchy_tetra(m ** 3, phi=0.05, seed_pos=seed, seed_q=seed + 1), m, 1.15,
                   seed + 2)
@@ -169,15 +190,17 @@ def oracle_bind_run(s, nsteps, noise, on, off, draws, tables=None):
     return {"x": x, "p": p, "Q": Q, "L": L}, ib, bt
 
 
-@pytest.mark.parametrize("skin", [0.0, 0.4])
-def test_bind_steps_fp64_vs_oracle(skin):
+@pytest.mark.parametrize("skin,walls", [(0.0, False), (0.4, False), (0.4, True)])
+def test_bind_steps_fp64_vs_oracle(skin, walls):
     """20 steps of the binding loop with injected noise (cell walk, and the
-    neighbour list with its rebuilds): the bound state equal to the oracle's
-    and the state within 1e-10"""
+    neighbour list with its rebuilds; periodic box and reflecting walls): the
+    bound state equal to the oracle's and the state within 1e-10"""
     s = wl.thermal_momenta(lattice_system(11, 1.0, 1.1, 61), seed=5)
+    if walls:
+        s = g.walled(s)
     N, nsteps = s["N"], 20
     noise = po.noise(123, 0, (nsteps + 1) * N * 6).reshape(nsteps + 1, N, 6)
-    say("bind steps fp64 skin", skin)
+    say("bind steps fp64 skin", skin, "walls", walls)
     e, nh = bind_engine(s, mdx.PREC_FP64, 0.7, 0.3)
     e.set_neighbor_list(skin, 64, 200)
     c0 = e.noise_counter()

@@ -32,6 +32,26 @@ def test_oracle_bind_matches_reference(case):
         assert st[2] > 0 and st[3] > 3   # components of 4+ patches were cut
 
 
+W = np.load(os.path.join(HERE, "golden", "bind_wall.npz"))
+
+
+@pytest.mark.parametrize("case", ["wall0", "wall1"])
+def test_oracle_bind_walls_match_reference(case):
+    """reflecting walls (the condensate's default box): the reference's
+    routine on the wide-cone system without periodic images, reproduced bit
+    for bit; the periodic box gives another bound set on the same inputs"""
+    s = g.walled(WIDE)
+    ib0, bt0, seed = W[case + "_isbound_in"], W[case + "_boundto_in"], int(W[case + "_seed"])
+    F, T, ib, bt, st = po.bind_forces(s, ib0, bt0, g.ON, g.OFF, seed=seed)
+    assert np.array_equal(ib, W[case + "_isbound_out"])
+    assert np.array_equal(bt, W[case + "_boundto_out"])
+    Fr, Tr = W[case + "_F_out"], W[case + "_T_out"]
+    assert np.abs(F - Fr).max() <= 1e-12 * max(1.0, np.abs(Fr).max())
+    assert np.abs(T - Tr).max() <= 1e-12 * max(1.0, np.abs(Tr).max())
+    _, _, ibp, btp, _ = po.bind_forces(WIDE, ib0, bt0, g.ON, g.OFF, seed=seed)
+    assert not (np.array_equal(ibp, ib) and np.array_equal(btp, bt))
+
+
 def test_oracle_bind_fixture_has_every_path():
     """the wide-cone fixture reaches the large-component cut (sizes to 27),
     breaks bonds and re-forms them"""
