@@ -253,6 +253,19 @@ int mdx_get_bindings(mdx_engine *e, int *isbound, int *boundto);
  * of smallest patches).  MDX_ERR_CAPACITY when a patch has more than 8
  * candidates or a component more than 64 patches. */
 int mdx_bind_forces(mdx_engine *e, unsigned long long draw, double *F, double *T);
+/* LangevinNVTR::calculate_patch_list(pairs, ComboPatch&)
+ * (Rotational/LangevinR.cpp:1702-1853): for each of the npairs particle
+ * pairs (original indices, 2 per row) closer than max_check + 1, each patch
+ * pair whose cone arguments satisfy (a_i - max_ang) + (a_j - max_ang) > -0.4
+ * (max_check: 1 raised to the largest 1.4 d; max_ang: 0.6 lowered to the
+ * smallest cos theta_m) gives a row of out (5 ints: p1 < p2, the potential
+ * index in the patch-row order of mdx_set_patches, the patch of p1, the
+ * patch of p2, patches numbered by particle as in mdx_set_bindings), in pair
+ * order and ascending potential.  *count = the number of rows;
+ * MDX_ERR_CAPACITY (and *count) when it exceeds cap.  A single rotor system
+ * whose particles are numbered by type (GeneralPatch's type ranges). */
+int mdx_patch_list(mdx_engine *e, const int *pairs, long long npairs, int *out, long long cap,
+                   long long *count);
 
 /* Time stepping. */
 /* Rotor integrator: computes the carried F, T at the current state

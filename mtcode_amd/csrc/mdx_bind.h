@@ -882,4 +882,64 @@ __global__ void __launch_bounds__(256) k_bind_forces(KArgs<R> a, BindArgs b, dou
         bind_forces_one(a, b, Fb, Tb, 0, off[t]);
 }
 
+// LangevinNVTR::calculate_patch_list (Rotational/LangevinR.cpp:1702-1853) on
+// a caller's pair list (original indices), one thread per pair: a pair
+// closer than maxdis (max_check + 1) sets bit ia * np(t2) + ib of mask[k] for
+// each patch ia of p1 and ib of p2 whose cone arguments satisfy
+// (a_i - maxang) + (a_j - maxang) > -0.4, with p1 < p2 (particles numbered
+// by type, so t1 <= t2, checked by the host).  FP64 in the reference's
+// expression order; orientations as the engine holds them.
+template <class R>
+__global__ void __launch_bounds__(256) k_patch_list(KArgs<R> a, const int *pairs, long long npairs,
+                                                    double maxdis, double maxang,
+                                                    unsigned long long *mask)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npairs)
+        return;
+    int p1 = pairs[2 * k], p2 = pairs[2 * k + 1];
+    if (p2 < p1) {
+        const int t = p1;
+        p1 = p2;
+        p2 = t;
+    }
+    const int s1 = a.slot[p1], s2 = a.slot[p2];
+    const PosRec x1 = a.X[s1], x2 = a.X[s2];
+    const DevRep rp = rep_rec(a, x1.meta);
+    double un[3];
+    min_image(a, rp, x1.x, x1.y, x1.z, x2, un);
+    double dis = un[0] * un[0] + un[1] * un[1] + un[2] * un[2];
+    dis = sqrt(dis);
+    unsigned long long m = 0ull;
+    if (dis < maxdis) {
+        for (int c = 0; c < 3; c++)
+            un[c] /= dis;
+        R q1[9], q2[9];
+        load_orient(a.Q, (size_t)s1, q1);
+        load_orient(a.Q, (size_t)s2, q2);
+        const int t1 = x1.meta & 255, t2 = x2.meta & 255;
+        const int n1 = a.np[t1], n2 = a.np[t2];
+        for (int ia = 0; ia < n1; ia++) {
+            const double *b1 = a.pdirs + 3 * (a.pstart[t1] + ia);
+            const double nx1 = b1[0] * (double)q1[0] + b1[1] * (double)q1[3] + b1[2] * (double)q1[6];
+            const double ny1 = b1[0] * (double)q1[1] + b1[1] * (double)q1[4] + b1[2] * (double)q1[7];
+            const double nz1 = b1[0] * (double)q1[2] + b1[1] * (double)q1[5] + b1[2] * (double)q1[8];
+            const double ai = -(nx1 * un[0] + ny1 * un[1] + nz1 * un[2]);
+            for (int ib = 0; ib < n2; ib++) {
+                const double *b2 = a.pdirs + 3 * (a.pstart[t2] + ib);
+                const double nx2 =
+                    b2[0] * (double)q2[0] + b2[1] * (double)q2[3] + b2[2] * (double)q2[6];
+                const double ny2 =
+                    b2[0] * (double)q2[1] + b2[1] * (double)q2[4] + b2[2] * (double)q2[7];
+                const double nz2 =
+                    b2[0] * (double)q2[2] + b2[1] * (double)q2[5] + b2[2] * (double)q2[8];
+                const double aj = nx2 * un[0] + ny2 * un[1] + nz2 * un[2];
+                if ((ai - maxang) + (aj - maxang) > -0.4)
+                    m |= 1ull << (ia * n2 + ib);
+            }
+        }
+    }
+    mask[k] = m;
+}
+
 } // namespace mdx

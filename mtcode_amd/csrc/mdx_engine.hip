@@ -159,6 +159,7 @@ struct mdx_engine {
     double bind_on_rate = 0, bind_off_rate = 0, bind_mk = 0, bind_cmin = 1;
     int nh = 0;
     Buf bpoff, bowner, bisbound, bboundto, badj_n, badj_p, badj_s, bF, bT, bscr, bnscr;
+    Buf plpairs, plmask; // mdx_patch_list: the caller's pairs, a bit mask per pair
     int bind_table = 0; // 1: the species-table model (mdx_set_binding_tables)
     Buf bspecies, bdtab, bttab, btr4;
 
@@ -1172,7 +1173,8 @@ void mdx_destroy(mdx_engine *e)
                    &e->ncand,   &e->cand,    &e->nbr,    &e->nbrc,   &e->nn,     &e->bcell,
                    &e->dslot,   &e->Tp,      &e->bpoff,  &e->bowner, &e->bisbound, &e->bboundto,
                    &e->badj_n,  &e->badj_p,  &e->badj_s, &e->bF,     &e->bT,      &e->bscr,
-                   &e->bnscr,   &e->bspecies, &e->bdtab, &e->bttab,  &e->btr4};
+                   &e->bnscr,   &e->bspecies, &e->bdtab, &e->bttab,  &e->btr4,
+                   &e->plpairs, &e->plmask};
     for (Buf *b : bufs)
         dfree(*b);
     for (GraphSlot &g : e->graphs)
@@ -2253,6 +2255,86 @@ int mdx_bind_forces(mdx_engine *e, unsigned long long draw, double *F, double *T
     HIPCHK(hipSetDevice(e->device));
     return e->prec == MDX_PREC_FP64 ? bind_forces_t<double>(e, draw, F, T)
                                     : bind_forces_t<float>(e, draw, F, T);
+}
+
+// calculate_patch_list on the device (k_patch_list), the masks expanded into
+// rows (p1, p2, potn, patch of p1, patch of p2) in pair order, ascending potn
+template <class R>
+static int patch_list_t(mdx_engine *e, const int *pairs, long long npairs, int *out,
+                        long long cap, long long *count)
+{
+    int rc;
+    if (!e->binned && (rc = bin_current<R>(e)))
+        return rc;
+    if (npairs == 0)
+        return MDX_OK;
+    if ((rc = upload(e, e->plpairs, pairs, 2 * (size_t)npairs)) ||
+        (rc = dalloc(e, e->plmask, 8 * (size_t)npairs)))
+        return rc;
+    double mk, cmin;
+    bind_limits(e, mk, cmin);
+    // ComboPatch::max_ang starts at 0.6 (combopatch.h:15); GeneralPatch
+    // lowers it to the smallest cos(theta_m) (GeneralPatch.cpp:98-99)
+    const double maxang = std::min(0.6, cmin);
+    const KArgs<R> a = make_args<R>(e);
+    const unsigned int nb = (unsigned int)((npairs + 255) / 256);
+    hipLaunchKernelGGL(k_patch_list<R>, dim3(nb), dim3(256), 0, e->stream, a,
+                       (const int *)e->plpairs.p, npairs, mk + 1.0, maxang,
+                       (unsigned long long *)e->plmask.p);
+    HIPCHK(hipGetLastError());
+    std::vector<unsigned long long> m((size_t)npairs);
+    if ((rc = download(e, e->plmask, m.data(), m.size())))
+        return rc;
+    std::vector<int> poff((size_t)e->N + 1, 0);
+    for (int i = 0; i < e->N; i++)
+        poff[i + 1] = poff[i] + e->np[e->meta[i] & 255];
+    long long n = 0;
+    for (long long k = 0; k < npairs; k++) {
+        if (!m[k])
+            continue;
+        const int p1 = std::min(pairs[2 * k], pairs[2 * k + 1]);
+        const int p2 = std::max(pairs[2 * k], pairs[2 * k + 1]);
+        const int t1 = e->meta[p1] & 255, t2 = e->meta[p2] & 255;
+        for (int bit = 0; bit < 64; bit++) {
+            if (!((m[k] >> bit) & 1ull))
+                continue;
+            if (n < cap) {
+                int *r = out + 5 * n;
+                r[0] = p1;
+                r[1] = p2;
+                r[2] = a.rstart[t1 * MAXT + t2] + bit;
+                r[3] = poff[p1] + bit / e->np[t2];
+                r[4] = poff[p2] + bit % e->np[t2];
+            }
+            n++;
+        }
+    }
+    *count = n;
+    if (n > cap)
+        return set_err(MDX_ERR_CAPACITY, "patch list: %lld rows, room for %lld", n, cap);
+    return MDX_OK;
+}
+
+int mdx_patch_list(mdx_engine *e, const int *pairs, long long npairs, int *out, long long cap,
+                   long long *count)
+{
+    if (!e || !count || npairs < 0 || cap < 0 || (npairs > 0 && !pairs) || (cap > 0 && !out))
+        return set_err(MDX_ERR_ARG, "bad arguments");
+    *count = 0;
+    if (e->integ != MDX_INTEG_ROTOR || e->ntypes == 0)
+        return set_err(MDX_ERR_STATE, "the patch list needs the rotor integrator and patches");
+    if (e->nrep != 1)
+        return set_err(MDX_ERR_STATE, "the patch list is for a single system");
+    for (int i = 1; i < e->N; i++)
+        if ((e->meta[i] & 255) < (e->meta[i - 1] & 255))
+            return set_err(MDX_ERR_STATE,
+                           "the patch list needs particles numbered by type (type ranges)");
+    for (long long k = 0; k < 2 * npairs; k++)
+        if (pairs[k] < 0 || pairs[k] >= e->N)
+            return set_err(MDX_ERR_ARG, "pairs[%lld] = %d outside 0..%d", k, pairs[k], e->N - 1);
+    HIPCHK(hipSetDevice(e->device));
+    return e->prec == MDX_PREC_FP64 ? patch_list_t<double>(e, pairs, npairs, out, cap, count)
+                                    : patch_list_t<float>(e, pairs, npairs, out, cap, count);
 }
 
 int mdx_prime(mdx_engine *e, const double *noise0)

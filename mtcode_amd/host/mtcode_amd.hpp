@@ -339,6 +339,50 @@ struct GeneralPatch : ComboPatch {
 // ------------------------------------------------------------ binding
 // BinaryBindStore (Bindings/BinaryBindStore.h:7-11): per patch, bound or not
 // and the partner patch (kept while unbound)
+// one patch pair of the patch list (Rotational/LangevinR.h:20-57): the
+// particles (index1 < index2), their patches and the potential's index
+struct patchint {
+    unsigned int particle_index1;
+    unsigned int particle_index2;
+    unsigned int patch_index1;
+    unsigned int patch_index2;
+    unsigned int potn;
+    void get_all(int &a1, int &a2, int &p, int &pi1, int &pi2) const
+    {
+        a1 = (int)particle_index1;
+        a2 = (int)particle_index2;
+        p = (int)potn;
+        pi1 = (int)patch_index1;
+        pi2 = (int)patch_index2;
+    }
+    void get_particle(int &a1, int &a2) const
+    {
+        a1 = (int)particle_index1;
+        a2 = (int)particle_index2;
+    }
+    void get_patch(int &p, int &pi1, int &pi2) const
+    {
+        p = (int)potn;
+        pi1 = (int)patch_index1;
+        pi2 = (int)patch_index2;
+    }
+    friend bool check_same_particle(const patchint &a, const patchint &b)
+    {
+        return a.particle_index1 == b.particle_index1 && a.particle_index2 == b.particle_index2;
+    }
+};
+
+// adjacency (Rotational/LangevinR.cpp:2000-2036): the positions in a patch
+// list where the particle pair differs from the entry before
+inline std::vector<int> adjacency(const std::vector<patchint> &c)
+{
+    std::vector<int> out;
+    for (size_t i = 1; i < c.size(); i++)
+        if (!check_same_particle(c[i], c[i - 1]))
+            out.push_back((int)i);
+    return out;
+}
+
 struct BinaryBindStore {
     vector1<bool> isbound;
     vector1<int> boundto;
@@ -935,6 +979,40 @@ class LangevinNVTR : public LangevinNVT {
         bind_pull(bo);
         F += Fb;
         T += Tb;
+    }
+    // calculate_patch_list(pairs, ComboPatch&) (LangevinR.cpp:1702-1853):
+    // the patch pairs of the listed particle pairs whose cones may bring them
+    // into binding range, computed on the device (mdx_patch_list), in pair
+    // order and ascending potential; particles numbered by type
+    std::vector<patchint> calculate_patch_list(matrix<int> &pairs, ComboPatch &iny)
+    {
+        StepTerms t;
+        t.patches = &iny;
+        apply_terms(t);
+        const long long np = pairs.getNsafe();
+        if (np > 0 && pairs.getncols() < 2)
+            throw mdx_error(MDX_ERR_ARG, "pairs must have two columns");
+        std::vector<int> ij((size_t)(2 * np));
+        for (long long k = 0; k < np; k++) {
+            ij[(size_t)(2 * k)] = pairs((int)k, 0);
+            ij[(size_t)(2 * k + 1)] = pairs((int)k, 1);
+        }
+        long long cap = 8 * np, n = 0;
+        std::vector<int> rows((size_t)(5 * std::max(1LL, cap)));
+        int rc = mdx_patch_list(e_, ij.data(), np, rows.data(), cap, &n);
+        if (rc == MDX_ERR_CAPACITY && n > cap) {
+            cap = n;
+            rows.assign((size_t)(5 * cap), 0);
+            rc = mdx_patch_list(e_, ij.data(), np, rows.data(), cap, &n);
+        }
+        check(rc);
+        std::vector<patchint> out((size_t)n);
+        for (long long k = 0; k < n; k++) {
+            const int *r = rows.data() + 5 * k;
+            out[(size_t)k] = {(unsigned int)r[0], (unsigned int)r[1], (unsigned int)r[3],
+                              (unsigned int)r[4], (unsigned int)r[2]};
+        }
+        return out;
     }
     // the loop of Condensate::run_singlebond (Condensate.cpp:464-478) on the
     // device: the set-up evaluation and nsteps steps of WCA + the bound pairs'

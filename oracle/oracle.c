@@ -1865,3 +1865,57 @@ int orc_bind_forces(const orc_box *b, int N, const double *x, const double *Q,
     free(o.pairs);
     return 0;
 }
+
+long long orc_patch_list(const orc_box *b, int N, const double *x, const double *Q,
+                         const int *ij, long long np, const orc_patchset *ps, double maxdis,
+                         double maxang, int *out, long long cap)
+{
+    int pstart[ORC_MAXTYPES], rstart[ORC_MAXTYPES][ORC_MAXTYPES];
+    patch_layout(ps, pstart, rstart);
+    int *poff = (int *)malloc(sizeof(int) * (size_t)(N + 1));
+    poff[0] = 0;
+    for (int i = 0; i < N; i++)
+        poff[i + 1] = poff[i] + ps->npatch[ps->type[i]];
+    long long n = 0;
+    for (long long k = 0; k < np; k++) {
+        int p1 = ij[2 * k], p2 = ij[2 * k + 1];
+        if (p2 < p1) {
+            int t = p1;
+            p1 = p2;
+            p2 = t;
+        }
+        double un[3], dis;
+        orc_distance_vector(b, x + 3 * p1, x + 3 * p2, un, &dis);
+        dis = sqrt(dis);
+        if (!(dis < maxdis))
+            continue;
+        for (int c = 0; c < 3; c++)
+            un[c] /= dis;
+        const int k1 = ps->type[p1], k2 = ps->type[p2];
+        if (k1 > k2) {
+            free(poff);
+            return -1;
+        }
+        for (int ia = 0; ia < ps->npatch[k1]; ia++)
+            for (int ib = 0; ib < ps->npatch[k2]; ib++) {
+                double n1[3], n2[3];
+                lab_patch(Q + 9 * p1, ps->dirs + 3 * (pstart[k1] + ia), n1);
+                lab_patch(Q + 9 * p2, ps->dirs + 3 * (pstart[k2] + ib), n2);
+                double ai = -(n1[0] * un[0] + n1[1] * un[1] + n1[2] * un[2]);
+                double aj = (n2[0] * un[0] + n2[1] * un[1] + n2[2] * un[2]);
+                if ((ai - maxang) + (aj - maxang) > -0.4) {
+                    if (n < cap) {
+                        int *r = out + 5 * n;
+                        r[0] = p1;
+                        r[1] = p2;
+                        r[2] = rstart[k1][k2] + ia * ps->npatch[k2] + ib;
+                        r[3] = poff[p1] + ia;
+                        r[4] = poff[p2] + ib;
+                    }
+                    n++;
+                }
+            }
+    }
+    free(poff);
+    return n;
+}

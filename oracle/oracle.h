@@ -35,6 +35,9 @@
  *   orc_bind_forces              : golden fixtures from the compiled reference
  *                                  (bound sets, forces, torques after one
  *                                  call with the reference's rand() draws)
+ *   orc_patch_list               : golden fixtures from the compiled reference
+ *                                  (calculate_patch_list on the reference's
+ *                                  own pairs, rows bit-exact)
  *   orc_dgraph_labels            : golden fixture from the compiled reference
  *                                  (distance_graph); scipy's connected
  *                                  components in the tests
@@ -322,6 +325,19 @@ int orc_bind_forces(const orc_box *b, int N, const double *x, const double *Q,
                     double *F, double *T, long long *stats);
 void orc_measured_temperature(const orc_params *P, int N, const double *p, const double *L,
                               double out[6]);
+/* LangevinNVTR::calculate_patch_list (Rotational/LangevinR.cpp:1702-1853) on
+ * one thread: for each listed pair (sorted p1 < p2) closer than maxdis
+ * (ComboPatch::max_check + 1), each patch pair of the pair's potentials whose
+ * cone arguments satisfy (a_i - maxang) + (a_j - maxang) > -0.4 (maxang =
+ * ComboPatch::max_ang: 0.6 lowered to the smallest cos theta_m,
+ * combopatch.h:15, GeneralPatch.cpp:93-99) gives a row (p1, p2, potn, patch
+ * of p1, patch of p2) (GeneralPatch::which_patch, GeneralPatch.cpp:297-320),
+ * in pair order and ascending potn.  Particles must be numbered by type
+ * (GeneralPatch's type ranges).  Returns the row count (rows beyond cap are
+ * counted, not written), or -1 for a pair of descending types. */
+long long orc_patch_list(const orc_box *b, int N, const double *x, const double *Q,
+                         const int *ij, long long np, const orc_patchset *ps, double maxdis,
+                         double maxang, int *out, long long cap);
 
 #ifdef __cplusplus
 }

@@ -133,6 +133,10 @@ def lib():
         L.orc_bind_forces.argtypes = [C.POINTER(Box), C.c_int, _dp, _dp, C.POINTER(PatchSet),
                                       C.POINTER(BindModel), _ip, _ip, _dp, _dp,
                                       C.POINTER(C.c_longlong)]
+        L.orc_patch_list.restype = C.c_longlong
+        L.orc_patch_list.argtypes = [C.POINTER(Box), C.c_int, _dp, _dp, _ip, C.c_longlong,
+                                     C.POINTER(PatchSet), C.c_double, C.c_double, _ip,
+                                     C.c_longlong]
         _lib = L
     return _lib
 
@@ -411,6 +415,47 @@ def bind_forces(s, isbound, boundto, on_rate, off_rate, seed=None, key=None, dra
     if rc != 0:
         raise RuntimeError(f"orc_bind_forces: {rc}")
     return F, T, ib, bt, st
+
+
+def patch_limits(params):
+    """ComboPatch::max_check and max_ang of a GeneralPatch with these rows
+    (eps, d, theta_m): 1.0 raised to the largest 1.4 d, 0.6 lowered to the
+    smallest cos theta_m (combopatch.h:15, GeneralPatch.cpp:93-99)"""
+    params = np.asarray(params, dtype=np.float64).reshape(-1, 3)
+    mk = max(1.0, float(np.max(1.4 * params[:, 1])))
+    ma = min(0.6, float(np.min(np.cos(params[:, 2]))))
+    return mk, ma
+
+
+def patch_list(s, pairs):
+    """calculate_patch_list on the given pairs (orc_patch_list): an M x 5
+    int32 array (p1, p2, potn, patch of p1, patch of p2)"""
+    keep = _Keep()
+    N = len(s["x"])
+    ps = make_patchset(s["patches"], keep)
+    mk, ma = patch_limits(s["patches"]["params"])
+    b = make_box(s["box"], s.get("periodic", True))
+    xa, xp = _d(s["x"])
+    qa, qp = _d(s["Q"])
+    ij, ijp = _i(np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2)))
+    n = lib().orc_patch_list(C.byref(b), N, xp, qp, ijp, len(ij), C.byref(ps), mk + 1.0, ma,
+                             None, 0)
+    if n < 0:
+        raise RuntimeError("orc_patch_list: particles not numbered by type")
+    out = np.zeros((max(n, 1), 5), dtype=np.int32)
+    lib().orc_patch_list(C.byref(b), N, xp, qp, ijp, len(ij), C.byref(ps), mk + 1.0, ma,
+                         out.ctypes.data_as(_ip), n)
+    return out[:n]
+
+
+def adjacency(rows):
+    """adjacency (LangevinR.cpp:2000-2036): the rows whose particle pair
+    differs from the row before"""
+    rows = np.asarray(rows).reshape(-1, 5)
+    if len(rows) < 2:
+        return np.zeros(0, dtype=np.int32)
+    d = (rows[1:, 0] != rows[:-1, 0]) | (rows[1:, 1] != rows[:-1, 1])
+    return (np.nonzero(d)[0] + 1).astype(np.int32)
 
 
 def bind_uniform(key, draw, root, k):

@@ -844,6 +844,43 @@ static int cmd_bind()
     return 0;
 }
 
+/* LangevinNVTR::calculate_patch_list (LangevinR.cpp:1702-1853) on the pairs
+ * of calculatepairs(cut), and adjacency (LangevinR.cpp:2000-2036): the pairs,
+ * the patch list (p1, p2, potn, patch1, patch2 per row) and the run starts */
+static int cmd_plist()
+{
+    int N = (int)ji("N");
+    cube geo = make_cube();
+    LangevinNVTR b(geo);
+    matrix<double> x = to_mat(rd<double>("x", (size_t)N * 3), N, 3);
+    matrix<double> Q = to_mat(rd<double>("Q", (size_t)N * 9), N, 9);
+    matrix<double> p0(N, 3), L0(N, 3);
+    b.initialize(x, p0, Q, L0);
+    ComboPatch *pots = make_patches();
+    if (!pots)
+        die("plist needs patches");
+    matrix<int> *pairs = make_pairs(b, geo, (int)ji("n"), jd("cut"));
+    write_pairs(pairs);
+    vector<patchint> opairs = b.calculate_patch_list(*pairs, *pots);
+    vector<int> rows(5 * opairs.size());
+    for (size_t k = 0; k < opairs.size(); k++) {
+        int a1, a2, pn, w1, w2;
+        opairs[k].get_all(a1, a2, pn, w1, w2);
+        rows[5 * k] = a1;
+        rows[5 * k + 1] = a2;
+        rows[5 * k + 2] = pn;
+        rows[5 * k + 3] = w1;
+        rows[5 * k + 4] = w2;
+    }
+    wr("plist", rows.data(), rows.size());
+    vector<int> runs = adjacency(opairs);
+    wr("adjacency", runs.data(), runs.size());
+    double lim[2] = {pots->max_check, pots->max_ang};
+    wr("limits", lim, 2);
+    delete pairs;
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc != 3)
@@ -881,6 +918,8 @@ int main(int argc, char **argv)
         return cmd_stress();
     if (cmd == "bind")
         return cmd_bind();
+    if (cmd == "plist")
+        return cmd_plist();
     die("unknown command " + cmd);
     return 2;
 }

@@ -183,6 +183,22 @@ static int rotor_main()
         }
         wr("out_isbound.bin", ib.data(), (size_t)nh);
         wr("out_boundto.bin", bt.data(), (size_t)nh);
+    } else if (mode == "plist") {
+        // calculate_patch_list and adjacency (LangevinR.cpp:1702-2036) on the
+        // pairs of pairs.bin
+        const int np = ji("npairs");
+        matrix<int> pairs = to_mat(rd<int>("pairs.bin", (size_t)np * 2), np, 2);
+        const std::vector<patchint> pl = b.calculate_patch_list(pairs, pots);
+        const std::vector<int> runs = adjacency(pl);
+        std::vector<int> rows(5 * pl.size() + 1);
+        for (size_t k = 0; k < pl.size(); k++) {
+            int a1, a2, pn, w1, w2;
+            pl[k].get_all(a1, a2, pn, w1, w2);
+            const int r[5] = {a1, a2, pn, w1, w2};
+            std::copy(r, r + 5, rows.begin() + 5 * k);
+        }
+        wr("out_plist.bin", rows.data(), 5 * pl.size());
+        wr("out_adjacency.bin", runs.data(), runs.size());
     } else if (mode == "run") {
         StepTerms t;
         t.wca = &wsa;

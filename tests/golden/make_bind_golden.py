@@ -24,6 +24,13 @@ Cases (x, Q, types and patch rows as in mtcode_amd.workloads):
 writes tests/golden/bind_wall.npz: wall0, wall1, the wide-cone system of
 wide0 / wide1 in a box with reflecting walls (cube periodicity false, the
 default of Condensate.cpp:17-18): no candidate pair across a face.
+
+    python tests/golden/make_bind_golden.py plist
+
+writes tests/golden/plist.npz: LangevinNVTR::calculate_patch_list on the
+reference's own calculatepairs(2.5) of ab0, wide0 and wall0's configuration
+(the cut of Condensate.cpp:457), with adjacency of each list and the patch
+set's max_check and max_ang.
 """
 from __future__ import annotations
 
@@ -182,8 +189,36 @@ def main_wall():
     np.savez_compressed(os.path.join(OUT, "bind_wall.npz"), **out)
 
 
+def main_plist():
+    if not po.have_ref():
+        raise SystemExit("build oracle/_ref first: make -C oracle ref")
+    ab, wide = systems()
+    out = {}
+    for k, s in (("ab", ab), ("wide", wide), ("wall", walled(wide))):
+        d = os.path.join(WORK, "plist_" + k)
+        shutil.rmtree(d, ignore_errors=True)
+        os.makedirs(d)
+        L = float(s["box"][0])
+        cut = 2.5
+        job = {"N": s["N"], "L": repr(L), "periodic": 1 if s.get("periodic", True) else 0,
+               "n": int(L // cut), "cut": repr(cut)}
+        job.update(general_job(s))
+        ps = s["patches"]
+        po.run_ref("plist", d, job, {"x": s["x"], "Q": s["Q"], "params": ps["params"],
+                                     "dirs": np.asarray(ps["dirs"]).reshape(-1, 3)})
+        out[k + "_pairs"] = po.read_ref(d, "pairs", np.int32).reshape(-1, 2)
+        out[k + "_plist"] = po.read_ref(d, "plist", np.int32).reshape(-1, 5)
+        out[k + "_adjacency"] = po.read_ref(d, "adjacency", np.int32)
+        out[k + "_limits"] = po.read_ref(d, "limits")
+        print(k, "pairs", len(out[k + "_pairs"]), "patch pairs", len(out[k + "_plist"]),
+              "limits", out[k + "_limits"])
+    np.savez_compressed(os.path.join(OUT, "plist.npz"), **out)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["wall"]:
         main_wall()
+    elif sys.argv[1:] == ["plist"]:
+        main_plist()
     else:
         main()

@@ -265,3 +265,30 @@ def test_distance_graph_matches_reference(tmp_path, golden, case):
                                "ex_lo": lo, "ex_hi": hi}, {"x": x})
     lab = np.fromfile(os.path.join(str(tmp_path), "out_labels.bin"), dtype=np.int32)
     assert np.array_equal(lab, g[f"{case}_labels"])
+
+
+@pytest.mark.gpu
+def test_patch_list_through_dropin(tmp_path):
+    """LangevinNVTR::calculate_patch_list and adjacency through the C++
+    classes (TetrahedralPatch) on the reference's own pairs of the wide-cone
+    fixture: rows, their order and the run starts equal to the compiled
+    reference's (tests/golden/plist.npz)"""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import make_bind_golden as g
+    PL = np.load(os.path.join(ROOT, "tests", "golden", "plist.npz"))
+    _, s = g.systems()
+    N = s["N"]
+    pairs = np.ascontiguousarray(PL["wide_pairs"], dtype=np.int32)
+    e, d, th = s["patches"]["params"][0]
+    say("drop-in patch list", N, len(pairs))
+    job = {"integ": "rotor", "mode": "plist", "N": N, "L": repr(float(s["box"][0])),
+           "nsteps": 0, "dt": s["dt"], "gamma": s["gamma"], "gammar": repr(float(s["gammar"])),
+           "kT": s["kT"], "precision": 0, "eps_p": repr(float(e)), "d": repr(float(d)),
+           "thetam": repr(float(th)), "npairs": len(pairs)}
+    run_driver(str(tmp_path), job, {"x": s["x"], "Q": s["Q"], "noise": np.zeros(N * 6),
+                                    "pairs": pairs})
+    rows = np.fromfile(os.path.join(tmp_path, "out_plist.bin"), dtype=np.int32).reshape(-1, 5)
+    runs = np.fromfile(os.path.join(tmp_path, "out_adjacency.bin"), dtype=np.int32)
+    assert np.array_equal(rows, PL["wide_plist"])
+    assert np.array_equal(runs, PL["wide_adjacency"])

@@ -103,6 +103,34 @@ def test_bind_walls_fp64_vs_oracle(case):
         assert rel(F, Fo) < 1e-12 and rel(T, To) < 1e-12
 
 
+PL = np.load(os.path.join(HERE, "golden", "plist.npz"))
+
+
+@pytest.mark.parametrize("case", ["ab", "wide", "wall"])
+def test_patch_list_vs_reference(case):
+    """mdx_patch_list on the reference's own pairs: FP64 rows bit-exact
+    against the compiled reference's calculate_patch_list (order included);
+    mixed precision (FP32 orientations) differs only in rows within rounding
+    of the -0.4 threshold (at most 0.1 % of the rows)"""
+    s = {"ab": AB, "wide": WIDE, "wall": g.walled(WIDE)}[case]
+    ref = PL[case + "_plist"]
+    say("patch list fp64", case)
+    e = mdx.engine_from_system(s, precision=mdx.PREC_FP64, seed=SEED)
+    rows = e.patch_list(PL[case + "_pairs"])
+    assert np.array_equal(rows, ref)
+    assert np.array_equal(po.adjacency(rows), PL[case + "_adjacency"])
+    # a small cap reports the row count with MDX_ERR_CAPACITY
+    with pytest.raises(mdx.MdxError) as ex:
+        e.patch_list(PL[case + "_pairs"], cap=10)
+    assert ex.value.code == mdx.MDX_ERR_CAPACITY
+    say("patch list mixed", case)
+    e = mdx.engine_from_system(s, precision=mdx.PREC_MIXED, seed=SEED)
+    rows = e.patch_list(PL[case + "_pairs"])
+    a = {tuple(r) for r in rows}
+    b = {tuple(r) for r in ref}
+    assert len(a ^ b) <= 1e-3 * len(b)
+
+
 def lattice_system(m, thetam, d, seed):
     s = g.lattice(wl.patchy_tetra(m ** 3, phi=0.05, seed_pos=seed, seed_q=seed + 1), m, 1.15,
                   seed + 2)

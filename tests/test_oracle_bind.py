@@ -52,6 +52,24 @@ def test_oracle_bind_walls_match_reference(case):
     assert not (np.array_equal(ibp, ib) and np.array_equal(btp, bt))
 
 
+PL = np.load(os.path.join(HERE, "golden", "plist.npz"))
+
+
+@pytest.mark.parametrize("case", ["ab", "wide", "wall"])
+def test_oracle_patch_list_matches_reference(case):
+    """calculate_patch_list (LangevinR.cpp:1702-1853) on the reference's own
+    pairs at cut 2.5: every row (p1, p2, potn, patches) and its order, the
+    adjacency run starts, and the patch set's max_check / max_ang (0.6 for
+    the condensate's cos(theta_m) = 0.95, cos(1.0) for the wide cones)"""
+    s = {"ab": AB, "wide": WIDE, "wall": g.walled(WIDE)}[case]
+    rows = po.patch_list(s, PL[case + "_pairs"])
+    assert len(rows) > 1000
+    assert np.array_equal(rows, PL[case + "_plist"])
+    assert np.array_equal(po.adjacency(rows), PL[case + "_adjacency"])
+    assert np.allclose(po.patch_limits(s["patches"]["params"]), PL[case + "_limits"],
+                       rtol=0, atol=1e-15)
+
+
 def test_oracle_bind_fixture_has_every_path():
     """the wide-cone fixture reaches the large-component cut (sizes to 27),
     breaks bonds and re-forms them"""
