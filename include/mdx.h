@@ -266,6 +266,12 @@ int mdx_bind_forces(mdx_engine *e, unsigned long long draw, double *F, double *T
  * whose particles are numbered by type (GeneralPatch's type ranges). */
 int mdx_patch_list(mdx_engine *e, const int *pairs, long long npairs, int *out, long long cap,
                    long long *count);
+/* LangevinNVTR::calculate_bound_pairs(pairs, ComboPatch&)
+ * (Rotational/LangevinR.cpp:1854-1998): as mdx_patch_list, with the pairs
+ * closer than max_check and the patch pairs whose two cone arguments both
+ * exceed their potential's cos(theta_m). */
+int mdx_bound_pairs(mdx_engine *e, const int *pairs, long long npairs, int *out, long long cap,
+                    long long *count);
 
 /* Time stepping. */
 /* Rotor integrator: computes the carried F, T at the current state

@@ -889,9 +889,12 @@ __global__ void __launch_bounds__(256) k_bind_forces(KArgs<R> a, BindArgs b, dou
 // (a_i - maxang) + (a_j - maxang) > -0.4, with p1 < p2 (particles numbered
 // by type, so t1 <= t2, checked by the host).  FP64 in the reference's
 // expression order; orientations as the engine holds them.
+// bound: LangevinNVTR::calculate_bound_pairs (LangevinR.cpp:1854-1998)
+// instead: closer than maxdis = max_check, and both cone arguments above the
+// row's cos(theta_m) (computed on the host, as the reference's cos).
 template <class R>
 __global__ void __launch_bounds__(256) k_patch_list(KArgs<R> a, const int *pairs, long long npairs,
-                                                    double maxdis, double maxang,
+                                                    double maxdis, double maxang, int bound,
                                                     unsigned long long *mask)
 {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -919,6 +922,7 @@ __global__ void __launch_bounds__(256) k_patch_list(KArgs<R> a, const int *pairs
         load_orient(a.Q, (size_t)s2, q2);
         const int t1 = x1.meta & 255, t2 = x2.meta & 255;
         const int n1 = a.np[t1], n2 = a.np[t2];
+        const int rowbase = rp.prow + a.rstart[t1 * MAXT + t2];
         for (int ia = 0; ia < n1; ia++) {
             const double *b1 = a.pdirs + 3 * (a.pstart[t1] + ia);
             const double nx1 = b1[0] * (double)q1[0] + b1[1] * (double)q1[3] + b1[2] * (double)q1[6];
@@ -934,7 +938,14 @@ __global__ void __launch_bounds__(256) k_patch_list(KArgs<R> a, const int *pairs
                 const double nz2 =
                     b2[0] * (double)q2[2] + b2[1] * (double)q2[5] + b2[2] * (double)q2[8];
                 const double aj = nx2 * un[0] + ny2 * un[1] + nz2 * un[2];
-                if ((ai - maxang) + (aj - maxang) > -0.4)
+                bool pass;
+                if (bound) {
+                    const double ctm = a.prows[rowbase + ia * n2 + ib].costm;
+                    pass = ai > ctm && aj > ctm;
+                } else {
+                    pass = (ai - maxang) + (aj - maxang) > -0.4;
+                }
+                if (pass)
                     m |= 1ull << (ia * n2 + ib);
             }
         }

@@ -2257,11 +2257,12 @@ int mdx_bind_forces(mdx_engine *e, unsigned long long draw, double *F, double *T
                                     : bind_forces_t<float>(e, draw, F, T);
 }
 
-// calculate_patch_list on the device (k_patch_list), the masks expanded into
-// rows (p1, p2, potn, patch of p1, patch of p2) in pair order, ascending potn
+// calculate_patch_list (bound 0) or calculate_bound_pairs (bound 1) on the
+// device (k_patch_list), the masks expanded into rows (p1, p2, potn, patch of
+// p1, patch of p2) in pair order, ascending potn
 template <class R>
 static int patch_list_t(mdx_engine *e, const int *pairs, long long npairs, int *out,
-                        long long cap, long long *count)
+                        long long cap, long long *count, int bound)
 {
     int rc;
     if (!e->binned && (rc = bin_current<R>(e)))
@@ -2279,7 +2280,7 @@ static int patch_list_t(mdx_engine *e, const int *pairs, long long npairs, int *
     const KArgs<R> a = make_args<R>(e);
     const unsigned int nb = (unsigned int)((npairs + 255) / 256);
     hipLaunchKernelGGL(k_patch_list<R>, dim3(nb), dim3(256), 0, e->stream, a,
-                       (const int *)e->plpairs.p, npairs, mk + 1.0, maxang,
+                       (const int *)e->plpairs.p, npairs, bound ? mk : mk + 1.0, maxang, bound,
                        (unsigned long long *)e->plmask.p);
     HIPCHK(hipGetLastError());
     std::vector<unsigned long long> m((size_t)npairs);
@@ -2311,12 +2312,13 @@ static int patch_list_t(mdx_engine *e, const int *pairs, long long npairs, int *
     }
     *count = n;
     if (n > cap)
-        return set_err(MDX_ERR_CAPACITY, "patch list: %lld rows, room for %lld", n, cap);
+        return set_err(MDX_ERR_CAPACITY, "%s: %lld rows, room for %lld",
+                       bound ? "bound pairs" : "patch list", n, cap);
     return MDX_OK;
 }
 
-int mdx_patch_list(mdx_engine *e, const int *pairs, long long npairs, int *out, long long cap,
-                   long long *count)
+static int patch_pairs(mdx_engine *e, const int *pairs, long long npairs, int *out,
+                       long long cap, long long *count, int bound)
 {
     if (!e || !count || npairs < 0 || cap < 0 || (npairs > 0 && !pairs) || (cap > 0 && !out))
         return set_err(MDX_ERR_ARG, "bad arguments");
@@ -2333,8 +2335,21 @@ int mdx_patch_list(mdx_engine *e, const int *pairs, long long npairs, int *out, 
         if (pairs[k] < 0 || pairs[k] >= e->N)
             return set_err(MDX_ERR_ARG, "pairs[%lld] = %d outside 0..%d", k, pairs[k], e->N - 1);
     HIPCHK(hipSetDevice(e->device));
-    return e->prec == MDX_PREC_FP64 ? patch_list_t<double>(e, pairs, npairs, out, cap, count)
-                                    : patch_list_t<float>(e, pairs, npairs, out, cap, count);
+    return e->prec == MDX_PREC_FP64
+               ? patch_list_t<double>(e, pairs, npairs, out, cap, count, bound)
+               : patch_list_t<float>(e, pairs, npairs, out, cap, count, bound);
+}
+
+int mdx_patch_list(mdx_engine *e, const int *pairs, long long npairs, int *out, long long cap,
+                   long long *count)
+{
+    return patch_pairs(e, pairs, npairs, out, cap, count, 0);
+}
+
+int mdx_bound_pairs(mdx_engine *e, const int *pairs, long long npairs, int *out, long long cap,
+                    long long *count)
+{
+    return patch_pairs(e, pairs, npairs, out, cap, count, 1);
 }
 
 int mdx_prime(mdx_engine *e, const double *noise0)

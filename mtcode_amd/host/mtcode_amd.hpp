@@ -986,6 +986,21 @@ class LangevinNVTR : public LangevinNVT {
     // order and ascending potential; particles numbered by type
     std::vector<patchint> calculate_patch_list(matrix<int> &pairs, ComboPatch &iny)
     {
+        return patch_rows(pairs, iny, mdx_patch_list);
+    }
+    // calculate_bound_pairs(pairs, ComboPatch&) (LangevinR.cpp:1854-1998):
+    // the patch pairs within max_check whose cones both pass their
+    // potential's cos(theta_m), on the device (mdx_bound_pairs)
+    std::vector<patchint> calculate_bound_pairs(matrix<int> &pairs, ComboPatch &iny)
+    {
+        return patch_rows(pairs, iny, mdx_bound_pairs);
+    }
+
+  private:
+    std::vector<patchint> patch_rows(matrix<int> &pairs, ComboPatch &iny,
+                                     int (*fn)(mdx_engine *, const int *, long long, int *,
+                                               long long, long long *))
+    {
         StepTerms t;
         t.patches = &iny;
         apply_terms(t);
@@ -999,11 +1014,11 @@ class LangevinNVTR : public LangevinNVT {
         }
         long long cap = 8 * np, n = 0;
         std::vector<int> rows((size_t)(5 * std::max(1LL, cap)));
-        int rc = mdx_patch_list(e_, ij.data(), np, rows.data(), cap, &n);
+        int rc = fn(e_, ij.data(), np, rows.data(), cap, &n);
         if (rc == MDX_ERR_CAPACITY && n > cap) {
             cap = n;
             rows.assign((size_t)(5 * cap), 0);
-            rc = mdx_patch_list(e_, ij.data(), np, rows.data(), cap, &n);
+            rc = fn(e_, ij.data(), np, rows.data(), cap, &n);
         }
         check(rc);
         std::vector<patchint> out((size_t)n);
@@ -1014,6 +1029,8 @@ class LangevinNVTR : public LangevinNVT {
         }
         return out;
     }
+
+  public:
     // the loop of Condensate::run_singlebond (Condensate.cpp:464-478) on the
     // device: the set-up evaluation and nsteps steps of WCA + the bound pairs'
     // terms (t: the WCA and the patches), the bound state carried in bo.

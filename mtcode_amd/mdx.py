@@ -108,6 +108,7 @@ def lib():
     L.mdx_bind_forces.argtypes = [C.c_void_p, C.c_ulonglong, _dp, _dp]
     L.mdx_patch_list.argtypes = [C.c_void_p, _ip, C.c_longlong, _ip, C.c_longlong,
                                  C.POINTER(C.c_longlong)]
+    L.mdx_bound_pairs.argtypes = L.mdx_patch_list.argtypes
     _lib = L
     return L
 
@@ -407,17 +408,19 @@ class Engine:
                                      T.ctypes.data_as(_dp)))
         return F, T
 
-    def patch_list(self, pairs, cap=None):
-        """calculate_patch_list on the given particle pairs: an M x 5 int32
-        array (p1, p2, potn, patch of p1, patch of p2)"""
+    def patch_list(self, pairs, cap=None, bound=False):
+        """calculate_patch_list (bound: calculate_bound_pairs) on the given
+        particle pairs: an M x 5 int32 array (p1, p2, potn, patch of p1,
+        patch of p2)"""
+        fn = lib().mdx_bound_pairs if bound else lib().mdx_patch_list
         pairs = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
         grow = cap is None   # no cap given: call again with room for every row
         cap = 8 * len(pairs) if cap is None else int(cap)
         n = C.c_longlong(0)
         while True:
             out = np.zeros((max(cap, 1), 5), dtype=np.int32)
-            rc = lib().mdx_patch_list(self.h, pairs.ctypes.data_as(_ip), len(pairs),
-                                      out.ctypes.data_as(_ip), cap, C.byref(n))
+            rc = fn(self.h, pairs.ctypes.data_as(_ip), len(pairs), out.ctypes.data_as(_ip), cap,
+                    C.byref(n))
             if grow and rc == MDX_ERR_CAPACITY and n.value > cap:
                 cap = n.value
                 grow = False

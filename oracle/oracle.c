@@ -1866,9 +1866,12 @@ int orc_bind_forces(const orc_box *b, int N, const double *x, const double *Q,
     return 0;
 }
 
-long long orc_patch_list(const orc_box *b, int N, const double *x, const double *Q,
-                         const int *ij, long long np, const orc_patchset *ps, double maxdis,
-                         double maxang, int *out, long long cap)
+/* the loop of calculate_patch_list (bound 0) and calculate_bound_pairs
+ * (bound 1, LangevinR.cpp:1854-1998: both cone arguments above the row's
+ * cos(theta_m)) */
+static long long patch_rows(const orc_box *b, int N, const double *x, const double *Q,
+                            const int *ij, long long np, const orc_patchset *ps, double maxdis,
+                            double maxang, int bound, int *out, long long cap)
 {
     int pstart[ORC_MAXTYPES], rstart[ORC_MAXTYPES][ORC_MAXTYPES];
     patch_layout(ps, pstart, rstart);
@@ -1903,12 +1906,18 @@ long long orc_patch_list(const orc_box *b, int N, const double *x, const double 
                 lab_patch(Q + 9 * p2, ps->dirs + 3 * (pstart[k2] + ib), n2);
                 double ai = -(n1[0] * un[0] + n1[1] * un[1] + n1[2] * un[2]);
                 double aj = (n2[0] * un[0] + n2[1] * un[1] + n2[2] * un[2]);
-                if ((ai - maxang) + (aj - maxang) > -0.4) {
+                const int row = rstart[k1][k2] + ia * ps->npatch[k2] + ib;
+                int pass;
+                if (bound)
+                    pass = ai > cos(ps->params[3 * row + 2]) && aj > cos(ps->params[3 * row + 2]);
+                else
+                    pass = (ai - maxang) + (aj - maxang) > -0.4;
+                if (pass) {
                     if (n < cap) {
                         int *r = out + 5 * n;
                         r[0] = p1;
                         r[1] = p2;
-                        r[2] = rstart[k1][k2] + ia * ps->npatch[k2] + ib;
+                        r[2] = row;
                         r[3] = poff[p1] + ia;
                         r[4] = poff[p2] + ib;
                     }
@@ -1918,4 +1927,18 @@ long long orc_patch_list(const orc_box *b, int N, const double *x, const double 
     }
     free(poff);
     return n;
+}
+
+long long orc_patch_list(const orc_box *b, int N, const double *x, const double *Q,
+                         const int *ij, long long np, const orc_patchset *ps, double maxdis,
+                         double maxang, int *out, long long cap)
+{
+    return patch_rows(b, N, x, Q, ij, np, ps, maxdis, maxang, 0, out, cap);
+}
+
+long long orc_bound_pairs(const orc_box *b, int N, const double *x, const double *Q,
+                          const int *ij, long long np, const orc_patchset *ps, double maxdis,
+                          int *out, long long cap)
+{
+    return patch_rows(b, N, x, Q, ij, np, ps, maxdis, 0.0, 1, out, cap);
 }

@@ -35,8 +35,10 @@
  *   orc_bind_forces              : golden fixtures from the compiled reference
  *                                  (bound sets, forces, torques after one
  *                                  call with the reference's rand() draws)
- *   orc_patch_list               : golden fixtures from the compiled reference
- *                                  (calculate_patch_list on the reference's
+ *   orc_patch_list,
+ *   orc_bound_pairs              : golden fixtures from the compiled reference
+ *                                  (calculate_patch_list and
+ *                                  calculate_bound_pairs on the reference's
  *                                  own pairs, rows bit-exact)
  *   orc_dgraph_labels            : golden fixture from the compiled reference
  *                                  (distance_graph); scipy's connected
@@ -338,6 +340,12 @@ void orc_measured_temperature(const orc_params *P, int N, const double *p, const
 long long orc_patch_list(const orc_box *b, int N, const double *x, const double *Q,
                          const int *ij, long long np, const orc_patchset *ps, double maxdis,
                          double maxang, int *out, long long cap);
+/* LangevinNVTR::calculate_bound_pairs (Rotational/LangevinR.cpp:1854-1998):
+ * as orc_patch_list with maxdis = max_check and the test a_i > cos theta_m
+ * and a_j > cos theta_m of each row */
+long long orc_bound_pairs(const orc_box *b, int N, const double *x, const double *Q,
+                          const int *ij, long long np, const orc_patchset *ps, double maxdis,
+                          int *out, long long cap);
 
 #ifdef __cplusplus
 }

@@ -137,6 +137,9 @@ def lib():
         L.orc_patch_list.argtypes = [C.POINTER(Box), C.c_int, _dp, _dp, _ip, C.c_longlong,
                                      C.POINTER(PatchSet), C.c_double, C.c_double, _ip,
                                      C.c_longlong]
+        L.orc_bound_pairs.restype = C.c_longlong
+        L.orc_bound_pairs.argtypes = [C.POINTER(Box), C.c_int, _dp, _dp, _ip, C.c_longlong,
+                                      C.POINTER(PatchSet), C.c_double, _ip, C.c_longlong]
         _lib = L
     return _lib
 
@@ -445,6 +448,26 @@ def patch_list(s, pairs):
     out = np.zeros((max(n, 1), 5), dtype=np.int32)
     lib().orc_patch_list(C.byref(b), N, xp, qp, ijp, len(ij), C.byref(ps), mk + 1.0, ma,
                          out.ctypes.data_as(_ip), n)
+    return out[:n]
+
+
+def bound_pairs(s, pairs):
+    """calculate_bound_pairs on the given pairs (orc_bound_pairs): an M x 5
+    int32 array as patch_list's"""
+    keep = _Keep()
+    N = len(s["x"])
+    ps = make_patchset(s["patches"], keep)
+    mk, _ = patch_limits(s["patches"]["params"])
+    b = make_box(s["box"], s.get("periodic", True))
+    xa, xp = _d(s["x"])
+    qa, qp = _d(s["Q"])
+    ij, ijp = _i(np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2)))
+    n = lib().orc_bound_pairs(C.byref(b), N, xp, qp, ijp, len(ij), C.byref(ps), mk, None, 0)
+    if n < 0:
+        raise RuntimeError("orc_bound_pairs: particles not numbered by type")
+    out = np.zeros((max(n, 1), 5), dtype=np.int32)
+    lib().orc_bound_pairs(C.byref(b), N, xp, qp, ijp, len(ij), C.byref(ps), mk,
+                          out.ctypes.data_as(_ip), n)
     return out[:n]
 
 

@@ -846,7 +846,8 @@ static int cmd_bind()
 
 /* LangevinNVTR::calculate_patch_list (LangevinR.cpp:1702-1853) on the pairs
  * of calculatepairs(cut), and adjacency (LangevinR.cpp:2000-2036): the pairs,
- * the patch list (p1, p2, potn, patch1, patch2 per row) and the run starts */
+ * the patch list (p1, p2, potn, patch1, patch2 per row) and the run starts;
+ * calculate_bound_pairs (LangevinR.cpp:1854-1998) on the same pairs */
 static int cmd_plist()
 {
     int N = (int)ji("N");
@@ -875,6 +876,18 @@ static int cmd_plist()
     wr("plist", rows.data(), rows.size());
     vector<int> runs = adjacency(opairs);
     wr("adjacency", runs.data(), runs.size());
+    vector<patchint> bpairs = b.calculate_bound_pairs(*pairs, *pots);
+    vector<int> brows(5 * bpairs.size());
+    for (size_t k = 0; k < bpairs.size(); k++) {
+        int a1, a2, pn, w1, w2;
+        bpairs[k].get_all(a1, a2, pn, w1, w2);
+        brows[5 * k] = a1;
+        brows[5 * k + 1] = a2;
+        brows[5 * k + 2] = pn;
+        brows[5 * k + 3] = w1;
+        brows[5 * k + 4] = w2;
+    }
+    wr("blist", brows.data(), brows.size());
     double lim[2] = {pots->max_check, pots->max_ang};
     wr("limits", lim, 2);
     delete pairs;

@@ -184,8 +184,8 @@ static int rotor_main()
         wr("out_isbound.bin", ib.data(), (size_t)nh);
         wr("out_boundto.bin", bt.data(), (size_t)nh);
     } else if (mode == "plist") {
-        // calculate_patch_list and adjacency (LangevinR.cpp:1702-2036) on the
-        // pairs of pairs.bin
+        // calculate_patch_list, adjacency and calculate_bound_pairs
+        // (LangevinR.cpp:1702-2036) on the pairs of pairs.bin
         const int np = ji("npairs");
         matrix<int> pairs = to_mat(rd<int>("pairs.bin", (size_t)np * 2), np, 2);
         const std::vector<patchint> pl = b.calculate_patch_list(pairs, pots);
@@ -199,6 +199,15 @@ static int rotor_main()
         }
         wr("out_plist.bin", rows.data(), 5 * pl.size());
         wr("out_adjacency.bin", runs.data(), runs.size());
+        const std::vector<patchint> bp = b.calculate_bound_pairs(pairs, pots);
+        std::vector<int> brows(5 * bp.size() + 1);
+        for (size_t k = 0; k < bp.size(); k++) {
+            int a1, a2, pn, w1, w2;
+            bp[k].get_all(a1, a2, pn, w1, w2);
+            const int r[5] = {a1, a2, pn, w1, w2};
+            std::copy(r, r + 5, brows.begin() + 5 * k);
+        }
+        wr("out_blist.bin", brows.data(), 5 * bp.size());
     } else if (mode == "run") {
         StepTerms t;
         t.wca = &wsa;

@@ -29,8 +29,8 @@ default of Condensate.cpp:17-18): no candidate pair across a face.
 
 writes tests/golden/plist.npz: LangevinNVTR::calculate_patch_list on the
 reference's own calculatepairs(2.5) of ab0, wide0 and wall0's configuration
-(the cut of Condensate.cpp:457), with adjacency of each list and the patch
-set's max_check and max_ang.
+(the cut of Condensate.cpp:457), with adjacency of each list, the patch
+set's max_check and max_ang, and calculate_bound_pairs on the same pairs.
 """
 from __future__ import annotations
 
@@ -210,6 +210,7 @@ def main_plist():
         out[k + "_plist"] = po.read_ref(d, "plist", np.int32).reshape(-1, 5)
         out[k + "_adjacency"] = po.read_ref(d, "adjacency", np.int32)
         out[k + "_limits"] = po.read_ref(d, "limits")
+        out[k + "_blist"] = po.read_ref(d, "blist", np.int32).reshape(-1, 5)
         print(k, "pairs", len(out[k + "_pairs"]), "patch pairs", len(out[k + "_plist"]),
               "limits", out[k + "_limits"])
     np.savez_compressed(os.path.join(OUT, "plist.npz"), **out)

@@ -269,10 +269,11 @@ def test_distance_graph_matches_reference(tmp_path, golden, case):
 
 @pytest.mark.gpu
 def test_patch_list_through_dropin(tmp_path):
-    """LangevinNVTR::calculate_patch_list and adjacency through the C++
-    classes (TetrahedralPatch) on the reference's own pairs of the wide-cone
-    fixture: rows, their order and the run starts equal to the compiled
-    reference's (tests/golden/plist.npz)"""
+    """LangevinNVTR::calculate_patch_list, adjacency and
+    calculate_bound_pairs through the C++ classes (TetrahedralPatch) on the
+    reference's own pairs of the wide-cone fixture: rows, their order and
+    the run starts equal to the compiled reference's
+    (tests/golden/plist.npz)"""
     import sys
     sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
     import make_bind_golden as g
@@ -292,3 +293,5 @@ def test_patch_list_through_dropin(tmp_path):
     runs = np.fromfile(os.path.join(tmp_path, "out_adjacency.bin"), dtype=np.int32)
     assert np.array_equal(rows, PL["wide_plist"])
     assert np.array_equal(runs, PL["wide_adjacency"])
+    brows = np.fromfile(os.path.join(tmp_path, "out_blist.bin"), dtype=np.int32).reshape(-1, 5)
+    assert np.array_equal(brows, PL["wide_blist"])

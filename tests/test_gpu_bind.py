@@ -108,8 +108,9 @@ PL = np.load(os.path.join(HERE, "golden", "plist.npz"))
 
 @pytest.mark.parametrize("case", ["ab", "wide", "wall"])
 def test_patch_list_vs_reference(case):
-    """mdx_patch_list on the reference's own pairs: FP64 rows bit-exact
-    against the compiled reference's calculate_patch_list (order included);
+    """mdx_patch_list and mdx_bound_pairs on the reference's own pairs: FP64
+    rows bit-exact against the compiled reference's calculate_patch_list and
+    calculate_bound_pairs (order included);
     mixed precision (FP32 orientations) differs only in rows within rounding
     of the -0.4 threshold (at most 0.1 % of the rows)"""
     s = {"ab": AB, "wide": WIDE, "wall": g.walled(WIDE)}[case]
@@ -119,6 +120,7 @@ def test_patch_list_vs_reference(case):
     rows = e.patch_list(PL[case + "_pairs"])
     assert np.array_equal(rows, ref)
     assert np.array_equal(po.adjacency(rows), PL[case + "_adjacency"])
+    assert np.array_equal(e.patch_list(PL[case + "_pairs"], bound=True), PL[case + "_blist"])
     # a small cap reports the row count with MDX_ERR_CAPACITY
     with pytest.raises(mdx.MdxError) as ex:
         e.patch_list(PL[case + "_pairs"], cap=10)

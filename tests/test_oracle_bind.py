@@ -68,6 +68,10 @@ def test_oracle_patch_list_matches_reference(case):
     assert np.array_equal(po.adjacency(rows), PL[case + "_adjacency"])
     assert np.allclose(po.patch_limits(s["patches"]["params"]), PL[case + "_limits"],
                        rtol=0, atol=1e-15)
+    # calculate_bound_pairs (LangevinR.cpp:1854-1998) on the same pairs
+    b = po.bound_pairs(s, PL[case + "_pairs"])
+    assert len(b) > 10
+    assert np.array_equal(b, PL[case + "_blist"])
 
 
 def test_oracle_bind_fixture_has_every_path():
