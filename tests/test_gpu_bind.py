@@ -133,6 +133,24 @@ def test_patch_list_vs_reference(case):
     assert len(a ^ b) <= 1e-3 * len(b)
 
 
+def test_patch_list_rejects_bad_input():
+    """host checks of mdx_patch_list: a pair index outside 0..N-1 is
+    MDX_ERR_ARG; particles not numbered by type (GeneralPatch's type ranges)
+    are MDX_ERR_STATE; both before any kernel runs"""
+    say("patch list errors")
+    e = mdx.engine_from_system(AB, precision=mdx.PREC_FP64, seed=SEED)
+    with pytest.raises(mdx.MdxError) as ex:
+        e.patch_list(np.array([[0, AB["N"]]]))
+    assert ex.value.code == mdx.MDX_ERR_ARG
+    assert len(e.patch_list(np.zeros((0, 2), np.int32))) == 0
+    s = dict(AB)
+    s["patches"] = dict(AB["patches"], types=AB["patches"]["types"][::-1].copy())
+    e = mdx.engine_from_system(s, precision=mdx.PREC_FP64, seed=SEED)
+    with pytest.raises(mdx.MdxError) as ex:
+        e.patch_list(PL["ab_pairs"][:10])
+    assert ex.value.code == mdx.MDX_ERR_STATE
+
+
 def lattice_system(m, thetam, d, seed):
     s = g.lattice(wl.patchy_tetra(m ** 3, phi=0.05, seed_pos=seed, seed_q=seed + 1), m, 1.15,
                   seed + 2)
