@@ -133,6 +133,38 @@ def test_patch_list_vs_reference(case):
     assert len(a ^ b) <= 1e-3 * len(b)
 
 
+def test_patch_lists_full_size_properties():
+    """the 1,048,576-particle condensate (mixed, the bench system) on the
+    pairs of calculatepairs(2.5) (mdx_neighbor_pairs): every row has
+    p1 < p2 and patches of its own particles; the bound-pair list lies inside
+    the patch list (both cones above cos(theta_m) = 0.95 pass the -0.4
+    allowance at max_ang 0.6, and max_check < max_check + 1); every bond the
+    binding routine forms is a row of the bound-pair list"""
+    s = wl.condensate_ab()
+    say("patch lists full size")
+    e, nh = bind_engine(s, mdx.PREC_MIXED, 0.5, 0.5)
+    n = int(float(s["box"][0]) // 2.5)
+    pairs = e.neighbor_pairs(n, 2.5)
+    e.bind_forces(draw=0)
+    ib, bt = e.get_bindings()
+    rows_p = e.patch_list(pairs, cap=2 * len(pairs))
+    rows_b = e.patch_list(pairs, cap=len(pairs), bound=True)
+    say("  pairs", len(pairs), "patch list", len(rows_p), "bound pairs", len(rows_b))
+    owner = np.repeat(np.arange(s["N"]), [s["patches"]["npatch"][t] for t in s["patches"]["types"]])
+    for rows in (rows_p, rows_b):
+        assert np.all(rows[:, 0] < rows[:, 1])
+        assert np.all(owner[rows[:, 3]] == rows[:, 0]) and np.all(owner[rows[:, 4]] == rows[:, 1])
+    key_p = rows_p[:, 3].astype(np.int64) * nh + rows_p[:, 4]
+    key_b = rows_b[:, 3].astype(np.int64) * nh + rows_b[:, 4]
+    assert np.all(np.isin(key_b, key_p))
+    a = np.nonzero(ib)[0]
+    b = bt[a]
+    lo = np.where(owner[a] < owner[b], a, b)
+    hi = np.where(owner[a] < owner[b], b, a)
+    assert len(a) > 0
+    assert np.all(np.isin(lo.astype(np.int64) * nh + hi, key_b))
+
+
 def test_patch_list_rejects_bad_input():
     """host checks of mdx_patch_list: a pair index outside 0..N-1 is
     MDX_ERR_ARG; particles not numbered by type (GeneralPatch's type ranges)
