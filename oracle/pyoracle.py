@@ -430,13 +430,18 @@ def patch_limits(params):
     return mk, ma
 
 
-def patch_list(s, pairs):
+def patch_list(s, pairs, maxdis=None, maxang=None):
     """calculate_patch_list on the given pairs (orc_patch_list): an M x 5
-    int32 array (p1, p2, potn, patch of p1, patch of p2)"""
+    int32 array (p1, p2, potn, patch of p1, patch of p2).  maxdis / maxang
+    replace max_check + 1 / max_ang (for tests that plant a mistake)."""
     keep = _Keep()
     N = len(s["x"])
     ps = make_patchset(s["patches"], keep)
     mk, ma = patch_limits(s["patches"]["params"])
+    if maxdis is not None:
+        mk = maxdis - 1.0
+    if maxang is not None:
+        ma = maxang
     b = make_box(s["box"], s.get("periodic", True))
     xa, xp = _d(s["x"])
     qa, qp = _d(s["Q"])

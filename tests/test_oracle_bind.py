@@ -74,6 +74,20 @@ def test_oracle_patch_list_matches_reference(case):
     assert np.array_equal(b, PL[case + "_blist"])
 
 
+def test_oracle_patch_list_fixture_pins_the_constants():
+    """plausible slips in the patch-list restatement change the two-species
+    list, so the reference fixture rejects each: max_ang taken as the
+    smallest cos(theta_m) (0.95) without its 0.6 start, the cut at max_check
+    instead of max_check + 1, and the bound-pair cone test used in place of
+    the -0.4 allowance"""
+    ref = PL["ab_plist"]
+    mk, ma = po.patch_limits(AB["patches"]["params"])
+    assert ma == 0.6 and mk == pytest.approx(1.68)
+    for kw in ({"maxang": 0.95}, {"maxdis": mk}):
+        assert not np.array_equal(po.patch_list(AB, PL["ab_pairs"], **kw), ref)
+    assert not np.array_equal(po.bound_pairs(AB, PL["ab_pairs"]), ref)
+
+
 def test_oracle_bind_fixture_has_every_path():
     """the wide-cone fixture reaches the large-component cut (sizes to 27),
     breaks bonds and re-forms them"""
