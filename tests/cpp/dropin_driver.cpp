@@ -152,7 +152,11 @@ static int rotor_main()
             const int n = std::max(1, (int)(geo.l / 4.0));
             matrix<int> boxes = geo.generate_boxes_relationships(n, tb);
             matrix<int> *pairs = b.calculatepairs(boxes, 3.5);
-            std::vector<int> opairs, runs_diff; // (patch list: not read by the engine)
+            // the patch list and its runs, as the reference's loop builds them
+            // (Condensate.cpp:474-479, every 10 steps); the engine's routine
+            // takes its candidates from its own pair walk
+            std::vector<patchint> opairs = b.calculate_patch_list(*pairs, pots);
+            std::vector<int> runs_diff = adjacency(opairs);
             matrix<double> F = b.calculateforces(*pairs, wsa);
             matrix<double> T(N, 3);
             b.calculate_forces_and_torques3D_onlyone_nonlets(opairs, runs_diff, pots, bbs, bm, F, T);
@@ -162,6 +166,10 @@ static int rotor_main()
                 if (i > 0 && i % 20 == 0) {
                     delete pairs;
                     pairs = b.calculatepairs(boxes, 3.5);
+                }
+                if (i > 0 && i % 10 == 0) {
+                    opairs = b.calculate_patch_list(*pairs, pots);
+                    runs_diff = adjacency(opairs);
                 }
                 b.advancemom_halfstep(F, T);
                 b.advance_pos();
