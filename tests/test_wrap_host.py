@@ -3,7 +3,9 @@ mtcode_amd/csrc/mdx_device.h, cube::correct_position_and_momentum of the
 reference's MDBase/geometry.h:403-445), fed NaN, +-inf and huge values: every
 call returns, non-finite inputs are reported (the kernels then keep the
 particle's previous state), finite results lie in the box, and a coordinate
-one box length out rounds as the reference's loop does.  CPU only."""
+one box length out rounds as the reference's loop does.  Also the drop-in's
+patch-list types (patchint, adjacency) on the reference's patch lists.  CPU
+only."""
 import os
 import shutil
 import subprocess
@@ -43,3 +45,25 @@ def test_orientation_store_on_host(tmp_path):
     print(r.stdout)
     assert r.returncode == 0, r.stdout
     assert r.stdout.strip().endswith("ok")
+
+
+def test_patch_list_types_on_host(tmp_path):
+    """the drop-in's patchint accessors and adjacency (host code of
+    mtcode_amd/host/mtcode_amd.hpp) on the compiled reference's patch lists
+    (tests/golden/plist.npz): the run starts equal the reference's adjacency"""
+    import numpy as np
+
+    exe = str(tmp_path / "plist_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}",
+                    f"-I{os.path.join(ROOT, 'mtcode_amd', 'host')}",
+                    os.path.join(ROOT, "tests", "cpp", "plist_check.cpp"), "-o", exe],
+                   check=True, capture_output=True)
+    PL = np.load(os.path.join(ROOT, "tests", "golden", "plist.npz"))
+    for case in ("ab", "wide", "wall"):
+        rows = np.ascontiguousarray(PL[case + "_plist"], dtype=np.int32)
+        src, dst = tmp_path / f"{case}.bin", tmp_path / f"{case}_adj.bin"
+        rows.tofile(src)
+        r = subprocess.run([exe, str(src), str(dst)], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0, r.stdout + r.stderr
+        assert r.stdout.strip().endswith("ok")
+        assert np.array_equal(np.fromfile(dst, dtype=np.int32), PL[case + "_adjacency"])
